@@ -1,0 +1,30 @@
+#!/bin/bash
+# Compile the reference's linear-solve sources (unmodified, read in place) and
+# link them with ref_solve_driver.f90.  Outputs only under oracle/_ref/.
+set -euo pipefail
+cd "$(dirname "$0")"
+: "${REF:=/root/reference}"
+: "${FC:=amdflang}"
+: "${FFLAGS:=-O2 -ffixed-line-length-none -fno-automatic -fopenmp -module-dir _ref/mod -I _ref/mod}"
+if [ ! -d "$REF/SolveEquations" ]; then
+  echo "reference sources not present at $REF; skipping reference build" >&2
+  exit 0
+fi
+mkdir -p _ref/obj _ref/mod
+SRCS=()
+for f in "$REF"/DeclareVariables/*; do SRCS+=("$f"); done
+SRCS+=("$REF/ProblemDesciption/User0.F90")
+for f in "$REF"/SolveEquations/AlgebraicSolvers/* "$REF"/SolveEquations/AlgebraicMultiGrid/* \
+         "$REF"/SolveEquations/FlowInArteries/*; do SRCS+=("$f"); done
+for f in "$REF"/Utilities/*; do
+  case "$(basename "$f")" in TimeStampRoutine.for) ;; *) SRCS+=("$f") ;; esac
+done
+OBJS=()
+for f in "${SRCS[@]}"; do
+  b=$(basename "$f"); o="_ref/obj/${b%.*}.o"
+  if [ ! -f "$o" ] || [ "$f" -nt "$o" ]; then $FC $FFLAGS -c "$f" -o "$o"; fi
+  OBJS+=("$o")
+done
+$FC $FFLAGS -c ref_solve_driver.f90 -o _ref/ref_solve_driver.o
+$FC -fopenmp -o _ref/ref_solve.x _ref/ref_solve_driver.o "${OBJS[@]}"
+echo "built _ref/ref_solve.x"

@@ -1,0 +1,113 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fsamd.h"
+
+namespace fs {
+
+void set_error(const std::string &msg);
+
+#define FS_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::fs::set_error(std::string(#call) + ": " + hipGetErrorString(e_));     \
+      return -2;                                                              \
+    }                                                                         \
+  } while (0)
+
+#define FS_HIP_THROW(call)                                                    \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw std::runtime_error(std::string(#call) + ": " +                    \
+                               hipGetErrorString(e_));                        \
+  } while (0)
+
+constexpr int kBlock = 256;     // threads per block for streaming kernels
+constexpr int kPartials = 1024; // fixed number of reduction partials
+
+// Device buffer owning a hipMalloc allocation.
+template <class T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    FS_HIP_THROW(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+    n = count;
+  }
+  void upload(const T *h, size_t count, hipStream_t s) {
+    alloc(count);
+    FS_HIP_THROW(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
+};
+
+struct Hierarchy; // fs_amg.hip
+
+} // namespace fs
+
+struct fs_mesh {
+  int n = 0, w = 0;
+  hipStream_t stream = nullptr;
+  std::vector<int> h_nbr;   // reference layout, 1-based, 0 = boundary
+  std::vector<int> h_nnbr;
+  std::vector<int> h_nbr_dev; // 0-based, -1 = none (the device layout)
+  fs::DevBuf<int> d_nbr;    // 0-based, -1 = none (slots past nnbr as well)
+  fs::DevBuf<int> d_mir;    // transpose map, linear anb index or -1
+  // exact natural-order D-ILU schedule (FactorizeUsingILU.for:29-75)
+  int ilu_nlevels = 0;
+  std::vector<int> ilu_level_ptr;
+  fs::DevBuf<int> d_ilu_cells;     // cells sorted by level
+  fs::DevBuf<int> d_ilu_ptr;       // [n+1] per-cell lower terms
+  fs::DevBuf<int> d_ilu_a, d_ilu_i, d_ilu_b; // anb(j,s) idx, i, anb(i,s') idx
+  // face topology (optional)
+  int nif = 0, nfaces_total = 0;
+  std::vector<int> h_owner, h_neigh;
+  fs::DevBuf<int> d_owner, d_neigh, d_iown, d_inbr;
+  fs::DevBuf<int> d_face_code;  // per (cell, slot): face*4 + role
+  // work vectors
+  fs::DevBuf<double> v[12];
+  fs::DevBuf<double> partial;  // kPartials * 4
+  fs::DevBuf<double> scal;     // scalar slots
+  fs::DevBuf<int> flags;       // [0] done, [1] iterations
+  // options
+  int mg_pre = 1, mg_post = 1, mg_maxcycles = 30, mg_maxlevels = 10;
+  int mg_minparents = 10, mg_reagg = 300, mg_fp32 = 0, mg_smoother = 1;
+  double mg_omega = 0.8;
+  int gmres_m = 30, check_every = 8;
+  int ncalls_mg = 0;
+  void *solve_work = nullptr;           // fs_solve.hip
+  std::shared_ptr<fs::Hierarchy> amg;   // fs_amg.hip
+};

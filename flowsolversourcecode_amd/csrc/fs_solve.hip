@@ -1,0 +1,793 @@
+// Linear-solve path on the device: the reference's SolveEquation
+// (SolveEquations/AlgebraicSolvers/SolveAlegbraicEquations.for:4) with its
+// residual norms, PBCG, Gauss-Seidel and D-ILU solvers, and the Krylov
+// solvers that carry the AMG preconditioner.
+//
+// Layout: every cell-indexed vector is FP64 in HBM; the ELL matrix keeps the
+// reference's column-major anb(N,W), so slot j of consecutive cells is one
+// contiguous stream and every thread's loads coalesce.  Gathers of x through
+// the neighbour table are the only irregular accesses.
+//
+// Reductions are deterministic: each block reduces with wavefront shuffles in
+// a fixed order into one of kPartials partials (grid-stride loop over a fixed
+// grid), and a one-block kernel combines the partials in index order.  Per
+// cell, every sum is formed in the reference's slot order, so a sweep,
+// residual or product of one cell is bitwise the reference's.
+//
+// Convergence never needs the host inside a batch: a one-block kernel
+// updates the device flags (done, iteration count), every iteration kernel
+// returns immediately once done is set, and the host reads the flags only
+// every check_every iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <stdexcept>
+
+#include "fs_common.h"
+#include "fs_kernels.h"
+#include "fs_solve.h"
+
+namespace fs {
+
+// ---------------------------------------------------------------- scalars
+enum Slot {
+  S_IRES = 0, S_FRES, S_RRF, S_BKNUM, S_BKDEN, S_AKDEN, S_BK, S_AK,
+  S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_TS, S_TT, S_RV, S_RES_ABS, S_RES_MAX,
+  S_RES_RMS, S_RES_SCALED, S_ACPHI, S_NSLOTS
+};
+enum Flag { F_DONE = 0, F_ITER = 1, F_ITMAX = 2, F_NFLAGS = 4 };
+
+static inline int grid_for(int n) {
+  int g = (n + kBlock - 1) / kBlock;
+  return g < 1 ? 1 : g;
+}
+static inline int grid_red(int n) {
+  int g = (n + kBlock - 1) / kBlock;
+  if (g > kPartials) g = kPartials;
+  return g < 1 ? 1 : g;
+}
+
+// y = A*x in the order of dsprsax (PBCGMethod.for:324-329): diagonal first,
+// then the off-diagonals in slot order.
+__device__ __forceinline__ double ell_ax(int i, int n, int w, const int *__restrict__ nbr,
+                                         const double *__restrict__ ac,
+                                         const double *__restrict__ anb,
+                                         const double *__restrict__ x) {
+  double s = ac[i] * x[i];
+  for (int j = 0; j < w; ++j) {
+    const int k = nbr[(size_t)j * n + i];
+    if (k >= 0) s = s + anb[(size_t)j * n + i] * x[k];
+  }
+  return s;
+}
+
+// r = b - A*x in the order of CheckConvergence (CheckConvergenceRoutine.for:
+// 23-34): b - ac*x first, then each off-diagonal subtracted in slot order.
+__device__ __forceinline__ double ell_res(int i, int n, int w, const int *__restrict__ nbr,
+                                          const double *__restrict__ ac,
+                                          const double *__restrict__ anb,
+                                          const double *__restrict__ b,
+                                          const double *__restrict__ x) {
+  double r = b[i] - ac[i] * x[i];
+  for (int j = 0; j < w; ++j) {
+    const int k = nbr[(size_t)j * n + i];
+    if (k >= 0) r = r - anb[(size_t)j * n + i] * x[k];
+  }
+  return r;
+}
+
+// y = A^T*x in the order of dsprstx (PBCGMethod.for:359-368): diagonal first,
+// then the contributions of source rows in increasing row order.  tlin/tsrc
+// list, per target cell, the linear anb index and source row of each
+// transposed entry, pre-sorted by (row, slot).
+__device__ __forceinline__ double ell_atx(int i, int n, int wt, const int *__restrict__ tlin,
+                                          const int *__restrict__ tsrc,
+                                          const double *__restrict__ ac,
+                                          const double *__restrict__ anb,
+                                          const double *__restrict__ x) {
+  double s = ac[i] * x[i];
+  for (int j = 0; j < wt; ++j) {
+    const int l = tlin[(size_t)j * n + i];
+    if (l >= 0) s = s + anb[l] * x[tsrc[(size_t)j * n + i]];
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Initial residual norms of SolveEquation (CalculateResidualsRoutine.for:
+// 19-67) with dphi = 0: r_i = bc_i.  Partials: sum|r|, max|r|, sum r^2,
+// max|ac*fit|.
+__global__ void k_init_norms(int n, const double *__restrict__ ac, const double *__restrict__ bc,
+                             const double *__restrict__ fit, double *__restrict__ part) {
+  double s1 = 0.0, m1 = -1.0, s2 = 0.0, m2 = -1.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double r = fabs(bc[i]);
+    s1 += r;
+    m1 = fmax(m1, r);
+    s2 += r * r;
+    m2 = fmax(m2, fabs(ac[i] * fit[i]));
+  }
+  s1 = block_sum<kBlock>(s1);
+  m1 = block_max<kBlock>(m1);
+  s2 = block_sum<kBlock>(s2);
+  m2 = block_max<kBlock>(m2);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1;
+    part[kPartials + blockIdx.x] = m1;
+    part[2 * kPartials + blockIdx.x] = s2;
+    part[3 * kPartials + blockIdx.x] = m2;
+  }
+}
+
+// L1 norm of the true residual bc - A*x (CheckConvergence).
+__global__ void k_resid_l1(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+                           const double *__restrict__ anb, const double *__restrict__ b,
+                           const double *__restrict__ x, double *__restrict__ part,
+                           const int *__restrict__ flags) {
+  if (flags && flags[F_DONE]) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    s += fabs(ell_res(i, n, w, nbr, ac, anb, b, x));
+  s = block_sum<kBlock>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// r = b - A*x, also copied to r0; partial L1 of r.
+__global__ void k_resid_vec(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+                            const double *__restrict__ anb, const double *__restrict__ b,
+                            const double *__restrict__ x, double *__restrict__ r,
+                            double *__restrict__ r0, double *__restrict__ part) {
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double v = ell_res(i, n, w, nbr, ac, anb, b, x);
+    r[i] = v;
+    if (r0) r0[i] = v;
+    s += fabs(v);
+  }
+  s = block_sum<kBlock>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_spmv(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+                       const double *__restrict__ anb, const double *__restrict__ x,
+                       double *__restrict__ y) {
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i < n) y[i] = ell_ax(i, n, w, nbr, ac, anb, x);
+}
+
+// PBCG start (PBCGMethod.for:147-161): r = bc - A*dphi, rr = r.
+__global__ void k_pbcg_start(int n, const double *__restrict__ bc, double *__restrict__ r,
+                             double *__restrict__ rr) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const double v = bc[i] - r[i];
+    r[i] = v;
+    rr[i] = v;
+  }
+}
+
+// bknum partials: sum (r*dc)*rr (PBCGMethod.for:163-170).
+__global__ void k_pbcg_bknum(int n, const double *__restrict__ r, const double *__restrict__ rr,
+                             const double *__restrict__ dc, double *__restrict__ part,
+                             const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    s += (r[i] * dc[i]) * rr[i];
+  s = block_sum<kBlock>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// p = bk*p + r*dc, pp = bk*pp + rr*dc; on the first iteration p = r*dc,
+// pp = rr*dc (PBCGMethod.for:172-192).
+__global__ void k_pbcg_p(int n, const double *__restrict__ r, const double *__restrict__ rr,
+                         const double *__restrict__ dc, double *__restrict__ p,
+                         double *__restrict__ pp, const double *__restrict__ scal,
+                         const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double z = r[i] * dc[i], zz = rr[i] * dc[i];
+  if (flags[F_ITER] == 0) {
+    p[i] = z;
+    pp[i] = zz;
+  } else {
+    const double bk = scal[S_BK];
+    p[i] = bk * p[i] + z;
+    pp[i] = bk * pp[i] + zz;
+  }
+}
+
+// z = A*p, zz = A^T*pp, akden partials = sum z*pp (PBCGMethod.for:195-205).
+__global__ void k_pbcg_ap(int n, int w, int wt, const int *__restrict__ nbr,
+                          const int *__restrict__ tlin, const int *__restrict__ tsrc,
+                          const double *__restrict__ ac, const double *__restrict__ anb,
+                          const double *__restrict__ p, const double *__restrict__ pp,
+                          double *__restrict__ z, double *__restrict__ zz,
+                          double *__restrict__ part, const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double zi = ell_ax(i, n, w, nbr, ac, anb, p);
+    z[i] = zi;
+    zz[i] = ell_atx(i, n, wt, tlin, tsrc, ac, anb, pp);
+    s += zi * pp[i];
+  }
+  s = block_sum<kBlock>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// dphi += ak*p, r -= ak*z, rr -= ak*zz (PBCGMethod.for:207-213).
+__global__ void k_pbcg_axpy(int n, double *__restrict__ dphi, double *__restrict__ r,
+                            double *__restrict__ rr, const double *__restrict__ p,
+                            const double *__restrict__ z, const double *__restrict__ zz,
+                            const double *__restrict__ scal, const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double ak = scal[S_AK];
+  dphi[i] = dphi[i] + ak * p[i];
+  r[i] = r[i] - ak * z[i];
+  rr[i] = rr[i] - ak * zz[i];
+}
+
+// Exact natural-order D-ILU diagonal, one dependency level per launch
+// (FactorizeUsingILU.for:23-75 in pull form): dc_j = 1/(ac_j - sum over the
+// lower couplings i of anb(j,i)*dc_i*anb(i,j)), terms in the reference's
+// push order.
+__global__ void k_ilu_level(int cnt, const int *__restrict__ cells, const int *__restrict__ ptr,
+                            const int *__restrict__ ta, const int *__restrict__ ti,
+                            const int *__restrict__ tb, const double *__restrict__ ac,
+                            const double *__restrict__ anb, double *__restrict__ dc) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= cnt) return;
+  const int j = cells[t];
+  double d = ac[j];
+  for (int e = ptr[j]; e < ptr[j + 1]; ++e) d = d - anb[ta[e]] * dc[ti[e]] * anb[tb[e]];
+  dc[j] = 1.0 / d;
+}
+
+// One colour of a multicolour Gauss-Seidel sweep: dphi_i = (bc_i - sum
+// anb*dphi_nb)/ac_i for the cells of the colour (SORMethod.for:22-36 per cell).
+__global__ void k_gs_color(int cnt, const int *__restrict__ cells, int n, int w,
+                           const int *__restrict__ nbr, const double *__restrict__ ac,
+                           const double *__restrict__ anb, const double *__restrict__ bc,
+                           double *__restrict__ x, const int *__restrict__ flags) {
+  if (flags && flags[F_DONE]) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = cells[t];
+  double s = bc[i];
+  for (int j = 0; j < w; ++j) {
+    const int k = nbr[(size_t)j * n + i];
+    if (k >= 0) s = s - anb[(size_t)j * n + i] * x[k];
+  }
+  x[i] = s / ac[i];
+}
+
+// ---- generic vector kernels for the Krylov solvers
+
+// partial dots: out0 = sum a*b, and if c: out1 = sum c*d
+__global__ void k_dot2(int n, const double *__restrict__ a, const double *__restrict__ b,
+                       const double *__restrict__ c, const double *__restrict__ d,
+                       double *__restrict__ part, const int *__restrict__ flags) {
+  if (flags && flags[F_DONE]) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    s0 += a[i] * b[i];
+    if (c) s1 += c[i] * d[i];
+  }
+  s0 = block_sum<kBlock>(s0);
+  s1 = block_sum<kBlock>(s1);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s0;
+    part[kPartials + blockIdx.x] = s1;
+  }
+}
+
+// BiCGSTAB: p = r + beta*(p - omega*v)
+__global__ void k_bicg_p(int n, const double *__restrict__ r, const double *__restrict__ v,
+                         double *__restrict__ p, const double *__restrict__ scal,
+                         const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (flags[F_ITER] == 0) {
+    p[i] = r[i];
+  } else {
+    p[i] = r[i] + scal[S_BETA] * (p[i] - scal[S_OMEGA] * v[i]);
+  }
+}
+
+// v = A*phat and partial (r0, v)
+__global__ void k_spmv_dot(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+                           const double *__restrict__ anb, const double *__restrict__ x,
+                           double *__restrict__ y, const double *__restrict__ d,
+                           double *__restrict__ part, const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double yi = ell_ax(i, n, w, nbr, ac, anb, x);
+    y[i] = yi;
+    s += yi * d[i];
+  }
+  s = block_sum<kBlock>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// s = r - alpha*v
+__global__ void k_bicg_s(int n, const double *__restrict__ r, const double *__restrict__ v,
+                         double *__restrict__ s, const double *__restrict__ scal,
+                         const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) s[i] = r[i] - scal[S_ALPHA] * v[i];
+}
+
+// t = A*shat, partials (t,s) and (t,t)
+__global__ void k_spmv_dot_ts(int n, int w, const int *__restrict__ nbr,
+                              const double *__restrict__ ac, const double *__restrict__ anb,
+                              const double *__restrict__ x, double *__restrict__ t,
+                              const double *__restrict__ s, double *__restrict__ part,
+                              const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double ti = ell_ax(i, n, w, nbr, ac, anb, x);
+    t[i] = ti;
+    a0 += ti * s[i];
+    a1 += ti * ti;
+  }
+  a0 = block_sum<kBlock>(a0);
+  a1 = block_sum<kBlock>(a1);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a0;
+    part[kPartials + blockIdx.x] = a1;
+  }
+}
+
+// x += alpha*phat + omega*shat; r = s - omega*t; partial L1(r)
+__global__ void k_bicg_update(int n, double *__restrict__ x, double *__restrict__ r,
+                              const double *__restrict__ phat, const double *__restrict__ shat,
+                              const double *__restrict__ s, const double *__restrict__ t,
+                              const double *__restrict__ scal, double *__restrict__ part,
+                              const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const double al = scal[S_ALPHA], om = scal[S_OMEGA];
+  double a = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[i] = x[i] + al * phat[i] + om * shat[i];
+    const double ri = s[i] - om * t[i];
+    r[i] = ri;
+    a += fabs(ri);
+  }
+  a = block_sum<kBlock>(a);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+// ---- GMRES helpers: w = A*z, Gram-Schmidt dots against V[0..k]
+__global__ void k_gmres_dots(int n, int k, const double *__restrict__ V, size_t ldv,
+                             const double *__restrict__ wv, double *__restrict__ part,
+                             const int *__restrict__ flags) {
+  // part[j*kPartials + block] = (V_j, w) for j = 0..k
+  if (flags[F_DONE]) return;
+  for (int j = 0; j <= k; ++j) {
+    double s = 0.0;
+    const double *vj = V + (size_t)j * ldv;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += vj[i] * wv[i];
+    s = block_sum<kBlock>(s);
+    if (threadIdx.x == 0) part[(size_t)j * kPartials + blockIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------- finalisers
+enum FinOp {
+  FIN_INIT_NORMS = 0, // res4, ires, done-before-start
+  FIN_PBCG_BK,        // bknum, bk, bkden
+  FIN_PBCG_AK,        // ak
+  FIN_CONV,           // fres from partials, iteration count, done
+  FIN_BICG_RHO,       // rho, beta
+  FIN_BICG_ALPHA,     // alpha = rho/(r0,v)
+  FIN_BICG_OMEGA,     // omega = (t,s)/(t,t)
+  FIN_TRUE_RES        // fres only
+};
+
+__device__ double reduce_partials(const double *part, int np) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
+  return block_sum<kBlock>(s);
+}
+__device__ double reduce_partials_max(const double *part, int np) {
+  double s = -1.0;
+  for (int i = threadIdx.x; i < np; i += kBlock) s = fmax(s, part[i]);
+  return block_max<kBlock>(s);
+}
+
+__global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ part,
+                      double *__restrict__ scal, int *__restrict__ flags) {
+  if (op != FIN_INIT_NORMS && op != FIN_TRUE_RES && flags[F_DONE]) return;
+  const double a = reduce_partials(part, np);
+  double b = 0.0, c = 0.0, d = 0.0;
+  if (op == FIN_INIT_NORMS) {
+    b = reduce_partials_max(part + kPartials, np);
+    c = reduce_partials(part + 2 * kPartials, np);
+    d = reduce_partials_max(part + 3 * kPartials, np);
+  } else if (op == FIN_BICG_OMEGA) {
+    b = reduce_partials(part + kPartials, np);
+  }
+  if (threadIdx.x != 0) return;
+  switch (op) {
+  case FIN_INIT_NORMS: {
+    scal[S_RES_ABS] = a;
+    scal[S_RES_MAX] = b;
+    scal[S_RES_RMS] = sqrt(c / n);
+    scal[S_RES_SCALED] = (nf == 4) ? 0.0 : b / (d + 1.0e-15);
+    scal[S_IRES] = a;
+    scal[S_FRES] = a;
+    flags[F_ITER] = 0;
+    flags[F_DONE] = !(a > scal[S_RRF] * a && 0 < flags[F_ITMAX]);
+    break;
+  }
+  case FIN_PBCG_BK: {
+    scal[S_BKNUM] = a;
+    if (flags[F_ITER] > 0) scal[S_BK] = a / (scal[S_BKDEN] + 1.0e-10);
+    scal[S_BKDEN] = a;
+    break;
+  }
+  case FIN_PBCG_AK:
+    scal[S_AKDEN] = a;
+    scal[S_AK] = scal[S_BKNUM] / (a + 1.0e-10);
+    break;
+  case FIN_CONV: {
+    scal[S_FRES] = a;
+    const int it = flags[F_ITER] + 1;
+    flags[F_ITER] = it;
+    // NaN residuals end the loop as well
+    flags[F_DONE] = !(a > scal[S_RRF] * scal[S_IRES] && it < flags[F_ITMAX]);
+    break;
+  }
+  case FIN_BICG_RHO: {
+    if (flags[F_ITER] > 0) scal[S_BETA] = (a / scal[S_RHO]) * (scal[S_ALPHA] / scal[S_OMEGA]);
+    scal[S_RHO] = a;
+    if (a == 0.0) flags[F_DONE] = 1; // breakdown: keep the current iterate
+    break;
+  }
+  case FIN_BICG_ALPHA:
+    scal[S_RV] = a;
+    scal[S_ALPHA] = (a != 0.0) ? scal[S_RHO] / a : 0.0;
+    break;
+  case FIN_BICG_OMEGA:
+    scal[S_TS] = a;
+    scal[S_TT] = b;
+    scal[S_OMEGA] = (b != 0.0) ? a / b : 0.0;
+    break;
+  case FIN_TRUE_RES:
+    scal[S_FRES] = a;
+    break;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+
+static void precompute_transpose(fs_mesh *m, std::vector<int> &tlin, std::vector<int> &tsrc, int &wt) {
+  const int n = m->n, w = m->w;
+  std::vector<int> deg(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int s = 0; s < m->h_nnbr[i]; ++s) {
+      const int k = m->h_nbr[(size_t)s * n + i];
+      if (k > 0 && k != i + 1) deg[k - 1]++;
+    }
+  wt = 1;
+  for (int i = 0; i < n; ++i) wt = std::max(wt, deg[i]);
+  tlin.assign((size_t)wt * n, -1);
+  tsrc.assign((size_t)wt * n, 0);
+  std::fill(deg.begin(), deg.end(), 0);
+  for (int i = 0; i < n; ++i) // source rows ascending, slots ascending
+    for (int s = 0; s < m->h_nnbr[i]; ++s) {
+      const int k = m->h_nbr[(size_t)s * n + i];
+      if (k > 0 && k != i + 1) {
+        const int j = k - 1, t = deg[j]++;
+        tlin[(size_t)t * n + j] = s * n + i;
+        tsrc[(size_t)t * n + j] = i;
+      }
+    }
+  (void)w;
+}
+
+// Emulates the loops of FactorizeUsingILU.for:29-75 to list, per cell, the
+// terms of its D-ILU diagonal in the order the reference applies them, and
+// groups the cells into dependency levels.
+static void precompute_ilu(fs_mesh *m) {
+  const int n = m->n, w = m->w;
+  const size_t total = (size_t)n * w;
+  std::vector<std::vector<int>> ta(n), ti(n), tb(n);
+  for (int i1 = 1; i1 <= n; ++i1) {
+    if (i1 == n - 1) continue; // FactorizeUsingILU.for:36
+    const int i3 = m->h_nnbr[i1 - 1];
+    for (int j2 = 1; j2 <= i3; ++j2) {
+      const int j4 = m->h_nbr[(size_t)(j2 - 1) * n + (i1 - 1)];
+      if (!(j4 > i1 && j4 <= n)) continue;
+      const int j3 = m->h_nnbr[j4 - 1];
+      int i2 = 0, i4 = -1;
+      while (i2 <= j3 && i4 != i1) {
+        i2 = i2 + 1;
+        const size_t lin = (size_t)(i2 - 1) * n + (j4 - 1);
+        i4 = lin < total ? m->h_nbr[lin] : 0;
+      }
+      if (i4 == i1) {
+        ta[j4 - 1].push_back((i2 - 1) * n + (j4 - 1));
+        ti[j4 - 1].push_back(i1 - 1);
+        tb[j4 - 1].push_back((j2 - 1) * n + (i1 - 1));
+      }
+    }
+  }
+  std::vector<int> level(n, 0), ptr(n + 1, 0), A, I, B;
+  int maxl = 0;
+  for (int j = 0; j < n; ++j) {
+    int l = 0;
+    for (int i : ti[j]) l = std::max(l, level[i] + 1);
+    level[j] = l;
+    maxl = std::max(maxl, l);
+    ptr[j + 1] = ptr[j] + (int)ta[j].size();
+    A.insert(A.end(), ta[j].begin(), ta[j].end());
+    I.insert(I.end(), ti[j].begin(), ti[j].end());
+    B.insert(B.end(), tb[j].begin(), tb[j].end());
+  }
+  m->ilu_nlevels = maxl + 1;
+  m->ilu_level_ptr.assign(maxl + 2, 0);
+  for (int j = 0; j < n; ++j) m->ilu_level_ptr[level[j] + 1]++;
+  for (int l = 0; l <= maxl; ++l) m->ilu_level_ptr[l + 1] += m->ilu_level_ptr[l];
+  std::vector<int> cells(n), fill(m->ilu_level_ptr.begin(), m->ilu_level_ptr.end() - 1);
+  for (int j = 0; j < n; ++j) cells[fill[level[j]]++] = j;
+  if (A.empty()) {
+    A.push_back(0);
+    I.push_back(0);
+    B.push_back(0);
+  }
+  m->d_ilu_cells.upload(cells, m->stream);
+  m->d_ilu_ptr.upload(ptr, m->stream);
+  m->d_ilu_a.upload(A, m->stream);
+  m->d_ilu_i.upload(I, m->stream);
+  m->d_ilu_b.upload(B, m->stream);
+}
+
+// Greedy colouring in cell order: each cell takes the smallest colour not
+// used by an already coloured neighbour.
+void greedy_colors(int n, int w, const int *nbr0, std::vector<int> &ptr, std::vector<int> &cells) {
+  std::vector<int> col(n, -1);
+  int nc = 0;
+  std::vector<int> used;
+  for (int i = 0; i < n; ++i) {
+    used.assign(nc + 1, 0);
+    for (int j = 0; j < w; ++j) {
+      const int k = nbr0[(size_t)j * n + i];
+      if (k >= 0 && col[k] >= 0) used[col[k]] = 1;
+    }
+    int c = 0;
+    while (c < nc && used[c]) ++c;
+    col[i] = c;
+    if (c == nc) ++nc;
+  }
+  ptr.assign(nc + 1, 0);
+  for (int i = 0; i < n; ++i) ptr[col[i] + 1]++;
+  for (int c = 0; c < nc; ++c) ptr[c + 1] += ptr[c];
+  cells.resize(n);
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  for (int i = 0; i < n; ++i) cells[fill[col[i]]++] = i;
+}
+
+struct SolveWork {
+  int wt = 1;
+  DevBuf<int> tlin, tsrc;
+  std::vector<int> color_ptr;
+  DevBuf<int> color_cells;
+  DevBuf<double> dc;
+};
+
+static SolveWork &work(fs_mesh *m) {
+  static_assert(sizeof(SolveWork) > 0, "");
+  return *reinterpret_cast<SolveWork *>(m->solve_work);
+}
+
+int mesh_prepare_solver(fs_mesh *m) {
+  auto *sw = new SolveWork();
+  m->solve_work = sw;
+  std::vector<int> tlin, tsrc;
+  precompute_transpose(m, tlin, tsrc, sw->wt);
+  sw->tlin.upload(tlin, m->stream);
+  sw->tsrc.upload(tsrc, m->stream);
+  precompute_ilu(m);
+  std::vector<int> nbr0((size_t)m->n * m->w);
+  for (size_t t = 0; t < nbr0.size(); ++t) nbr0[t] = m->h_nbr_dev[t];
+  std::vector<int> cells;
+  greedy_colors(m->n, m->w, nbr0.data(), sw->color_ptr, cells);
+  sw->color_cells.upload(cells, m->stream);
+  sw->dc.alloc(m->n);
+  for (auto &v : m->v) v.alloc(m->n);
+  m->partial.alloc(4 * kPartials);
+  m->scal.alloc(S_NSLOTS);
+  m->flags.alloc(F_NFLAGS);
+  FS_HIP_THROW(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+void mesh_release_solver(fs_mesh *m) {
+  delete reinterpret_cast<SolveWork *>(m->solve_work);
+  m->solve_work = nullptr;
+}
+
+// D-ILU diagonal on the device (exact reference order).
+static void ilu_factor(fs_mesh *m, const double *ac, const double *anb, double *dc) {
+  for (int l = 0; l < m->ilu_nlevels; ++l) {
+    const int b = m->ilu_level_ptr[l], e = m->ilu_level_ptr[l + 1];
+    if (e > b)
+      hipLaunchKernelGGL(k_ilu_level, dim3(grid_for(e - b)), dim3(kBlock), 0, m->stream, e - b,
+                         m->d_ilu_cells.p + b, m->d_ilu_ptr.p, m->d_ilu_a.p, m->d_ilu_i.p,
+                         m->d_ilu_b.p, ac, anb, dc);
+  }
+}
+
+static void host_flags(fs_mesh *m, int *fl) {
+  FS_HIP_THROW(hipMemcpyAsync(fl, m->flags.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, m->stream));
+  FS_HIP_THROW(hipStreamSynchronize(m->stream));
+}
+
+// Runs `body` in batches of check_every iterations until the device sets done.
+template <class F> static int run_batches(fs_mesh *m, int itmax, F body) {
+  int fl[F_NFLAGS];
+  host_flags(m, fl);
+  int launched = 0;
+  while (!fl[F_DONE] && launched < itmax) {
+    const int batch = std::min(m->check_every, itmax - launched);
+    for (int b = 0; b < batch; ++b) body();
+    launched += batch;
+    host_flags(m, fl);
+  }
+  return fl[F_ITER];
+}
+
+int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int solver, int lmg,
+                 const double *ac, const double *anb, const double *bc, double *dphi, double *res4,
+                 fs_solve_stats *st) {
+  const int n = m->n, w = m->w;
+  hipStream_t s = m->stream;
+  auto t0 = std::chrono::steady_clock::now();
+  const int gr = grid_red(n), ge = grid_for(n);
+  double *part = m->partial.p, *scal = m->scal.p;
+  int *flags = m->flags.p;
+  const int *nbr = m->d_nbr.p;
+  SolveWork &sw = work(m);
+
+  double h_sc[S_NSLOTS] = {0};
+  h_sc[S_RRF] = rrf;
+  int h_fl[F_NFLAGS] = {0, 0, itmax, 0};
+  FS_HIP(hipMemcpyAsync(scal, h_sc, sizeof(h_sc), hipMemcpyHostToDevice, s));
+  FS_HIP(hipMemcpyAsync(flags, h_fl, sizeof(h_fl), hipMemcpyHostToDevice, s));
+  FS_HIP(hipMemsetAsync(dphi, 0, sizeof(double) * n, s)); // dphi=0. (:47)
+  hipLaunchKernelGGL(k_init_norms, dim3(gr), dim3(kBlock), 0, s, n, ac, bc, fit, part);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_INIT_NORMS, gr, n, nf, part, scal, flags);
+
+  int iters = 0;
+  double *V0 = m->v[0].p, *V1 = m->v[1].p, *V2 = m->v[2].p, *V3 = m->v[3].p, *V4 = m->v[4].p,
+         *V5 = m->v[5].p, *V6 = m->v[6].p, *V7 = m->v[7].p;
+
+  if (lmg == FS_MG_SOLVER) {
+    iters = amg_solve(m, ac, anb, bc, dphi, solver);
+  } else if (solver == FS_PBCG) {
+    // r=V0 rr=V1 p=V2 pp=V3 z=V4 zz=V5 (PBCG1.F90:4)
+    ilu_factor(m, ac, anb, sw.dc.p);
+    hipLaunchKernelGGL(k_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, dphi, V0);
+    hipLaunchKernelGGL(k_pbcg_start, dim3(ge), dim3(kBlock), 0, s, n, bc, V0, V1);
+    hipLaunchKernelGGL(k_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, V0, V1);
+    const bool pre = (lmg == FS_MG_PRECOND);
+    if (pre) amg_prepare(m, ac, anb);
+    iters = run_batches(m, itmax, [&]() {
+      hipLaunchKernelGGL(k_pbcg_bknum, dim3(gr), dim3(kBlock), 0, s, n, V0, V1, sw.dc.p, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_PBCG_BK, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_pbcg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V1, sw.dc.p, V2, V3, scal, flags);
+      hipLaunchKernelGGL(k_pbcg_ap, dim3(gr), dim3(kBlock), 0, s, n, w, sw.wt, nbr, sw.tlin.p,
+                         sw.tsrc.p, ac, anb, V2, V3, V4, V5, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_PBCG_AK, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_pbcg_axpy, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, V1, V2, V4, V5, scal, flags);
+      hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+    });
+  } else if (solver == FS_SOR) {
+    // multicolour symmetric Gauss-Seidel: colours forward then backward
+    iters = run_batches(m, itmax, [&]() {
+      const int nc = (int)sw.color_ptr.size() - 1;
+      for (int c = 0; c < nc; ++c) {
+        const int b = sw.color_ptr[c], e = sw.color_ptr[c + 1];
+        hipLaunchKernelGGL(k_gs_color, dim3(grid_for(e - b)), dim3(kBlock), 0, s, e - b,
+                           sw.color_cells.p + b, n, w, nbr, ac, anb, bc, dphi, flags);
+      }
+      for (int c = nc - 1; c >= 0; --c) {
+        const int b = sw.color_ptr[c], e = sw.color_ptr[c + 1];
+        hipLaunchKernelGGL(k_gs_color, dim3(grid_for(e - b)), dim3(kBlock), 0, s, e - b,
+                           sw.color_cells.p + b, n, w, nbr, ac, anb, bc, dphi, flags);
+      }
+      hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+    });
+  } else if (solver == FS_BICGS) {
+    // right-preconditioned BiCGSTAB; r=V0 r0=V1 p=V2 v=V3 s=V4 t=V5 phat=V6 shat=V7
+    const bool pre = (lmg == FS_MG_PRECOND);
+    if (pre) amg_prepare(m, ac, anb);
+    hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, V1, part);
+    FS_HIP(hipMemsetAsync(V3, 0, sizeof(double) * n, s));
+    iters = run_batches(m, itmax, [&]() {
+      hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V1, V0, nullptr, nullptr, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_BICG_RHO, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V2, scal, flags);
+      if (pre) amg_apply(m, V2, V6, flags);
+      const double *ph = pre ? V6 : V2;
+      hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, ph, V3, V1, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_BICG_ALPHA, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_bicg_s, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V4, scal, flags);
+      if (pre) amg_apply(m, V4, V7, flags);
+      const double *sh = pre ? V7 : V4;
+      hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, sh, V5, V4, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_BICG_OMEGA, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, ph, sh, V4, V5, scal, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+    });
+  } else if (solver == FS_ILU) {
+    set_error("solver 'ilu' (sequential D-ILU sweeps) is not offered on the device; use pbcg, bicgs or sor");
+    return -3;
+  } else {
+    set_error("unsupported solver code " + std::to_string(solver));
+    return -3;
+  }
+  (void)V5;
+  (void)V6;
+  (void)V7;
+
+  // true final residual, fit += dphi (:180)
+  int fl[F_NFLAGS];
+  host_flags(m, fl);
+  hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, nullptr);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_TRUE_RES, gr, n, nf, part, scal, flags);
+  add_inplace(m, fit, dphi);
+  double h_out[S_NSLOTS];
+  FS_HIP(hipMemcpyAsync(h_out, scal, sizeof(h_out), hipMemcpyDeviceToHost, s));
+  FS_HIP(hipStreamSynchronize(s));
+  if (res4) {
+    res4[0] = h_out[S_RES_ABS];
+    res4[1] = h_out[S_RES_MAX];
+    res4[2] = h_out[S_RES_RMS];
+    res4[3] = h_out[S_RES_SCALED];
+  }
+  if (lmg != FS_MG_SOLVER) iters = fl[F_ITER];
+  if (st) {
+    st->iterations = iters;
+    st->levels = (lmg != FS_MG_OFF) ? amg_levels(m) : 1;
+    st->initial_l1 = h_out[S_IRES];
+    st->final_l1 = h_out[S_FRES];
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  const bool converged = !(h_out[S_FRES] > rrf * h_out[S_IRES]);
+  if (std::isnan(h_out[S_FRES])) {
+    set_error("residual is NaN");
+    return -4;
+  }
+  return converged ? 0 : 1;
+}
+
+__global__ void k_add(int n, double *__restrict__ a, const double *__restrict__ b) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) a[i] = a[i] + b[i];
+}
+
+void launch_fin_conv(fs_mesh *m, int np) {
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, m->stream, (int)FIN_CONV, np, m->n, 4,
+                     m->partial.p, m->scal.p, m->flags.p);
+}
+
+void add_inplace(fs_mesh *m, double *a, const double *b) {
+  hipLaunchKernelGGL(k_add, dim3(grid_for(m->n)), dim3(kBlock), 0, m->stream, m->n, a, b);
+}
+
+} // namespace fs

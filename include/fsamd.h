@@ -1,0 +1,159 @@
+/*
+ * fsamd.h -- C ABI of the MI355X (gfx950) library for the per-iteration hot
+ * path of the FlowSolver finite-volume code.
+ *
+ * Array conventions are the reference's module globals, unchanged, so a
+ * Fortran host can pass its own arrays through ISO_C_BINDING:
+ *   - cell arrays ac(N), bc(N), FiT(N), dphi(N)            (double)
+ *   - ELL arrays anb(N,W), ElementNeighbor(N,W)             (column-major:
+ *     element (i,j) at [j*N + i]); neighbour numbers are 1-based, 0 = the slot
+ *     of a boundary face (DeclareVariables/Variables2.F90:10-20,
+ *     DeclareVariables/Geometry3.F90:34-35).
+ *   - default INTEGER is int32; LOGICAL is passed as int32 (0 / nonzero).
+ * Strings of the reference (solver names, variable names) become enum codes.
+ *
+ * Return codes: 0 = converged; 1 = stopped at itmax (the residual is still
+ * reported, as the reference does); negative = invalid input or device error
+ * (fs_last_error() gives the message).  A negative code leaves the caller's
+ * arrays untouched so the host can fall back to its own solver.
+ *
+ * Threading: one call in flight per fs_mesh; each mesh owns one HIP stream
+ * and every entry point synchronises before it returns.
+ */
+#ifndef FSAMD_H
+#define FSAMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fs_mesh fs_mesh;
+
+/* solver codes: the reference's ASSolver* strings (User0.F90:288-301) */
+enum fs_solver {
+  FS_SOR = 0,    /* 'sor'    symmetric Gauss-Seidel (SORMethod.for:4)        */
+  FS_ILU = 1,    /* 'ilu'    D-ILU (FactorizeUsingILU.for:4, ILUMethod.for:4) */
+  FS_PBCG = 2,   /* 'pbcg'   ILU-diagonal biconjugate gradient (PBCGMethod.for:96) */
+  FS_DIRECT = 3, /* 'direct' dense LU -- not offered on the device             */
+  FS_GMRES = 4,  /* 'gmres'  restarted GMRES (matrixSolverOMP.f90:48)          */
+  FS_BICGS = 5   /* 'bicgs'  BiCGSTAB (matrixSolverOMP.f90:306)                */
+};
+
+/* multigrid modes for the LMultigrid argument */
+enum fs_multigrid {
+  FS_MG_OFF = 0,     /* no multigrid                                           */
+  FS_MG_SOLVER = 1,  /* the reference's AlgebraicMultigrid: V-cycles as the
+                        solver, smoother chosen by the solver code
+                        (AlgebraicMultiGridMainRoutine.for:4)                 */
+  FS_MG_PRECOND = 2  /* one AMG V-cycle as the preconditioner of the Krylov
+                        solver (PBCG, GMRES, BiCGSTAB) -- no reference
+                        counterpart                                           */
+};
+
+/* option keys for fs_set_option (defaults in brackets) */
+enum fs_option {
+  FS_OPT_MG_PRESWEEP = 1,     /* MultiGridpreSweep [1 on device]                */
+  FS_OPT_MG_POSTSWEEP = 2,    /* MultiGridpostSweep [1 on device]               */
+  FS_OPT_MG_MAXCYCLES = 3,    /* MaxMultiGridCycles [30]                        */
+  FS_OPT_MG_MAXLEVELS = 4,    /* maxNumberofCoarseLevels [10]                   */
+  FS_OPT_MG_MINPARENTS = 5,   /* minNumberofParents [10]                        */
+  FS_OPT_MG_REAGGLOMERATE = 6,/* reAgglomorate: rebuild the aggregates every n
+                                 calls; otherwise only the Galerkin operators
+                                 are recomputed [300]                          */
+  FS_OPT_MG_FP32 = 7,         /* 1 = hierarchy and smoother in FP32 inside the
+                                 FP64 Krylov loop [0]                          */
+  FS_OPT_MG_OMEGA = 8,        /* damping of the device Jacobi smoother [0.8]    */
+  FS_OPT_MG_SMOOTHER = 9,     /* 0 = damped Jacobi, 1 = multicolour symmetric
+                                 Gauss-Seidel [1]                              */
+  FS_OPT_GMRES_RESTART = 10,  /* GMRES restart length [30]                      */
+  FS_OPT_CHECK_EVERY = 11     /* iterations launched between host checks [8]    */
+};
+
+/* Statistics of the last solve. */
+typedef struct fs_solve_stats {
+  int iterations;       /* Krylov / stationary iterations or MG cycles       */
+  int levels;           /* AMG levels in use (1 = no hierarchy)              */
+  double initial_l1;    /* sum |bc - A*0|                                    */
+  double final_l1;      /* sum |bc - A*dphi| (true residual)                 */
+  double seconds;       /* wall time of the solve on the device              */
+} fs_solve_stats;
+
+/* Topology of the mesh (ReadFoamMesh.f90:288-619; Geometry3.F90:21-37).
+ * elem_nbr: ElementNeighbor(N,W) column-major, 1-based, 0 = boundary slot.
+ * n_nbr   : NumberofElementNeighbors(N).
+ * The face arrays may be NULL when only the linear-solve entry points are
+ * used: owner/neighbour = NIFaceOwner/NIFaceNeighbor(Fi), i_own_nbr /
+ * i_nbr_own = iOwnerNeighbor/iNeighborOwner(Fi) (1-based slots). */
+int fs_mesh_create(int n_cells, int width, const int *elem_nbr,
+                   const int *n_nbr, int n_ifaces, const int *owner,
+                   const int *neighbour, const int *i_own_nbr,
+                   const int *i_nbr_own, fs_mesh **out);
+int fs_mesh_destroy(fs_mesh *mesh);
+int fs_set_option(fs_mesh *mesh, int key, double value);
+
+/* Drop-in for SUBROUTINE SolveEquation(Variable,FiT,rrF,itmax,solver,
+ * LMultigrid) (SolveEquations/AlgebraicSolvers/SolveAlegbraicEquations.for:4)
+ * with its module inputs ac, anb, bc (Variables2) passed explicitly.
+ * nf is the GetNFfromVariable code (AssignNFvaluesToVariables.for:4; 4 = p').
+ * On return fit += dphi, dphi holds the correction, res4 = ResorAbs, ResorMax,
+ * ResorRMS, ResorScaled of the initial residual (CalculateResidualsRoutine.for
+ * :19-67).  Host arrays; copies in and out over PCIe. */
+int fs_solve_equation(fs_mesh *mesh, int nf, double *fit, double rrf,
+                      int itmax, int solver, int lmultigrid, const double *ac,
+                      const double *anb, const double *bc, double *dphi,
+                      double *res4, fs_solve_stats *stats);
+
+/* The same solve on device-resident arrays (hipMalloc'ed, same layouts). */
+int fs_solve_equation_device(fs_mesh *mesh, int nf, double *d_fit, double rrf,
+                             int itmax, int solver, int lmultigrid,
+                             const double *d_ac, const double *d_anb,
+                             const double *d_bc, double *d_dphi, double *res4,
+                             fs_solve_stats *stats);
+
+/* Cell-to-face map NGlobalEFaces(N,W) (column-major, 1-based global face
+ * numbers: interior faces 1..Fi first, then boundary faces patch by patch)
+ * with NumbOfElementFaces(N) (ReadFoamMesh.f90:329-342; Geometry1.F90:14).
+ * Needed by the face-flux assembly; slot j of a cell must be the face across
+ * which ElementNeighbor(cell,j) lies.  Requires the interior owner arrays of
+ * fs_mesh_create. */
+int fs_mesh_set_cell_faces(fs_mesh *mesh, int n_faces_total,
+                           const int *global_efaces, const int *n_efaces);
+
+/* Scatter of face fluxes into the ELL system, AssembleGlobalMatrixFaceFluxes
+ * and AssembleGlobalMatrixElementFluxes (Assemble/
+ * AssembleGolbalElementAndFaceFluxes.for:43-125, :4-39), as a deterministic
+ * per-cell gather.  flux_* are FluxCf/FluxFf/FluxTf(NFacesTotal) with the
+ * interior faces first (Variables2.F90:4-7); only interior faces and the
+ * boundary faces listed in the mesh are used; flux_ce/flux_te are
+ * FluxCE/FluxTE(N) (Variables3.F90:4-6).  ac/anb/bc are accumulated into
+ * (the reference zeroes them in InitializeCoefficients before assembly).
+ * Device pointers. */
+int fs_assemble_face_fluxes_device(fs_mesh *mesh, const double *d_flux_cf,
+                                   const double *d_flux_ff,
+                                   const double *d_flux_tf,
+                                   const double *d_flux_ce,
+                                   const double *d_flux_te, double *d_ac,
+                                   double *d_anb, double *d_bc);
+
+/* Inspection of the AMG hierarchy held by the mesh (after a multigrid
+ * solve): dims = {n, w, total children, levels, colours} of level `level`;
+ * maps in the reference's numbering (1-based): parent (this level -> next),
+ * seed/nchild/children of this level's aggregates (ElementParent,
+ * NumberOfChildren, Children), nbr/nnbr (ElementNeighborMG,
+ * NumberofElementNeighborsMG), ac/anb (acMG/anbMG of the last solve).
+ * Any pointer may be NULL.  (AgglomorateALevelAlgebraicly.for:48-331) */
+int fs_amg_export(fs_mesh *mesh, int level, int *dims, int *parent, int *seed,
+                  int *nchild, int *child_ptr, int *child, int *nbr,
+                  int *nnbr, double *ac, double *anb);
+
+/* Message of the last failing call on this thread. */
+const char *fs_last_error(void);
+/* Library version string. */
+const char *fs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

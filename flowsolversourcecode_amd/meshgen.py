@@ -1,0 +1,135 @@
+"""Seeded synthetic meshes in polyMesh form (the reference reads OpenFOAM
+polyMesh directories, SolveEquations/GridAndGeometry/ReadFoamMesh.f90:31-250).
+
+``box_hex(nx, ny, nz, ...)`` builds an nx*ny*nz hexahedral block on
+[0,lx]x[0,ly]x[0,lz] with the polyMesh conventions the reference relies on:
+
+* interior faces first, ordered by owner and then by neighbour (owner <
+  neighbour), with the face normal (right-hand node order) pointing from
+  owner to neighbour;
+* boundary faces after them, grouped by patch in the order xmin, xmax, ymin,
+  ymax, zmin, zmax, each patch ordered by owner cell, normals outward.
+
+Interior nodes can be moved by ``perturb`` times the local spacing per axis,
+uniformly in [-perturb, perturb], from ``numpy.random.default_rng(seed)``
+(boundary nodes stay on the box faces so the domain is unchanged).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PATCHES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
+
+
+class PolyMesh:
+    def __init__(self, points, face_ptr, face_nodes, owner, neighbour, patch_names, patch_sizes, shape):
+        self.points = points            # (P, 3) float64
+        self.face_ptr = face_ptr        # (F+1,) int32
+        self.face_nodes = face_nodes    # (sum,) int32, 0-based
+        self.owner = owner              # (F,) int32, 0-based
+        self.neighbour = neighbour      # (Fi,) int32, 0-based
+        self.patch_names = list(patch_names)
+        self.patch_sizes = np.asarray(patch_sizes, np.int32)
+        self.shape = shape
+
+    @property
+    def n_cells(self):
+        return int(max(self.owner.max(), self.neighbour.max() if len(self.neighbour) else -1)) + 1
+
+    @property
+    def n_faces(self):
+        return len(self.owner)
+
+    @property
+    def n_internal(self):
+        return len(self.neighbour)
+
+    def patch_face_range(self, name):
+        i = self.patch_names.index(name)
+        start = self.n_internal + int(self.patch_sizes[:i].sum())
+        return start, start + int(self.patch_sizes[i])
+
+
+def box_hex(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, perturb=0.0, seed=0):
+    ix = np.arange(nx + 1)
+    iy = np.arange(ny + 1)
+    iz = np.arange(nz + 1)
+    X, Y, Z = np.meshgrid(ix * (lx / nx), iy * (ly / ny), iz * (lz / nz), indexing="ij")
+    # point index p(i,j,k) = i + (nx+1)*(j + (ny+1)*k): order points with i fastest
+    pts = np.stack([X.transpose(2, 1, 0).ravel(), Y.transpose(2, 1, 0).ravel(), Z.transpose(2, 1, 0).ravel()], 1)
+    if perturb > 0.0:
+        rng = np.random.default_rng(seed)
+        I, J, K = np.meshgrid(ix, iy, iz, indexing="ij")
+        I, J, K = (a.transpose(2, 1, 0).ravel() for a in (I, J, K))
+        interior = (I > 0) & (I < nx) & (J > 0) & (J < ny) & (K > 0) & (K < nz)
+        d = rng.uniform(-perturb, perturb, size=(pts.shape[0], 3))
+        d *= np.array([lx / nx, ly / ny, lz / nz])
+        pts = pts + d * interior[:, None]
+
+    def P(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    def C(i, j, k):
+        return i + nx * (j + ny * k)
+
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci, cj, ck = (a.transpose(2, 1, 0).ravel() for a in (ci, cj, ck))  # cell order, i fastest
+    cell = C(ci, cj, ck)
+    # interior faces of each cell towards +x, +y, +z (increasing neighbour index)
+    fx_mask = ci < nx - 1
+    fy_mask = cj < ny - 1
+    fz_mask = ck < nz - 1
+    quads, own, nbr, key = [], [], [], []
+    # +x face at i+1: (i+1,j,k),(i+1,j+1,k),(i+1,j+1,k+1),(i+1,j,k+1): normal +x
+    q = np.stack([P(ci + 1, cj, ck), P(ci + 1, cj + 1, ck), P(ci + 1, cj + 1, ck + 1), P(ci + 1, cj, ck + 1)], 1)
+    quads.append(q[fx_mask]); own.append(cell[fx_mask]); nbr.append(cell[fx_mask] + 1)
+    # +y face at j+1: (i,j+1,k),(i,j+1,k+1),(i+1,j+1,k+1),(i+1,j+1,k): normal +y
+    q = np.stack([P(ci, cj + 1, ck), P(ci, cj + 1, ck + 1), P(ci + 1, cj + 1, ck + 1), P(ci + 1, cj + 1, ck)], 1)
+    quads.append(q[fy_mask]); own.append(cell[fy_mask]); nbr.append(cell[fy_mask] + nx)
+    # +z face at k+1: (i,j,k+1),(i+1,j,k+1),(i+1,j+1,k+1),(i,j+1,k+1): normal +z
+    q = np.stack([P(ci, cj, ck + 1), P(ci + 1, cj, ck + 1), P(ci + 1, cj + 1, ck + 1), P(ci, cj + 1, ck + 1)], 1)
+    quads.append(q[fz_mask]); own.append(cell[fz_mask]); nbr.append(cell[fz_mask] + nx * ny)
+    q_int = np.concatenate(quads)
+    o_int = np.concatenate(own)
+    n_int = np.concatenate(nbr)
+    order = np.lexsort((n_int, o_int))  # by owner, then neighbour
+    q_int, o_int, n_int = q_int[order], o_int[order], n_int[order]
+
+    bq, bo, sizes = [], [], []
+    # outward normals
+    m = ci == 0  # xmin: normal -x (reverse of the +x quad)
+    q = np.stack([P(ci, cj, ck), P(ci, cj, ck + 1), P(ci, cj + 1, ck + 1), P(ci, cj + 1, ck)], 1)
+    bq.append(q[m]); bo.append(cell[m]); sizes.append(int(m.sum()))
+    m = ci == nx - 1
+    q = np.stack([P(ci + 1, cj, ck), P(ci + 1, cj + 1, ck), P(ci + 1, cj + 1, ck + 1), P(ci + 1, cj, ck + 1)], 1)
+    bq.append(q[m]); bo.append(cell[m]); sizes.append(int(m.sum()))
+    m = cj == 0
+    q = np.stack([P(ci, cj, ck), P(ci + 1, cj, ck), P(ci + 1, cj, ck + 1), P(ci, cj, ck + 1)], 1)
+    bq.append(q[m]); bo.append(cell[m]); sizes.append(int(m.sum()))
+    m = cj == ny - 1
+    q = np.stack([P(ci, cj + 1, ck), P(ci, cj + 1, ck + 1), P(ci + 1, cj + 1, ck + 1), P(ci + 1, cj + 1, ck)], 1)
+    bq.append(q[m]); bo.append(cell[m]); sizes.append(int(m.sum()))
+    m = ck == 0
+    q = np.stack([P(ci, cj, ck), P(ci, cj + 1, ck), P(ci + 1, cj + 1, ck), P(ci + 1, cj, ck)], 1)
+    bq.append(q[m]); bo.append(cell[m]); sizes.append(int(m.sum()))
+    m = ck == nz - 1
+    q = np.stack([P(ci, cj, ck + 1), P(ci + 1, cj, ck + 1), P(ci + 1, cj + 1, ck + 1), P(ci, cj + 1, ck + 1)], 1)
+    bq.append(q[m]); bo.append(cell[m]); sizes.append(int(m.sum()))
+
+    quads = np.concatenate([q_int] + bq).astype(np.int32)
+    owner = np.concatenate([o_int] + bo).astype(np.int32)
+    face_ptr = (np.arange(len(quads) + 1) * 4).astype(np.int32)
+    return PolyMesh(pts.astype(np.float64), face_ptr, quads.ravel().copy(), owner, n_int.astype(np.int32),
+                    PATCHES, sizes, (nx, ny, nz))
+
+
+def lid_driven_cavity_walls(mesh, lid="ymax", lid_velocity=(1.0, 0.0, 0.0)):
+    """Wall velocity of every boundary face: the lid patch moves, the other
+    walls are at rest.  Returns (bu, bv, bw), one entry per boundary face."""
+    nb = mesh.n_faces - mesh.n_internal
+    bu, bv, bw = np.zeros(nb), np.zeros(nb), np.zeros(nb)
+    s, e = mesh.patch_face_range(lid)
+    s -= mesh.n_internal
+    e -= mesh.n_internal
+    bu[s:e], bv[s:e], bw[s:e] = lid_velocity
+    return bu, bv, bw

@@ -1,0 +1,84 @@
+/*
+ * oracle/or_simple.h -- CPU restatement of the reference's mesh topology,
+ * geometry and steady SIMPLE iteration for a laminar, incompressible,
+ * constant-property flow in a box bounded by no-slip walls (moving or at
+ * rest).  TEST INFRASTRUCTURE ONLY.
+ */
+#ifndef OR_SIMPLE_H
+#define OR_SIMPLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  /* polyMesh input (0-based) */
+  int npoints, nfaces, nif, ncells, npatches;
+  double *x, *y, *z;
+  int *fptr, *fnodes, *owner, *neigh, *patch_start;
+  /* topology in the reference's numbering (cell numbers 1-based) */
+  int w;        /* NElementFaces */
+  int *nef;     /* NumbOfElementFaces(N) */
+  int *gef;     /* NGlobalEFaces(N,W) column-major, 1-based face numbers */
+  int *enbr;    /* ElementNeighbor(N,W), 0 = boundary slot */
+  int *iown, *inbr; /* iOwnerNeighbor / iNeighborOwner (Fi), 1-based slots */
+  int *nen, *len, len_w; /* NumbOfElementNodes, ListOfElementNodes (N,len_w), 0-based nodes */
+  /* geometry over all faces (interior first) and cells */
+  double *Sx, *Sy, *Sz, *A, *fcx, *fcy, *fcz, *nx, *ny, *nz;
+  double *vol, *xc, *yc, *zc;
+  double *dcf, *dx, *dy, *dz, *ex, *ey, *ez, *g;
+  double *Ex, *Ey, *Ez, *E, *gdiff, *Tx, *Ty, *Tz, *T;
+} or_mesh;
+
+or_mesh *or_mesh_new(int npoints, const double *x, const double *y, const double *z, int nfaces,
+                     const int *fptr, const int *fnodes, const int *owner, int nif,
+                     const int *neigh, int npatches, const int *patch_size);
+void or_mesh_delete(or_mesh *m);
+void or_mesh_free(or_mesh *m);
+int or_mesh_topology(or_mesh *m);
+int or_mesh_geometry(or_mesh *m);
+
+/* one SIMPLE case on a mesh */
+typedef struct {
+  or_mesh *m;
+  int N, Fi, Fb, W;
+  /* boundary faces (index b = face - Fi): wall velocity */
+  double *bu0, *bv0, *bw0;
+  /* settings */
+  double rho, mu, urf_u, urf_p;
+  int msolver, mitmax, psolver, pitmax, plmg, ngrad, refcell;
+  double mrrf, prrf;
+  /* fields */
+  double *u, *v, *w, *p, *pc, *mdot;
+  double *gu, *gv, *gw, *gp, *gpc;      /* 3N: x block, y block, z block */
+  double *bu, *bv, *bw, *bp, *bpc;      /* Fb */
+  double *bgu, *bgv, *bgw, *bgp, *bgpc; /* 3Fb */
+  double *fgu, *fgv, *fgw, *fgp;        /* 3Fi face gradients */
+  double *du, *dv, *dw, *us, *vs, *ws;  /* D1 (= D2 for SIMPLE), star */
+  double *visc;                          /* cell viscosity */
+  /* assembly work */
+  double *fcf, *fff, *fvf, *ftf;        /* all faces */
+  double *fce, *fte, *effdiv;           /* cells */
+  double *ac, *anb, *bc, *dphi;
+  double res_mom[3][4], res_pc[4], mass_l1;
+  int it_mom[3], it_pc;
+} or_case;
+
+or_case *or_case_new(or_mesh *m, const double *wall_u, const double *wall_v,
+                     const double *wall_w, double rho, double mu, double urf_u,
+                     double urf_p, int msolver, double mrrf, int mitmax,
+                     int psolver, double prrf, int pitmax);
+void or_case_delete(or_case *c);
+void or_gradient(or_case *c, const double *phi, const double *bphi, double *gr, double *bgr);
+void or_case_momentum_gradients(or_case *c);
+void or_case_momentum_assemble(or_case *c, int comp);
+void or_case_momentum_solve(or_case *c, int comp);
+void or_case_pc_assemble(or_case *c);
+void or_case_pc_solve_correct(or_case *c);
+void or_case_iterate(or_case *c, int niter);
+double or_case_mass_imbalance(or_case *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -221,3 +221,138 @@ class Amg:
         hist = np.zeros(max(maxcycles, 1))
         k = lib().or_amg_solve(self.h, _c(bc, np.float64), ires, rrf, maxcycles, npre, npost, dphi, hist)
         return dphi, hist[:k].copy()
+
+
+# ----------------------------------------------------------------------------
+# mesh topology / geometry and the SIMPLE case (or_geom.c, or_simple.c)
+
+class _OrMesh(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("npoints", "nfaces", "nif", "ncells", "npatches")] + \
+               [(n, C.POINTER(C.c_double)) for n in ("x", "y", "z")] + \
+               [(n, C.POINTER(C.c_int)) for n in ("fptr", "fnodes", "owner", "neigh", "patch_start")] + \
+               [("w", C.c_int)] + \
+               [(n, C.POINTER(C.c_int)) for n in ("nef", "gef", "enbr", "iown", "inbr", "nen", "len")] + \
+               [("len_w", C.c_int)] + \
+               [(n, C.POINTER(C.c_double)) for n in (
+                   "Sx", "Sy", "Sz", "A", "fcx", "fcy", "fcz", "nx", "ny", "nz", "vol", "xc", "yc", "zc",
+                   "dcf", "dx", "dy", "dz", "ex", "ey", "ez", "g", "Ex", "Ey", "Ez", "E", "gdiff", "Tx", "Ty",
+                   "Tz", "T")]
+
+
+FACE_FIELDS = ("Sx", "Sy", "Sz", "A", "fcx", "fcy", "fcz", "nx", "ny", "nz", "dcf", "dx", "dy", "dz", "ex", "ey",
+               "ez", "g", "Ex", "Ey", "Ez", "E", "gdiff", "Tx", "Ty", "Tz", "T")
+CELL_FIELDS = ("vol", "xc", "yc", "zc")
+
+
+class OrMesh:
+    """Topology and geometry of a polyMesh computed by the restatement."""
+
+    def __init__(self, pm):
+        L = lib()
+        L.or_mesh_new.restype = C.POINTER(_OrMesh)
+        L.or_mesh_new.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, _ip, _ip, C.c_int, _ip, C.c_int, _ip]
+        L.or_mesh_delete.argtypes = [C.POINTER(_OrMesh)]
+        pts = np.ascontiguousarray(pm.points)
+        self.ptr = L.or_mesh_new(len(pts), _c(pts[:, 0], np.float64), _c(pts[:, 1], np.float64),
+                                 _c(pts[:, 2], np.float64), pm.n_faces, _c(pm.face_ptr, np.int32),
+                                 _c(pm.face_nodes, np.int32), _c(pm.owner, np.int32), pm.n_internal,
+                                 _c(pm.neighbour, np.int32), len(pm.patch_sizes), _c(pm.patch_sizes, np.int32))
+        s = self.ptr.contents
+        self.N, self.W, self.nf, self.nif = s.ncells, s.w, s.nfaces, s.nif
+
+        def arr(p, n, dt):
+            return np.ctypeslib.as_array(p, shape=(n,)).astype(dt).copy()
+
+        self.nef = arr(s.nef, self.N, np.int32)
+        self.gef = arr(s.gef, self.N * self.W, np.int32).reshape(self.W, self.N)
+        self.enbr = arr(s.enbr, self.N * self.W, np.int32).reshape(self.W, self.N)
+        self.iown = arr(s.iown, self.nif, np.int32)
+        self.inbr = arr(s.inbr, self.nif, np.int32)
+        self.face = {k: arr(getattr(s, k), self.nf, np.float64) for k in FACE_FIELDS}
+        self.cell = {k: arr(getattr(s, k), self.N, np.float64) for k in CELL_FIELDS}
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().or_mesh_delete(self.ptr)
+            self.ptr = None
+
+
+class _OrCase(C.Structure):
+    _fields_ = [("m", C.c_void_p)] + [(n, C.c_int) for n in ("N", "Fi", "Fb", "W")] + \
+               [(n, C.POINTER(C.c_double)) for n in ("bu0", "bv0", "bw0")] + \
+               [(n, C.c_double) for n in ("rho", "mu", "urf_u", "urf_p")] + \
+               [(n, C.c_int) for n in ("msolver", "mitmax", "psolver", "pitmax", "plmg", "ngrad", "refcell")] + \
+               [(n, C.c_double) for n in ("mrrf", "prrf")] + \
+               [(n, C.POINTER(C.c_double)) for n in (
+                   "u", "v", "w", "p", "pc", "mdot", "gu", "gv", "gw", "gp", "gpc", "bu", "bv", "bw", "bp", "bpc",
+                   "bgu", "bgv", "bgw", "bgp", "bgpc", "fgu", "fgv", "fgw", "fgp", "du", "dv", "dw", "us", "vs",
+                   "ws", "visc", "fcf", "fff", "fvf", "ftf", "fce", "fte", "effdiv", "ac", "anb", "bc", "dphi")] + \
+               [("res_mom", C.c_double * 12), ("res_pc", C.c_double * 4), ("mass_l1", C.c_double),
+                ("it_mom", C.c_int * 3), ("it_pc", C.c_int)]
+
+
+class OrCase:
+    """The SIMPLE restatement on a mesh (or_simple.c)."""
+
+    SIZES = {"u": "N", "v": "N", "w": "N", "p": "N", "pc": "N", "mdot": "Fi", "gu": "3N", "gv": "3N",
+             "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bpc": "Fb",
+             "bgu": "3Fb", "bgv": "3Fb", "bgw": "3Fb", "bgp": "3Fb", "fgu": "3Fi", "fgv": "3Fi", "fgw": "3Fi",
+             "fgp": "3Fi", "du": "N", "dv": "N", "dw": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
+             "anb": "NW", "bc": "N", "fcf": "F", "fff": "F", "ftf": "F", "fce": "N", "fte": "N"}
+
+    def __init__(self, pm, bu, bv, bw, rho=1.0, mu=0.01, urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1,
+                 mitmax=20, psolver=SOR, prrf=0.01, pitmax=200):
+        L = lib()
+        self.mesh = OrMesh(pm)
+        L.or_case_new.restype = C.POINTER(_OrCase)
+        L.or_case_new.argtypes = [C.POINTER(_OrMesh), _dp, _dp, _dp] + [C.c_double] * 4 + \
+            [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.or_case_delete.argtypes = [C.POINTER(_OrCase)]
+        for f in ("or_case_iterate", "or_case_momentum_assemble", "or_case_momentum_solve"):
+            getattr(L, f).argtypes = [C.POINTER(_OrCase), C.c_int]
+        for f in ("or_case_momentum_gradients", "or_case_pc_assemble", "or_case_pc_solve_correct"):
+            getattr(L, f).argtypes = [C.POINTER(_OrCase)]
+        L.or_case_mass_imbalance.argtypes = [C.POINTER(_OrCase)]
+        L.or_case_mass_imbalance.restype = C.c_double
+        self.ptr = L.or_case_new(self.mesh.ptr, _c(bu, np.float64), _c(bv, np.float64), _c(bw, np.float64), rho, mu,
+                                 urf_u, urf_p, msolver, mrrf, mitmax, psolver, prrf, pitmax)
+        s = self.ptr.contents
+        self.dims = {"N": s.N, "Fi": s.Fi, "Fb": s.Fb, "3N": 3 * s.N, "3Fb": 3 * s.Fb, "3Fi": 3 * s.Fi,
+                     "NW": s.N * s.W, "F": s.Fi + s.Fb}
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().or_case_delete(self.ptr)
+            self.ptr = None
+
+    def view(self, name):
+        """numpy view (no copy) of a field of the case."""
+        n = self.dims[self.SIZES[name]]
+        return np.ctypeslib.as_array(getattr(self.ptr.contents, name), shape=(n,))
+
+    def get(self, name):
+        return self.view(name).copy()
+
+    def set(self, name, values):
+        self.view(name)[:] = values
+
+    def iterate(self, n):
+        lib().or_case_iterate(self.ptr, n)
+
+    def momentum_gradients(self):
+        lib().or_case_momentum_gradients(self.ptr)
+
+    def momentum_assemble(self, comp):
+        lib().or_case_momentum_assemble(self.ptr, comp)
+
+    def momentum_solve(self, comp):
+        lib().or_case_momentum_solve(self.ptr, comp)
+
+    def pc_assemble(self):
+        lib().or_case_pc_assemble(self.ptr)
+
+    def pc_solve_correct(self):
+        lib().or_case_pc_solve_correct(self.ptr)
+
+    def mass_imbalance(self):
+        return lib().or_case_mass_imbalance(self.ptr)

@@ -180,3 +180,158 @@ class Mesh:
                                    _dptr(out["anb"])), "fs_amg_export")
         out["child"] = out["child"][:nch]
         return out
+
+
+# ----------------------------------------------------------------------------
+# topology / geometry (host) and the device SIMPLE case (include/fsamd_simple.h)
+
+class SimpleStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("momentum_iterations", C.c_int), ("pc_iterations", C.c_int),
+                ("pc_unconverged", C.c_int), ("pc_levels", C.c_int), ("seconds", C.c_double),
+                ("momentum_seconds", C.c_double), ("pc_seconds", C.c_double), ("pc_initial_l1", C.c_double),
+                ("pc_final_l1", C.c_double), ("res_momentum", C.c_double * 3), ("mass_l1", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "res_momentum"}
+        d["res_momentum"] = list(self.res_momentum)
+        return d
+
+
+PHASES = {"momentum_gradients": 0, "assemble_u": 1, "assemble_v": 2, "assemble_w": 3, "solve_u": 4,
+          "solve_v": 5, "solve_w": 6, "pc_assemble": 7, "pc_solve_correct": 8}
+TOPOLOGY = {"nef": 0, "gef": 1, "enbr": 2, "iown": 3, "inbr": 4}
+
+
+def _mesh_args(pm):
+    pts = np.ascontiguousarray(pm.points, dtype=np.float64)
+    keep = [np.ascontiguousarray(pts[:, k]) for k in range(3)]
+    keep += [np.ascontiguousarray(a, dtype=np.int32) for a in (pm.face_ptr, pm.face_nodes, pm.owner, pm.neighbour)]
+    return keep
+
+
+def _setup_case_symbols(L):
+    if getattr(L, "_case_ready", False):
+        return
+    L.fs_geom_create.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, _ip, _ip, C.c_int, _ip,
+                                 C.POINTER(C.c_void_p)]
+    L.fs_geom_destroy.argtypes = [C.c_void_p]
+    L.fs_geom_dims.argtypes = [C.c_void_p, _ip]
+    L.fs_geom_get.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_longlong]
+    L.fs_geom_topology.argtypes = [C.c_void_p, C.c_int, _ip, C.c_longlong]
+    L.fs_case_create.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, _ip, _ip, C.c_int, _ip, _dp, _dp, _dp,
+                                 C.c_double, C.c_double, C.POINTER(C.c_void_p)]
+    L.fs_case_destroy.argtypes = [C.c_void_p]
+    L.fs_case_configure.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
+                                    C.c_int, C.c_double, C.c_int, C.c_int]
+    L.fs_case_mesh.argtypes = [C.c_void_p]
+    L.fs_case_mesh.restype = C.c_void_p
+    L.fs_case_iterate.argtypes = [C.c_void_p, C.c_int, C.POINTER(SimpleStats)]
+    L.fs_case_phase.argtypes = [C.c_void_p, C.c_int]
+    L.fs_case_get.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_longlong]
+    L.fs_case_set.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_longlong]
+    L.fs_case_dims.argtypes = [C.c_void_p, _ip]
+    L.fs_case_topology.argtypes = [C.c_void_p, C.c_int, _ip, C.c_longlong]
+    L._case_ready = True
+
+
+class Geometry:
+    """Host-side topology and geometry of a polyMesh (fs_geom_*)."""
+
+    def __init__(self, pm):
+        L = lib()
+        _setup_case_symbols(L)
+        x, y, z, fp, fn, ow, nb = _mesh_args(pm)
+        h = C.c_void_p()
+        _check(L.fs_geom_create(len(x), _dptr(x), _dptr(y), _dptr(z), pm.n_faces, _iptr(fp), _iptr(fn), _iptr(ow),
+                                pm.n_internal, _iptr(nb), C.byref(h)), "fs_geom_create")
+        self.h = h
+        d = np.zeros(4, np.int32)
+        L.fs_geom_dims(h, _iptr(d))
+        self.N, self.W, self.F, self.Fi = (int(v) for v in d)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().fs_geom_destroy(self.h)
+            self.h = None
+
+    def get(self, name):
+        n = self.N if name in ("vol", "xc", "yc", "zc") else self.F
+        out = np.zeros(n)
+        _check(lib().fs_geom_get(self.h, name.encode(), _dptr(out), n), f"fs_geom_get({name})")
+        return out
+
+    def topology(self, name):
+        n = {"nef": self.N, "gef": self.N * self.W, "enbr": self.N * self.W, "iown": self.Fi, "inbr": self.Fi}[name]
+        out = np.zeros(n, np.int32)
+        _check(lib().fs_geom_topology(self.h, TOPOLOGY[name], _iptr(out), n), f"fs_geom_topology({name})")
+        return out.reshape(self.W, self.N) if name in ("gef", "enbr") else out
+
+
+class Case:
+    """Device-resident SIMPLE case (fs_case_*) on a polyMesh with no-slip walls."""
+
+    SIZES = {"u": "N", "v": "N", "w": "N", "p": "N", "pc": "N", "mdot": "Fi", "gu": "3N", "gv": "3N",
+             "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bgp": "3Fb",
+             "fg": "12Fi", "du": "N", "dv": "N", "dw": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
+             "anb": "NW", "bc": "N", "cf": "F", "ff": "F", "tf": "F", "vol": "N", "Sx": "F", "Sy": "F",
+             "Sz": "F", "g": "F", "gdiff": "F", "Tx": "F", "Ty": "F", "Tz": "F", "dcf": "F", "A": "F",
+             "kx": "Fi", "ky": "Fi", "kz": "Fi"}
+
+    def __init__(self, pm, wall_u, wall_v, wall_w, rho=1.0, mu=0.01):
+        L = lib()
+        _setup_case_symbols(L)
+        x, y, z, fp, fn, ow, nb = _mesh_args(pm)
+        bu, bv, bw = (np.ascontiguousarray(a, dtype=np.float64) for a in (wall_u, wall_v, wall_w))
+        h = C.c_void_p()
+        _check(L.fs_case_create(len(x), _dptr(x), _dptr(y), _dptr(z), pm.n_faces, _iptr(fp), _iptr(fn), _iptr(ow),
+                                pm.n_internal, _iptr(nb), _dptr(bu), _dptr(bv), _dptr(bw), float(rho), float(mu),
+                                C.byref(h)), "fs_case_create")
+        self.h = h
+        d = np.zeros(4, np.int32)
+        L.fs_case_dims(h, _iptr(d))
+        N, W, F, Fi = (int(v) for v in d)
+        self.N, self.W, self.F, self.Fi, self.Fb = N, W, F, Fi, F - Fi
+        self.dims = {"N": N, "3N": 3 * N, "Fi": Fi, "12Fi": 12 * Fi, "Fb": F - Fi, "3Fb": 3 * (F - Fi),
+                     "F": F, "NW": N * W}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fs_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, urf_u=0.7, urf_p=0.3, msolver="sor", mrrf=0.1, mitmax=20, mlmg=0, psolver="bicgs",
+                  prrf=1e-8, pitmax=500, plmg=MG_PRECOND):
+        ms = SOLVERS[msolver] if isinstance(msolver, str) else int(msolver)
+        ps = SOLVERS[psolver] if isinstance(psolver, str) else int(psolver)
+        _check(lib().fs_case_configure(self.h, urf_u, urf_p, ms, mrrf, int(mitmax), int(mlmg), ps, prrf,
+                                       int(pitmax), int(plmg)), "fs_case_configure")
+
+    def set_option(self, key, value):
+        k = OPTIONS[key] if isinstance(key, str) else int(key)
+        _check(lib().fs_set_option(lib().fs_case_mesh(self.h), k, float(value)), "fs_set_option")
+
+    def iterate(self, n):
+        st = SimpleStats()
+        _check(lib().fs_case_iterate(self.h, int(n), C.byref(st)), "fs_case_iterate")
+        return st.as_dict()
+
+    def phase(self, name):
+        _check(lib().fs_case_phase(self.h, PHASES[name] if isinstance(name, str) else int(name)), "fs_case_phase")
+
+    def get(self, name):
+        n = self.dims[self.SIZES[name]]
+        out = np.zeros(n)
+        _check(lib().fs_case_get(self.h, name.encode(), _dptr(out), n), f"fs_case_get({name})")
+        return out
+
+    def set(self, name, values):
+        n = self.dims[self.SIZES[name]]
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        assert a.size == n
+        _check(lib().fs_case_set(self.h, name.encode(), _dptr(a), n), f"fs_case_set({name})")

@@ -1,0 +1,927 @@
+// One steady SIMPLE outer iteration on the device (MainProgram.for:257-348):
+// SolveMomentumX/Y/Z (Momentum/SolveXMomentumEquation.for:175-299 and twins)
+// and SolveContinuity (ContinuityEquation/SolveContinuityEquation.for:37-114)
+// for laminar incompressible constant-property flow bounded by no-slip walls.
+//
+// Face loops run face-ordered over structure-of-arrays geometry (coalesced);
+// every face quantity the reference accumulates (FluxCf/FluxFf/FluxTf, the
+// Green-Gauss face values) is formed by one thread in the reference's order
+// and written once.  Cells then gather their faces through the per-slot face
+// code (face*4 + role) in slot order, which is the reference's face-loop
+// order, so the scatter AssembleGlobalMatrixFaceFluxes becomes an atomic-free
+// gather that is bitwise the reference's per-cell sum.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+
+#include "fs_common.h"
+#include "fs_geom.h"
+#include "fs_kernels.h"
+#include "fs_solve.h"
+#include "fs_simple.h"
+
+namespace fs {
+
+namespace {
+
+enum { ROLE_NONE = 0, ROLE_OWNER = 1, ROLE_NEIGH = 2, ROLE_BOUND = 3 };
+
+inline int blocks(int n) { return std::max(1, (n + kBlock - 1) / kBlock); }
+
+struct GradArgs { // up to 4 fields at once
+  const double *phi[4];
+  const double *bval[4]; // boundary values per boundary face, or nullptr =
+                         // the owner cell's own value
+  double *grad[4];       // [gx | gy | gz], 3N each
+  const double *fv[4];   // face values (interior)
+};
+
+// Green-Gauss face value phi_f = phi_C g + phi_F (1-g)
+// (StandardGreenGaussCorrection1Gradient.for:51-53)
+template <int NF>
+__global__ void k_gg_face0(int Fi, const int *__restrict__ own, const int *__restrict__ nei,
+                           const double *__restrict__ g, GradArgs a, double *__restrict__ fp) {
+  const int f = xcd_block() * kBlock + threadIdx.x;
+  if (f >= Fi) return;
+  const int i = own[f], j = nei[f];
+  const double gf = g[f];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) fp[(size_t)k * Fi + f] = a.phi[k][i] * gf + a.phi[k][j] * (1. - gf);
+}
+
+// corrected face value at the face centroid (:108-147):
+// phi_f = phi_f' + sum_d (grad_d C g + grad_d F (1-g)) (r_f - r_f')_d
+template <int NF>
+__global__ void k_gg_facecorr(int Fi, int N, const int *__restrict__ own,
+                              const int *__restrict__ nei, const double *__restrict__ g,
+                              const double *__restrict__ kx, const double *__restrict__ ky,
+                              const double *__restrict__ kz, const double *__restrict__ fp,
+                              GradArgs a, double *__restrict__ fv) {
+  const int f = xcd_block() * kBlock + threadIdx.x;
+  if (f >= Fi) return;
+  const int i = own[f], j = nei[f];
+  const double gf = g[f], og = 1. - gf;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const double *gr = a.grad[k];
+    double cx = gr[i] * gf + gr[j] * og;
+    cx = cx * kx[f];
+    double cy = gr[N + i] * gf + gr[N + j] * og;
+    cy = cy * ky[f];
+    double cz = gr[2 * N + i] * gf + gr[2 * N + j] * og;
+    cz = cz * kz[f];
+    fv[(size_t)k * Fi + f] = fp[(size_t)k * Fi + f] + cx + cy + cz;
+  }
+}
+
+// Cell sums of phi_f S_f over the cell's faces in slot (= face) order, then
+// division by the volume (:35-64, :148-172).  Interior faces add +phi_f S_f
+// at the owner and -phi_f S_f at the neighbour; boundary faces add the
+// boundary value times S_b.
+template <int NF>
+__global__ void k_gg_cell(int N, int W, int Fi, const int *__restrict__ code,
+                          const double *__restrict__ Sx, const double *__restrict__ Sy,
+                          const double *__restrict__ Sz, const double *__restrict__ vol,
+                          const double *__restrict__ fv, GradArgs a) {
+  const int c = xcd_block() * kBlock + threadIdx.x;
+  if (c >= N) return;
+  double gx[NF], gy[NF], gz[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) gx[k] = gy[k] = gz[k] = 0.0;
+  for (int s = 0; s < W; ++s) {
+    const int cd = code[(size_t)s * N + c];
+    const int role = cd & 3, f = cd >> 2;
+    if (role == ROLE_NONE) continue;
+    const double sx = Sx[f], sy = Sy[f], sz = Sz[f];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      double v;
+      if (role == ROLE_BOUND) v = a.bval[k] ? a.bval[k][f - Fi] : a.phi[k][c];
+      else v = fv[(size_t)k * Fi + f];
+      const double tx = v * sx, ty = v * sy, tz = v * sz;
+      if (role == ROLE_NEIGH) {
+        gx[k] = gx[k] - tx;
+        gy[k] = gy[k] - ty;
+        gz[k] = gz[k] - tz;
+      } else {
+        gx[k] = gx[k] + tx;
+        gy[k] = gy[k] + ty;
+        gz[k] = gz[k] + tz;
+      }
+    }
+  }
+  const double v = vol[c];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    a.grad[k][c] = gx[k] / v;
+    a.grad[k][N + c] = gy[k] / v;
+    a.grad[k][2 * N + c] = gz[k] / v;
+  }
+}
+
+// Wall projection of the boundary pressure gradient (CorrectGradients.for,
+// CorrectPresureGradientAtWalls) over all boundary faces (all are walls here).
+__global__ void k_wall_pgrad(int Fb, int Fi, int N, const int *__restrict__ own,
+                             const double *__restrict__ nx, const double *__restrict__ ny,
+                             const double *__restrict__ nz, const double *__restrict__ gp,
+                             double *__restrict__ bgp) {
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= Fb) return;
+  const int f = Fi + b, i1 = own[f];
+  const double x = nx[f], y = ny[f], z = nz[f];
+  const double gx = gp[i1], gy = gp[N + i1], gz = gp[2 * N + i1];
+  bgp[b] = (1 - x * x) * gx - x * y * gy - x * z * gz;
+  bgp[Fb + b] = (1 - y * y) * gy - y * x * gx - y * z * gz;
+  bgp[2 * Fb + b] = (1 - z * z) * gz - z * x * gx - z * y * gy;
+}
+
+// Face gradients (InterpolateGradientFromElementToFace.for:16-93): 'average'
+// for u, v, w and 'averagecorrected' for p.  fg = 12 blocks of Fi:
+// [u x,y,z | v x,y,z | w x,y,z | p x,y,z].
+__global__ void k_face_grads(int Fi, int N, const int *__restrict__ own,
+                             const int *__restrict__ nei, const double *__restrict__ g,
+                             const double *__restrict__ d, const double *__restrict__ ex,
+                             const double *__restrict__ ey, const double *__restrict__ ez,
+                             const double *__restrict__ gu, const double *__restrict__ gv,
+                             const double *__restrict__ gw, const double *__restrict__ gp,
+                             const double *__restrict__ p, double *__restrict__ fg) {
+  const int f = xcd_block() * kBlock + threadIdx.x;
+  if (f >= Fi) return;
+  const int i = own[f], j = nei[f];
+  const double gf = g[f], og = 1. - gf;
+  const double *G[4] = {gu, gv, gw, gp};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double fx = gf * G[k][i] + og * G[k][j];
+    double fy = gf * G[k][N + i] + og * G[k][N + j];
+    double fz = gf * G[k][2 * N + i] + og * G[k][2 * N + j];
+    if (k == 3) {
+      double corr = (p[j] - p[i]) / d[f];
+      corr = corr - (fx * ex[f] + fy * ey[f] + fz * ez[f]);
+      fx = fx + corr * ex[f];
+      fy = fy + corr * ey[f];
+      fz = fz + corr * ez[f];
+    }
+    fg[(size_t)(3 * k) * Fi + f] = fx;
+    fg[(size_t)(3 * k + 1) * Fi + f] = fy;
+    fg[(size_t)(3 * k + 2) * Fi + f] = fz;
+  }
+}
+
+struct MomArgs {
+  int comp;
+  double mu, c23;
+  const double *phi, *u, *v, *w;          // current velocity components
+  const double *bu, *bv, *bw;              // wall velocities
+  const double *gu, *gv, *gw;              // cell gradients (boundary copies)
+  const double *fg;                        // face gradients (12 blocks)
+  const double *mdot;
+};
+
+// Momentum face fluxes of interior faces in the reference's order:
+// diffusion (AssembleIsotropicDiffusion.for:99-117), explicit diffusion
+// (AssembleExplicitDiffusion.for), bulk viscosity (AssembleBulkViscosity.for),
+// upwind convection (AssembleConvection.for:72-88).
+__global__ void k_mom_face(int Fi, const int *__restrict__ own, const int *__restrict__ nei,
+                           const double *__restrict__ g, const double *__restrict__ gdiff,
+                           const double *__restrict__ Tx, const double *__restrict__ Ty,
+                           const double *__restrict__ Tz, const double *__restrict__ Sx,
+                           const double *__restrict__ Sy, const double *__restrict__ Sz, MomArgs a,
+                           double *__restrict__ cf, double *__restrict__ ff,
+                           double *__restrict__ tf) {
+  const int f = xcd_block() * kBlock + threadIdx.x;
+  if (f >= Fi) return;
+  const int i = own[f], j = nei[f];
+  const int c = a.comp;
+  const double gf = g[f];
+  const double gamf = gf * a.mu + (1. - gf) * a.mu;
+  const double pi = a.phi[i], pj = a.phi[j];
+  const double *fgc = a.fg + (size_t)(3 * c) * Fi; // face gradient of this component
+  // diffusion
+  const double cfl = gamf * gdiff[f], ffl = -(gamf * gdiff[f]);
+  const double vfl = -(gamf * (fgc[f] * Tx[f] + fgc[Fi + f] * Ty[f] + fgc[2 * Fi + f] * Tz[f]));
+  double C = 0.0 + cfl, Fv = 0.0 + ffl, T = 0.0 + cfl * pi + ffl * pj + vfl;
+  // explicit diffusion: -Gamma_f (grad u_c)^T . S
+  const double ga = a.fg[(size_t)c * Fi + f], gb = a.fg[(size_t)(3 + c) * Fi + f],
+               gc = a.fg[(size_t)(6 + c) * Fi + f];
+  const double v2 = -(gamf * (ga * Sx[f] + gb * Sy[f] + gc * Sz[f]));
+  C = C + 0.;
+  Fv = Fv + 0.;
+  T = T + v2;
+  // bulk viscosity
+  const double dv = a.fg[f] + a.fg[(size_t)4 * Fi + f] + a.fg[(size_t)8 * Fi + f];
+  const double S = c == 0 ? Sx[f] : c == 1 ? Sy[f] : Sz[f];
+  const double v3 = -(a.c23 * gamf * dv * S);
+  C = C + 0.;
+  Fv = Fv + 0.;
+  T = T + v3;
+  // upwind convection
+  const double md = a.mdot[f];
+  const double ccl = md > 0. ? md : 0.;
+  const double fcl = -(-md > 0. ? -md : 0.);
+  C = C + ccl;
+  Fv = Fv + fcl;
+  T = T + ccl * pi + fcl * pj;
+  cf[f] = C;
+  ff[f] = Fv;
+  tf[f] = T;
+}
+
+// No-slip wall faces (AssembleIsotropicDiffusion.for:148-219), then the
+// explicit and bulk terms with the owner's gradient as boundary gradient.
+__global__ void k_mom_wall(int Fb, int Fi, int N, const int *__restrict__ own,
+                           const double *__restrict__ A, const double *__restrict__ nx,
+                           const double *__restrict__ ny, const double *__restrict__ nz,
+                           const double *__restrict__ dx, const double *__restrict__ dy,
+                           const double *__restrict__ dz, const double *__restrict__ Sx,
+                           const double *__restrict__ Sy, const double *__restrict__ Sz, MomArgs a,
+                           double *__restrict__ cf, double *__restrict__ tf) {
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= Fb) return;
+  const int f = Fi + b, i1 = own[f], c = a.comp;
+  const double gamf = a.mu;
+  const double x = nx[f], y = ny[f], z = nz[f], area = A[f];
+  const double dn = dx[f] * x + dy[f] * y + dz[f] * z;
+  const double bphi = c == 0 ? a.bu[b] : c == 1 ? a.bv[b] : a.bw[b];
+  double cfl, vfl;
+  if (c == 0) {
+    cfl = gamf * area * (1. - x * x) / dn;
+    vfl = -((gamf * area / dn) * (bphi * (1. - x * x) + (a.v[i1] - a.bv[b]) * x * y +
+                                   (a.w[i1] - a.bw[b]) * x * z));
+  } else if (c == 1) {
+    cfl = gamf * area * (1. - y * y) / dn;
+    vfl = -((gamf * area / dn) * ((a.u[i1] - a.bu[b]) * x * y + bphi * (1. - y * y) +
+                                   (a.w[i1] - a.bw[b]) * z * y));
+  } else {
+    cfl = gamf * area * (1. - z * z) / dn;
+    vfl = -((gamf * area / dn) * ((a.u[i1] - a.bu[b]) * x * z + (a.v[i1] - a.bv[b]) * y * z +
+                                   bphi * (1. - z * z)));
+  }
+  const double ffl = 0.;
+  double C = 0.0 + cfl, T = 0.0 + cfl * a.phi[i1] + ffl * bphi + vfl;
+  const double bga = a.gu[(size_t)c * N + i1], bgb = a.gv[(size_t)c * N + i1],
+               bgc = a.gw[(size_t)c * N + i1];
+  const double v2 = -(gamf * (bga * Sx[f] + bgb * Sy[f] + bgc * Sz[f]));
+  C = C + 0.;
+  T = T + v2;
+  const double dv = a.gu[i1] + a.gv[N + i1] + a.gw[2 * N + i1];
+  const double S = c == 0 ? Sx[f] : c == 1 ? Sy[f] : Sz[f];
+  const double v3 = -(a.c23 * gamf * dv * S);
+  C = C + 0.;
+  T = T + v3;
+  cf[f] = C;
+  tf[f] = T;
+}
+
+// Per-cell gather of the momentum system: face fluxes in slot order
+// (AssembleGlobalMatrixFaceFluxes), effective divergence and element fluxes
+// (CalculateEffectiveDivergence.for, AssembleConvection.for:169-178), the
+// pressure gradient (AssemblePressureGradient.for), under-relaxation
+// (UnderRelaxation.for) and D = V/a_P, u* = u (StoreTheDValues.for).
+__global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
+                           const double *__restrict__ cf, const double *__restrict__ ff,
+                           const double *__restrict__ tf, const double *__restrict__ mdot,
+                           const double *__restrict__ phi, const double *__restrict__ gpc,
+                           const double *__restrict__ vol, double urf, double *__restrict__ ac,
+                           double *__restrict__ anb, double *__restrict__ bc,
+                           double *__restrict__ D, double *__restrict__ star) {
+  const int c = xcd_block() * kBlock + threadIdx.x;
+  if (c >= N) return;
+  double a = 0.0, b = 0.0, e = 0.0;
+  for (int s = 0; s < W; ++s) {
+    const int cd = code[(size_t)s * N + c];
+    const int role = cd & 3, f = cd >> 2;
+    double an = 0.0;
+    if (role == ROLE_OWNER) {
+      a = a + cf[f];
+      an = an + ff[f];
+      b = b - tf[f];
+      e = e + mdot[f];
+    } else if (role == ROLE_NEIGH) {
+      a = a - ff[f];
+      an = an - cf[f];
+      b = b + tf[f];
+      e = e - mdot[f];
+    } else if (role == ROLE_BOUND) {
+      a = a + cf[f];
+      b = b - tf[f];
+      e = e + 0.;
+    }
+    anb[(size_t)s * N + c] = an;
+  }
+  const double pc = phi[c];
+  const double ep = e > 0.0 ? e : 0.0;
+  const double fce = 0.0 + (ep - e);
+  double fte = 0.0 + ((ep - e) * pc + (-(ep * pc)));
+  fte = fte + gpc[c] * vol[c];
+  a = a + fce;
+  b = b - fte;
+  a = a / urf;
+  ac[c] = a;
+  bc[c] = b;
+  D[c] = vol[c] / a;
+  star[c] = pc;
+}
+
+struct RcArgs {
+  double rho, urf_u;
+  const double *u, *v, *w, *us, *vs, *ws, *du, *dv, *dw, *gp, *fgp, *p;
+};
+
+// Rhie-Chow face mass flux (AssembleMdotUsingRhieChow.for:554-759, steady,
+// LRelaxMomentum) followed by the pressure-correction face coefficients
+// (AssemblePressureCorrectionFluxes.for:85-153).
+__global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
+                            const int *__restrict__ nei, const double *__restrict__ g,
+                            const double *__restrict__ Sx, const double *__restrict__ Sy,
+                            const double *__restrict__ Sz, const double *__restrict__ ex,
+                            const double *__restrict__ ey, const double *__restrict__ ez,
+                            const double *__restrict__ dcf, RcArgs a, double *__restrict__ mdot,
+                            double *__restrict__ cf, double *__restrict__ ff,
+                            double *__restrict__ tf) {
+  const int f = xcd_block() * kBlock + threadIdx.x;
+  if (f >= Fi) return;
+  const int i = own[f], j = nei[f];
+  const double gf = g[f], og = 1. - gf;
+  const double du = gf * a.du[i] + og * a.du[j], dv = gf * a.dv[i] + og * a.dv[j],
+               dw = gf * a.dw[i] + og * a.dw[j];
+  const double ub = gf * a.u[i] + og * a.u[j], vb = gf * a.v[i] + og * a.v[j],
+               wb = gf * a.w[i] + og * a.w[j];
+  const double usf = gf * a.us[i] + og * a.us[j], vsf = gf * a.vs[i] + og * a.vs[j],
+               wsf = gf * a.ws[i] + og * a.ws[j];
+  const double pbx = gf * a.gp[i] + og * a.gp[j], pby = gf * a.gp[N + i] + og * a.gp[N + j],
+               pbz = gf * a.gp[2 * N + i] + og * a.gp[2 * N + j];
+  const double sx = Sx[f], sy = Sy[f], sz = Sz[f];
+  const double rhof = a.rho;
+  double fv = rhof * (ub * sx + vb * sy + wb * sz);
+  const double dusf = du * sx, dvsf = dv * sy, dwsf = dw * sz;
+  const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
+  const double eu = dusf / mag, ev = dvsf / mag, ew = dwsf / mag;
+  const double dot = ex[f] * eu + ey[f] * ev + ez[f] * ew;
+  const double due = mag * ex[f] / dot, dve = mag * ey[f] / dot, dwe = mag * ez[f] / dot;
+  const double geo = sqrt(due * due + dve * dve + dwe * dwe) / dcf[f];
+  const double dut = dusf - due, dvt = dvsf - dve, dwt = dwsf - dwe;
+  const double cfl = rhof * geo, ffl = -(rhof * geo);
+  fv = fv - rhof * (a.fgp[f] * dut + a.fgp[Fi + f] * dvt + a.fgp[2 * Fi + f] * dwt);
+  fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
+  const double mstar = rhof * (usf * sx + vsf * sy + wsf * sz);
+  fv = fv + (1. - a.urf_u) * (mdot[f] - mstar);
+  const double m = cfl * a.p[i] + ffl * a.p[j] + fv;
+  mdot[f] = m;
+  cf[f] = cfl;
+  ff[f] = ffl;
+  tf[f] = m;
+}
+
+// p' system gather (walls carry no flux: FluxCf = 0, Bmdot = 0).
+__global__ void k_pc_cell(int N, int W, const int *__restrict__ code,
+                          const double *__restrict__ cf, const double *__restrict__ ff,
+                          const double *__restrict__ tf, double *__restrict__ ac,
+                          double *__restrict__ anb, double *__restrict__ bc) {
+  const int c = xcd_block() * kBlock + threadIdx.x;
+  if (c >= N) return;
+  double a = 0.0, b = 0.0;
+  for (int s = 0; s < W; ++s) {
+    const int cd = code[(size_t)s * N + c];
+    const int role = cd & 3, f = cd >> 2;
+    double an = 0.0;
+    if (role == ROLE_OWNER) {
+      a = a + cf[f];
+      an = an + ff[f];
+      b = b - tf[f];
+    } else if (role == ROLE_NEIGH) {
+      a = a - ff[f];
+      an = an - cf[f];
+      b = b + tf[f];
+    }
+    anb[(size_t)s * N + c] = an;
+  }
+  ac[c] = a;
+  bc[c] = b;
+}
+
+// CorrectPressure (CorrectPressure.for:21-42)
+__global__ void k_correct_p(int N, double urf_p, int ref, const double *__restrict__ pc,
+                            double *__restrict__ p) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= N) return;
+  const double r = ref >= 0 ? pc[ref] : 0.0;
+  p[c] = p[c] + urf_p * (pc[c] - r);
+}
+
+// CorrectVelocity (CorrectMdotandVelocity.for:774-778)
+__global__ void k_correct_u(int N, const double *__restrict__ gpc, const double *__restrict__ du,
+                            const double *__restrict__ dv, const double *__restrict__ dw,
+                            double *__restrict__ u, double *__restrict__ v, double *__restrict__ w) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= N) return;
+  u[c] = u[c] - du[c] * gpc[c];
+  v[c] = v[c] - dv[c] * gpc[N + c];
+  w[c] = w[c] - dw[c] * gpc[2 * N + c];
+}
+
+// Correctmdot (CorrectMdotandVelocity.for:55-63)
+__global__ void k_correct_mdot(int Fi, const int *__restrict__ own, const int *__restrict__ nei,
+                               const double *__restrict__ cf, const double *__restrict__ ff,
+                               const double *__restrict__ pc, double *__restrict__ mdot) {
+  const int f = xcd_block() * kBlock + threadIdx.x;
+  if (f >= Fi) return;
+  mdot[f] = mdot[f] + cf[f] * pc[own[f]] + ff[f] * pc[nei[f]];
+}
+
+// UpdateBoundaryPressure at walls (CorrectPressure.for:49-90)
+__global__ void k_wall_p(int Fb, int Fi, const int *__restrict__ own, const double *__restrict__ dx,
+                         const double *__restrict__ dy, const double *__restrict__ dz,
+                         const double *__restrict__ p, const double *__restrict__ bgp,
+                         double *__restrict__ bp) {
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= Fb) return;
+  const int f = Fi + b;
+  bp[b] = p[own[f]] + bgp[b] * dx[f] + bgp[Fb + b] * dy[f] + bgp[2 * Fb + b] * dz[f];
+}
+
+// L1 of the cell mass imbalance: per-cell sum of outgoing mdot in slot order
+__global__ void k_mass_l1(int N, int W, const int *__restrict__ code,
+                          const double *__restrict__ mdot, double *__restrict__ part) {
+  double s = 0.0;
+  for (int c = blockIdx.x * kBlock + threadIdx.x; c < N; c += gridDim.x * kBlock) {
+    double r = 0.0;
+    for (int t = 0; t < W; ++t) {
+      const int cd = code[(size_t)t * N + c];
+      const int role = cd & 3, f = cd >> 2;
+      if (role == ROLE_OWNER) r = r + mdot[f];
+      else if (role == ROLE_NEIGH) r = r - mdot[f];
+    }
+    s += fabs(r);
+  }
+  s = block_sum<kBlock>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+} // namespace
+
+struct SimpleCase {
+  HostGeom geom;
+  fs_mesh *mesh = nullptr;
+  hipStream_t s = nullptr;
+  int N = 0, W = 0, F = 0, Fi = 0, Fb = 0;
+  DevBuf<int> own, nei, code;
+  DevBuf<double> Sx, Sy, Sz, A, nx, ny, nz, dx, dy, dz, ex, ey, ez, d, g, gdiff, Tx, Ty, Tz, kx,
+      ky, kz, vol;
+  DevBuf<double> bu, bv, bw;
+  DevBuf<double> u, v, w, p, pc, mdot, gu, gv, gw, gp, gpc, bp, bgp, fg, du, dv, dw, us, vs, ws;
+  DevBuf<double> fp, fv, cf, ff, tf, ac, anb, bc, dphi, part;
+  double rho = 1.0, mu = 0.01, urf_u = 0.7, urf_p = 0.3;
+  int msolver = FS_SOR, mlmg = 0, mitmax = 20, psolver = FS_BICGS, plmg = FS_MG_PRECOND,
+      pitmax = 500, refcell = 0;
+  double mrrf = 0.1, prrf = 1e-8;
+  fs_simple_stats last{};
+};
+
+static void upload_geometry(SimpleCase &C) {
+  hipStream_t s = C.s;
+  const FaceGeom &F = C.geom.face;
+  C.Sx.upload(F.Sx, s); C.Sy.upload(F.Sy, s); C.Sz.upload(F.Sz, s); C.A.upload(F.A, s);
+  C.nx.upload(F.nx, s); C.ny.upload(F.ny, s); C.nz.upload(F.nz, s);
+  C.dx.upload(F.dx, s); C.dy.upload(F.dy, s); C.dz.upload(F.dz, s);
+  C.ex.upload(F.ex, s); C.ey.upload(F.ey, s); C.ez.upload(F.ez, s); C.d.upload(F.d, s);
+  C.g.upload(F.g, s); C.gdiff.upload(F.gdiff, s);
+  C.Tx.upload(F.Tx, s); C.Ty.upload(F.Ty, s); C.Tz.upload(F.Tz, s);
+  C.kx.upload(F.kx, s); C.ky.upload(F.ky, s); C.kz.upload(F.kz, s);
+  C.vol.upload(C.geom.vol, s);
+}
+
+// Gradient of up to four fields: Green-Gauss plus nIterGradient = 2
+// face-centroid corrections (CalculateGradients.for:212-215).
+template <int NF> static void gradients(SimpleCase &C, GradArgs a) {
+  const int Fi = C.Fi, N = C.N;
+  hipLaunchKernelGGL(k_gg_face0<NF>, dim3(blocks(Fi)), dim3(kBlock), 0, C.s, Fi, C.own.p, C.nei.p,
+                     C.g.p, a, C.fp.p);
+  hipLaunchKernelGGL(k_gg_cell<NF>, dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.W, Fi, C.code.p,
+                     C.Sx.p, C.Sy.p, C.Sz.p, C.vol.p, C.fp.p, a);
+  for (int it = 0; it < 2; ++it) {
+    hipLaunchKernelGGL(k_gg_facecorr<NF>, dim3(blocks(Fi)), dim3(kBlock), 0, C.s, Fi, N, C.own.p,
+                       C.nei.p, C.g.p, C.kx.p, C.ky.p, C.kz.p, C.fp.p, a, C.fv.p);
+    hipLaunchKernelGGL(k_gg_cell<NF>, dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.W, Fi, C.code.p,
+                       C.Sx.p, C.Sy.p, C.Sz.p, C.vol.p, C.fv.p, a);
+  }
+}
+
+static void phase_momentum_gradients(SimpleCase &C) {
+  GradArgs a{};
+  a.phi[0] = C.u.p; a.phi[1] = C.v.p; a.phi[2] = C.w.p; a.phi[3] = C.p.p;
+  a.bval[0] = C.bu.p; a.bval[1] = C.bv.p; a.bval[2] = C.bw.p; a.bval[3] = C.bp.p;
+  a.grad[0] = C.gu.p; a.grad[1] = C.gv.p; a.grad[2] = C.gw.p; a.grad[3] = C.gp.p;
+  gradients<4>(C, a);
+  hipLaunchKernelGGL(k_wall_pgrad, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N,
+                     C.own.p, C.nx.p, C.ny.p, C.nz.p, C.gp.p, C.bgp.p);
+  hipLaunchKernelGGL(k_face_grads, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p,
+                     C.nei.p, C.g.p, C.d.p, C.ex.p, C.ey.p, C.ez.p, C.gu.p, C.gv.p, C.gw.p, C.gp.p,
+                     C.p.p, C.fg.p);
+}
+
+static void phase_momentum_assemble(SimpleCase &C, int comp) {
+  MomArgs a{};
+  a.comp = comp;
+  a.mu = C.mu;
+  a.c23 = (double)(-2.0f / 3.0f); // REAL(4) literal -2./3. (AssembleBulkViscosity.for:49)
+  a.phi = comp == 0 ? C.u.p : comp == 1 ? C.v.p : C.w.p;
+  a.u = C.u.p; a.v = C.v.p; a.w = C.w.p;
+  a.bu = C.bu.p; a.bv = C.bv.p; a.bw = C.bw.p;
+  a.gu = C.gu.p; a.gv = C.gv.p; a.gw = C.gw.p;
+  a.fg = C.fg.p;
+  a.mdot = C.mdot.p;
+  hipLaunchKernelGGL(k_mom_face, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.own.p, C.nei.p,
+                     C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p, C.Sx.p, C.Sy.p, C.Sz.p, a, C.cf.p,
+                     C.ff.p, C.tf.p);
+  hipLaunchKernelGGL(k_mom_wall, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N, C.own.p,
+                     C.A.p, C.nx.p, C.ny.p, C.nz.p, C.dx.p, C.dy.p, C.dz.p, C.Sx.p, C.Sy.p, C.Sz.p, a,
+                     C.cf.p, C.tf.p);
+  double *D = comp == 0 ? C.du.p : comp == 1 ? C.dv.p : C.dw.p;
+  double *st = comp == 0 ? C.us.p : comp == 1 ? C.vs.p : C.ws.p;
+  hipLaunchKernelGGL(k_mom_cell, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W, C.code.p,
+                     C.cf.p, C.ff.p, C.tf.p, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
+                     C.urf_u, C.ac.p, C.anb.p, C.bc.p, D, st);
+}
+
+static int phase_momentum_solve(SimpleCase &C, int comp, fs_simple_stats *st) {
+  double *phi = comp == 0 ? C.u.p : comp == 1 ? C.v.p : C.w.p;
+  fs_solve_stats ss{};
+  double res4[4];
+  const int rc = solve_device(C.mesh, comp + 1, phi, C.mrrf, C.mitmax, C.msolver, C.mlmg, C.ac.p,
+                              C.anb.p, C.bc.p, C.dphi.p, res4, &ss);
+  if (rc < 0) return rc;
+  if (st) {
+    st->momentum_iterations += ss.iterations;
+    st->momentum_seconds += ss.seconds;
+    st->res_momentum[comp] = res4[0];
+  }
+  return 0;
+}
+
+static void phase_pc_assemble(SimpleCase &C) {
+  RcArgs a{};
+  a.rho = C.rho;
+  a.urf_u = C.urf_u;
+  a.u = C.u.p; a.v = C.v.p; a.w = C.w.p;
+  a.us = C.us.p; a.vs = C.vs.p; a.ws = C.ws.p;
+  a.du = C.du.p; a.dv = C.dv.p; a.dw = C.dw.p;
+  a.gp = C.gp.p;
+  a.fgp = C.fg.p + (size_t)9 * C.Fi;
+  a.p = C.p.p;
+  hipLaunchKernelGGL(k_rhie_chow, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p,
+                     C.nei.p, C.g.p, C.Sx.p, C.Sy.p, C.Sz.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, a,
+                     C.mdot.p, C.cf.p, C.ff.p, C.tf.p);
+  hipLaunchKernelGGL(k_pc_cell, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W, C.code.p, C.cf.p,
+                     C.ff.p, C.tf.p, C.ac.p, C.anb.p, C.bc.p);
+}
+
+static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
+  FS_HIP(hipMemsetAsync(C.pc.p, 0, sizeof(double) * C.N, C.s)); // InitializePressureCorrection
+  fs_solve_stats ss{};
+  double res4[4];
+  const int rc = solve_device(C.mesh, 4, C.pc.p, C.prrf, C.pitmax, C.psolver, C.plmg, C.ac.p,
+                              C.anb.p, C.bc.p, C.dphi.p, res4, &ss);
+  if (rc < 0) return rc;
+  if (st) {
+    st->pc_iterations += ss.iterations;
+    st->pc_seconds += ss.seconds;
+    st->pc_initial_l1 = ss.initial_l1;
+    st->pc_final_l1 = ss.final_l1;
+    st->pc_levels = ss.levels;
+    if (rc == 1) st->pc_unconverged++;
+  }
+  const int N = C.N;
+  hipLaunchKernelGGL(k_correct_p, dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.urf_p, C.refcell,
+                     C.pc.p, C.p.p);
+  GradArgs a{};
+  a.phi[0] = C.pc.p;
+  a.bval[0] = nullptr; // walls: BPressureC = PressureC(owner)
+  a.grad[0] = C.gpc.p;
+  gradients<1>(C, a);
+  hipLaunchKernelGGL(k_correct_u, dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.gpc.p, C.du.p, C.dv.p,
+                     C.dw.p, C.u.p, C.v.p, C.w.p);
+  hipLaunchKernelGGL(k_correct_mdot, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.own.p, C.nei.p,
+                     C.cf.p, C.ff.p, C.pc.p, C.mdot.p);
+  hipLaunchKernelGGL(k_wall_p, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.own.p, C.dx.p,
+                     C.dy.p, C.dz.p, C.p.p, C.bgp.p, C.bp.p);
+  return 0;
+}
+
+static double mass_l1(SimpleCase &C) {
+  const int np = std::min(kPartials, blocks(C.N));
+  hipLaunchKernelGGL(k_mass_l1, dim3(np), dim3(kBlock), 0, C.s, C.N, C.W, C.code.p, C.mdot.p, C.part.p);
+  std::vector<double> h(np);
+  FS_HIP_THROW(hipMemcpyAsync(h.data(), C.part.p, sizeof(double) * np, hipMemcpyDeviceToHost, C.s));
+  FS_HIP_THROW(hipStreamSynchronize(C.s));
+  double t = 0.0;
+  for (double x : h) t += x;
+  return t;
+}
+
+int simple_iterate(SimpleCase &C, int niter, fs_simple_stats *st) {
+  auto t0 = std::chrono::steady_clock::now();
+  fs_simple_stats loc{};
+  for (int it = 0; it < niter; ++it) {
+    phase_momentum_gradients(C);
+    for (int comp = 0; comp < 3; ++comp) {
+      phase_momentum_assemble(C, comp);
+      const int rc = phase_momentum_solve(C, comp, &loc);
+      if (rc < 0) return rc;
+    }
+    phase_pc_assemble(C);
+    const int rc = phase_pc_solve_correct(C, &loc);
+    if (rc < 0) return rc;
+  }
+  FS_HIP(hipStreamSynchronize(C.s));
+  loc.iterations = niter;
+  loc.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  loc.mass_l1 = mass_l1(C);
+  C.last = loc;
+  if (st) *st = loc;
+  return 0;
+}
+
+} // namespace fs
+
+using fs::set_error;
+
+struct fs_case {
+  fs::SimpleCase c;
+};
+
+#define CASE_GUARD_BEGIN try {
+#define CASE_GUARD_END                                                        \
+  }                                                                           \
+  catch (const std::exception &e) {                                           \
+    set_error(e.what());                                                      \
+    return -2;                                                                \
+  }
+
+extern "C" {
+
+int fs_case_create(int npoints, const double *x, const double *y, const double *z, int nfaces,
+                   const int *face_ptr, const int *face_nodes, const int *owner, int nif,
+                   const int *neighbour, const double *wall_u, const double *wall_v,
+                   const double *wall_w, double rho, double mu, fs_case **out) {
+  CASE_GUARD_BEGIN
+  if (!out || !x || !y || !z || !face_ptr || !face_nodes || !owner || !neighbour || !wall_u ||
+      !wall_v || !wall_w || npoints <= 0 || nfaces <= 0 || nif <= 0 || nif > nfaces) {
+    set_error("fs_case_create: invalid arguments");
+    return -1;
+  }
+  *out = nullptr;
+  if (face_ptr[0] != 0) {
+    set_error("fs_case_create: face_ptr[0] must be 0");
+    return -1;
+  }
+  for (int f = 0; f < nfaces; ++f) {
+    if (face_ptr[f + 1] - face_ptr[f] < 3 || owner[f] < 0 || (f < nif && neighbour[f] < 0)) {
+      set_error("fs_case_create: bad face " + std::to_string(f));
+      return -1;
+    }
+    for (int t = face_ptr[f]; t < face_ptr[f + 1]; ++t)
+      if (face_nodes[t] < 0 || face_nodes[t] >= npoints) {
+        set_error("fs_case_create: face node out of range at face " + std::to_string(f));
+        return -1;
+      }
+  }
+  auto *cs = new fs_case();
+  fs::SimpleCase &C = cs->c;
+  try {
+    fs::PolyInput in;
+    in.npoints = npoints;
+    in.nfaces = nfaces;
+    in.nif = nif;
+    in.x = x;
+    in.y = y;
+    in.z = z;
+    in.fptr = face_ptr;
+    in.fnodes = face_nodes;
+    in.owner = owner;
+    in.neigh = neighbour;
+    fs::build_topology(in, C.geom);
+    fs::build_geometry(in, C.geom);
+    C.N = C.geom.N;
+    C.W = C.geom.W;
+    C.F = C.geom.F;
+    C.Fi = C.geom.Fi;
+    C.Fb = C.F - C.Fi;
+    std::vector<int> own1(nif), nei1(nif);
+    for (int f = 0; f < nif; ++f) {
+      own1[f] = owner[f] + 1;
+      nei1[f] = neighbour[f] + 1;
+    }
+    int rc = fs_mesh_create(C.N, C.W, C.geom.enbr.data(), C.geom.nef.data(), nif, own1.data(),
+                            nei1.data(), C.geom.iown.data(), C.geom.inbr.data(), &C.mesh);
+    if (rc == 0) rc = fs_mesh_set_cell_faces(C.mesh, nfaces, C.geom.gef.data(), C.geom.nef.data());
+    if (rc != 0) {
+      std::string msg = fs_last_error();
+      if (C.mesh) fs_mesh_destroy(C.mesh);
+      delete cs;
+      set_error("fs_case_create: " + msg);
+      return rc;
+    }
+    C.s = C.mesh->stream;
+    C.own.upload(owner, nfaces, C.s);
+    C.nei.upload(neighbour, nif, C.s);
+    // the per-slot face code of the mesh context
+    C.code.alloc((size_t)C.N * C.W);
+    FS_HIP_THROW(hipMemcpyAsync(C.code.p, C.mesh->d_face_code.p, sizeof(int) * C.N * C.W,
+                                hipMemcpyDeviceToDevice, C.s));
+    fs::upload_geometry(C);
+    C.bu.upload(wall_u, C.Fb, C.s);
+    C.bv.upload(wall_v, C.Fb, C.s);
+    C.bw.upload(wall_w, C.Fb, C.s);
+    C.rho = rho;
+    C.mu = mu;
+    const size_t N = C.N, Fi = C.Fi, Fb = C.Fb, F = C.F;
+    for (auto *b : {&C.u, &C.v, &C.w, &C.p, &C.pc, &C.du, &C.dv, &C.dw, &C.us, &C.vs, &C.ws, &C.ac,
+                    &C.bc, &C.dphi}) {
+      b->alloc(N);
+      FS_HIP_THROW(hipMemsetAsync(b->p, 0, sizeof(double) * N, C.s));
+    }
+    for (auto *b : {&C.gu, &C.gv, &C.gw, &C.gp, &C.gpc}) {
+      b->alloc(3 * N);
+      FS_HIP_THROW(hipMemsetAsync(b->p, 0, sizeof(double) * 3 * N, C.s));
+    }
+    C.mdot.alloc(Fi);
+    FS_HIP_THROW(hipMemsetAsync(C.mdot.p, 0, sizeof(double) * Fi, C.s));
+    C.bp.alloc(Fb);
+    FS_HIP_THROW(hipMemsetAsync(C.bp.p, 0, sizeof(double) * Fb, C.s));
+    C.bgp.alloc(3 * Fb);
+    FS_HIP_THROW(hipMemsetAsync(C.bgp.p, 0, sizeof(double) * 3 * Fb, C.s));
+    C.fg.alloc(12 * Fi);
+    FS_HIP_THROW(hipMemsetAsync(C.fg.p, 0, sizeof(double) * 12 * Fi, C.s));
+    C.fp.alloc(4 * Fi);
+    C.fv.alloc(4 * Fi);
+    C.cf.alloc(F);
+    C.ff.alloc(F);
+    C.tf.alloc(F);
+    FS_HIP_THROW(hipMemsetAsync(C.ff.p, 0, sizeof(double) * F, C.s));
+    C.anb.alloc(N * C.W);
+    C.part.alloc(fs::kPartials);
+    FS_HIP_THROW(hipStreamSynchronize(C.s));
+  } catch (...) {
+    if (C.mesh) fs_mesh_destroy(C.mesh);
+    delete cs;
+    throw;
+  }
+  *out = cs;
+  return 0;
+  CASE_GUARD_END
+}
+
+int fs_case_destroy(fs_case *cs) {
+  if (!cs) return 0;
+  fs_mesh *m = cs->c.mesh;
+  if (m) (void)hipStreamSynchronize(m->stream);
+  delete cs; // device buffers first, the mesh (and its stream) last
+  if (m) fs_mesh_destroy(m);
+  return 0;
+}
+
+fs_mesh *fs_case_mesh(fs_case *cs) { return cs ? cs->c.mesh : nullptr; }
+
+int fs_case_configure(fs_case *cs, double urf_u, double urf_p, int msolver, double mrrf,
+                      int mitmax, int mlmg, int psolver, double prrf, int pitmax, int plmg) {
+  if (!cs) return -1;
+  auto &C = cs->c;
+  if (!(urf_u > 0.0 && urf_u <= 1.0) || !(urf_p > 0.0 && urf_p <= 1.0) || mitmax < 0 || pitmax < 0) {
+    set_error("fs_case_configure: invalid relaxation factors or iteration limits");
+    return -1;
+  }
+  C.urf_u = urf_u;
+  C.urf_p = urf_p;
+  C.msolver = msolver;
+  C.mrrf = mrrf;
+  C.mitmax = mitmax;
+  C.mlmg = mlmg;
+  C.psolver = psolver;
+  C.prrf = prrf;
+  C.pitmax = pitmax;
+  C.plmg = plmg;
+  return 0;
+}
+
+int fs_case_iterate(fs_case *cs, int niter, fs_simple_stats *st) {
+  if (!cs || niter < 0) return -1;
+  CASE_GUARD_BEGIN
+  return fs::simple_iterate(cs->c, niter, st);
+  CASE_GUARD_END
+}
+
+int fs_case_phase(fs_case *cs, int phase) {
+  if (!cs) return -1;
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  switch (phase) {
+  case FS_PHASE_MOMENTUM_GRADIENTS: fs::phase_momentum_gradients(C); break;
+  case FS_PHASE_ASSEMBLE_U: fs::phase_momentum_assemble(C, 0); break;
+  case FS_PHASE_ASSEMBLE_V: fs::phase_momentum_assemble(C, 1); break;
+  case FS_PHASE_ASSEMBLE_W: fs::phase_momentum_assemble(C, 2); break;
+  case FS_PHASE_SOLVE_U: { int rc = fs::phase_momentum_solve(C, 0, nullptr); if (rc < 0) return rc; break; }
+  case FS_PHASE_SOLVE_V: { int rc = fs::phase_momentum_solve(C, 1, nullptr); if (rc < 0) return rc; break; }
+  case FS_PHASE_SOLVE_W: { int rc = fs::phase_momentum_solve(C, 2, nullptr); if (rc < 0) return rc; break; }
+  case FS_PHASE_PC_ASSEMBLE: fs::phase_pc_assemble(C); break;
+  case FS_PHASE_PC_SOLVE_CORRECT: { int rc = fs::phase_pc_solve_correct(C, nullptr); if (rc < 0) return rc; break; }
+  default:
+    set_error("fs_case_phase: unknown phase");
+    return -1;
+  }
+  FS_HIP(hipStreamSynchronize(C.s));
+  return 0;
+  CASE_GUARD_END
+}
+
+// name -> (buffer, length)
+static bool case_field(fs::SimpleCase &C, const char *name, double **p, size_t *n) {
+  const size_t N = C.N, Fi = C.Fi, Fb = C.Fb, F = C.F;
+  const std::map<std::string, std::pair<fs::DevBuf<double> *, size_t>> tab = {
+      {"u", {&C.u, N}}, {"v", {&C.v, N}}, {"w", {&C.w, N}}, {"p", {&C.p, N}}, {"pc", {&C.pc, N}},
+      {"mdot", {&C.mdot, Fi}}, {"gu", {&C.gu, 3 * N}}, {"gv", {&C.gv, 3 * N}},
+      {"gw", {&C.gw, 3 * N}}, {"gp", {&C.gp, 3 * N}}, {"gpc", {&C.gpc, 3 * N}},
+      {"bu", {&C.bu, Fb}}, {"bv", {&C.bv, Fb}}, {"bw", {&C.bw, Fb}}, {"bp", {&C.bp, Fb}},
+      {"bgp", {&C.bgp, 3 * Fb}}, {"fg", {&C.fg, 12 * Fi}}, {"du", {&C.du, N}}, {"dv", {&C.dv, N}},
+      {"dw", {&C.dw, N}}, {"us", {&C.us, N}}, {"vs", {&C.vs, N}}, {"ws", {&C.ws, N}},
+      {"ac", {&C.ac, N}}, {"anb", {&C.anb, N * C.W}}, {"bc", {&C.bc, N}}, {"cf", {&C.cf, F}},
+      {"ff", {&C.ff, F}}, {"tf", {&C.tf, F}}, {"vol", {&C.vol, N}}, {"Sx", {&C.Sx, F}},
+      {"Sy", {&C.Sy, F}}, {"Sz", {&C.Sz, F}}, {"g", {&C.g, F}}, {"gdiff", {&C.gdiff, F}},
+      {"Tx", {&C.Tx, F}}, {"Ty", {&C.Ty, F}}, {"Tz", {&C.Tz, F}}, {"dcf", {&C.d, F}},
+      {"A", {&C.A, F}}, {"kx", {&C.kx, Fi}}, {"ky", {&C.ky, Fi}}, {"kz", {&C.kz, Fi}}};
+  auto it = tab.find(name);
+  if (it == tab.end()) return false;
+  *p = it->second.first->p;
+  *n = it->second.second;
+  return true;
+}
+
+int fs_case_get(fs_case *cs, const char *name, double *out, long long count) {
+  if (!cs || !name || !out) return -1;
+  CASE_GUARD_BEGIN
+  double *p;
+  size_t n;
+  if (!case_field(cs->c, name, &p, &n)) {
+    set_error(std::string("fs_case_get: unknown field ") + name);
+    return -1;
+  }
+  if ((size_t)count != n) {
+    set_error(std::string("fs_case_get: field ") + name + " has " + std::to_string(n) + " entries");
+    return -1;
+  }
+  FS_HIP(hipMemcpyAsync(out, p, sizeof(double) * n, hipMemcpyDeviceToHost, cs->c.s));
+  FS_HIP(hipStreamSynchronize(cs->c.s));
+  return 0;
+  CASE_GUARD_END
+}
+
+int fs_case_set(fs_case *cs, const char *name, const double *in, long long count) {
+  if (!cs || !name || !in) return -1;
+  CASE_GUARD_BEGIN
+  double *p;
+  size_t n;
+  if (!case_field(cs->c, name, &p, &n)) {
+    set_error(std::string("fs_case_set: unknown field ") + name);
+    return -1;
+  }
+  if ((size_t)count != n) {
+    set_error(std::string("fs_case_set: field ") + name + " has " + std::to_string(n) + " entries");
+    return -1;
+  }
+  FS_HIP(hipMemcpyAsync(p, in, sizeof(double) * n, hipMemcpyHostToDevice, cs->c.s));
+  FS_HIP(hipStreamSynchronize(cs->c.s));
+  return 0;
+  CASE_GUARD_END
+}
+
+int fs_case_dims(fs_case *cs, int *dims) {
+  if (!cs || !dims) return -1;
+  dims[0] = cs->c.N;
+  dims[1] = cs->c.W;
+  dims[2] = cs->c.F;
+  dims[3] = cs->c.Fi;
+  return 0;
+}
+
+// Host copies of the topology arrays the reference builds in Readit
+// (reference numbering).  which: 0 NumbOfElementFaces(N), 1 NGlobalEFaces(N,W),
+// 2 ElementNeighbor(N,W), 3 iOwnerNeighbor(Fi), 4 iNeighborOwner(Fi).
+int fs_case_topology(fs_case *cs, int which, int *out, long long count) {
+  if (!cs || !out) return -1;
+  const auto &g = cs->c.geom;
+  const std::vector<int> *v = which == 0 ? &g.nef : which == 1 ? &g.gef : which == 2 ? &g.enbr
+                              : which == 3 ? &g.iown : which == 4 ? &g.inbr : nullptr;
+  if (!v || (size_t)count != v->size()) {
+    set_error("fs_case_topology: bad selector or size");
+    return -1;
+  }
+  std::copy(v->begin(), v->end(), out);
+  return 0;
+}
+
+} // extern "C"

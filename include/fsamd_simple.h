@@ -1,0 +1,122 @@
+/*
+ * fsamd_simple.h -- C ABI of the device-resident SIMPLE outer iteration.
+ *
+ * The reference runs one steady SIMPLE outer iteration per pass of its main
+ * loop (MainProgram/MainProgram.for:257-348): SolveMomentumX/Y/Z
+ * (SolveEquations/Momentum/SolveXMomentumEquation.for:4 and twins), then
+ * SolveContinuity (SolveEquations/ContinuityEquation/
+ * SolveContinuityEquation.for:4).  fs_case_iterate replaces that pass for a
+ * laminar, incompressible, constant-property flow bounded by no-slip walls
+ * (moving or at rest), with every array resident in HBM:
+ *   gradients  Green-Gauss + 2 face-centroid corrections
+ *              (GradientFiles/StandardGreenGaussCorrection1Gradient.for:3)
+ *   face grads 'average' (velocity) / 'averagecorrected' (pressure)
+ *              (Interpolation/InterpolateGradientFromElementToFace.for:3)
+ *   momentum   diffusion with E/T split, explicit and bulk-viscosity terms,
+ *              upwind convection, effective divergence, pressure gradient,
+ *              under-relaxation, D = V/aP
+ *              (Diffusion/AssembleIsotropicDiffusion.for:3,
+ *               Diffusion/AssembleExplicitDiffusion.for:3,
+ *               Diffusion/AssembleBulkViscosity.for:3,
+ *               Convection/AssembleConvection.for:4,
+ *               Momentum/AssemblePressureGradient.for:4,
+ *               UnderRelaxVariables/UnderRelaxation.for:4,
+ *               Momentum/StoreTheDValues.for:4)
+ *   continuity Rhie-Chow mass flux, p' coefficients, p' solve, pressure,
+ *              velocity and mass-flux corrections, wall pressure
+ *              (ContinuityEquation/AssembleMdotUsingRhieChow.for:4,
+ *               AssemblePressureCorrectionFluxes.for:4, CorrectPressure.for:4,
+ *               CorrectMdotandVelocity.for:4,722)
+ * with the reference's settings MethodCalcGradient = 2, nIterGradient = 2,
+ * MethodDecomposeS = 3, Algorithm 'simple', LRelaxMomentum, upwind
+ * convection (User0.F90).
+ *
+ * The mesh is a polyMesh in memory (0-based): face_ptr/face_nodes list each
+ * face's nodes, faces 0..nif-1 are interior (owner < neighbour), the rest are
+ * boundary faces patch by patch.  Topology and geometry follow the
+ * reference's Readit (GridAndGeometry/ReadFoamMesh.f90:254-619) and
+ * ProcessGeometry (GridAndGeometry/ProcessGeometryRoutine.for:3).
+ */
+#ifndef FSAMD_SIMPLE_H
+#define FSAMD_SIMPLE_H
+
+#include "fsamd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fs_case fs_case;
+
+typedef struct fs_simple_stats {
+  int iterations;            /* SIMPLE outer iterations run                  */
+  int momentum_iterations;   /* summed over u, v, w solves                   */
+  int pc_iterations;         /* summed p' solver iterations                  */
+  int pc_unconverged;        /* p' solves that stopped at itmax              */
+  int pc_levels;             /* AMG levels of the last p' solve              */
+  double seconds;            /* wall time of the iterations                  */
+  double momentum_seconds;
+  double pc_seconds;
+  double pc_initial_l1;      /* last p' solve: sum|bc|                       */
+  double pc_final_l1;        /* last p' solve: sum|bc - A p'|                */
+  double res_momentum[3];    /* last ResorAbs of u, v, w                     */
+  double mass_l1;            /* sum over cells of |net mass outflow| at end  */
+} fs_simple_stats;
+
+enum fs_phase {
+  FS_PHASE_MOMENTUM_GRADIENTS = 0,
+  FS_PHASE_ASSEMBLE_U = 1,
+  FS_PHASE_ASSEMBLE_V = 2,
+  FS_PHASE_ASSEMBLE_W = 3,
+  FS_PHASE_SOLVE_U = 4,
+  FS_PHASE_SOLVE_V = 5,
+  FS_PHASE_SOLVE_W = 6,
+  FS_PHASE_PC_ASSEMBLE = 7,
+  FS_PHASE_PC_SOLVE_CORRECT = 8
+};
+
+/* wall_u/v/w: velocity of every boundary face (nfaces - nif entries). */
+int fs_case_create(int npoints, const double *x, const double *y, const double *z,
+                   int nfaces, const int *face_ptr, const int *face_nodes,
+                   const int *owner, int nif, const int *neighbour,
+                   const double *wall_u, const double *wall_v,
+                   const double *wall_w, double rho, double mu, fs_case **out);
+int fs_case_destroy(fs_case *c);
+/* urfMomentum, urfPressure, ASSolverMomentum/rrFMomentum/ASIterMomentum/
+ * LMultigridMomentum and the Continuity equivalents (User0.F90:271-318). */
+int fs_case_configure(fs_case *c, double urf_u, double urf_p, int msolver,
+                      double mrrf, int mitmax, int mlmg, int psolver,
+                      double prrf, int pitmax, int plmg);
+/* The linear-solve context of the case (for fs_set_option). */
+fs_mesh *fs_case_mesh(fs_case *c);
+int fs_case_iterate(fs_case *c, int niter, fs_simple_stats *stats);
+/* One phase of the iteration, for tests (enum fs_phase). */
+int fs_case_phase(fs_case *c, int phase);
+/* Host copies of device fields by name (u v w p pc mdot gu gv gw gp gpc bu bv
+ * bw bp bgp fg du dv dw us vs ws ac anb bc cf ff tf and geometry vol Sx Sy
+ * Sz g gdiff Tx Ty Tz dcf A kx ky kz); count must equal the field length. */
+int fs_case_get(fs_case *c, const char *name, double *out, long long count);
+int fs_case_set(fs_case *c, const char *name, const double *in, long long count);
+/* dims = {N, W, NFacesTotal, NIFaces} */
+int fs_case_dims(fs_case *c, int *dims);
+/* Topology arrays in the reference's numbering: which = 0 NumbOfElementFaces,
+ * 1 NGlobalEFaces, 2 ElementNeighbor, 3 iOwnerNeighbor, 4 iNeighborOwner. */
+int fs_case_topology(fs_case *c, int which, int *out, long long count);
+
+/* Host-only topology and geometry (no device needed): the same arrays the
+ * case uploads.  Face fields over NFacesTotal (interior first): Sx Sy Sz A
+ * fcx fcy fcz nx ny nz dcf dx dy dz ex ey ez g Ex Ey Ez E gdiff Tx Ty Tz T;
+ * cell fields: vol xc yc zc (ProcessGeometryRoutine.for:76-499). */
+typedef struct fs_geom fs_geom;
+int fs_geom_create(int npoints, const double *x, const double *y, const double *z,
+                   int nfaces, const int *face_ptr, const int *face_nodes,
+                   const int *owner, int nif, const int *neighbour, fs_geom **out);
+int fs_geom_destroy(fs_geom *g);
+int fs_geom_dims(fs_geom *g, int *dims);
+int fs_geom_get(fs_geom *g, const char *name, double *out, long long count);
+int fs_geom_topology(fs_geom *g, int which, int *out, long long count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -231,6 +231,7 @@ def _setup_case_symbols(L):
     L.fs_case_set.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_longlong]
     L.fs_case_dims.argtypes = [C.c_void_p, _ip]
     L.fs_case_topology.argtypes = [C.c_void_p, C.c_int, _ip, C.c_longlong]
+    L.fs_case_time_spmv.argtypes = [C.c_void_p, C.c_int, _dp]
     L._case_ready = True
 
 
@@ -320,6 +321,12 @@ class Case:
         st = SimpleStats()
         _check(lib().fs_case_iterate(self.h, int(n), C.byref(st)), "fs_case_iterate")
         return st.as_dict()
+
+    def time_spmv(self, reps=50):
+        """Average ms of one ELL product with the last assembled system."""
+        ms = np.zeros(1)
+        _check(lib().fs_case_time_spmv(self.h, int(reps), _dptr(ms)), "fs_case_time_spmv")
+        return float(ms[0])
 
     def phase(self, name):
         _check(lib().fs_case_phase(self.h, PHASES[name] if isinstance(name, str) else int(name)), "fs_case_phase")

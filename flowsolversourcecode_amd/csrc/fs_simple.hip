@@ -899,6 +899,15 @@ int fs_case_set(fs_case *cs, const char *name, const double *in, long long count
   CASE_GUARD_END
 }
 
+int fs_case_time_spmv(fs_case *cs, int reps, double *avg_ms) {
+  if (!cs || reps <= 0 || !avg_ms) return -1;
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  // the p' operator of the last iteration applied to the current pressure
+  return fs::time_spmv(C.mesh, C.ac.p, C.anb.p, C.p.p, C.dphi.p, reps, avg_ms);
+  CASE_GUARD_END
+}
+
 int fs_case_dims(fs_case *cs, int *dims) {
   if (!cs || !dims) return -1;
   dims[0] = cs->c.N;

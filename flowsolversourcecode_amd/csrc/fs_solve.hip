@@ -791,3 +791,32 @@ void add_inplace(fs_mesh *m, double *a, const double *b) {
 }
 
 } // namespace fs
+
+// ---- kernel timing for the roofline figure: the ELL product y = A*x of the
+// Krylov loop (k_spmv_dot), launched reps times between two events on the
+// mesh's stream.
+namespace fs {
+int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, double *y,
+              int reps, double *avg_ms) {
+  const int n = m->n, w = m->w, gr = grid_red(n);
+  hipEvent_t e0, e1;
+  FS_HIP(hipEventCreate(&e0));
+  FS_HIP(hipEventCreate(&e1));
+  int h_fl[F_NFLAGS] = {0, 0, 1, 0};
+  FS_HIP(hipMemcpyAsync(m->flags.p, h_fl, sizeof(h_fl), hipMemcpyHostToDevice, m->stream));
+  hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, m->d_nbr.p, ac, anb, x,
+                     y, x, m->partial.p, m->flags.p);
+  FS_HIP(hipEventRecord(e0, m->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, m->d_nbr.p, ac, anb,
+                       x, y, x, m->partial.p, m->flags.p);
+  FS_HIP(hipEventRecord(e1, m->stream));
+  FS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+} // namespace fs

@@ -97,6 +97,10 @@ int fs_case_phase(fs_case *c, int phase);
  * Sz g gdiff Tx Ty Tz dcf A kx ky kz); count must equal the field length. */
 int fs_case_get(fs_case *c, const char *name, double *out, long long count);
 int fs_case_set(fs_case *c, const char *name, const double *in, long long count);
+/* Average duration (ms) of the ELL product y = A*x with the last assembled
+ * system (the p' operator after an iteration), reps launches timed with HIP
+ * events on the case's stream. */
+int fs_case_time_spmv(fs_case *c, int reps, double *avg_ms);
 /* dims = {N, W, NFacesTotal, NIFaces} */
 int fs_case_dims(fs_case *c, int *dims);
 /* Topology arrays in the reference's numbering: which = 0 NumbOfElementFaces,

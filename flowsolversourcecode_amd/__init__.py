@@ -33,7 +33,8 @@ NF_CODES = {"velx": 1, "vely": 2, "velz": 3, "pressc": 4, "press": 5, "temp": 6,
             "tretheta": 14, "tv2": 15, "tzeta": 16, "frelax": 17, "lambda": 18}
 OPTIONS = {"mg_presweep": 1, "mg_postsweep": 2, "mg_maxcycles": 3, "mg_maxlevels": 4,
            "mg_minparents": 5, "mg_reagglomerate": 6, "mg_fp32": 7, "mg_omega": 8,
-           "mg_smoother": 9, "gmres_restart": 10, "check_every": 11}
+           "mg_smoother": 9, "gmres_restart": 10, "check_every": 11,
+           "mg_tail_cells": 12, "mg_coarse_sweeps": 13}
 
 
 class FsError(RuntimeError):

@@ -370,6 +370,136 @@ __global__ void k_add2(int n, double *__restrict__ x, const double *__restrict__
   if (i < n) x[i] = x[i] + c[i];
 }
 
+
+
+// ---- lanes-per-row forms for the wide coarse levels: a row's slots are
+// spread over LPR consecutive lanes and combined with shuffles, so a row
+// costs one dependent-load chain instead of w of them.
+template <int LPR, class T>
+__device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, const int *__restrict__ nbr,
+                                         const T *__restrict__ anb, const T *__restrict__ x) {
+  T s = T(0);
+  for (int j = lane; j < w; j += LPR) {
+    const int k = nbr[(size_t)j * n + i];
+    if (k >= 0) s += anb[(size_t)j * n + i] * x[k];
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+  return s;
+}
+
+template <int LPR, class T>
+__global__ void k_lresid_lpr(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac,
+                             const T *__restrict__ anb, const T *__restrict__ b,
+                             const T *__restrict__ x, T *__restrict__ r, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int i = t / LPR, lane = t % LPR;
+  const bool ok = i < n;
+  const T s = row_offdiag<LPR, T>(ok ? i : 0, lane, ok ? n : 1, ok ? w : 0, nbr, anb, x);
+  if (ok && lane == 0) r[i] = b[i] - ac[i] * x[i] - s;
+}
+
+template <int LPR, class T>
+__global__ void k_jac_lpr(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac,
+                          const T *__restrict__ anb, const T *__restrict__ b, const T *__restrict__ x,
+                          T *__restrict__ xn, T omega, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int i = t / LPR, lane = t % LPR;
+  const bool ok = i < n;
+  const T s = row_offdiag<LPR, T>(ok ? i : 0, lane, ok ? n : 1, ok ? w : 0, nbr, anb, x);
+  if (ok && lane == 0) xn[i] = x[i] + omega * (b[i] - ac[i] * x[i] - s) / ac[i];
+}
+
+
+// ------------------------------------------------------------ small levels
+// Every level below the tail threshold is processed by ONE workgroup in one
+// launch: restriction, smoothing, the coarsest level and the prolongations of
+// those levels, separated by workgroup barriers.  The small levels are
+// latency-bound (a few thousand cells, tens of slots), so one resident
+// workgroup with its data in L2 replaces dozens of tiny launches.
+struct TailLevel {
+  int n, w, ncolors;
+  const int *nbr;
+  const void *ac, *anb;
+  void *x, *b, *r, *t;
+  const int *parent;              // to the next coarser level, or null
+  const int *mptr, *mem;          // members of this level's aggregates in the finer level
+  const int *color_ptr, *color_cells;
+};
+
+constexpr int kTailThreads = 1024;
+
+constexpr int kTailLPR = 8;
+
+// y_i = b_i - ac_i x_i - sum_j anb_ij x_j for the rows of one level, LPR lanes
+// per row (all 1024 threads of the workgroup take part)
+template <class T>
+__device__ void tail_rows(const TailLevel &L, const T *x, T omega, T *out, int mode) {
+  const T *ac = (const T *)L.ac, *anb = (const T *)L.anb, *b = (const T *)L.b;
+  const int lane = threadIdx.x % kTailLPR;
+  const int rows_per_pass = blockDim.x / kTailLPR;
+  for (int base = 0; base < L.n; base += rows_per_pass) {
+    const int i = base + threadIdx.x / kTailLPR;
+    const bool ok = i < L.n;
+    const T s = row_offdiag<kTailLPR, T>(ok ? i : 0, lane, ok ? L.n : 1, ok ? L.w : 0, L.nbr, anb, x);
+    if (ok && lane == 0) {
+      const T r = b[i] - ac[i] * x[i] - s;
+      out[i] = mode == 0 ? r : x[i] + omega * r / ac[i];
+    }
+  }
+}
+
+// damped Jacobi sweeps on one level (x and t alternate; the result ends in x)
+template <class T> __device__ void tail_smooth(const TailLevel &L, int sweeps, bool zero, T omega) {
+  const T *ac = (const T *)L.ac, *b = (const T *)L.b;
+  T *x = (T *)L.x, *t = (T *)L.t;
+  int sw = 0;
+  if (zero) {
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) x[i] = sweeps > 0 ? omega * b[i] / ac[i] : T(0);
+    __syncthreads();
+    sw = 1;
+  }
+  for (; sw < sweeps; ++sw) {
+    tail_rows<T>(L, x, omega, t, 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) x[i] = t[i];
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kTailThreads)
+k_tail(const TailLevel *__restrict__ lv, int nlev, int pre, int post, int coarse, T omega,
+       const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  for (int l = 0; l + 1 < nlev; ++l) {
+    const TailLevel &L = lv[l];
+    tail_smooth<T>(L, pre, true, omega);
+    tail_rows<T>(L, (const T *)L.x, omega, (T *)L.r, 0);
+    __syncthreads();
+    const TailLevel &C = lv[l + 1];
+    const T *r = (const T *)L.r;
+    T *bc = (T *)C.b;
+    for (int p = threadIdx.x; p < C.n; p += blockDim.x) {
+      T s = T(0);
+      for (int q = C.mptr[p]; q < C.mptr[p + 1]; ++q) s = s + r[C.mem[q]];
+      bc[p] = s;
+    }
+    __syncthreads();
+  }
+  tail_smooth<T>(lv[nlev - 1], coarse, true, omega);
+  for (int l = nlev - 2; l >= 0; --l) {
+    const TailLevel &L = lv[l];
+    T *x = (T *)L.x;
+    const T *xc = (const T *)lv[l + 1].x;
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) x[i] = x[i] + xc[L.parent[i]];
+    __syncthreads();
+    tail_smooth<T>(L, post, false, omega);
+  }
+}
+
 } // namespace
 
 // ------------------------------------------------------------ hierarchy
@@ -383,7 +513,7 @@ struct DLevel {
   DevBuf<double> x64, b64, r64, t64;
   DevBuf<float> x32, b32, r32, t32;
   std::vector<int> color_ptr;
-  DevBuf<int> color_cells;
+  DevBuf<int> color_cells, color_ptr_dev;
   DevBuf<int> parent;            // 0-based parent in the next level
   DevBuf<int> mptr, mem;         // members of this level's aggregates (coarse)
   DevBuf<int> gptr;
@@ -396,6 +526,8 @@ struct Hierarchy {
   std::vector<DLevel> lev;
   int calls = 0;
   bool fp32 = false;
+  int tail = -1;                 // first level run by the one-workgroup tail
+  DevBuf<TailLevel> tail32, tail64;
 };
 
 static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
@@ -469,6 +601,7 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
     std::vector<int> cells;
     greedy_colors(h.n, h.w, nbr0.data(), D.color_ptr, cells);
     D.color_cells.upload(cells, s);
+    D.color_ptr_dev.upload(D.color_ptr, s);
     D.x64.alloc(h.n);
     D.b64.alloc(h.n);
     D.r64.alloc(h.n);
@@ -479,6 +612,41 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
     D.t32.alloc(h.n);
     D.ac32.alloc(h.n);
     D.anb32.alloc((size_t)h.n * h.w);
+  }
+  // levels from the first one at or below the threshold run in the tail
+  H->tail = nl; // level 0 always runs grid-wide (it is the caller's operator)
+  for (int l = 1; l < nl; ++l)
+    if (H->lev[l].n <= m->mg_tail_cells) {
+      H->tail = l;
+      break;
+    }
+  if (H->tail < nl) {
+    const int nt = nl - H->tail;
+    std::vector<TailLevel> d32(nt), d64(nt);
+    for (int q = 0; q < nt; ++q) {
+      DLevel &D = H->lev[H->tail + q];
+      for (int prec = 0; prec < 2; ++prec) {
+        TailLevel t{};
+        t.n = D.n;
+        t.w = D.w;
+        t.ncolors = (int)D.color_ptr.size() - 1;
+        t.nbr = D.nbr;
+        t.ac = prec ? (const void *)D.ac64_own.p : (const void *)D.ac32.p;
+        t.anb = prec ? (const void *)D.anb64_own.p : (const void *)D.anb32.p;
+        t.x = prec ? (void *)D.x64.p : (void *)D.x32.p;
+        t.b = prec ? (void *)D.b64.p : (void *)D.b32.p;
+        t.r = prec ? (void *)D.r64.p : (void *)D.r32.p;
+        t.t = prec ? (void *)D.t64.p : (void *)D.t32.p;
+        t.parent = (H->tail + q + 1 < nl) ? D.parent.p : nullptr;
+        t.mptr = q > 0 ? D.mptr.p : nullptr;
+        t.mem = q > 0 ? D.mem.p : nullptr;
+        t.color_ptr = D.color_ptr_dev.p;
+        t.color_cells = D.color_cells.p;
+        (prec ? d64 : d32)[q] = t;
+      }
+    }
+    H->tail32.upload(d32, s);
+    H->tail64.upload(d64, s);
   }
   FS_HIP_THROW(hipStreamSynchronize(s));
   m->amg = H;
@@ -536,12 +704,49 @@ template <> struct LV<float> {
 };
 
 template <class T>
+static void launch_jac(fs_mesh *m, DLevel &D, const T *b, const T *x, T *xn, T om, const int *flags) {
+  hipStream_t s = m->stream;
+  const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
+  if (D.w <= 8)
+    hipLaunchKernelGGL(k_jac<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x, xn,
+                       om, flags);
+  else if (D.w <= 16)
+    hipLaunchKernelGGL((k_jac_lpr<4, T>), dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
+                       anb, b, x, xn, om, flags);
+  else if (D.w <= 32)
+    hipLaunchKernelGGL((k_jac_lpr<8, T>), dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
+                       anb, b, x, xn, om, flags);
+  else
+    hipLaunchKernelGGL((k_jac_lpr<16, T>), dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, D.w, D.nbr,
+                       ac, anb, b, x, xn, om, flags);
+}
+
+template <class T>
+static void launch_resid(fs_mesh *m, DLevel &D, const T *b, const T *x, T *r, const int *flags) {
+  hipStream_t s = m->stream;
+  const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
+  if (D.w <= 8)
+    hipLaunchKernelGGL(k_lresid<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x, r,
+                       flags);
+  else if (D.w <= 16)
+    hipLaunchKernelGGL((k_lresid_lpr<4, T>), dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
+                       anb, b, x, r, flags);
+  else if (D.w <= 32)
+    hipLaunchKernelGGL((k_lresid_lpr<8, T>), dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
+                       anb, b, x, r, flags);
+  else
+    hipLaunchKernelGGL((k_lresid_lpr<16, T>), dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, D.w, D.nbr,
+                       ac, anb, b, x, r, flags);
+}
+
+template <class T>
 static void smooth(fs_mesh *m, DLevel &D, const T *b, T *&x, T *&tmp, int sweeps, bool zero_start,
                    const int *flags) {
   hipStream_t s = m->stream;
   const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
   const int nc = (int)D.color_ptr.size() - 1;
-  const bool gs = m->mg_smoother == 1 && nc <= 8;
+  // multicolour Gauss-Seidel only where it is cheap: few colours, narrow rows
+  const bool gs = m->mg_smoother == 1 && nc <= 4 && D.w <= 8;
   const T om = (T)m->mg_omega;
   if (gs) {
     if (zero_start) hipLaunchKernelGGL(k_zero<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, x, flags);
@@ -566,18 +771,20 @@ static void smooth(fs_mesh *m, DLevel &D, const T *b, T *&x, T *&tmp, int sweeps
       hipLaunchKernelGGL(k_zero<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, x, flags);
     }
     for (; sw < sweeps; ++sw) {
-      hipLaunchKernelGGL(k_jac<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b,
-                         x, tmp, om, flags);
+      launch_jac<T>(m, D, b, x, tmp, om, flags);
       std::swap(x, tmp);
     }
   }
 }
 
 // One V-cycle: x_0 = M^{-1} b_0 on the hierarchy held in precision T.
+// Levels 0..tail-1 are swept by grid-wide kernels, levels tail..L-1 by the
+// one-workgroup tail kernel.
 template <class T> static void vcycle(fs_mesh *m, const T *b0, T *x0, const int *flags) {
   Hierarchy &H = *m->amg;
   hipStream_t s = m->stream;
   const int L = (int)H.lev.size();
+  const int tail = std::min(H.tail, L); // >= 1 by construction
   std::vector<T *> X(L), Tm(L);
   std::vector<const T *> B(L);
   for (int l = 0; l < L; ++l) {
@@ -586,21 +793,26 @@ template <class T> static void vcycle(fs_mesh *m, const T *b0, T *x0, const int 
     B[l] = LV<T>::b(H.lev[l]);
   }
   B[0] = b0;
-  for (int l = 0; l + 1 < L; ++l) {
+  const int top = tail < L ? tail : L - 1; // levels swept by grid-wide kernels: 0..top-1
+  for (int l = 0; l < top; ++l) {
     DLevel &D = H.lev[l];
     smooth<T>(m, D, B[l], X[l], Tm[l], m->mg_pre, true, flags);
-    hipLaunchKernelGGL(k_lresid<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr,
-                       LV<T>::ac(D), LV<T>::anb(D), B[l], X[l], LV<T>::r(D), flags);
+    launch_resid<T>(m, D, B[l], X[l], LV<T>::r(D), flags);
     DLevel &C = H.lev[l + 1];
     hipLaunchKernelGGL(k_restrict<T>, dim3(gfor(C.n)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p,
                        LV<T>::r(D), LV<T>::b(C), flags);
   }
-  {
+  const int coarse = std::max(m->mg_coarse_sweeps, 1);
+  if (tail < L) {
+    const TailLevel *desc = (sizeof(T) == 4) ? H.tail32.p : H.tail64.p;
+    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), 0, s, desc, L - tail, m->mg_pre,
+                       m->mg_post, coarse, (T)m->mg_omega, flags);
+    X[tail] = LV<T>::x(H.lev[tail]); // the tail keeps its result in x
+  } else {
     DLevel &D = H.lev[L - 1];
-    const int coarse = std::max(4, 2 * (m->mg_pre + m->mg_post));
     smooth<T>(m, D, B[L - 1], X[L - 1], Tm[L - 1], coarse, true, flags);
   }
-  for (int l = L - 2; l >= 0; --l) {
+  for (int l = top - 1; l >= 0; --l) {
     DLevel &D = H.lev[l];
     hipLaunchKernelGGL(k_prolong<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.parent.p, X[l + 1],
                        X[l], flags);

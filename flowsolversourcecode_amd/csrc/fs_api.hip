@@ -121,6 +121,9 @@ int fs_set_option(fs_mesh *m, int key, double value) {
     return -1;
   }
   const int iv = (int)value;
+  // keys that change the hierarchy's structure force a rebuild, but only when
+  // their value actually changes
+  const int old_levels = m->mg_maxlevels, old_parents = m->mg_minparents, old_tail = m->mg_tail_cells;
   switch (key) {
   case FS_OPT_MG_PRESWEEP: m->mg_pre = iv; break;
   case FS_OPT_MG_POSTSWEEP: m->mg_post = iv; break;
@@ -133,11 +136,14 @@ int fs_set_option(fs_mesh *m, int key, double value) {
   case FS_OPT_MG_SMOOTHER: m->mg_smoother = iv; break;
   case FS_OPT_GMRES_RESTART: m->gmres_m = iv; break;
   case FS_OPT_CHECK_EVERY: m->check_every = iv < 1 ? 1 : iv; break;
+  case FS_OPT_MG_TAIL_CELLS: m->mg_tail_cells = iv; break;
+  case FS_OPT_MG_COARSE_SWEEPS: m->mg_coarse_sweeps = iv < 1 ? 1 : iv; break;
   default:
     set_error("fs_set_option: unknown key " + std::to_string(key));
     return -1;
   }
-  if (key == FS_OPT_MG_MAXLEVELS || key == FS_OPT_MG_MINPARENTS) m->amg.reset();
+  if (m->mg_maxlevels != old_levels || m->mg_minparents != old_parents || m->mg_tail_cells != old_tail)
+    m->amg.reset();
   return 0;
 }
 

@@ -103,9 +103,10 @@ struct fs_mesh {
   fs::DevBuf<double> scal;     // scalar slots
   fs::DevBuf<int> flags;       // [0] done, [1] iterations
   // options
-  int mg_pre = 1, mg_post = 1, mg_maxcycles = 30, mg_maxlevels = 10;
-  int mg_minparents = 10, mg_reagg = 300, mg_fp32 = 0, mg_smoother = 1;
+  int mg_pre = 2, mg_post = 2, mg_maxcycles = 30, mg_maxlevels = 10;
+  int mg_minparents = 10, mg_reagg = 300, mg_fp32 = 0, mg_smoother = 0;
   double mg_omega = 0.8;
+  int mg_tail_cells = 1024, mg_coarse_sweeps = 16;
   int gmres_m = 30, check_every = 8;
   int ncalls_mg = 0;
   void *solve_work = nullptr;           // fs_solve.hip

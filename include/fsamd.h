@@ -54,8 +54,8 @@ enum fs_multigrid {
 
 /* option keys for fs_set_option (defaults in brackets) */
 enum fs_option {
-  FS_OPT_MG_PRESWEEP = 1,     /* MultiGridpreSweep [1 on device]                */
-  FS_OPT_MG_POSTSWEEP = 2,    /* MultiGridpostSweep [1 on device]               */
+  FS_OPT_MG_PRESWEEP = 1,     /* MultiGridpreSweep [2 on device]                */
+  FS_OPT_MG_POSTSWEEP = 2,    /* MultiGridpostSweep [2 on device]               */
   FS_OPT_MG_MAXCYCLES = 3,    /* MaxMultiGridCycles [30]                        */
   FS_OPT_MG_MAXLEVELS = 4,    /* maxNumberofCoarseLevels [10]                   */
   FS_OPT_MG_MINPARENTS = 5,   /* minNumberofParents [10]                        */
@@ -66,9 +66,12 @@ enum fs_option {
                                  FP64 Krylov loop [0]                          */
   FS_OPT_MG_OMEGA = 8,        /* damping of the device Jacobi smoother [0.8]    */
   FS_OPT_MG_SMOOTHER = 9,     /* 0 = damped Jacobi, 1 = multicolour symmetric
-                                 Gauss-Seidel [1]                              */
+                                 Gauss-Seidel on levels with <= 4 colours [0]  */
   FS_OPT_GMRES_RESTART = 10,  /* GMRES restart length [30]                      */
-  FS_OPT_CHECK_EVERY = 11     /* iterations launched between host checks [8]    */
+  FS_OPT_CHECK_EVERY = 11,    /* iterations launched between host checks [8]    */
+  FS_OPT_MG_TAIL_CELLS = 12,  /* levels with at most this many cells run in one
+                                 single-workgroup kernel [1024]                 */
+  FS_OPT_MG_COARSE_SWEEPS = 13 /* smoothing sweeps on the coarsest level [16]   */
 };
 
 /* Statistics of the last solve. */

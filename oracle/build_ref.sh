@@ -28,3 +28,18 @@ done
 $FC $FFLAGS -c ref_solve_driver.f90 -o _ref/ref_solve_driver.o
 $FC -fopenmp -o _ref/ref_solve.x _ref/ref_solve_driver.o "${OBJS[@]}"
 echo "built _ref/ref_solve.x"
+
+# The same driver linked with the device drop-in SolveEquation (the Fortran
+# shim of the product) instead of SolveAlegbraicEquations.for.
+LIBDIR="$(cd .. && pwd)/flowsolversourcecode_amd"
+if [ -f "$LIBDIR/libfsamd.so" ]; then
+  $FC $FFLAGS -c "$LIBDIR/fortran/fsamd_iface.f90" -o _ref/fsamd_iface.o
+  $FC $FFLAGS -c "$LIBDIR/fortran/SolveEquationDevice.f90" -o _ref/SolveEquationDevice.o
+  OBJS2=()
+  for o in "${OBJS[@]}"; do
+    case "$(basename "$o")" in SolveAlegbraicEquations.o) ;; *) OBJS2+=("$o") ;; esac
+  done
+  $FC -fopenmp -o _ref/ref_solve_device.x _ref/ref_solve_driver.o _ref/fsamd_iface.o \
+      _ref/SolveEquationDevice.o "${OBJS2[@]}" -L"$LIBDIR" -lfsamd -Wl,-rpath,"\$ORIGIN/../../flowsolversourcecode_amd"
+  echo "built _ref/ref_solve_device.x"
+fi

@@ -75,6 +75,8 @@ program ref_solve_driver
   case (0); solver = 'sor'
   case (1); solver = 'ilu'
   case (2); solver = 'pbcg'
+  case (4); solver = 'gmres'
+  case (5); solver = 'bicgs'
   case default; solver = 'sor'
   end select
   select case (icycle)

@@ -53,6 +53,23 @@ def cpu_baseline(args):
                       f"{k} iterations in {el:.1f} s ({rate:.3f} it/s), scaled by cell count {n}^3/{args.n}^3"}
 
 
+COPY_BYTES = 1 << 30
+
+
+def traffic_probe(case, args):
+    """Launch sequence for the FETCH_SIZE / WRITE_SIZE counter runs: one SIMPLE
+    iteration (to assemble a real p' system), then the ELL product alone, then
+    a plain device-to-device copy of COPY_BYTES (k_copy16, 4 launches), whose
+    counters give the scale from counter units to bytes."""
+    import flowsolversourcecode_amd as fs
+    case.iterate(1)
+    ms = case.time_spmv(args.spmv_reps)
+    copy_ms = fs.time_device_copy(COPY_BYTES, 3)
+    print(json.dumps({"probe": "traffic", "cells": case.N, "W": case.W, "spmv_ms": ms, "spmv_reps": args.spmv_reps,
+                      "spmv_bytes": case.N * (8 + case.W * 12 + 24), "copy_bytes": COPY_BYTES,
+                      "copy_ms": copy_ms, "copy_GBps": 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +81,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="set up the workload, run one iteration, then only the ELL product (--spmv-reps launches) "
+                         "and a 1 GiB device copy; for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the mesh is a design "
@@ -83,6 +103,9 @@ def main():
     setup_s = time.perf_counter() - t0
     print(f"# setup {setup_s:.1f} s: N={case.N} faces={case.F}", file=sys.stderr, flush=True)
 
+    if args.traffic_probe:
+        traffic_probe(case, args)
+        return
     if args.warmup:
         st = case.iterate(args.warmup)
         print(f"# warmup {args.warmup} its: {st['seconds']:.2f} s pc_its={st['pc_iterations']}",

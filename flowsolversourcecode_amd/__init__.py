@@ -73,12 +73,20 @@ def lib():
                                                C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p, _dp, C.POINTER(SolveStats)]
         L.fs_amg_export.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]
+        L.fs_time_device_copy.argtypes = [C.c_longlong, C.c_int, _dp]
         _lib = L
     return _lib
 
 
 def last_error():
     return lib().fs_last_error().decode()
+
+
+def time_device_copy(nbytes, reps=3):
+    """Average ms of one device-to-device copy of nbytes (streaming kernel)."""
+    ms = np.zeros(1)
+    _check(lib().fs_time_device_copy(int(nbytes), int(reps), _dptr(ms)), "fs_time_device_copy")
+    return float(ms[0])
 
 
 def _iptr(a):

@@ -903,8 +903,9 @@ int fs_case_time_spmv(fs_case *cs, int reps, double *avg_ms) {
   if (!cs || reps <= 0 || !avg_ms) return -1;
   CASE_GUARD_BEGIN
   auto &C = cs->c;
-  // the p' operator of the last iteration applied to the current pressure
-  return fs::time_spmv(C.mesh, C.ac.p, C.anb.p, C.p.p, C.dphi.p, reps, avg_ms);
+  // the p' operator of the last iteration applied to the current pressure,
+  // dotted with its right-hand side (three distinct N-vectors, as in BiCGSTAB)
+  return fs::time_spmv(C.mesh, C.ac.p, C.anb.p, C.p.p, C.bc.p, C.dphi.p, reps, avg_ms);
   CASE_GUARD_END
 }
 

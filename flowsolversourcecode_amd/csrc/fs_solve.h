@@ -28,8 +28,9 @@ void amg_apply(fs_mesh *m, const double *rhs, double *out, const int *flags);
 int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc, double *dphi,
               int solver);
 int amg_levels(fs_mesh *m);
-int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, double *y,
-              int reps, double *avg_ms);
+// Average ms of one k_spmv_dot launch (y = A x with (dot_with, y) partials).
+int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, const double *dot_with,
+              double *y, int reps, double *avg_ms);
 // Host copies of the aggregation maps of level l -> l+1 for tests.
 int amg_export(fs_mesh *m, int level, int *dims, int *parent, int *seed, int *nchild,
                int *child_ptr, int *child, int *nbr, int *nnbr, double *ac, double *anb);

@@ -796,8 +796,8 @@ void add_inplace(fs_mesh *m, double *a, const double *b) {
 // Krylov loop (k_spmv_dot), launched reps times between two events on the
 // mesh's stream.
 namespace fs {
-int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, double *y,
-              int reps, double *avg_ms) {
+int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, const double *dot_with,
+              double *y, int reps, double *avg_ms) {
   const int n = m->n, w = m->w, gr = grid_red(n);
   hipEvent_t e0, e1;
   FS_HIP(hipEventCreate(&e0));
@@ -805,11 +805,11 @@ int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, 
   int h_fl[F_NFLAGS] = {0, 0, 1, 0};
   FS_HIP(hipMemcpyAsync(m->flags.p, h_fl, sizeof(h_fl), hipMemcpyHostToDevice, m->stream));
   hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, m->d_nbr.p, ac, anb, x,
-                     y, x, m->partial.p, m->flags.p);
+                     y, dot_with, m->partial.p, m->flags.p);
   FS_HIP(hipEventRecord(e0, m->stream));
   for (int r = 0; r < reps; ++r)
     hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, m->d_nbr.p, ac, anb,
-                       x, y, x, m->partial.p, m->flags.p);
+                       x, y, dot_with, m->partial.p, m->flags.p);
   FS_HIP(hipEventRecord(e1, m->stream));
   FS_HIP(hipEventSynchronize(e1));
   float ms = 0.f;

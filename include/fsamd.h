@@ -151,6 +151,12 @@ int fs_amg_export(fs_mesh *mesh, int level, int *dims, int *parent, int *seed,
                   int *nchild, int *child_ptr, int *child, int *nbr,
                   int *nnbr, double *ac, double *anb);
 
+/* Average ms of one plain device-to-device copy of `bytes` (a multiple of
+ * 16) over `reps` launches of a streaming copy kernel on a private stream.
+ * No counterpart in the reference: it calibrates HBM byte counters
+ * (FETCH_SIZE / WRITE_SIZE) and gives the achievable copy bandwidth. */
+int fs_time_device_copy(long long bytes, int reps, double *avg_ms);
+
 /* Message of the last failing call on this thread. */
 const char *fs_last_error(void);
 /* Library version string. */

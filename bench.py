@@ -54,6 +54,24 @@ def cpu_baseline(args):
 
 
 COPY_BYTES = 1 << 30
+HBM_PEAK_GBPS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_spmv_dot_cavity128_hbm_traffic.md")
+
+
+def measured_traffic(cells):
+    """HBM bytes per k_spmv_dot launch from the committed FETCH_SIZE /
+    WRITE_SIZE counter summary of this workload (a line
+    'traffic_bytes_per_launch cells=<N>: <bytes>'), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    head, val = line.split(":", 1)
+                    if int(head.split("cells=")[1]) == cells:
+                        return float(val)
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
 
 
 def traffic_probe(case, args):
@@ -66,7 +84,8 @@ def traffic_probe(case, args):
     ms = case.time_spmv(args.spmv_reps)
     copy_ms = fs.time_device_copy(COPY_BYTES, 3)
     print(json.dumps({"probe": "traffic", "cells": case.N, "W": case.W, "spmv_ms": ms, "spmv_reps": args.spmv_reps,
-                      "spmv_bytes": case.N * (8 + case.W * 12 + 24), "copy_bytes": COPY_BYTES,
+                      "spmv_bytes": 8 * case.N + 4 * case.N * case.W + 8 * 2 * case.Fi + 24 * case.N,
+                      "copy_bytes": COPY_BYTES,
                       "copy_ms": copy_ms, "copy_GBps": 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9}), flush=True)
 
 
@@ -80,7 +99,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--spmv-reps", type=int, default=20, help="ELL-product launches of --traffic-probe")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="set up the workload, run one iteration, then only the ELL product (--spmv-reps launches) "
                          "and a 1 GiB device copy; for hardware-counter runs")
@@ -110,22 +129,35 @@ def main():
         st = case.iterate(args.warmup)
         print(f"# warmup {args.warmup} its: {st['seconds']:.2f} s pc_its={st['pc_iterations']}",
               file=sys.stderr, flush=True)
+    # per-launch HIP events (recorded by the dispatches themselves on the
+    # case's stream) for the ELL product of the Krylov loop and the sweeps of
+    # the finest AMG level, over the timed region
+    case.kernel_clock_start(["spmv", "fine_sweep"], capacity=16384)
     st = case.iterate(args.steps)  # synchronises before and after
     el = st["seconds"]
     value = args.steps / el
     print(f"# timed {args.steps} its: {el:.3f} s, p' its {st['pc_iterations']}, p' s {st['pc_seconds']:.3f}, "
           f"mom s {st['momentum_seconds']:.3f}, mass {st['mass_l1']:.3e}", file=sys.stderr, flush=True)
+    clk = {k: case.kernel_clock_read(k) for k in ("spmv", "fine_sweep")}
+    case.kernel_clock_start("off")
 
-    # dominant kernel class: the ELL operator products of the p' Krylov loop
-    # and smoothers; algorithmic bytes per product = per cell 8 (ac) +
-    # W*(8 anb + 4 nbr) + 8 (x) + 8 (y) + 8 (dot operand)
-    W = case.W
-    ms = case.time_spmv(args.spmv_reps)
-    bytes_per = case.N * (8 + W * 12 + 8 + 8 + 8)
-    achieved = bytes_per / (ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
-            "traffic": None, "kernel": "k_spmv_dot (ELL y=A*x with (r0,y) partials)",
-            "bytes_per_launch": bytes_per, "avg_ms": ms}
+    def roof(c, kernel, per_cell):
+        if not c["launches"]:
+            return None
+        achieved = c["bytes"] / (c["avg_ms"] * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": kernel,
+                "bytes_per_launch": c["bytes"], "bytes_per_cell": per_cell, "avg_ms": c["avg_ms"],
+                "launches": c["launches"], "idle_launches": c["untimed"],
+                "share_of_step": c["launches"] * c["avg_ms"] * 1e-3 / el}
+
+    roof_spmv = roof(clk["spmv"], "k_spmv_dot (FP64 ELL y=A*x with (r0,y) partials, BiCGSTAB loop)",
+                     clk["spmv"]["bytes"] / case.N if clk["spmv"]["launches"] else None)
+    traffic = measured_traffic(case.N)
+    if roof_spmv is not None and traffic is not None:
+        roof_spmv["traffic"] = traffic
+    roof_sweep = roof(clk["fine_sweep"], "k_sweep<1, float, *> (FP32 damped-Jacobi sweeps of the finest AMG level)",
+                      clk["fine_sweep"]["bytes"] / case.N if clk["fine_sweep"]["launches"] else None)
 
     cpu = None if args.no_cpu else cpu_baseline(args)
     out = {
@@ -149,7 +181,8 @@ def main():
         "momentum_ms": 1e3 * st["momentum_seconds"] / args.steps,
         "pc_levels": st["pc_levels"],
         "mass_imbalance_l1": st["mass_l1"],
-        "roofline": roof,
+        "roofline": roof_spmv,
+        "roofline_fine_sweep": roof_sweep,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

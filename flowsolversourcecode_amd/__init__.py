@@ -25,7 +25,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfsamd.so")
 
-SOLVERS = {"sor": 0, "ilu": 1, "pbcg": 2, "direct": 3, "gmres": 4, "bicgs": 5}
+SOLVERS = {"sor": 0, "ilu": 1, "pbcg": 2, "direct": 3, "gmres": 4, "bicgs": 5, "cg": 6}
 MG_OFF, MG_SOLVER, MG_PRECOND = 0, 1, 2
 # GetNFfromVariable (AssignNFvaluesToVariables.for:20-108)
 NF_CODES = {"velx": 1, "vely": 2, "velz": 3, "pressc": 4, "press": 5, "temp": 6, "htotal": 7,
@@ -34,7 +34,8 @@ NF_CODES = {"velx": 1, "vely": 2, "velz": 3, "pressc": 4, "press": 5, "temp": 6,
 OPTIONS = {"mg_presweep": 1, "mg_postsweep": 2, "mg_maxcycles": 3, "mg_maxlevels": 4,
            "mg_minparents": 5, "mg_reagglomerate": 6, "mg_fp32": 7, "mg_omega": 8,
            "mg_smoother": 9, "gmres_restart": 10, "check_every": 11,
-           "mg_tail_cells": 12, "mg_coarse_sweeps": 13}
+           "mg_tail_cells": 12, "mg_coarse_sweeps": 13, "mg_alpha": 14, "mg_coarse_presweep": 15,
+           "mg_coarse_postsweep": 16}
 
 
 class FsError(RuntimeError):
@@ -74,12 +75,33 @@ def lib():
                                                C.c_void_p, _dp, C.POINTER(SolveStats)]
         L.fs_amg_export.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]
         L.fs_time_device_copy.argtypes = [C.c_longlong, C.c_int, _dp]
+        L.fs_kernel_clock_start.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.fs_kernel_clock_read.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.fs_kernel_clock_clear.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
 
 def last_error():
     return lib().fs_last_error().decode()
+
+
+CLOCK_SITES = {"off": 0, "spmv": 1, "fine_sweep": 2}
+
+
+def _clock_start(mesh_handle, sites, capacity):
+    mask = 0
+    for st in ([sites] if isinstance(sites, (str, int)) else sites):
+        mask |= 1 << (CLOCK_SITES[st] if isinstance(st, str) else int(st))
+    _check(lib().fs_kernel_clock_start(mesh_handle, mask & ~1, int(capacity)), "fs_kernel_clock_start")
+    _check(lib().fs_kernel_clock_clear(mesh_handle), "fs_kernel_clock_clear")
+
+
+def _clock_read(mesh_handle, site):
+    out = np.zeros(4)
+    st = CLOCK_SITES[site] if isinstance(site, str) else int(site)
+    _check(lib().fs_kernel_clock_read(mesh_handle, st, _dptr(out)), "fs_kernel_clock_read")
+    return {"launches": int(out[0]), "avg_ms": float(out[1]), "bytes": float(out[2]), "untimed": int(out[3])}
 
 
 def time_device_copy(nbytes, reps=3):
@@ -134,6 +156,14 @@ class Mesh:
     def set_option(self, key, value):
         k = OPTIONS[key] if isinstance(key, str) else int(key)
         _check(lib().fs_set_option(self.h, k, float(value)), "fs_set_option")
+
+    def kernel_clock_start(self, sites, capacity=4096):
+        """Arm per-launch HIP-event timing of solve-loop kernel sites ("spmv", "fine_sweep")."""
+        _clock_start(self.h, sites, capacity)
+
+    def kernel_clock_read(self, site):
+        """{launches, avg_ms, bytes, untimed} of one site since the clock was started."""
+        return _clock_read(self.h, site)
 
     def solve_equation(self, variable, fit, rrf, itmax, solver, lmultigrid, ac, anb, bc):
         """SolveEquation on host arrays.  Returns (rc, fit_new, dphi, res4, stats)."""
@@ -325,6 +355,14 @@ class Case:
     def set_option(self, key, value):
         k = OPTIONS[key] if isinstance(key, str) else int(key)
         _check(lib().fs_set_option(lib().fs_case_mesh(self.h), k, float(value)), "fs_set_option")
+
+    def kernel_clock_start(self, sites, capacity=4096):
+        """Arm per-launch HIP-event timing of solve-loop kernel sites ("spmv", "fine_sweep")."""
+        _clock_start(lib().fs_case_mesh(self.h), sites, capacity)
+
+    def kernel_clock_read(self, site):
+        """{launches, avg_ms, bytes, untimed} of one site since the clock was started."""
+        return _clock_read(lib().fs_case_mesh(self.h), site)
 
     def iterate(self, n):
         st = SimpleStats()

@@ -16,12 +16,14 @@
 // Cycle (device): the additive-correction V-cycle of MultiGridCycle.for:32-47
 // with restriction = sum of member residuals in fine-cell order
 // (AssembleNodeAndElementBasedAgglomoratedRHS.for:25-33) and prolongation =
-// injection (CorrectSolutionOnFinerLevel.for:17-24).  The reference's
-// sequential SOR smoother becomes multicolour symmetric Gauss-Seidel (or
-// damped Jacobi), so cycle iterates differ while the converged solution does
-// not.  Each visit of a coarse level starts from a zero correction, the
-// standard V-cycle.  The hierarchy can be held in FP32 (FS_OPT_MG_FP32):
-// Galerkin sums stay FP64 and are rounded once to FP32.
+// injection (CorrectSolutionOnFinerLevel.for:17-24), optionally scaled by an
+// over-correction factor alpha (FS_OPT_MG_ALPHA; 1 = the reference).  The
+// reference's sequential SOR smoother becomes damped Jacobi (or multicolour
+// symmetric Gauss-Seidel), so cycle iterates differ while the converged
+// solution does not.  Each visit of a coarse level starts from a zero
+// correction, the standard V-cycle.  The coarsest level (at most 64 cells) is
+// solved directly in FP64.  The hierarchy can be held in FP32
+// (FS_OPT_MG_FP32): Galerkin sums stay FP64 and are rounded once to FP32.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -249,49 +251,125 @@ __global__ void k_to(int n, const double *__restrict__ a, T *__restrict__ b) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) b[i] = (T)a[i];
 }
+
+// ------------------------------------------------------------ cycle kernels
+// Grid-wide kernels for the levels above the tail.  Every kernel returns at
+// once when the Krylov loop has set its done flag, so the batches launched
+// after convergence cost only their launch.
+
+// sum_j anb_ij x'_j over the slots of row i, spread over LPR consecutive
+// lanes and combined with shuffles; x' = x + alpha*xc[parent] when PRO
+template <int LPR, class T, bool PRO>
+__device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, const int *__restrict__ nbr,
+                                         const T *__restrict__ anb, const T *x, const int *__restrict__ parent,
+                                         const T *__restrict__ xc, T alpha) {
+  T s = T(0);
+  for (int j = lane; j < w; j += LPR) {
+    const int k = nbr[(size_t)j * n + i];
+    if (k >= 0) {
+      T xk = x[k];
+      if (PRO) xk = xk + alpha * xc[parent[k]];
+      s += anb[(size_t)j * n + i] * xk;
+    }
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+  return s;
+}
+
+// One damped-Jacobi sweep xn = x' + omega*(b - A x')/ac, x' = x (PRO false)
+// or x + alpha*xc[parent] (PRO true: the prolongation of the coarse
+// correction fused into the first post-smoothing sweep).  TO = double writes
+// the FP64 result of the cycle directly.
+template <int LPR, class T, class TO, bool PRO>
+__global__ void __launch_bounds__(kBlock)
+k_sweep(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac, const T *__restrict__ anb,
+        const T *__restrict__ b, const T *x, const int *__restrict__ parent, const T *__restrict__ xc,
+        T alpha, T omega, TO *xn, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int t = xcd_block() * kBlock + threadIdx.x;
+  const int i = t / LPR, lane = t % LPR;
+  const bool ok = i < n;
+  const T s = row_offdiag<LPR, T, PRO>(ok ? i : 0, lane, ok ? n : 1, ok ? w : 0, nbr, anb, x, parent, xc, alpha);
+  if (ok && lane == 0) {
+    T xi = x[i];
+    if (PRO) xi = xi + alpha * xc[parent[i]];
+    xn[i] = (TO)(xi + omega * (b[i] - ac[i] * xi - s) / ac[i]);
+  }
+}
+
+// r = b - A x
+template <int LPR, class T>
+__global__ void __launch_bounds__(kBlock)
+k_resid(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac, const T *__restrict__ anb,
+        const T *__restrict__ b, const T *__restrict__ x, T *__restrict__ r, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int t = xcd_block() * kBlock + threadIdx.x;
+  const int i = t / LPR, lane = t % LPR;
+  const bool ok = i < n;
+  const T s = row_offdiag<LPR, T, false>(ok ? i : 0, lane, ok ? n : 1, ok ? w : 0, nbr, anb, x, nullptr,
+                                         nullptr, T(0));
+  if (ok && lane == 0) r[i] = b[i] - ac[i] * x[i] - s;
+}
+
+// Entry of the cycle on the finest level: b = (T)rhs (when the hierarchy is
+// FP32; b may be null in FP64) and the first sweep from a zero start,
+// x = omega*b/ac, or x = 0 when no pre-smoothing sweep is asked for.
 template <class T>
-__global__ void k_from(int n, const T *__restrict__ a, double *__restrict__ b,
-                       const int *__restrict__ flags) {
+__global__ void k_enter(int n, const T *__restrict__ ac, const double *__restrict__ rhs, T *__restrict__ b,
+                        T *__restrict__ x, T omega, int sweep, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T bi = (T)rhs[i];
+  if (b) b[i] = bi;
+  x[i] = sweep ? omega * bi / ac[i] : T(0);
+}
+
+// Restriction b_c[p] = sum of the member residuals in increasing fine-cell
+// order; mode 1 also takes the coarse level's first sweep from a zero start
+// (x_c = omega*b_c/ac_c), mode 2 sets x_c = 0, mode 0 leaves x_c alone.
+template <class T>
+__global__ void k_restrict(int nc, const int *__restrict__ mptr, const int *__restrict__ mem,
+                           const T *__restrict__ rf, T *__restrict__ bc, const T *__restrict__ acc,
+                           T *__restrict__ xc, T omega, int mode, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nc) return;
+  T s = T(0);
+  for (int t = mptr[p]; t < mptr[p + 1]; ++t) s = s + rf[mem[t]];
+  bc[p] = s;
+  if (mode == 1) xc[p] = omega * s / acc[p];
+  else if (mode == 2) xc[p] = T(0);
+}
+
+// out = x + alpha*xc[parent] (out may be x)
+template <class T, class TO>
+__global__ void k_prolong(int nf, const int *__restrict__ parent, const T *__restrict__ xc, const T *x,
+                          T alpha, TO *out, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i < nf) out[i] = (TO)(x[i] + alpha * xc[parent[i]]);
+}
+
+template <class T, class TO>
+__global__ void k_convert(int n, const T *__restrict__ x, TO *__restrict__ out, const int *__restrict__ flags) {
   if (flags && flags[0]) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) b[i] = (double)a[i];
+  if (i < n) out[i] = (TO)x[i];
 }
 
 template <class T>
-__device__ __forceinline__ T lres(int i, int n, int w, const int *__restrict__ nbr,
-                                  const T *__restrict__ ac, const T *__restrict__ anb,
-                                  const T *__restrict__ b, const T *__restrict__ x) {
-  T r = b[i] - ac[i] * x[i];
-  for (int j = 0; j < w; ++j) {
-    const int k = nbr[(size_t)j * n + i];
-    if (k >= 0) r = r - anb[(size_t)j * n + i] * x[k];
-  }
-  return r;
-}
-
-// first damped-Jacobi sweep from a zero start: x = omega*b/ac
-template <class T>
-__global__ void k_jac0(int n, const T *__restrict__ ac, const T *__restrict__ b, T *__restrict__ x,
-                       T omega, const int *__restrict__ flags) {
+__global__ void k_zero(int n, T *__restrict__ x, const int *__restrict__ flags) {
   if (flags && flags[0]) return;
-  const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i < n) x[i] = omega * b[i] / ac[i];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = T(0);
 }
 
-// x_new = x + omega*(b - A x)/ac
+// one colour of a multicolour Gauss-Seidel sweep, in place
 template <class T>
-__global__ void k_jac(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac,
-                      const T *__restrict__ anb, const T *__restrict__ b, const T *__restrict__ x,
-                      T *__restrict__ xn, T omega, const int *__restrict__ flags) {
-  if (flags && flags[0]) return;
-  const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i < n) xn[i] = x[i] + omega * lres(i, n, w, nbr, ac, anb, b, x) / ac[i];
-}
-
-template <class T>
-__global__ void k_gsc(int cnt, const int *__restrict__ cells, int n, int w,
-                      const int *__restrict__ nbr, const T *__restrict__ ac,
-                      const T *__restrict__ anb, const T *__restrict__ b, T *__restrict__ x,
+__global__ void k_gsc(int cnt, const int *__restrict__ cells, int n, int w, const int *__restrict__ nbr,
+                      const T *__restrict__ ac, const T *__restrict__ anb, const T *__restrict__ b, T *x,
                       const int *__restrict__ flags) {
   if (flags && flags[0]) return;
   const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -305,48 +383,10 @@ __global__ void k_gsc(int cnt, const int *__restrict__ cells, int n, int w,
   x[i] = s / ac[i];
 }
 
-template <class T>
-__global__ void k_zero(int n, T *__restrict__ x, const int *__restrict__ flags) {
-  if (flags && flags[0]) return;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) x[i] = T(0);
-}
-
-template <class T>
-__global__ void k_lresid(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac,
-                         const T *__restrict__ anb, const T *__restrict__ b,
-                         const T *__restrict__ x, T *__restrict__ r, const int *__restrict__ flags) {
-  if (flags && flags[0]) return;
-  const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i < n) r[i] = lres(i, n, w, nbr, ac, anb, b, x);
-}
-
-// b_c[p] = sum of member residuals in increasing fine-cell order
-template <class T>
-__global__ void k_restrict(int nc, const int *__restrict__ mptr, const int *__restrict__ mem,
-                           const T *__restrict__ rf, T *__restrict__ bc,
-                           const int *__restrict__ flags) {
-  if (flags && flags[0]) return;
-  const int p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= nc) return;
-  T s = T(0);
-  for (int t = mptr[p]; t < mptr[p + 1]; ++t) s = s + rf[mem[t]];
-  bc[p] = s;
-}
-
-template <class T>
-__global__ void k_prolong(int nf, const int *__restrict__ parent, const T *__restrict__ xc,
-                          T *__restrict__ xf, const int *__restrict__ flags) {
-  if (flags && flags[0]) return;
-  const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i < nf) xf[i] = xf[i] + xc[parent[i]];
-}
-
 // AMG-as-solver residual: r = b - A x and partial L1(r)
-__global__ void k_amg_res(int n, int w, const int *__restrict__ nbr,
-                          const double *__restrict__ ac, const double *__restrict__ anb,
-                          const double *__restrict__ b, const double *__restrict__ x,
-                          double *__restrict__ r, double *__restrict__ part,
+__global__ void k_amg_res(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+                          const double *__restrict__ anb, const double *__restrict__ b,
+                          const double *__restrict__ x, double *__restrict__ r, double *__restrict__ part,
                           const int *__restrict__ flags) {
   if (flags[0]) return;
   double s = 0.0;
@@ -370,133 +410,210 @@ __global__ void k_add2(int n, double *__restrict__ x, const double *__restrict__
   if (i < n) x[i] = x[i] + c[i];
 }
 
-
-
-// ---- lanes-per-row forms for the wide coarse levels: a row's slots are
-// spread over LPR consecutive lanes and combined with shuffles, so a row
-// costs one dependent-load chain instead of w of them.
-template <int LPR, class T>
-__device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, const int *__restrict__ nbr,
-                                         const T *__restrict__ anb, const T *__restrict__ x) {
-  T s = T(0);
-  for (int j = lane; j < w; j += LPR) {
-    const int k = nbr[(size_t)j * n + i];
-    if (k >= 0) s += anb[(size_t)j * n + i] * x[k];
-  }
-#pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
-  return s;
-}
-
-template <int LPR, class T>
-__global__ void k_lresid_lpr(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac,
-                             const T *__restrict__ anb, const T *__restrict__ b,
-                             const T *__restrict__ x, T *__restrict__ r, const int *__restrict__ flags) {
-  if (flags && flags[0]) return;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  const int i = t / LPR, lane = t % LPR;
-  const bool ok = i < n;
-  const T s = row_offdiag<LPR, T>(ok ? i : 0, lane, ok ? n : 1, ok ? w : 0, nbr, anb, x);
-  if (ok && lane == 0) r[i] = b[i] - ac[i] * x[i] - s;
-}
-
-template <int LPR, class T>
-__global__ void k_jac_lpr(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac,
-                          const T *__restrict__ anb, const T *__restrict__ b, const T *__restrict__ x,
-                          T *__restrict__ xn, T omega, const int *__restrict__ flags) {
-  if (flags && flags[0]) return;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  const int i = t / LPR, lane = t % LPR;
-  const bool ok = i < n;
-  const T s = row_offdiag<LPR, T>(ok ? i : 0, lane, ok ? n : 1, ok ? w : 0, nbr, anb, x);
-  if (ok && lane == 0) xn[i] = x[i] + omega * (b[i] - ac[i] * x[i] - s) / ac[i];
-}
-
-
 // ------------------------------------------------------------ small levels
-// Every level below the tail threshold is processed by ONE workgroup in one
-// launch: restriction, smoothing, the coarsest level and the prolongations of
-// those levels, separated by workgroup barriers.  The small levels are
-// latency-bound (a few thousand cells, tens of slots), so one resident
-// workgroup with its data in L2 replaces dozens of tiny launches.
-struct TailLevel {
-  int n, w, ncolors;
-  const int *nbr;
-  const void *ac, *anb;
-  void *x, *b, *r, *t;
-  const int *parent;              // to the next coarser level, or null
-  const int *mptr, *mem;          // members of this level's aggregates in the finer level
-  const int *color_ptr, *color_cells;
+// Every level from the first one below the tail threshold down to the
+// coarsest runs in ONE workgroup in one launch, with the levels' operators
+// held in LDS.  A byte image of the tail is kept in device memory: the static
+// part (CSR columns, row pointers, parents, aggregate members as uint16) is
+// written at setup, the coefficients (ac and the CSR values in the
+// hierarchy's precision) are packed after each Galerkin update, and the
+// coarsest level's FP64 inverse is written there too.  The kernel copies the
+// image into LDS with 16-byte loads, runs restriction, smoothing, the
+// coarsest solve and the prolongations between workgroup barriers, and
+// writes the top tail level's correction back to device memory.
+constexpr int kTailThreads = 1024;
+constexpr int kTailMaxLev = 8;
+constexpr int kTailDirectMax = 64; // coarsest levels up to this size are solved directly
+
+struct TailLvl {
+  int n, nnz, lpr;
+  int o_ac, o_val, o_col, o_rp, o_par, o_mp, o_mem; // byte offsets of the image sections
+  int o_x, o_t, o_r, o_b;                           // byte offsets of the LDS-only vectors
+};
+struct TailDesc {
+  int nlev, img_bytes, lds_bytes, direct, o_g;
+  TailLvl L[kTailMaxLev];
 };
 
-constexpr int kTailThreads = 1024;
-
-constexpr int kTailLPR = 8;
-
-// y_i = b_i - ac_i x_i - sum_j anb_ij x_j for the rows of one level, LPR lanes
-// per row (all 1024 threads of the workgroup take part)
+// one pass over the rows of a tail level: out = b - A x (jac false) or
+// x + omega*(b - A x)/ac (jac true); lpr lanes per row, all threads take part
 template <class T>
-__device__ void tail_rows(const TailLevel &L, const T *x, T omega, T *out, int mode) {
-  const T *ac = (const T *)L.ac, *anb = (const T *)L.anb, *b = (const T *)L.b;
-  const int lane = threadIdx.x % kTailLPR;
-  const int rows_per_pass = blockDim.x / kTailLPR;
-  for (int base = 0; base < L.n; base += rows_per_pass) {
-    const int i = base + threadIdx.x / kTailLPR;
-    const bool ok = i < L.n;
-    const T s = row_offdiag<kTailLPR, T>(ok ? i : 0, lane, ok ? L.n : 1, ok ? L.w : 0, L.nbr, anb, x);
-    if (ok && lane == 0) {
+__device__ __forceinline__ void tail_pass(unsigned char *lds, const TailLvl &L, const T *x, T *out, T omega, bool jac) {
+  const T *ac = (const T *)(lds + L.o_ac), *val = (const T *)(lds + L.o_val), *b = (const T *)(lds + L.o_b);
+  const unsigned short *col = (const unsigned short *)(lds + L.o_col);
+  const unsigned short *rp = (const unsigned short *)(lds + L.o_rp);
+  const int lpr = L.lpr, lane = threadIdx.x % lpr, rows = blockDim.x / lpr;
+  for (int base = 0; base < L.n; base += rows) {
+    const int i = base + threadIdx.x / lpr;
+    T s = T(0);
+    if (i < L.n)
+      for (int q = rp[i] + lane; q < rp[i + 1]; q += lpr) s += val[q] * x[col[q]];
+    for (int off = lpr / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, lpr);
+    if (i < L.n && lane == 0) {
       const T r = b[i] - ac[i] * x[i] - s;
-      out[i] = mode == 0 ? r : x[i] + omega * r / ac[i];
+      out[i] = jac ? x[i] + omega * r / ac[i] : r;
     }
   }
 }
 
-// damped Jacobi sweeps on one level (x and t alternate; the result ends in x)
-template <class T> __device__ void tail_smooth(const TailLevel &L, int sweeps, bool zero, T omega) {
-  const T *ac = (const T *)L.ac, *b = (const T *)L.b;
-  T *x = (T *)L.x, *t = (T *)L.t;
+// damped-Jacobi sweeps on a tail level; the result ends in x
+template <class T> __device__ __forceinline__ void tail_smooth(unsigned char *lds, const TailLvl &L, int sweeps, bool zero, T omega) {
+  T *x = (T *)(lds + L.o_x), *t = (T *)(lds + L.o_t);
+  const T *ac = (const T *)(lds + L.o_ac), *b = (const T *)(lds + L.o_b);
   int sw = 0;
   if (zero) {
     for (int i = threadIdx.x; i < L.n; i += blockDim.x) x[i] = sweeps > 0 ? omega * b[i] / ac[i] : T(0);
     __syncthreads();
     sw = 1;
   }
+  T *src = x, *dst = t;
   for (; sw < sweeps; ++sw) {
-    tail_rows<T>(L, x, omega, t, 1);
+    tail_pass<T>(lds, L, src, dst, omega, true);
     __syncthreads();
-    for (int i = threadIdx.x; i < L.n; i += blockDim.x) x[i] = t[i];
+    T *tmp = src;
+    src = dst;
+    dst = tmp;
+  }
+  if (src != x) {
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) x[i] = src[i];
     __syncthreads();
   }
 }
 
 template <class T>
 __global__ void __launch_bounds__(kTailThreads)
-k_tail(const TailLevel *__restrict__ lv, int nlev, int pre, int post, int coarse, T omega,
-       const int *__restrict__ flags) {
+k_tail(const TailDesc *__restrict__ d, const unsigned char *__restrict__ img, const T *__restrict__ b_top,
+       T *__restrict__ x_top, int pre, int post, int coarse, T omega, T alpha, const int *__restrict__ flags) {
   if (flags && flags[0]) return;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  const int nlev = d->nlev;
+  {
+    const uint4 *src = (const uint4 *)img;
+    uint4 *dst = (uint4 *)lds;
+    const int nv = d->img_bytes / 16;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) dst[i] = src[i];
+    const TailLvl &L = d->L[0];
+    T *b = (T *)(lds + L.o_b);
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) b[i] = b_top[i];
+  }
+  __syncthreads();
   for (int l = 0; l + 1 < nlev; ++l) {
-    const TailLevel &L = lv[l];
-    tail_smooth<T>(L, pre, true, omega);
-    tail_rows<T>(L, (const T *)L.x, omega, (T *)L.r, 0);
+    const TailLvl &L = d->L[l];
+    tail_smooth<T>(lds, L, pre, true, omega);
+    tail_pass<T>(lds, L, (const T *)(lds + L.o_x), (T *)(lds + L.o_r), omega, false);
     __syncthreads();
-    const TailLevel &C = lv[l + 1];
-    const T *r = (const T *)L.r;
-    T *bc = (T *)C.b;
+    const TailLvl &C = d->L[l + 1];
+    const T *r = (const T *)(lds + L.o_r);
+    T *bc = (T *)(lds + C.o_b);
+    const unsigned short *mp = (const unsigned short *)(lds + C.o_mp);
+    const unsigned short *mem = (const unsigned short *)(lds + C.o_mem);
     for (int p = threadIdx.x; p < C.n; p += blockDim.x) {
       T s = T(0);
-      for (int q = C.mptr[p]; q < C.mptr[p + 1]; ++q) s = s + r[C.mem[q]];
+      for (int q = mp[p]; q < mp[p + 1]; ++q) s = s + r[mem[q]];
       bc[p] = s;
     }
     __syncthreads();
   }
-  tail_smooth<T>(lv[nlev - 1], coarse, true, omega);
-  for (int l = nlev - 2; l >= 0; --l) {
-    const TailLevel &L = lv[l];
-    T *x = (T *)L.x;
-    const T *xc = (const T *)lv[l + 1].x;
-    for (int i = threadIdx.x; i < L.n; i += blockDim.x) x[i] = x[i] + xc[L.parent[i]];
+  const TailLvl &Lc = d->L[nlev - 1];
+  if (d->direct) {
+    // coarsest level: x = G b with the FP64 (pseudo-)inverse G
+    const double *G = (const double *)(lds + d->o_g);
+    const T *b = (const T *)(lds + Lc.o_b);
+    T *x = (T *)(lds + Lc.o_x);
+    for (int i = threadIdx.x; i < Lc.n; i += blockDim.x) {
+      double s = 0.0;
+      for (int j = 0; j < Lc.n; ++j) s += G[(size_t)i * Lc.n + j] * (double)b[j];
+      x[i] = (T)s;
+    }
     __syncthreads();
-    tail_smooth<T>(L, post, false, omega);
+  } else {
+    tail_smooth<T>(lds, Lc, coarse, true, omega);
+  }
+  for (int l = nlev - 2; l >= 0; --l) {
+    const TailLvl &L = d->L[l];
+    T *x = (T *)(lds + L.o_x);
+    const T *xc = (const T *)(lds + d->L[l + 1].o_x);
+    const unsigned short *par = (const unsigned short *)(lds + L.o_par);
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) x[i] = x[i] + alpha * xc[par[i]];
+    __syncthreads();
+    tail_smooth<T>(lds, L, post, false, omega);
+  }
+  const TailLvl &L0 = d->L[0];
+  const T *x0 = (const T *)(lds + L0.o_x);
+  for (int i = threadIdx.x; i < L0.n; i += blockDim.x) x_top[i] = x0[i];
+}
+
+// coefficients of one tail level into its image sections (after Galerkin)
+template <class T>
+__global__ void k_tail_pack(int n, int nnz, const double *__restrict__ ac, const double *__restrict__ anb,
+                            const int *__restrict__ lin, T *__restrict__ img_ac, T *__restrict__ img_val) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t < n) img_ac[t] = (T)ac[t];
+  if (t < nnz) img_val[t] = (T)anb[lin[t]];
+}
+
+// FP64 inverse of the coarsest operator by Gauss-Jordan elimination without
+// pivoting (the coarse operators are diagonally dominant), one workgroup,
+// n <= kTailDirectMax.  A singular operator with zero row sums (the pure
+// Neumann p' problem, null space = constants) is solved with its last
+// unknown pinned to zero: G is the inverse of the leading (n-1)x(n-1) block,
+// bordered by zeros, a symmetric generalised inverse.  A zero or non-finite
+// pivot leaves G = 0 (no coarse correction) rather than garbage.
+constexpr int kInvThreads = 256;
+__global__ void __launch_bounds__(kInvThreads)
+k_coarse_inverse(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+                 const double *__restrict__ anb, double *__restrict__ G) {
+  __shared__ double M[kTailDirectMax * kTailDirectMax];
+  __shared__ double rowk[kTailDirectMax], colk[kTailDirectMax];
+  __shared__ int s_m, s_fail;
+  const int nn = n * n;
+  for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+    const int i = e / n, k = e % n;
+    M[e] = (i == k) ? ac[i] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // off-diagonals in slot order; singularity test on the row sums
+    bool zero_rows = true;
+    for (int i = 0; i < n; ++i) {
+      double rs = ac[i];
+      for (int j = 0; j < w; ++j) {
+        const int k = nbr[(size_t)j * n + i];
+        if (k >= 0) {
+          M[i * n + k] += anb[(size_t)j * n + i];
+          rs += anb[(size_t)j * n + i];
+        }
+      }
+      if (!(fabs(rs) <= 1.0e-9 * fabs(ac[i]))) zero_rows = false;
+    }
+    s_m = zero_rows ? n - 1 : n;
+    s_fail = 0;
+  }
+  __syncthreads();
+  const int m = s_m;
+  for (int k = 0; k < m; ++k) {
+    const double p = M[k * n + k];
+    if (!(fabs(p) > 0.0) || !isfinite(p)) {
+      if (threadIdx.x == 0) s_fail = 1;
+      break; // p is read from LDS by every thread: the branch is uniform
+    }
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+      rowk[j] = (j == k) ? 1.0 / p : M[k * n + j] / p;
+      colk[j] = M[j * n + k];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+      const int i = e / m, j = e % m;
+      if (i == k) M[k * n + j] = rowk[j];
+      else M[i * n + j] = ((j == k) ? 0.0 : M[i * n + j]) - colk[i] * rowk[j];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  const bool fail = s_fail != 0;
+  for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+    const int i = e / n, j = e % n;
+    G[e] = (!fail && i < m && j < m) ? M[i * n + j] : 0.0;
   }
 }
 
@@ -513,12 +630,23 @@ struct DLevel {
   DevBuf<double> x64, b64, r64, t64;
   DevBuf<float> x32, b32, r32, t32;
   std::vector<int> color_ptr;
-  DevBuf<int> color_cells, color_ptr_dev;
+  DevBuf<int> color_cells;
   DevBuf<int> parent;            // 0-based parent in the next level
   DevBuf<int> mptr, mem;         // members of this level's aggregates (coarse)
+  std::vector<int> h_mptr, h_mem;
   DevBuf<int> gptr;
   DevBuf<long long> gsrc;
   int gne = 0;
+};
+
+// The tail of one precision: its first level, descriptor and byte image.
+struct TailImage {
+  int ts = -1;                   // first tail level; == levels when there is no tail
+  TailDesc desc{};
+  DevBuf<unsigned char> img;
+  DevBuf<TailDesc> ddesc;
+  DevBuf<int> lin;               // CSR entry -> ELL linear index, all tail levels
+  std::vector<int> lin_off;      // per tail level
 };
 
 struct Hierarchy {
@@ -526,9 +654,150 @@ struct Hierarchy {
   std::vector<DLevel> lev;
   int calls = 0;
   bool fp32 = false;
-  int tail = -1;                 // first level run by the one-workgroup tail
-  DevBuf<TailLevel> tail32, tail64;
+  TailImage tail32, tail64;
 };
+
+static size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+static int lds_budget() {
+  static int budget = -1;
+  if (budget < 0) {
+    int dev = 0, v = 0;
+    budget = 64 * 1024;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      budget = v;
+    budget = std::min(budget, 160 * 1024);
+    // the tail kernels ask for more than the default dynamic LDS window
+    if (hipFuncSetAttribute((const void *)k_tail<float>, hipFuncAttributeMaxDynamicSharedMemorySize, budget) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void *)k_tail<double>, hipFuncAttributeMaxDynamicSharedMemorySize, budget) !=
+            hipSuccess) {
+      (void)hipGetLastError();
+      budget = 64 * 1024;
+    }
+  }
+  return budget;
+}
+
+// lanes per row for a tail level: fewest dependent steps per sweep
+static int tail_lpr(int n, int nnz) {
+  int best = 1;
+  double best_cost = 1e30;
+  const double avg = n > 0 ? (double)nnz / n : 0.0;
+  for (int lpr = 1; lpr <= 16; lpr *= 2) {
+    const int rows = kTailThreads / lpr;
+    const double passes = std::ceil((double)n / rows);
+    int lg = 0;
+    while ((1 << lg) < lpr) ++lg;
+    const double cost = passes * (std::ceil(avg / lpr) + lg);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = lpr;
+    }
+  }
+  return best;
+}
+
+// Lays out the tail of precision tsz (4 or 8 bytes) starting at the first
+// level at or below mg_tail_cells; moves the start down while the layout
+// does not fit the LDS budget, the level count or the uint16 index range.
+static void build_tail(fs_mesh *m, Hierarchy &H, TailImage &ti, size_t tsz) {
+  hipStream_t s = m->stream;
+  const int nl = (int)H.lev.size();
+  ti.ts = nl;
+  if (nl < 2) return;
+  int ts = nl;
+  for (int l = 1; l < nl; ++l)
+    if (H.lev[l].n <= m->mg_tail_cells) {
+      ts = l;
+      break;
+    }
+  const size_t budget = (size_t)lds_budget();
+  TailDesc d{};
+  for (; ts < nl; ++ts) {
+    const int nt = nl - ts;
+    if (nt > kTailMaxLev) continue;
+    d = TailDesc{};
+    d.nlev = nt;
+    size_t off = 0;
+    bool ok = true;
+    for (int q = 0; q < nt; ++q) {
+      const HostLevel &h = H.host[ts + q];
+      int nnz = 0;
+      for (int i = 0; i < h.n; ++i)
+        for (int j = 0; j < h.nnbr[i]; ++j)
+          if (h.nbr[(size_t)j * h.n + i] > 0) ++nnz;
+      if (h.n >= 65535 || nnz >= 65535 || (q > 0 && H.host[ts + q - 1].n >= 65535)) ok = false;
+      TailLvl &L = d.L[q];
+      L.n = h.n;
+      L.nnz = nnz;
+      L.lpr = tail_lpr(h.n, nnz);
+      L.o_ac = (int)off; off += align16(tsz * h.n);
+      L.o_val = (int)off; off += align16(tsz * std::max(nnz, 1));
+      L.o_col = (int)off; off += align16(2 * std::max(nnz, 1));
+      L.o_rp = (int)off; off += align16(2 * (h.n + 1));
+      L.o_par = (int)off; if (q + 1 < nt) off += align16(2 * h.n);
+      L.o_mp = (int)off; if (q > 0) off += align16(2 * (h.n + 1));
+      L.o_mem = (int)off; if (q > 0) off += align16(2 * H.host[ts + q - 1].n);
+    }
+    const int nc = d.L[nt - 1].n;
+    d.direct = nc <= kTailDirectMax ? 1 : 0;
+    d.o_g = (int)off;
+    if (d.direct) off += align16(sizeof(double) * nc * nc);
+    d.img_bytes = (int)off;
+    for (int q = 0; q < nt; ++q) {
+      TailLvl &L = d.L[q];
+      L.o_x = (int)off; off += align16(tsz * L.n);
+      L.o_t = (int)off; off += align16(tsz * L.n);
+      L.o_r = (int)off; off += align16(tsz * L.n);
+      L.o_b = (int)off; off += align16(tsz * L.n);
+    }
+    d.lds_bytes = (int)off;
+    if (ok && off <= budget) break;
+  }
+  ti.ts = ts;
+  if (ts >= nl) return;
+  // static sections of the image
+  std::vector<unsigned char> img(d.img_bytes, 0);
+  std::vector<int> lin;
+  ti.lin_off.assign(d.nlev, 0);
+  for (int q = 0; q < d.nlev; ++q) {
+    const HostLevel &h = H.host[ts + q];
+    const TailLvl &L = d.L[q];
+    unsigned short *col = (unsigned short *)(img.data() + L.o_col);
+    unsigned short *rp = (unsigned short *)(img.data() + L.o_rp);
+    ti.lin_off[q] = (int)lin.size();
+    int e = 0;
+    for (int i = 0; i < h.n; ++i) {
+      rp[i] = (unsigned short)e;
+      for (int j = 0; j < h.nnbr[i]; ++j) {
+        const int k = h.nbr[(size_t)j * h.n + i];
+        if (k > 0) {
+          col[e++] = (unsigned short)(k - 1);
+          lin.push_back(j * h.n + i);
+        }
+      }
+    }
+    rp[h.n] = (unsigned short)e;
+    if (q + 1 < d.nlev) {
+      unsigned short *par = (unsigned short *)(img.data() + L.o_par);
+      for (int i = 0; i < h.n; ++i) par[i] = (unsigned short)(h.parent[i] - 1);
+    }
+    if (q > 0) {
+      const DLevel &D = H.lev[ts + q];
+      unsigned short *mp = (unsigned short *)(img.data() + L.o_mp);
+      unsigned short *mem = (unsigned short *)(img.data() + L.o_mem);
+      for (int p = 0; p <= h.n; ++p) mp[p] = (unsigned short)D.h_mptr[p];
+      for (size_t t = 0; t < D.h_mem.size(); ++t) mem[t] = (unsigned short)D.h_mem[t];
+    }
+  }
+  if (lin.empty()) lin.push_back(0);
+  ti.desc = d;
+  ti.img.upload(img, s);
+  ti.ddesc.upload(&ti.desc, 1, s);
+  ti.lin.upload(lin, s);
+}
 
 static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
   auto H = std::make_shared<Hierarchy>();
@@ -585,13 +854,14 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
       D.gne = (int)gptrs[l].size() - 1;
       // members in increasing fine order
       const HostLevel &f = H->host[l - 1];
-      std::vector<int> cnt(h.n + 1, 0), mem(f.n);
-      for (int i = 0; i < f.n; ++i) cnt[f.parent[i]]++;
-      for (int p = 0; p < h.n; ++p) cnt[p + 1] += cnt[p];
-      std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-      for (int i = 0; i < f.n; ++i) mem[fill[f.parent[i] - 1]++] = i;
-      D.mptr.upload(cnt, s);
-      D.mem.upload(mem, s);
+      D.h_mptr.assign(h.n + 1, 0);
+      D.h_mem.assign(f.n, 0);
+      for (int i = 0; i < f.n; ++i) D.h_mptr[f.parent[i]]++;
+      for (int p = 0; p < h.n; ++p) D.h_mptr[p + 1] += D.h_mptr[p];
+      std::vector<int> fill(D.h_mptr.begin(), D.h_mptr.end() - 1);
+      for (int i = 0; i < f.n; ++i) D.h_mem[fill[f.parent[i] - 1]++] = i;
+      D.mptr.upload(D.h_mptr, s);
+      D.mem.upload(D.h_mem, s);
     }
     if (l + 1 < nl) {
       std::vector<int> par0(h.n);
@@ -601,7 +871,6 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
     std::vector<int> cells;
     greedy_colors(h.n, h.w, nbr0.data(), D.color_ptr, cells);
     D.color_cells.upload(cells, s);
-    D.color_ptr_dev.upload(D.color_ptr, s);
     D.x64.alloc(h.n);
     D.b64.alloc(h.n);
     D.r64.alloc(h.n);
@@ -613,41 +882,8 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
     D.ac32.alloc(h.n);
     D.anb32.alloc((size_t)h.n * h.w);
   }
-  // levels from the first one at or below the threshold run in the tail
-  H->tail = nl; // level 0 always runs grid-wide (it is the caller's operator)
-  for (int l = 1; l < nl; ++l)
-    if (H->lev[l].n <= m->mg_tail_cells) {
-      H->tail = l;
-      break;
-    }
-  if (H->tail < nl) {
-    const int nt = nl - H->tail;
-    std::vector<TailLevel> d32(nt), d64(nt);
-    for (int q = 0; q < nt; ++q) {
-      DLevel &D = H->lev[H->tail + q];
-      for (int prec = 0; prec < 2; ++prec) {
-        TailLevel t{};
-        t.n = D.n;
-        t.w = D.w;
-        t.ncolors = (int)D.color_ptr.size() - 1;
-        t.nbr = D.nbr;
-        t.ac = prec ? (const void *)D.ac64_own.p : (const void *)D.ac32.p;
-        t.anb = prec ? (const void *)D.anb64_own.p : (const void *)D.anb32.p;
-        t.x = prec ? (void *)D.x64.p : (void *)D.x32.p;
-        t.b = prec ? (void *)D.b64.p : (void *)D.b32.p;
-        t.r = prec ? (void *)D.r64.p : (void *)D.r32.p;
-        t.t = prec ? (void *)D.t64.p : (void *)D.t32.p;
-        t.parent = (H->tail + q + 1 < nl) ? D.parent.p : nullptr;
-        t.mptr = q > 0 ? D.mptr.p : nullptr;
-        t.mem = q > 0 ? D.mem.p : nullptr;
-        t.color_ptr = D.color_ptr_dev.p;
-        t.color_cells = D.color_cells.p;
-        (prec ? d64 : d32)[q] = t;
-      }
-    }
-    H->tail32.upload(d32, s);
-    H->tail64.upload(d64, s);
-  }
+  build_tail(m, *H, H->tail32, sizeof(float));
+  build_tail(m, *H, H->tail64, sizeof(double));
   FS_HIP_THROW(hipStreamSynchronize(s));
   m->amg = H;
 }
@@ -674,6 +910,30 @@ void amg_prepare(fs_mesh *m, const double *ac, const double *anb) {
                        C.gsrc.p, F.n, F.ac64, F.anb64, C.ac64_own.p, C.anb64_own.p,
                        H.fp32 ? C.ac32.p : nullptr, H.fp32 ? C.anb32.p : nullptr);
   }
+  // the tail of the precision in use: coefficients and the coarsest inverse
+  TailImage &ti = H.fp32 ? H.tail32 : H.tail64;
+  const int nl = (int)H.lev.size();
+  if (ti.ts < nl) {
+    const TailDesc &d = ti.desc;
+    for (int q = 0; q < d.nlev; ++q) {
+      DLevel &D = H.lev[ti.ts + q];
+      const TailLvl &L = d.L[q];
+      const int cnt = std::max(L.n, L.nnz);
+      unsigned char *base = ti.img.p;
+      if (H.fp32)
+        hipLaunchKernelGGL(k_tail_pack<float>, dim3(gfor(cnt)), dim3(kBlock), 0, s, L.n, L.nnz, D.ac64, D.anb64,
+                           ti.lin.p + ti.lin_off[q], (float *)(base + L.o_ac), (float *)(base + L.o_val));
+      else
+        hipLaunchKernelGGL(k_tail_pack<double>, dim3(gfor(cnt)), dim3(kBlock), 0, s, L.n, L.nnz, D.ac64,
+                           D.anb64, ti.lin.p + ti.lin_off[q], (double *)(base + L.o_ac),
+                           (double *)(base + L.o_val));
+    }
+    if (d.direct) {
+      DLevel &Dc = H.lev[nl - 1];
+      hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(kInvThreads), 0, s, Dc.n, Dc.w, Dc.nbr, Dc.ac64,
+                         Dc.anb64, (double *)(ti.img.p + d.o_g));
+    }
+  }
 }
 
 int amg_levels(fs_mesh *m) { return m->amg ? (int)m->amg->lev.size() : 1; }
@@ -685,6 +945,7 @@ template <class T> struct LV {
   static T *t(DLevel &d);
   static const T *ac(DLevel &d);
   static const T *anb(DLevel &d);
+  static TailImage &tail(Hierarchy &h);
 };
 template <> struct LV<double> {
   static double *x(DLevel &d) { return d.x64.p; }
@@ -693,6 +954,7 @@ template <> struct LV<double> {
   static double *t(DLevel &d) { return d.t64.p; }
   static const double *ac(DLevel &d) { return d.ac64; }
   static const double *anb(DLevel &d) { return d.anb64; }
+  static TailImage &tail(Hierarchy &h) { return h.tail64; }
 };
 template <> struct LV<float> {
   static float *x(DLevel &d) { return d.x32.p; }
@@ -701,24 +963,36 @@ template <> struct LV<float> {
   static float *t(DLevel &d) { return d.t32.p; }
   static const float *ac(DLevel &d) { return d.ac32.p; }
   static const float *anb(DLevel &d) { return d.anb32.p; }
+  static TailImage &tail(Hierarchy &h) { return h.tail32; }
 };
 
-template <class T>
-static void launch_jac(fs_mesh *m, DLevel &D, const T *b, const T *x, T *xn, T om, const int *flags) {
+// one damped-Jacobi sweep on a grid-wide level, lanes per row by row width;
+// sweeps on the finest level go through the kernel clock (site
+// FS_CLOCK_FINE_SWEEP) with their algorithmic bytes: ac, b, x, the result, the
+// neighbour indices and the off-diagonal values, plus parent and the coarse
+// correction when the prolongation is fused in
+template <class T, class TO, bool PRO>
+static void launch_sweep(fs_mesh *m, DLevel &D, int level, const T *b, const T *x, const int *parent, const T *xc,
+                         int nc, T alpha, T om, TO *xn, const int *flags) {
   hipStream_t s = m->stream;
   const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
+  const int site = level == 0 ? FS_CLOCK_FINE_SWEEP : 0;
+  double bytes = 0.0;
+  if (site)
+    bytes = (double)D.n * (3.0 * sizeof(T) + sizeof(TO)) + 4.0 * D.n * D.w + (double)sizeof(T) * m->nnz +
+            (PRO ? 4.0 * D.n + (double)sizeof(T) * nc : 0.0);
   if (D.w <= 8)
-    hipLaunchKernelGGL(k_jac<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x, xn,
-                       om, flags);
+    timed_launch(m, site, bytes, k_sweep<1, T, TO, PRO>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
+                 anb, b, x, parent, xc, alpha, om, xn, flags);
   else if (D.w <= 16)
-    hipLaunchKernelGGL((k_jac_lpr<4, T>), dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
-                       anb, b, x, xn, om, flags);
+    timed_launch(m, site, bytes, k_sweep<4, T, TO, PRO>, dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, D.nbr,
+                 ac, anb, b, x, parent, xc, alpha, om, xn, flags);
   else if (D.w <= 32)
-    hipLaunchKernelGGL((k_jac_lpr<8, T>), dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
-                       anb, b, x, xn, om, flags);
+    timed_launch(m, site, bytes, k_sweep<8, T, TO, PRO>, dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, D.nbr,
+                 ac, anb, b, x, parent, xc, alpha, om, xn, flags);
   else
-    hipLaunchKernelGGL((k_jac_lpr<16, T>), dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, D.w, D.nbr,
-                       ac, anb, b, x, xn, om, flags);
+    timed_launch(m, site, bytes, k_sweep<16, T, TO, PRO>, dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, D.w,
+                 D.nbr, ac, anb, b, x, parent, xc, alpha, om, xn, flags);
 }
 
 template <class T>
@@ -726,114 +1000,179 @@ static void launch_resid(fs_mesh *m, DLevel &D, const T *b, const T *x, T *r, co
   hipStream_t s = m->stream;
   const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
   if (D.w <= 8)
-    hipLaunchKernelGGL(k_lresid<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x, r,
+    hipLaunchKernelGGL((k_resid<1, T>), dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x, r,
                        flags);
   else if (D.w <= 16)
-    hipLaunchKernelGGL((k_lresid_lpr<4, T>), dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
-                       anb, b, x, r, flags);
+    hipLaunchKernelGGL((k_resid<4, T>), dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x,
+                       r, flags);
   else if (D.w <= 32)
-    hipLaunchKernelGGL((k_lresid_lpr<8, T>), dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
-                       anb, b, x, r, flags);
+    hipLaunchKernelGGL((k_resid<8, T>), dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x,
+                       r, flags);
   else
-    hipLaunchKernelGGL((k_lresid_lpr<16, T>), dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, D.w, D.nbr,
-                       ac, anb, b, x, r, flags);
+    hipLaunchKernelGGL((k_resid<16, T>), dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b,
+                       x, r, flags);
+}
+
+// multicolour symmetric Gauss-Seidel only where it is cheap: few colours,
+// narrow rows
+static bool use_gs(const fs_mesh *m, const DLevel &D) {
+  return m->mg_smoother == 1 && (int)D.color_ptr.size() - 1 <= 4 && D.w <= 8;
 }
 
 template <class T>
-static void smooth(fs_mesh *m, DLevel &D, const T *b, T *&x, T *&tmp, int sweeps, bool zero_start,
-                   const int *flags) {
+static void gs_sweeps(fs_mesh *m, DLevel &D, const T *b, T *x, int sweeps, const int *flags) {
   hipStream_t s = m->stream;
   const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
   const int nc = (int)D.color_ptr.size() - 1;
-  // multicolour Gauss-Seidel only where it is cheap: few colours, narrow rows
-  const bool gs = m->mg_smoother == 1 && nc <= 4 && D.w <= 8;
-  const T om = (T)m->mg_omega;
-  if (gs) {
-    if (zero_start) hipLaunchKernelGGL(k_zero<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, x, flags);
-    for (int sw = 0; sw < sweeps; ++sw) {
-      for (int c = 0; c < nc; ++c) {
-        const int bb = D.color_ptr[c], e = D.color_ptr[c + 1];
-        hipLaunchKernelGGL(k_gsc<T>, dim3(gfor(e - bb)), dim3(kBlock), 0, s, e - bb,
-                           D.color_cells.p + bb, D.n, D.w, D.nbr, ac, anb, b, x, flags);
-      }
-      for (int c = nc - 1; c >= 0; --c) {
-        const int bb = D.color_ptr[c], e = D.color_ptr[c + 1];
-        hipLaunchKernelGGL(k_gsc<T>, dim3(gfor(e - bb)), dim3(kBlock), 0, s, e - bb,
-                           D.color_cells.p + bb, D.n, D.w, D.nbr, ac, anb, b, x, flags);
-      }
+  for (int sw = 0; sw < sweeps; ++sw) {
+    for (int c = 0; c < nc; ++c) {
+      const int bb = D.color_ptr[c], e = D.color_ptr[c + 1];
+      hipLaunchKernelGGL(k_gsc<T>, dim3(gfor(e - bb)), dim3(kBlock), 0, s, e - bb, D.color_cells.p + bb, D.n,
+                         D.w, D.nbr, ac, anb, b, x, flags);
     }
-  } else {
-    int sw = 0;
-    if (zero_start && sweeps > 0) {
-      hipLaunchKernelGGL(k_jac0<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, ac, b, x, om, flags);
-      sw = 1;
-    } else if (zero_start) {
-      hipLaunchKernelGGL(k_zero<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, x, flags);
-    }
-    for (; sw < sweeps; ++sw) {
-      launch_jac<T>(m, D, b, x, tmp, om, flags);
-      std::swap(x, tmp);
+    for (int c = nc - 1; c >= 0; --c) {
+      const int bb = D.color_ptr[c], e = D.color_ptr[c + 1];
+      hipLaunchKernelGGL(k_gsc<T>, dim3(gfor(e - bb)), dim3(kBlock), 0, s, e - bb, D.color_cells.p + bb, D.n,
+                         D.w, D.nbr, ac, anb, b, x, flags);
     }
   }
 }
 
-// One V-cycle: x_0 = M^{-1} b_0 on the hierarchy held in precision T.
-// Levels 0..tail-1 are swept by grid-wide kernels, levels tail..L-1 by the
-// one-workgroup tail kernel.
-template <class T> static void vcycle(fs_mesh *m, const T *b0, T *x0, const int *flags) {
+// Remaining pre-smoothing sweeps after the first (which the entry or the
+// restriction kernel took); returns the buffer that holds the result.
+template <class T>
+static T *pre_smooth(fs_mesh *m, DLevel &D, int level, const T *b, int sweeps, T om, T *&other, const int *flags) {
+  T *cur = LV<T>::x(D), *oth = LV<T>::t(D);
+  if (use_gs(m, D)) {
+    gs_sweeps<T>(m, D, b, cur, sweeps, flags); // x was zeroed by the caller
+  } else {
+    for (int k = 1; k < sweeps; ++k) {
+      launch_sweep<T, T, false>(m, D, level, b, cur, nullptr, nullptr, 0, T(0), om, oth, flags);
+      std::swap(cur, oth);
+    }
+  }
+  other = oth;
+  return cur;
+}
+
+// Prolongation of the coarse correction xc into x plus the post-smoothing
+// sweeps; the last operation writes dst (type TO), or a level buffer when dst
+// is null.  Returns the buffer holding the result when dst is null.
+template <class T, class TO>
+static T *post_smooth(fs_mesh *m, DLevel &D, int level, int nc, const T *b, T *x, T *other, const T *xc, int sweeps,
+                      T om, T al, TO *dst, const int *flags) {
+  hipStream_t s = m->stream;
+  if (use_gs(m, D)) {
+    hipLaunchKernelGGL((k_prolong<T, T>), dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.parent.p, xc, x, al, x,
+                       flags);
+    gs_sweeps<T>(m, D, b, x, sweeps, flags);
+    if (dst) hipLaunchKernelGGL((k_convert<T, TO>), dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, x, dst, flags);
+    return x;
+  }
+  if (sweeps == 0) {
+    if (dst) {
+      hipLaunchKernelGGL((k_prolong<T, TO>), dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.parent.p, xc, x, al, dst,
+                         flags);
+    } else {
+      hipLaunchKernelGGL((k_prolong<T, T>), dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.parent.p, xc, x, al, x,
+                         flags);
+    }
+    return x;
+  }
+  T *cur = x, *oth = other;
+  for (int k = 0; k < sweeps; ++k) {
+    const bool last = k + 1 == sweeps;
+    if (last && dst) {
+      if (k == 0) launch_sweep<T, TO, true>(m, D, level, b, cur, D.parent.p, xc, nc, al, om, dst, flags);
+      else launch_sweep<T, TO, false>(m, D, level, b, cur, nullptr, nullptr, 0, T(0), om, dst, flags);
+      return cur;
+    }
+    if (k == 0) launch_sweep<T, T, true>(m, D, level, b, cur, D.parent.p, xc, nc, al, om, oth, flags);
+    else launch_sweep<T, T, false>(m, D, level, b, cur, nullptr, nullptr, 0, T(0), om, oth, flags);
+    std::swap(cur, oth);
+  }
+  return cur;
+}
+
+// out = M^{-1} rhs: one V-cycle on the hierarchy held in precision T.  The
+// first sweep of each level is fused into the kernel that produces its
+// right-hand side, the prolongation into the first post-smoothing sweep, the
+// FP64 output into the last one; the levels from the tail's first level down
+// run in the one-workgroup tail kernel.  out must not alias rhs.
+template <class T>
+static void vcycle(fs_mesh *m, const double *rhs, double *out, double alpha, const int *flags) {
   Hierarchy &H = *m->amg;
   hipStream_t s = m->stream;
   const int L = (int)H.lev.size();
-  const int tail = std::min(H.tail, L); // >= 1 by construction
-  std::vector<T *> X(L), Tm(L);
-  std::vector<const T *> B(L);
-  for (int l = 0; l < L; ++l) {
-    X[l] = LV<T>::x(H.lev[l]);
-    Tm[l] = LV<T>::t(H.lev[l]);
-    B[l] = LV<T>::b(H.lev[l]);
+  TailImage &ti = LV<T>::tail(H);
+  const int ts = std::min(ti.ts, L);
+  const T om = (T)m->mg_omega, al = (T)alpha;
+  const int pre0 = m->mg_pre, post0 = m->mg_post;
+  const int cpre = m->mg_cpre >= 0 ? m->mg_cpre : pre0, cpost = m->mg_cpost >= 0 ? m->mg_cpost : post0;
+  const int coarse = std::max(m->mg_coarse_sweeps, 1);
+  const bool fp64 = sizeof(T) == 8;
+
+  DLevel &D0 = H.lev[0];
+  const T *b0 = fp64 ? (const T *)rhs : LV<T>::b(D0);
+  // level 0 with no coarser level: smoothing only
+  if (L == 1) {
+    const bool gs = use_gs(m, D0);
+    hipLaunchKernelGGL(k_enter<T>, dim3(gfor(D0.n)), dim3(kBlock), 0, s, D0.n, LV<T>::ac(D0), rhs,
+                       fp64 ? nullptr : LV<T>::b(D0), LV<T>::x(D0), om, gs ? 0 : 1, flags);
+    T *oth = nullptr;
+    T *x = pre_smooth<T>(m, D0, 0, b0, coarse, om, oth, flags);
+    hipLaunchKernelGGL((k_convert<T, double>), dim3(gfor(D0.n)), dim3(kBlock), 0, s, D0.n, x, out, flags);
+    return;
   }
-  B[0] = b0;
-  const int top = tail < L ? tail : L - 1; // levels swept by grid-wide kernels: 0..top-1
+  std::vector<T *> res(L, nullptr), oth(L, nullptr);
+  // levels above the bottom: grid-wide kernels
+  const int top = ts < L ? ts : L - 1;
+  {
+    const bool gs = use_gs(m, D0);
+    hipLaunchKernelGGL(k_enter<T>, dim3(gfor(D0.n)), dim3(kBlock), 0, s, D0.n, LV<T>::ac(D0), rhs,
+                       fp64 ? nullptr : LV<T>::b(D0), LV<T>::x(D0), om, (!gs && pre0 > 0) ? 1 : 0, flags);
+  }
   for (int l = 0; l < top; ++l) {
     DLevel &D = H.lev[l];
-    smooth<T>(m, D, B[l], X[l], Tm[l], m->mg_pre, true, flags);
-    launch_resid<T>(m, D, B[l], X[l], LV<T>::r(D), flags);
+    const T *b = l == 0 ? b0 : LV<T>::b(D);
+    const int npre = l == 0 ? pre0 : cpre;
+    res[l] = pre_smooth<T>(m, D, l, b, npre, om, oth[l], flags);
+    launch_resid<T>(m, D, b, res[l], LV<T>::r(D), flags);
     DLevel &C = H.lev[l + 1];
-    hipLaunchKernelGGL(k_restrict<T>, dim3(gfor(C.n)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p,
-                       LV<T>::r(D), LV<T>::b(C), flags);
+    int mode;
+    if (l + 1 == ts) mode = 0;                                  // the tail kernel takes over
+    else if (l + 1 == L - 1) mode = 1;                          // grid-wide coarsest: `coarse` sweeps
+    else mode = (!use_gs(m, C) && cpre > 0) ? 1 : 2;
+    hipLaunchKernelGGL(k_restrict<T>, dim3(gfor(C.n)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p, LV<T>::r(D),
+                       LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
   }
-  const int coarse = std::max(m->mg_coarse_sweeps, 1);
-  if (tail < L) {
-    const TailLevel *desc = (sizeof(T) == 4) ? H.tail32.p : H.tail64.p;
-    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), 0, s, desc, L - tail, m->mg_pre,
-                       m->mg_post, coarse, (T)m->mg_omega, flags);
-    X[tail] = LV<T>::x(H.lev[tail]); // the tail keeps its result in x
+  if (ts < L) {
+    DLevel &Dt = H.lev[ts];
+    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), ti.desc.lds_bytes, s, ti.ddesc.p, ti.img.p,
+                       LV<T>::b(Dt), LV<T>::x(Dt), cpre, cpost, coarse, om, al, flags);
+    FS_HIP_THROW(hipGetLastError()); // a refused launch (LDS request) must not pass silently
+    res[ts] = LV<T>::x(Dt);
   } else {
-    DLevel &D = H.lev[L - 1];
-    smooth<T>(m, D, B[L - 1], X[L - 1], Tm[L - 1], coarse, true, flags);
+    DLevel &Dc = H.lev[L - 1];
+    // the restriction took the first of the coarse sweeps
+    T *o = nullptr;
+    res[L - 1] = pre_smooth<T>(m, Dc, L - 1, LV<T>::b(Dc), coarse, om, o, flags);
   }
   for (int l = top - 1; l >= 0; --l) {
     DLevel &D = H.lev[l];
-    hipLaunchKernelGGL(k_prolong<T>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.parent.p, X[l + 1],
-                       X[l], flags);
-    smooth<T>(m, D, B[l], X[l], Tm[l], m->mg_post, false, flags);
-  }
-  if (X[0] != x0) {
-    FS_HIP_THROW(hipMemcpyAsync(x0, X[0], sizeof(T) * H.lev[0].n, hipMemcpyDeviceToDevice, s));
+    const T *b = l == 0 ? b0 : LV<T>::b(D);
+    const int npost = l == 0 ? post0 : cpost;
+    const int nc = H.lev[l + 1].n;
+    if (l == 0)
+      post_smooth<T, double>(m, D, 0, nc, b, res[0], oth[0], res[1], npost, om, al, out, flags);
+    else
+      res[l] = post_smooth<T, T>(m, D, l, nc, b, res[l], oth[l], res[l + 1], npost, om, al, (T *)nullptr, flags);
   }
 }
 
-void amg_apply(fs_mesh *m, const double *rhs, double *out, const int *flags) {
-  Hierarchy &H = *m->amg;
-  hipStream_t s = m->stream;
-  DLevel &L0 = H.lev[0];
-  if (H.fp32) {
-    hipLaunchKernelGGL(k_to<float>, dim3(gfor(L0.n)), dim3(kBlock), 0, s, L0.n, rhs, L0.b32.p);
-    vcycle<float>(m, L0.b32.p, L0.x32.p, flags);
-    hipLaunchKernelGGL(k_from<float>, dim3(gfor(L0.n)), dim3(kBlock), 0, s, L0.n, L0.x32.p, out, flags);
-  } else {
-    vcycle<double>(m, rhs, out, flags);
-  }
+void amg_apply(fs_mesh *m, const double *rhs, double *out, const int *flags, double alpha) {
+  if (m->amg->fp32) vcycle<float>(m, rhs, out, alpha, flags);
+  else vcycle<double>(m, rhs, out, alpha, flags);
 }
 
 // The reference's multigrid solve loop (AlgebraicMultiGridMainRoutine.for:
@@ -847,6 +1186,8 @@ int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc,
   const int n = m->n;
   double *r = m->v[8].p, *c = m->v[9].p;
   int *flags = m->flags.p;
+  // additive correction as in the reference unless asked otherwise
+  const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.0;
   // flags[2] holds itmax for the Krylov solvers; the multigrid loop is
   // bounded by MaxMultiGridCycles instead
   int itmax_cycles = m->mg_maxcycles;
@@ -860,7 +1201,7 @@ int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc,
   while (!fl[0] && launched < itmax_cycles) {
     const int batch = std::min(m->check_every, itmax_cycles - launched);
     for (int b = 0; b < batch; ++b) {
-      amg_apply(m, r, c, flags);
+      amg_apply(m, r, c, flags, alpha);
       hipLaunchKernelGGL(k_add2, dim3(gfor(n)), dim3(kBlock), 0, s, n, dphi, c, flags);
       hipLaunchKernelGGL(k_amg_res, dim3(gr), dim3(kBlock), 0, s, n, m->w, m->d_nbr.p, ac, anb,
                          bc, dphi, r, m->partial.p, flags);

@@ -30,6 +30,36 @@ using fs::set_error;
 extern "C" {
 
 const char *fs_last_error(void) { return fs::g_err.c_str(); }
+
+int fs_kernel_clock_start(fs_mesh *m, int site_mask, int capacity) {
+  if (!m) {
+    set_error("fs_kernel_clock_start: mesh is NULL");
+    return -1;
+  }
+  FS_GUARD_BEGIN
+  return fs::kernel_clock_start(m, site_mask, capacity);
+  FS_GUARD_END
+}
+
+int fs_kernel_clock_read(fs_mesh *m, int site, double *out) {
+  if (!m || !out) {
+    set_error("fs_kernel_clock_read: NULL argument");
+    return -1;
+  }
+  FS_GUARD_BEGIN
+  return fs::kernel_clock_read(m, site, out);
+  FS_GUARD_END
+}
+
+int fs_kernel_clock_clear(fs_mesh *m) {
+  if (!m) {
+    set_error("fs_kernel_clock_clear: mesh is NULL");
+    return -1;
+  }
+  FS_GUARD_BEGIN
+  return fs::kernel_clock_clear(m);
+  FS_GUARD_END
+}
 const char *fs_version(void) { return "fsamd 0.1 gfx950"; }
 
 int fs_mesh_create(int n_cells, int width, const int *elem_nbr, const int *n_nbr, int n_ifaces,
@@ -125,8 +155,8 @@ int fs_set_option(fs_mesh *m, int key, double value) {
   // their value actually changes
   const int old_levels = m->mg_maxlevels, old_parents = m->mg_minparents, old_tail = m->mg_tail_cells;
   switch (key) {
-  case FS_OPT_MG_PRESWEEP: m->mg_pre = iv; break;
-  case FS_OPT_MG_POSTSWEEP: m->mg_post = iv; break;
+  case FS_OPT_MG_PRESWEEP: m->mg_pre = iv < 0 ? 0 : iv; break;
+  case FS_OPT_MG_POSTSWEEP: m->mg_post = iv < 0 ? 0 : iv; break;
   case FS_OPT_MG_MAXCYCLES: m->mg_maxcycles = iv; break;
   case FS_OPT_MG_MAXLEVELS: m->mg_maxlevels = iv; break;
   case FS_OPT_MG_MINPARENTS: m->mg_minparents = iv; break;
@@ -138,6 +168,15 @@ int fs_set_option(fs_mesh *m, int key, double value) {
   case FS_OPT_CHECK_EVERY: m->check_every = iv < 1 ? 1 : iv; break;
   case FS_OPT_MG_TAIL_CELLS: m->mg_tail_cells = iv; break;
   case FS_OPT_MG_COARSE_SWEEPS: m->mg_coarse_sweeps = iv < 1 ? 1 : iv; break;
+  case FS_OPT_MG_ALPHA:
+    if (!(value >= 0.0 && value <= 4.0)) {
+      set_error("fs_set_option: MG_ALPHA must lie in [0, 4]");
+      return -1;
+    }
+    m->mg_alpha = value;
+    break;
+  case FS_OPT_MG_COARSE_PRESWEEP: m->mg_cpre = iv < -1 ? -1 : iv; break;
+  case FS_OPT_MG_COARSE_POSTSWEEP: m->mg_cpost = iv < -1 ? -1 : iv; break;
   default:
     set_error("fs_set_option: unknown key " + std::to_string(key));
     return -1;

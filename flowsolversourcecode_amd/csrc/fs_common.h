@@ -76,6 +76,21 @@ template <class T> struct DevBuf {
 
 struct Hierarchy; // fs_amg.hip
 
+// Per-launch HIP-event timing of one kernel site of the solve loops: events
+// are recorded by the dispatch itself (hipExtLaunchKernelGGL), tagged with
+// the Krylov iteration that launched them, so launches after convergence
+// (which return at once) can be told from the ones that did the work.
+struct KernelClock {
+  int sites = 0, cap = 0, used = 0;  // sites: bit (1 << site) per armed site
+  int dropped[4] = {0, 0, 0, 0};
+  int cur_iter = -1;       // Krylov iteration being launched, -1 outside a loop
+  int solve_first = 0;     // first record of the solve in progress
+  std::vector<hipEvent_t> ev;  // 2 per record
+  std::vector<int> iter, site;
+  std::vector<double> bytes;
+  std::vector<char> live;
+};
+
 } // namespace fs
 
 struct fs_mesh {
@@ -107,8 +122,12 @@ struct fs_mesh {
   int mg_minparents = 10, mg_reagg = 300, mg_fp32 = 0, mg_smoother = 0;
   double mg_omega = 0.8;
   int mg_tail_cells = 1024, mg_coarse_sweeps = 16;
+  int mg_cpre = -1, mg_cpost = -1; // sweeps on the coarse levels (-1: as on the finest)
+  double mg_alpha = 0.0;           // over-correction (0: 1 as solver, 1.7 as preconditioner)
   int gmres_m = 30, check_every = 8;
   int ncalls_mg = 0;
+  long long nnz = 0;                    // off-diagonal entries of the ELL pattern
+  fs::KernelClock kclock;
   void *solve_work = nullptr;           // fs_solve.hip
   std::shared_ptr<fs::Hierarchy> amg;   // fs_amg.hip
 };

@@ -1,11 +1,37 @@
 // Entry points shared between the solver translation units.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include <vector>
 
 #include "fs_common.h"
 
 namespace fs {
+
+// Launch through the mesh's kernel clock when `site` is the armed one.
+template <class K, class... A>
+void timed_launch(fs_mesh *m, int site, double bytes, K kernel, dim3 grid, dim3 block, unsigned lds,
+                  hipStream_t s, A... args) {
+  KernelClock &c = m->kclock;
+  if (site != 0 && (c.sites >> site & 1)) {
+    if (c.used < c.cap) {
+      const int u = c.used++;
+      c.iter[u] = c.cur_iter;
+      c.site[u] = site;
+      c.bytes[u] = bytes;
+      c.live[u] = 1;
+      hipExtLaunchKernelGGL(kernel, grid, block, lds, s, c.ev[2 * u], c.ev[2 * u + 1], 0u, args...);
+      return;
+    }
+    c.dropped[site & 3]++;
+  }
+  hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+}
+int kernel_clock_start(fs_mesh *m, int site_mask, int capacity);
+int kernel_clock_read(fs_mesh *m, int site, double *out);
+int kernel_clock_clear(fs_mesh *m);
+void kernel_clock_release(fs_mesh *m);
 
 int mesh_prepare_solver(fs_mesh *m);
 void mesh_release_solver(fs_mesh *m);
@@ -22,8 +48,9 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
 // fine operator (ac, anb): aggregation on the first call and every
 // mg_reagg calls, device Galerkin operators otherwise.
 void amg_prepare(fs_mesh *m, const double *ac, const double *anb);
-// out = one V-cycle applied to rhs (both FP64 device vectors).
-void amg_apply(fs_mesh *m, const double *rhs, double *out, const int *flags);
+// out = one V-cycle applied to rhs (both FP64 device vectors, not aliased),
+// coarse corrections scaled by alpha.
+void amg_apply(fs_mesh *m, const double *rhs, double *out, const int *flags, double alpha);
 // The reference's multigrid-as-solver loop; returns cycles run.
 int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc, double *dphi,
               int solver);

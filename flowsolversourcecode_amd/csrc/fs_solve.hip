@@ -136,6 +136,34 @@ __global__ void k_resid_l1(int n, int w, const int *__restrict__ nbr, const doub
 }
 
 // r = b - A*x, also copied to r0; partial L1 of r.
+// p = z (first iteration) or z + beta*p
+__global__ void k_cg_p(int n, const double *__restrict__ z, double *__restrict__ p,
+                       const double *__restrict__ scal, const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const bool first = flags[F_ITER] == 0;
+  const double be = scal[S_BETA];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = first ? z[i] : z[i] + be * p[i];
+}
+
+// x += alpha p, r -= alpha q, partial L1(r)
+__global__ void k_cg_update(int n, double *__restrict__ x, double *__restrict__ r,
+                            const double *__restrict__ p, const double *__restrict__ q,
+                            const double *__restrict__ scal, double *__restrict__ part,
+                            const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const double al = scal[S_ALPHA];
+  double a = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[i] = x[i] + al * p[i];
+    const double ri = r[i] - al * q[i];
+    r[i] = ri;
+    a += fabs(ri);
+  }
+  a = block_sum<kBlock>(a);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
 __global__ void k_resid_vec(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
                             const double *__restrict__ anb, const double *__restrict__ b,
                             const double *__restrict__ x, double *__restrict__ r,
@@ -392,7 +420,9 @@ enum FinOp {
   FIN_BICG_RHO,       // rho, beta
   FIN_BICG_ALPHA,     // alpha = rho/(r0,v)
   FIN_BICG_OMEGA,     // omega = (t,s)/(t,t)
-  FIN_TRUE_RES        // fres only
+  FIN_TRUE_RES,       // fres only
+  FIN_CG_RHO,         // rz = (r,z), beta
+  FIN_CG_ALPHA        // alpha = rz/(p,q)
 };
 
 __device__ double reduce_partials(const double *part, int np) {
@@ -450,9 +480,19 @@ __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ 
     break;
   }
   case FIN_BICG_RHO: {
-    if (flags[F_ITER] > 0) scal[S_BETA] = (a / scal[S_RHO]) * (scal[S_ALPHA] / scal[S_OMEGA]);
+    // breakdown (rho or the last omega zero, or a non-finite beta, as on a
+    // singular system whose right-hand side carries round-off outside the
+    // range): stop with the current iterate
+    if (flags[F_ITER] > 0) {
+      const double beta = (a / scal[S_RHO]) * (scal[S_ALPHA] / scal[S_OMEGA]);
+      if (scal[S_OMEGA] == 0.0 || !isfinite(beta)) {
+        flags[F_DONE] = 1;
+        break;
+      }
+      scal[S_BETA] = beta;
+    }
     scal[S_RHO] = a;
-    if (a == 0.0) flags[F_DONE] = 1; // breakdown: keep the current iterate
+    if (a == 0.0) flags[F_DONE] = 1;
     break;
   }
   case FIN_BICG_ALPHA:
@@ -466,6 +506,27 @@ __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ 
     break;
   case FIN_TRUE_RES:
     scal[S_FRES] = a;
+    break;
+  case FIN_CG_RHO: {
+    if (flags[F_ITER] > 0) {
+      const double beta = a / scal[S_RHO];
+      if (!isfinite(beta)) {
+        flags[F_DONE] = 1; // breakdown: keep the iterate
+        break;
+      }
+      scal[S_BETA] = beta;
+    }
+    scal[S_RHO] = a;
+    if (a == 0.0) flags[F_DONE] = 1;
+    break;
+  }
+  case FIN_CG_ALPHA:
+    scal[S_RV] = a;
+    scal[S_ALPHA] = (a != 0.0) ? scal[S_RHO] / a : 0.0;
+    if (a == 0.0 || !isfinite(scal[S_ALPHA])) {
+      scal[S_ALPHA] = 0.0;
+      flags[F_DONE] = 1;
+    }
     break;
   }
 }
@@ -610,11 +671,90 @@ int mesh_prepare_solver(fs_mesh *m) {
   m->partial.alloc(4 * kPartials);
   m->scal.alloc(S_NSLOTS);
   m->flags.alloc(F_NFLAGS);
+  m->nnz = 0;
+  for (int i = 0; i < m->n; ++i)
+    for (int j = 0; j < m->w; ++j)
+      if (m->h_nbr_dev[(size_t)j * m->n + i] >= 0) ++m->nnz;
   FS_HIP_THROW(hipStreamSynchronize(m->stream));
   return 0;
 }
 
+int kernel_clock_start(fs_mesh *m, int site_mask, int capacity) {
+  KernelClock &c = m->kclock;
+  const int valid = (1 << FS_CLOCK_SPMV) | (1 << FS_CLOCK_FINE_SWEEP);
+  if (site_mask & ~valid) {
+    set_error("fs_kernel_clock_start: unknown site in mask " + std::to_string(site_mask));
+    return -1;
+  }
+  if (site_mask == 0) {
+    c.sites = 0;
+    return 0;
+  }
+  if (capacity < 1 || capacity > (1 << 20)) {
+    set_error("fs_kernel_clock_start: capacity must lie in [1, 2^20]");
+    return -1;
+  }
+  while ((int)c.ev.size() < 2 * capacity) {
+    hipEvent_t e;
+    FS_HIP(hipEventCreate(&e));
+    c.ev.push_back(e);
+  }
+  c.sites = site_mask;
+  c.cap = capacity;
+  c.used = 0;
+  for (int &d : c.dropped) d = 0;
+  c.solve_first = 0;
+  c.iter.assign(capacity, -1);
+  c.site.assign(capacity, 0);
+  c.bytes.assign(capacity, 0.0);
+  c.live.assign(capacity, 0);
+  return 0;
+}
+
+int kernel_clock_read(fs_mesh *m, int site, double *out) {
+  KernelClock &c = m->kclock;
+  if (site < 1 || site > 3) {
+    set_error("fs_kernel_clock_read: unknown site " + std::to_string(site));
+    return -1;
+  }
+  FS_HIP(hipStreamSynchronize(m->stream));
+  long long k = 0, idle = 0;
+  double ms = 0.0, by = 0.0;
+  for (int u = 0; u < c.used; ++u) {
+    if (c.site[u] != site) continue;
+    if (!c.live[u]) {
+      ++idle;
+      continue;
+    }
+    float t = 0.f;
+    FS_HIP(hipEventElapsedTime(&t, c.ev[2 * u], c.ev[2 * u + 1]));
+    ms += t;
+    by += c.bytes[u];
+    ++k;
+  }
+  out[0] = (double)k;
+  out[1] = k ? ms / k : 0.0;
+  out[2] = k ? by / k : 0.0;
+  out[3] = (double)(c.dropped[site & 3] + idle);
+  return 0;
+}
+
+int kernel_clock_clear(fs_mesh *m) {
+  KernelClock &c = m->kclock;
+  FS_HIP(hipStreamSynchronize(m->stream));
+  c.used = 0;
+  for (int &d : c.dropped) d = 0;
+  c.solve_first = 0;
+  return 0;
+}
+
+void kernel_clock_release(fs_mesh *m) {
+  for (hipEvent_t e : m->kclock.ev) (void)hipEventDestroy(e);
+  m->kclock = KernelClock{};
+}
+
 void mesh_release_solver(fs_mesh *m) {
+  kernel_clock_release(m);
   delete reinterpret_cast<SolveWork *>(m->solve_work);
   m->solve_work = nullptr;
 }
@@ -640,12 +780,21 @@ template <class F> static int run_batches(fs_mesh *m, int itmax, F body) {
   int fl[F_NFLAGS];
   host_flags(m, fl);
   int launched = 0;
+  m->kclock.solve_first = m->kclock.used;
   while (!fl[F_DONE] && launched < itmax) {
     const int batch = std::min(m->check_every, itmax - launched);
-    for (int b = 0; b < batch; ++b) body();
+    for (int b = 0; b < batch; ++b) {
+      m->kclock.cur_iter = launched + b;
+      body();
+    }
     launched += batch;
     host_flags(m, fl);
   }
+  // launches of iterations past the converged one returned at once
+  KernelClock &c = m->kclock;
+  for (int u = c.solve_first; u < c.used; ++u)
+    if (c.iter[u] >= fl[F_ITER]) c.live[u] = 0;
+  c.cur_iter = -1;
   return fl[F_ITER];
 }
 
@@ -682,8 +831,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     hipLaunchKernelGGL(k_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, dphi, V0);
     hipLaunchKernelGGL(k_pbcg_start, dim3(ge), dim3(kBlock), 0, s, n, bc, V0, V1);
     hipLaunchKernelGGL(k_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, V0, V1);
-    const bool pre = (lmg == FS_MG_PRECOND);
-    if (pre) amg_prepare(m, ac, anb);
+    // PBCG keeps the reference's D-ILU preconditioner in every multigrid mode
     iters = run_batches(m, itmax, [&]() {
       hipLaunchKernelGGL(k_pbcg_bknum, dim3(gr), dim3(kBlock), 0, s, n, V0, V1, sw.dc.p, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_PBCG_BK, gr, n, nf, part, scal, flags);
@@ -716,22 +864,50 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     // right-preconditioned BiCGSTAB; r=V0 r0=V1 p=V2 v=V3 s=V4 t=V5 phat=V6 shat=V7
     const bool pre = (lmg == FS_MG_PRECOND);
     if (pre) amg_prepare(m, ac, anb);
+    const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
+    // algorithmic bytes of y = A x with the (r0, y) partials: ac, nbr, anb, x, y, r0
+    const double spmv_bytes = 8.0 * n + 4.0 * n * w + 8.0 * m->nnz + 24.0 * n;
     hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, V1, part);
     FS_HIP(hipMemsetAsync(V3, 0, sizeof(double) * n, s));
     iters = run_batches(m, itmax, [&]() {
       hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V1, V0, nullptr, nullptr, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_BICG_RHO, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V2, scal, flags);
-      if (pre) amg_apply(m, V2, V6, flags);
+      if (pre) amg_apply(m, V2, V6, flags, alpha);
       const double *ph = pre ? V6 : V2;
-      hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, ph, V3, V1, part, flags);
+      timed_launch(m, FS_CLOCK_SPMV, spmv_bytes, k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, ph,
+                   V3, (const double *)V1, part, (const int *)flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_BICG_ALPHA, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_bicg_s, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V4, scal, flags);
-      if (pre) amg_apply(m, V4, V7, flags);
+      if (pre) amg_apply(m, V4, V7, flags, alpha);
       const double *sh = pre ? V7 : V4;
       hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, sh, V5, V4, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_BICG_OMEGA, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, ph, sh, V4, V5, scal, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+    });
+  } else if (solver == FS_CG) {
+    // preconditioned conjugate gradient; r=V0 z=V1 p=V2 q=V3
+    const bool pre = (lmg == FS_MG_PRECOND);
+    if (pre) amg_prepare(m, ac, anb);
+    const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
+    // y = A p with the (p, y) partials: ac, nbr, anb, p, y
+    const double spmv_bytes = 8.0 * n + 4.0 * n * w + 8.0 * m->nnz + 16.0 * n;
+    hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, nullptr,
+                       part);
+    iters = run_batches(m, itmax, [&]() {
+      const double *z = V0;
+      if (pre) {
+        amg_apply(m, V0, V1, flags, alpha);
+        z = V1;
+      }
+      hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V0, z, nullptr, nullptr, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CG_RHO, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_cg_p, dim3(ge), dim3(kBlock), 0, s, n, z, V2, scal, flags);
+      timed_launch(m, FS_CLOCK_SPMV, spmv_bytes, k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, V2,
+                   V3, (const double *)V2, part, (const int *)flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CG_ALPHA, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_cg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, V2, V3, scal, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
   } else if (solver == FS_ILU) {

@@ -38,7 +38,9 @@ enum fs_solver {
   FS_PBCG = 2,   /* 'pbcg'   ILU-diagonal biconjugate gradient (PBCGMethod.for:96) */
   FS_DIRECT = 3, /* 'direct' dense LU -- not offered on the device             */
   FS_GMRES = 4,  /* 'gmres'  restarted GMRES (matrixSolverOMP.f90:48)          */
-  FS_BICGS = 5   /* 'bicgs'  BiCGSTAB (matrixSolverOMP.f90:306)                */
+  FS_BICGS = 5,  /* 'bicgs'  BiCGSTAB (matrixSolverOMP.f90:306)                */
+  FS_CG = 6      /* preconditioned conjugate gradient for symmetric systems
+                    (the p' equation) -- no reference counterpart            */
 };
 
 /* multigrid modes for the LMultigrid argument */
@@ -71,7 +73,14 @@ enum fs_option {
   FS_OPT_CHECK_EVERY = 11,    /* iterations launched between host checks [8]    */
   FS_OPT_MG_TAIL_CELLS = 12,  /* levels with at most this many cells run in one
                                  single-workgroup kernel [1024]                 */
-  FS_OPT_MG_COARSE_SWEEPS = 13 /* smoothing sweeps on the coarsest level [16]   */
+  FS_OPT_MG_COARSE_SWEEPS = 13,/* smoothing sweeps on a coarsest level too large
+                                 for the direct FP64 solve (> 64 cells) [16]   */
+  FS_OPT_MG_ALPHA = 14,       /* over-correction of the prolongated coarse
+                                 correction; 0 = 1.0 with multigrid as the
+                                 solver (the reference), 1.7 as a
+                                 preconditioner [0]                           */
+  FS_OPT_MG_COARSE_PRESWEEP = 15, /* pre-sweeps on levels >= 1 (-1: as level 0) [-1] */
+  FS_OPT_MG_COARSE_POSTSWEEP = 16 /* post-sweeps on levels >= 1 (-1: as level 0) [-1] */
 };
 
 /* Statistics of the last solve. */
@@ -156,6 +165,27 @@ int fs_amg_export(fs_mesh *mesh, int level, int *dims, int *parent, int *seed,
  * No counterpart in the reference: it calibrates HBM byte counters
  * (FETCH_SIZE / WRITE_SIZE) and gives the achievable copy bandwidth. */
 int fs_time_device_copy(long long bytes, int reps, double *avg_ms);
+
+/* Per-launch timing of kernels of the solve loops, for the roofline figure:
+ * each launch of an armed site records two HIP events on the mesh's stream
+ * from its own dispatch (hipExtLaunchKernelGGL).  No counterpart in the
+ * reference.  fs_kernel_clock_start creates `capacity` event pairs (outside
+ * any timed region) and arms the sites of site_mask (bit 1 << FS_CLOCK_*; 0
+ * disarms).  fs_kernel_clock_read synchronises the stream and returns for one
+ * site out[0] launches that did work (launches of Krylov iterations after
+ * convergence, which return at once, are left out), out[1] their mean
+ * duration in ms, out[2] their mean algorithmic bytes, out[3] launches of the
+ * site not timed (idle or beyond the capacity).  fs_kernel_clock_clear
+ * empties the records; the sites stay armed. */
+enum fs_clock_site {
+  FS_CLOCK_OFF = 0,
+  FS_CLOCK_SPMV = 1,      /* k_spmv_dot, the FP64 operator product of the
+                             BiCGSTAB / CG loops                              */
+  FS_CLOCK_FINE_SWEEP = 2 /* k_sweep on the finest AMG level                 */
+};
+int fs_kernel_clock_start(fs_mesh *mesh, int site_mask, int capacity);
+int fs_kernel_clock_read(fs_mesh *mesh, int site, double *out);
+int fs_kernel_clock_clear(fs_mesh *mesh);
 
 /* Message of the last failing call on this thread. */
 const char *fs_last_error(void);

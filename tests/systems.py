@@ -93,3 +93,13 @@ def dense(nbr, nnbr, ac, anb):
             if s < nnbr[c] and nbr[s, c]:
                 A[c, nbr[s, c] - 1] += anb[s, c]
     return A
+
+
+def sparse(nbr, nnbr, ac, anb):
+    """The same operator as a scipy CSR matrix (for systems too large for dense)."""
+    import scipy.sparse as sp
+    w, n = nbr.shape
+    rows = np.repeat(np.arange(n)[None, :], w, 0)
+    m = (nbr > 0) & (np.arange(w)[:, None] < np.asarray(nnbr)[None, :])
+    A = sp.csr_matrix((np.asarray(anb)[m], (rows[m], nbr[m] - 1)), shape=(n, n))
+    return (A + sp.diags(np.asarray(ac, float))).tocsr()

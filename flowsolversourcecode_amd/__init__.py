@@ -389,3 +389,11 @@ class Case:
         a = np.ascontiguousarray(values, dtype=np.float64)
         assert a.size == n
         _check(lib().fs_case_set(self.h, name.encode(), _dptr(a), n), f"fs_case_set({name})")
+
+    def topology(self, name):
+        """Readit's topology arrays of the case's mesh (reference numbering):
+        "nef", "gef", "enbr" ((W, N) = ElementNeighbor), "iown", "inbr"."""
+        n = {"nef": self.N, "gef": self.N * self.W, "enbr": self.N * self.W, "iown": self.Fi, "inbr": self.Fi}[name]
+        out = np.zeros(n, np.int32)
+        _check(lib().fs_case_topology(self.h, TOPOLOGY[name], _iptr(out), n), f"fs_case_topology({name})")
+        return out.reshape(self.W, self.N) if name in ("gef", "enbr") else out

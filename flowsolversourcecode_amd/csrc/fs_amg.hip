@@ -229,7 +229,9 @@ __global__ void k_galerkin(int ne, int nc, const int *__restrict__ ptr,
                            const long long *__restrict__ src, int nf,
                            const double *__restrict__ acf, const double *__restrict__ anbf,
                            double *__restrict__ acc_out, double *__restrict__ anbc,
-                           float *__restrict__ ac32, float *__restrict__ anb32) {
+                           float *__restrict__ ac32, float *__restrict__ anb32,
+                           const int *__restrict__ csr_pos, double *__restrict__ val64,
+                           float *__restrict__ val32) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= ne) return;
   double a = 0.0;
@@ -243,6 +245,12 @@ __global__ void k_galerkin(int ne, int nc, const int *__restrict__ ptr,
   } else {
     anbc[e - nc] = a;
     if (anb32) anb32[e - nc] = (float)a;
+    // the same value at its place in the row-major CSR copy of the level
+    const int q = csr_pos[e - nc];
+    if (q >= 0) {
+      val64[q] = a;
+      if (val32) val32[q] = (float)a;
+    }
   }
 }
 
@@ -310,6 +318,71 @@ k_resid(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac, con
   const T s = row_offdiag<LPR, T, false>(ok ? i : 0, lane, ok ? n : 1, ok ? w : 0, nbr, anb, x, nullptr,
                                          nullptr, T(0));
   if (ok && lane == 0) r[i] = b[i] - ac[i] * x[i] - s;
+}
+
+// ---- coarse levels in row-major CSR (no padding; a row's entries are
+// contiguous, so the LPR lanes of a row read one segment)
+template <int LPR, class T, bool PRO>
+__device__ __forceinline__ T csr_offdiag(int i, int lane, const int *__restrict__ rp, const int *__restrict__ col,
+                                         const T *__restrict__ val, const T *x, const int *__restrict__ parent,
+                                         const T *__restrict__ xc, T alpha, bool ok) {
+  T s = T(0);
+  if (ok)
+    for (int q = rp[i] + lane; q < rp[i + 1]; q += LPR) {
+      const int k = col[q];
+      T xk = x[k];
+      if (PRO) xk = xk + alpha * xc[parent[k]];
+      s += val[q] * xk;
+    }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+  return s;
+}
+
+template <int LPR, class T, class TO, bool PRO>
+__global__ void __launch_bounds__(kBlock)
+k_sweep_csr(int n, const int *__restrict__ rp, const int *__restrict__ col, const T *__restrict__ val,
+            const T *__restrict__ ac, const T *__restrict__ b, const T *x, const int *__restrict__ parent,
+            const T *__restrict__ xc, T alpha, T omega, TO *xn, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int t = xcd_block() * kBlock + threadIdx.x;
+  const int i = t / LPR, lane = t % LPR;
+  const bool ok = i < n;
+  const T s = csr_offdiag<LPR, T, PRO>(ok ? i : 0, lane, rp, col, val, x, parent, xc, alpha, ok);
+  if (ok && lane == 0) {
+    T xi = x[i];
+    if (PRO) xi = xi + alpha * xc[parent[i]];
+    xn[i] = (TO)(xi + omega * (b[i] - ac[i] * xi - s) / ac[i]);
+  }
+}
+
+// Residual of a coarse level summed straight into the next level's
+// right-hand side: b_c[p] = sum over the members i of p (in increasing
+// order) of (b_i - A_i x); the coarse level's first sweep as k_restrict
+// (mode 1: x_c = omega b_c/ac_c, 2: x_c = 0, 0: untouched).  LPR lanes per
+// coarse cell work through each member row together.
+template <int LPR, class T>
+__global__ void __launch_bounds__(kBlock)
+k_resid_restrict_csr(int nc, const int *__restrict__ mptr, const int *__restrict__ mem, const int *__restrict__ rp,
+                     const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ ac,
+                     const T *__restrict__ b, const T *__restrict__ x, T *__restrict__ bc, const T *__restrict__ acc,
+                     T *__restrict__ xc, T omega, int mode, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int t = xcd_block() * kBlock + threadIdx.x;
+  const int p = t / LPR, lane = t % LPR;
+  const bool ok = p < nc;
+  const int q0 = ok ? mptr[p] : 0, q1 = ok ? mptr[p + 1] : 0;
+  T sum = T(0);
+  for (int q = q0; q < q1; ++q) { // the same trip count on the LPR lanes of p
+    const int i = mem[q];
+    const T s = csr_offdiag<LPR, T, false>(i, lane, rp, col, val, x, nullptr, nullptr, T(0), true);
+    sum = sum + (b[i] - ac[i] * x[i] - s);
+  }
+  if (ok && lane == 0) {
+    bc[p] = sum;
+    if (mode == 1) xc[p] = omega * sum / acc[p];
+    else if (mode == 2) xc[p] = T(0);
+  }
 }
 
 // Entry of the cycle on the finest level: b = (T)rhs (when the hierarchy is
@@ -637,6 +710,12 @@ struct DLevel {
   DevBuf<int> gptr;
   DevBuf<long long> gsrc;
   int gne = 0;
+  // levels >= 1: row-major CSR copy of the off-diagonals for the grid-wide
+  // Jacobi kernels (values written by k_galerkin through csr_pos)
+  long long nnz = 0;
+  DevBuf<int> csr_rp, csr_col, csr_pos;
+  DevBuf<double> val64;
+  DevBuf<float> val32;
 };
 
 // The tail of one precision: its first level, descriptor and byte image.
@@ -862,6 +941,23 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
       for (int i = 0; i < f.n; ++i) D.h_mem[fill[f.parent[i] - 1]++] = i;
       D.mptr.upload(D.h_mptr, s);
       D.mem.upload(D.h_mem, s);
+      std::vector<int> rp(h.n + 1, 0), col, pos((size_t)h.n * h.w, -1);
+      for (int i = 0; i < h.n; ++i) {
+        for (int j = 0; j < h.w; ++j) {
+          const int k = nbr0[(size_t)j * h.n + i];
+          if (k < 0) continue;
+          pos[(size_t)j * h.n + i] = (int)col.size();
+          col.push_back(k);
+        }
+        rp[i + 1] = (int)col.size();
+      }
+      D.nnz = (long long)col.size();
+      if (col.empty()) col.push_back(0);
+      D.csr_rp.upload(rp, s);
+      D.csr_col.upload(col, s);
+      D.csr_pos.upload(pos, s);
+      D.val64.alloc(col.size());
+      D.val32.alloc(col.size());
     }
     if (l + 1 < nl) {
       std::vector<int> par0(h.n);
@@ -908,7 +1004,8 @@ void amg_prepare(fs_mesh *m, const double *ac, const double *anb) {
     DLevel &F = H.lev[l - 1], &C = H.lev[l];
     hipLaunchKernelGGL(k_galerkin, dim3(gfor(C.gne)), dim3(kBlock), 0, s, C.gne, C.n, C.gptr.p,
                        C.gsrc.p, F.n, F.ac64, F.anb64, C.ac64_own.p, C.anb64_own.p,
-                       H.fp32 ? C.ac32.p : nullptr, H.fp32 ? C.anb32.p : nullptr);
+                       H.fp32 ? C.ac32.p : nullptr, H.fp32 ? C.anb32.p : nullptr, C.csr_pos.p, C.val64.p,
+                       H.fp32 ? C.val32.p : nullptr);
   }
   // the tail of the precision in use: coefficients and the coarsest inverse
   TailImage &ti = H.fp32 ? H.tail32 : H.tail64;
@@ -945,6 +1042,7 @@ template <class T> struct LV {
   static T *t(DLevel &d);
   static const T *ac(DLevel &d);
   static const T *anb(DLevel &d);
+  static const T *val(DLevel &d);
   static TailImage &tail(Hierarchy &h);
 };
 template <> struct LV<double> {
@@ -954,6 +1052,7 @@ template <> struct LV<double> {
   static double *t(DLevel &d) { return d.t64.p; }
   static const double *ac(DLevel &d) { return d.ac64; }
   static const double *anb(DLevel &d) { return d.anb64; }
+  static const double *val(DLevel &d) { return d.val64.p; }
   static TailImage &tail(Hierarchy &h) { return h.tail64; }
 };
 template <> struct LV<float> {
@@ -963,8 +1062,37 @@ template <> struct LV<float> {
   static float *t(DLevel &d) { return d.t32.p; }
   static const float *ac(DLevel &d) { return d.ac32.p; }
   static const float *anb(DLevel &d) { return d.anb32.p; }
+  static const float *val(DLevel &d) { return d.val32.p; }
   static TailImage &tail(Hierarchy &h) { return h.tail32; }
 };
+
+// lanes per row of a coarse level's CSR kernels by its mean row length
+static int csr_lpr(const DLevel &D) {
+  const double a = D.n ? (double)D.nnz / D.n : 0.0;
+  return a <= 6.0 ? 4 : a <= 16.0 ? 8 : 16;
+}
+
+// residual of coarse level D restricted into C (plus C's first sweep)
+template <class T>
+static void launch_resid_restrict(fs_mesh *m, DLevel &D, DLevel &C, const T *b, const T *x, T om, int mode,
+                                  const int *flags) {
+  hipStream_t s = m->stream;
+  const int *rp = D.csr_rp.p, *col = D.csr_col.p;
+  const T *val = LV<T>::val(D), *ac = LV<T>::ac(D);
+  switch (csr_lpr(D)) {
+  case 4:
+    hipLaunchKernelGGL((k_resid_restrict_csr<4, T>), dim3(gfor(C.n * 4)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p,
+                       rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    break;
+  case 8:
+    hipLaunchKernelGGL((k_resid_restrict_csr<8, T>), dim3(gfor(C.n * 8)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p,
+                       rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    break;
+  default:
+    hipLaunchKernelGGL((k_resid_restrict_csr<16, T>), dim3(gfor(C.n * 16)), dim3(kBlock), 0, s, C.n, C.mptr.p,
+                       C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+  }
+}
 
 // one damped-Jacobi sweep on a grid-wide level, lanes per row by row width;
 // sweeps on the finest level go through the kernel clock (site
@@ -976,6 +1104,24 @@ static void launch_sweep(fs_mesh *m, DLevel &D, int level, const T *b, const T *
                          int nc, T alpha, T om, TO *xn, const int *flags) {
   hipStream_t s = m->stream;
   const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
+  if (level > 0) {
+    const int *rp = D.csr_rp.p, *col = D.csr_col.p;
+    const T *val = LV<T>::val(D);
+    switch (csr_lpr(D)) {
+    case 4:
+      hipLaunchKernelGGL((k_sweep_csr<4, T, TO, PRO>), dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, rp, col, val,
+                         ac, b, x, parent, xc, alpha, om, xn, flags);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_sweep_csr<8, T, TO, PRO>), dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, rp, col, val,
+                         ac, b, x, parent, xc, alpha, om, xn, flags);
+      break;
+    default:
+      hipLaunchKernelGGL((k_sweep_csr<16, T, TO, PRO>), dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, rp, col,
+                         val, ac, b, x, parent, xc, alpha, om, xn, flags);
+    }
+    return;
+  }
   const int site = level == 0 ? FS_CLOCK_FINE_SWEEP : 0;
   double bytes = 0.0;
   if (site)
@@ -1137,14 +1283,18 @@ static void vcycle(fs_mesh *m, const double *rhs, double *out, double alpha, con
     const T *b = l == 0 ? b0 : LV<T>::b(D);
     const int npre = l == 0 ? pre0 : cpre;
     res[l] = pre_smooth<T>(m, D, l, b, npre, om, oth[l], flags);
-    launch_resid<T>(m, D, b, res[l], LV<T>::r(D), flags);
     DLevel &C = H.lev[l + 1];
     int mode;
     if (l + 1 == ts) mode = 0;                                  // the tail kernel takes over
     else if (l + 1 == L - 1) mode = 1;                          // grid-wide coarsest: `coarse` sweeps
     else mode = (!use_gs(m, C) && cpre > 0) ? 1 : 2;
-    hipLaunchKernelGGL(k_restrict<T>, dim3(gfor(C.n)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p, LV<T>::r(D),
-                       LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    if (l > 0 && !use_gs(m, D)) {
+      launch_resid_restrict<T>(m, D, C, b, res[l], om, mode, flags);
+    } else {
+      launch_resid<T>(m, D, b, res[l], LV<T>::r(D), flags);
+      hipLaunchKernelGGL(k_restrict<T>, dim3(gfor(C.n)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p, LV<T>::r(D),
+                         LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    }
   }
   if (ts < L) {
     DLevel &Dt = H.lev[ts];

@@ -23,6 +23,7 @@
 #include "fs_geom.h"
 #include "fs_kernels.h"
 #include "fs_solve.h"
+#include "fs_mpnn.h"
 #include "fs_simple.h"
 
 namespace fs {
@@ -476,11 +477,14 @@ struct SimpleCase {
   DevBuf<double> bu, bv, bw;
   DevBuf<double> u, v, w, p, pc, mdot, gu, gv, gw, gp, gpc, bp, bgp, fg, du, dv, dw, us, vs, ws;
   DevBuf<double> fp, fv, cf, ff, tf, ac, anb, bc, dphi, part;
+  DevBuf<double> pcprev; // the last converged p' (input of the warm-start network)
   double rho = 1.0, mu = 0.01, urf_u = 0.7, urf_p = 0.3;
   int msolver = FS_SOR, mlmg = 0, mitmax = 20, psolver = FS_BICGS, plmg = FS_MG_PRECOND,
       pitmax = 500, refcell = 0;
   double mrrf = 0.1, prrf = 1e-8;
   fs_simple_stats last{};
+  fs_mpnn *warm = nullptr;  // p' initial guess from the network
+  fs_mpnn *sink = nullptr;  // p' snapshots into the network's dataset
 };
 
 static void upload_geometry(SimpleCase &C) {
@@ -585,9 +589,20 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
   FS_HIP(hipMemsetAsync(C.pc.p, 0, sizeof(double) * C.N, C.s)); // InitializePressureCorrection
   fs_solve_stats ss{};
   double res4[4];
+  const bool warm = C.warm != nullptr;
+  if (C.pcprev.n != (size_t)C.N) {
+    C.pcprev.alloc(C.N);
+    FS_HIP(hipMemsetAsync(C.pcprev.p, 0, sizeof(double) * C.N, C.s));
+  }
+  if (warm) {
+    const int rp = mpnn_predict_device(C.warm, C.s, C.bc.p, C.ac.p, C.anb.p, C.vol.p, C.pcprev.p, C.dphi.p);
+    if (rp < 0) return rp;
+  }
   const int rc = solve_device(C.mesh, 4, C.pc.p, C.prrf, C.pitmax, C.psolver, C.plmg, C.ac.p,
-                              C.anb.p, C.bc.p, C.dphi.p, res4, &ss);
+                              C.anb.p, C.bc.p, C.dphi.p, res4, &ss, warm);
   if (rc < 0) return rc;
+  if (C.sink) mpnn_capture(C.sink, C.s, C.bc.p, C.dphi.p, C.ac.p, C.anb.p, C.vol.p, C.pcprev.p);
+  FS_HIP(hipMemcpyAsync(C.pcprev.p, C.dphi.p, sizeof(double) * C.N, hipMemcpyDeviceToDevice, C.s));
   if (st) {
     st->pc_iterations += ss.iterations;
     st->pc_seconds += ss.seconds;
@@ -907,6 +922,32 @@ int fs_case_time_spmv(fs_case *cs, int reps, double *avg_ms) {
   // dotted with its right-hand side (three distinct N-vectors, as in BiCGSTAB)
   return fs::time_spmv(C.mesh, C.ac.p, C.anb.p, C.p.p, C.bc.p, C.dphi.p, reps, avg_ms);
   CASE_GUARD_END
+}
+
+int fs_case_set_warm_start(fs_case *cs, fs_mpnn *net) {
+  if (!cs) {
+    set_error("fs_case_set_warm_start: case is NULL");
+    return -1;
+  }
+  if (net && !fs::mpnn_matches(net, cs->c.N, cs->c.W, cs->c.mesh->h_nbr_dev)) {
+    set_error("fs_case_set_warm_start: the network was built on another mesh topology");
+    return -1;
+  }
+  cs->c.warm = net;
+  return 0;
+}
+
+int fs_case_set_snapshot_sink(fs_case *cs, fs_mpnn *net) {
+  if (!cs) {
+    set_error("fs_case_set_snapshot_sink: case is NULL");
+    return -1;
+  }
+  if (net && !fs::mpnn_matches(net, cs->c.N, cs->c.W, cs->c.mesh->h_nbr_dev)) {
+    set_error("fs_case_set_snapshot_sink: the network was built on another mesh topology");
+    return -1;
+  }
+  cs->c.sink = net;
+  return 0;
 }
 
 int fs_case_dims(fs_case *cs, int *dims) {

@@ -40,9 +40,11 @@ void add_inplace(fs_mesh *m, double *a, const double *b);
 // one-block convergence update from np L1 partials (iteration count, done)
 void launch_fin_conv(fs_mesh *m, int np);
 
+// warm: dphi holds an initial guess (BiCGSTAB and CG drop it for a zero start
+// when its residual exceeds |bc|_1); otherwise dphi starts at zero (:47)
 int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int solver, int lmg,
                  const double *ac, const double *anb, const double *bc, double *dphi, double *res4,
-                 fs_solve_stats *st);
+                 fs_solve_stats *st, bool warm = false);
 
 // AMG (fs_amg.hip).  amg_prepare builds or refreshes the hierarchy for the
 // fine operator (ac, anb): aggregation on the first call and every

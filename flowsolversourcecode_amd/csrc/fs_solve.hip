@@ -35,9 +35,9 @@ namespace fs {
 enum Slot {
   S_IRES = 0, S_FRES, S_RRF, S_BKNUM, S_BKDEN, S_AKDEN, S_BK, S_AK,
   S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_TS, S_TT, S_RV, S_RES_ABS, S_RES_MAX,
-  S_RES_RMS, S_RES_SCALED, S_ACPHI, S_NSLOTS
+  S_RES_RMS, S_RES_SCALED, S_ACPHI, S_GAMMA, S_NSLOTS
 };
-enum Flag { F_DONE = 0, F_ITER = 1, F_ITMAX = 2, F_NFLAGS = 4 };
+enum Flag { F_DONE = 0, F_ITER = 1, F_ITMAX = 2, F_REJECT = 3, F_NFLAGS = 4 };
 
 static inline int grid_for(int n) {
   int g = (n + kBlock - 1) / kBlock;
@@ -136,6 +136,25 @@ __global__ void k_resid_l1(int n, int w, const int *__restrict__ nbr, const doub
 }
 
 // r = b - A*x, also copied to r0; partial L1 of r.
+// x *= gamma (the L2-optimal scaling of an initial guess)
+__global__ void k_warm_scale(int n, double *__restrict__ x, const double *__restrict__ scal,
+                             const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = scal[S_GAMMA] * x[i];
+}
+
+// a rejected initial guess becomes the zero start: x = 0, r = r0 = b
+__global__ void k_warm_fix(int n, double *__restrict__ x, double *__restrict__ r, double *__restrict__ r0,
+                           const double *__restrict__ b, const int *__restrict__ flags) {
+  if (!flags[F_REJECT]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  x[i] = 0.0;
+  r[i] = b[i];
+  if (r0) r0[i] = b[i];
+}
+
 // p = z (first iteration) or z + beta*p
 __global__ void k_cg_p(int n, const double *__restrict__ z, double *__restrict__ p,
                        const double *__restrict__ scal, const int *__restrict__ flags) {
@@ -422,7 +441,9 @@ enum FinOp {
   FIN_BICG_OMEGA,     // omega = (t,s)/(t,t)
   FIN_TRUE_RES,       // fres only
   FIN_CG_RHO,         // rz = (r,z), beta
-  FIN_CG_ALPHA        // alpha = rz/(p,q)
+  FIN_CG_ALPHA,       // alpha = rz/(p,q)
+  FIN_WARM,           // reject an initial guess whose L1 residual exceeds ires
+  FIN_WARM_SCALE      // gamma = (A x0, b)/(A x0, A x0)
 };
 
 __device__ double reduce_partials(const double *part, int np) {
@@ -445,7 +466,7 @@ __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ 
     b = reduce_partials_max(part + kPartials, np);
     c = reduce_partials(part + 2 * kPartials, np);
     d = reduce_partials_max(part + 3 * kPartials, np);
-  } else if (op == FIN_BICG_OMEGA) {
+  } else if (op == FIN_BICG_OMEGA || op == FIN_WARM_SCALE) {
     b = reduce_partials(part + kPartials, np);
   }
   if (threadIdx.x != 0) return;
@@ -520,6 +541,12 @@ __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ 
     if (a == 0.0) flags[F_DONE] = 1;
     break;
   }
+  case FIN_WARM:
+    flags[F_REJECT] = !(a <= scal[S_IRES]);
+    break;
+  case FIN_WARM_SCALE:
+    scal[S_GAMMA] = (b > 0.0 && isfinite(a / b)) ? a / b : 0.0;
+    break;
   case FIN_CG_ALPHA:
     scal[S_RV] = a;
     scal[S_ALPHA] = (a != 0.0) ? scal[S_RHO] / a : 0.0;
@@ -798,9 +825,22 @@ template <class F> static int run_batches(fs_mesh *m, int itmax, F body) {
   return fl[F_ITER];
 }
 
+// x0 *= (A x0, b)/(A x0, A x0): the scaling of an initial guess with the
+// smallest L2 residual (0 when A x0 = 0), so the guess is never worse than
+// a zero start in that norm.  v is scratch.
+static void scale_guess(fs_mesh *m, const double *ac, const double *anb, const double *bc, double *x, double *v) {
+  const int n = m->n, gr = grid_red(n);
+  hipStream_t s = m->stream;
+  hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, m->w, m->d_nbr.p, ac, anb, x, v, bc,
+                     m->partial.p, m->flags.p);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_WARM_SCALE, gr, n, 4, m->partial.p, m->scal.p,
+                     m->flags.p);
+  hipLaunchKernelGGL(k_warm_scale, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, m->scal.p, m->flags.p);
+}
+
 int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int solver, int lmg,
                  const double *ac, const double *anb, const double *bc, double *dphi, double *res4,
-                 fs_solve_stats *st) {
+                 fs_solve_stats *st, bool warm) {
   const int n = m->n, w = m->w;
   hipStream_t s = m->stream;
   auto t0 = std::chrono::steady_clock::now();
@@ -815,7 +855,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
   int h_fl[F_NFLAGS] = {0, 0, itmax, 0};
   FS_HIP(hipMemcpyAsync(scal, h_sc, sizeof(h_sc), hipMemcpyHostToDevice, s));
   FS_HIP(hipMemcpyAsync(flags, h_fl, sizeof(h_fl), hipMemcpyHostToDevice, s));
-  FS_HIP(hipMemsetAsync(dphi, 0, sizeof(double) * n, s)); // dphi=0. (:47)
+  if (!warm) FS_HIP(hipMemsetAsync(dphi, 0, sizeof(double) * n, s)); // dphi=0. (:47)
   hipLaunchKernelGGL(k_init_norms, dim3(gr), dim3(kBlock), 0, s, n, ac, bc, fit, part);
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_INIT_NORMS, gr, n, nf, part, scal, flags);
 
@@ -867,7 +907,12 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
     // algorithmic bytes of y = A x with the (r0, y) partials: ac, nbr, anb, x, y, r0
     const double spmv_bytes = 8.0 * n + 4.0 * n * w + 8.0 * m->nnz + 24.0 * n;
+    if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
     hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, V1, part);
+    if (warm) {
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, V1, bc, flags);
+    }
     FS_HIP(hipMemsetAsync(V3, 0, sizeof(double) * n, s));
     iters = run_batches(m, itmax, [&]() {
       hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V1, V0, nullptr, nullptr, part, flags);
@@ -893,8 +938,13 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
     // y = A p with the (p, y) partials: ac, nbr, anb, p, y
     const double spmv_bytes = 8.0 * n + 4.0 * n * w + 8.0 * m->nnz + 16.0 * n;
+    if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
     hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, nullptr,
                        part);
+    if (warm) {
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, (double *)nullptr, bc, flags);
+    }
     iters = run_batches(m, itmax, [&]() {
       const double *z = V0;
       if (pre) {

@@ -100,6 +100,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=20, help="ELL-product launches of --traffic-probe")
+    ap.add_argument("--no-variants", dest="variants", action="store_false",
+                    help="skip the FP64-plain and learned-warm-start runs")
+    ap.add_argument("--learn-samples", type=int, default=8)
+    ap.add_argument("--learn-steps", type=int, default=160)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="set up the workload, run one iteration, then only the ELL product (--spmv-reps launches) "
                          "and a 1 GiB device copy; for hardware-counter runs")
@@ -112,15 +116,21 @@ def main():
     import flowsolversourcecode_amd as fs
     from flowsolversourcecode_amd import meshgen
 
+    def make_case(fp32, alpha):
+        pm = meshgen.box_hex(args.n, args.n, args.n, perturb=0.15, seed=2026)
+        bu, bv, bw = meshgen.lid_driven_cavity_walls(pm)
+        case = fs.Case(pm, bu, bv, bw, rho=1.0, mu=1e-3)
+        case.configure(urf_u=0.7, urf_p=0.3, msolver="sor", mrrf=0.1, mitmax=10, psolver="bicgs", prrf=1e-8,
+                       pitmax=200, plmg=fs.MG_PRECOND)
+        case.set_option("mg_fp32", fp32)
+        case.set_option("mg_alpha", alpha)
+        return case
+
     t0 = time.perf_counter()
-    pm = meshgen.box_hex(args.n, args.n, args.n, perturb=0.15, seed=2026)
-    bu, bv, bw = meshgen.lid_driven_cavity_walls(pm)
-    case = fs.Case(pm, bu, bv, bw, rho=1.0, mu=1e-3)
-    case.configure(urf_u=0.7, urf_p=0.3, msolver="sor", mrrf=0.1, mitmax=10, psolver="bicgs", prrf=1e-8,
-                   pitmax=200, plmg=fs.MG_PRECOND)
-    case.set_option("mg_fp32", 1 if args.precision == "fp32amg" else 0)
+    case = make_case(1 if args.precision == "fp32amg" else 0, 0.0)
     setup_s = time.perf_counter() - t0
-    print(f"# setup {setup_s:.1f} s: N={case.N} faces={case.F}", file=sys.stderr, flush=True)
+    n_cells, n_faces = case.N, case.F
+    print(f"# setup {setup_s:.1f} s: N={n_cells} faces={n_faces}", file=sys.stderr, flush=True)
 
     if args.traffic_probe:
         traffic_probe(case, args)
@@ -159,6 +169,50 @@ def main():
     roof_sweep = roof(clk["fine_sweep"], "k_sweep<1, float, *> (FP32 damped-Jacobi sweeps of the finest AMG level)",
                       clk["fine_sweep"]["bytes"] / case.N if clk["fine_sweep"]["launches"] else None)
 
+    variants = {}
+    if args.variants and args.precision == "fp32amg":
+        head_fields = {k: case.get(k) for k in ("u", "v", "w", "p")}
+        head_pc = st["pc_iterations"] / args.steps
+        del case
+        # the plain form: FP64 hierarchy and the reference's additive correction (alpha 1)
+        plain = make_case(0, 1.0)
+        plain.iterate(args.warmup)
+        sp = plain.iterate(args.steps)
+        err = {k: float(np.abs(plain.get(k) - head_fields[k]).max() / np.abs(plain.get(k)).max())
+               for k in head_fields}
+        variants["fp64_plain"] = {"value": args.steps / sp["seconds"], "ms_per_step": 1e3 * sp["seconds"] / args.steps,
+                                  "pc_solve_ms": 1e3 * sp["pc_seconds"] / args.steps,
+                                  "pc_iterations_per_solve": sp["pc_iterations"] / args.steps,
+                                  "note": "FP64 AMG hierarchy, additive correction alpha = 1, same BiCGSTAB"}
+        variants["field_error_fp32amg_vs_fp64_plain"] = err
+        print(f"# fp64 plain: {args.steps / sp['seconds']:.2f} it/s, field error {err}", file=sys.stderr, flush=True)
+        del plain
+        # FP32 AMG + learned warm start, trained on the run's own first iterations
+        from flowsolversourcecode_amd import learn
+        lc = make_case(1, 0.0)
+        net = learn.Mpnn.for_case(lc, layers=3, seed=7, capacity=args.learn_samples, use_prev=True)
+        learn.attach(lc, sink=net)
+        t1 = time.perf_counter()
+        lc.iterate(args.learn_samples)
+        learn.attach(lc)
+        hist = net.train(epochs=max(1, args.learn_steps // args.learn_samples), batch=1, lr=3e-3, seed=0)
+        t_train = time.perf_counter() - t1
+        learn.attach(lc, warm=net)
+        lc.iterate(1)
+        sl = lc.iterate(args.steps)
+        variants["fp32amg_learned"] = {
+            "value": args.steps / sl["seconds"], "ms_per_step": 1e3 * sl["seconds"] / args.steps,
+            "pc_solve_ms": 1e3 * sl["pc_seconds"] / args.steps,
+            "pc_iterations_per_solve": sl["pc_iterations"] / args.steps,
+            "pc_iterations_per_solve_zero_start": head_pc,
+            "train": {"samples": args.learn_samples, "steps": len(hist) * args.learn_samples,
+                      "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1], "seconds": t_train},
+            "note": "3-layer MPNN (hidden 64, BF16 MFMA) with the previous p' among its inputs, trained on the "
+                    "run's own first iterations; timed iterations start after those"}
+        print(f"# learned: {args.steps / sl['seconds']:.2f} it/s, p' its {sl['pc_iterations'] / args.steps:.1f} "
+              f"(zero start {head_pc:.1f}), train {t_train:.1f} s", file=sys.stderr, flush=True)
+        del lc, net
+
     cpu = None if args.no_cpu else cpu_baseline(args)
     out = {
         "metric": "SIMPLE iter/s, p' solve ms to 1e-8 resid: FP64 plain vs FP32-AMG+learned",
@@ -175,7 +229,7 @@ def main():
         "data": "synthetic seeded mesh",
         "config": {"workload": f"lid-driven cavity {args.n}^3 hex, +/-15% node perturbation, Re=1000, "
                                f"p' BiCGSTAB+AMG({args.precision}) to 1e-8",
-                   "cells": case.N, "faces": case.F, "parallelism": "single GPU"},
+                   "cells": n_cells, "faces": n_faces, "parallelism": "single GPU"},
         "pc_solve_ms": 1e3 * st["pc_seconds"] / args.steps,
         "pc_iterations_per_solve": st["pc_iterations"] / args.steps,
         "momentum_ms": 1e3 * st["momentum_seconds"] / args.steps,
@@ -183,6 +237,7 @@ def main():
         "mass_imbalance_l1": st["mass_l1"],
         "roofline": roof_spmv,
         "roofline_fine_sweep": roof_sweep,
+        "variants": variants or None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -37,7 +37,10 @@ enum Slot {
   S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_TS, S_TT, S_RV, S_RES_ABS, S_RES_MAX,
   S_RES_RMS, S_RES_SCALED, S_ACPHI, S_GAMMA, S_NSLOTS
 };
-enum Flag { F_DONE = 0, F_ITER = 1, F_ITMAX = 2, F_REJECT = 3, F_NFLAGS = 4 };
+enum Flag { F_DONE = 0, F_ITER = 1, F_ITMAX = 2, F_REJECT = 3, F_INNER = 4, F_GATE = 5, F_NFLAGS = 8 };
+// F_INNER: the GMRES cycle in progress has finished its Arnoldi steps;
+// F_GATE = F_DONE | F_INNER, the flag the cycle's kernels (and the V-cycle
+// they call) test
 
 static inline int grid_for(int n) {
   int g = (n + kBlock - 1) / kBlock;
@@ -415,19 +418,210 @@ __global__ void k_bicg_update(int n, double *__restrict__ x, double *__restrict_
   if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
-// ---- GMRES helpers: w = A*z, Gram-Schmidt dots against V[0..k]
-__global__ void k_gmres_dots(int n, int k, const double *__restrict__ V, size_t ldv,
-                             const double *__restrict__ wv, double *__restrict__ part,
-                             const int *__restrict__ flags) {
-  // part[j*kPartials + block] = (V_j, w) for j = 0..k
-  if (flags[F_DONE]) return;
-  for (int j = 0; j <= k; ++j) {
-    double s = 0.0;
-    const double *vj = V + (size_t)j * ldv;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += vj[i] * wv[i];
-    s = block_sum<kBlock>(s);
-    if (threadIdx.x == 0) part[(size_t)j * kPartials + blockIdx.x] = s;
+// ---- restarted GMRES with right preconditioning (matrixSolverOMP.f90:48 is
+// the reference's GMRES; this one keeps its Krylov basis and Hessenberg
+// matrix on the device).  Classical Gram-Schmidt applied twice, all dot
+// products FP64 over fixed partials.  gm[] holds the small state:
+// H[(m+1) x m] column-major, then cs[m], sn[m], g[m+1], y[m], and the slots
+// below.
+enum GmSlot { GM_BETA = 0, GM_JEND, GM_TOL2, GM_NSLOT };
+inline __host__ __device__ size_t gm_h(int) { return 0; }
+inline __host__ __device__ size_t gm_cs(int m) { return (size_t)(m + 1) * m; }
+inline __host__ __device__ size_t gm_sn(int m) { return gm_cs(m) + m; }
+inline __host__ __device__ size_t gm_g(int m) { return gm_sn(m) + m; }
+inline __host__ __device__ size_t gm_y(int m) { return gm_g(m) + m + 1; }
+inline __host__ __device__ size_t gm_slot(int m) { return gm_y(m) + m; }
+inline __host__ __device__ size_t gm_hpass(int m) { return gm_slot(m) + GM_NSLOT; }
+inline __host__ __device__ size_t gm_size(int m) { return gm_hpass(m) + m + 1; }
+
+__global__ void k_gm_spmv(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+                          const double *__restrict__ anb, const double *__restrict__ x, double *__restrict__ y,
+                          const int *__restrict__ flags) {
+  if (flags[F_GATE]) return;
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i < n) y[i] = ell_ax(i, n, w, nbr, ac, anb, x);
+}
+
+// part[j*kPartials + block] = (V_j, w); blockIdx.y = j
+__global__ void k_gm_dots(int n, const double *__restrict__ V, size_t ldv, const double *__restrict__ wv,
+                          double *__restrict__ part, const int *__restrict__ flags) {
+  if (flags[F_GATE]) return;
+  const double *vj = V + (size_t)blockIdx.y * ldv;
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += vj[i] * wv[i];
+  s = block_sum<kBlock>(s);
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * kPartials + blockIdx.x] = s;
+}
+
+__device__ double reduce_partials(const double *part, int np);
+
+// hpass[j] = (V_j, w) from the partials; H[j][col] += hpass[j]; blockIdx.x = j
+__global__ void k_gm_hfin(int np, int m, int col, const double *__restrict__ part, double *__restrict__ gm,
+                          const int *__restrict__ flags) {
+  if (flags[F_GATE]) return;
+  const int j = blockIdx.x;
+  const double a = reduce_partials(part + (size_t)j * kPartials, np);
+  if (threadIdx.x == 0) {
+    gm[gm_hpass(m) + j] = a;
+    gm[gm_h(m) + (size_t)col * (m + 1) + j] += a;
   }
+}
+
+// w -= sum_{j < nv} hpass[j] V_j
+__global__ void k_gm_maxpy(int n, int nv, int m, const double *__restrict__ V, size_t ldv,
+                           const double *__restrict__ gm, double *__restrict__ wv, const int *__restrict__ flags) {
+  if (flags[F_GATE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double s = wv[i];
+  for (int j = 0; j < nv; ++j) s -= gm[gm_hpass(m) + j] * V[(size_t)j * ldv + i];
+  wv[i] = s;
+}
+
+// partials of (w, w)
+__global__ void k_gm_norm2(int n, const double *__restrict__ wv, double *__restrict__ part,
+                           const int *__restrict__ flags) {
+  if (flags[F_GATE]) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += wv[i] * wv[i];
+  s = block_sum<kBlock>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Column `col` of the Arnoldi relation: H[col+1][col] = ||w||, the previous
+// Givens rotations, a new one, the residual estimate |g[col+1]|; ends the
+// cycle's Arnoldi steps when the estimate meets the L2 target, the basis is
+// full or w vanished.
+__global__ void k_gm_col(int np, int m, int col, const double *__restrict__ part, double *__restrict__ gm,
+                         int *__restrict__ flags) {
+  if (flags[F_GATE]) return;
+  const double hn = sqrt(reduce_partials(part, np));
+  if (threadIdx.x != 0) return;
+  double *H = gm + gm_h(m) + (size_t)col * (m + 1);
+  double *cs = gm + gm_cs(m), *sn = gm + gm_sn(m), *g = gm + gm_g(m);
+  H[col + 1] = hn;
+  for (int i = 0; i < col; ++i) {
+    const double a = H[i], b = H[i + 1];
+    H[i] = cs[i] * a + sn[i] * b;
+    H[i + 1] = -sn[i] * a + cs[i] * b;
+  }
+  const double a = H[col], b = H[col + 1];
+  const double d = sqrt(a * a + b * b);
+  const double c = d > 0.0 ? a / d : 1.0, s = d > 0.0 ? b / d : 0.0;
+  cs[col] = c;
+  sn[col] = s;
+  H[col] = d;
+  H[col + 1] = 0.0;
+  const double gj = g[col];
+  g[col] = c * gj;
+  g[col + 1] = -s * gj;
+  gm[gm_slot(m) + GM_JEND] = col + 1;
+  gm[gm_hpass(m) + m] = hn; // the norm for the scaling of the next basis vector
+  flags[F_ITER] = flags[F_ITER] + 1;
+  if (!(fabs(g[col + 1]) > gm[gm_slot(m) + GM_TOL2]) || col + 1 == m || !(hn > 0.0) ||
+      flags[F_ITER] >= flags[F_ITMAX]) {
+    flags[F_INNER] = 1;
+    flags[F_GATE] = 1;
+  }
+}
+
+// v = w / ||w|| for the next basis vector
+__global__ void k_gm_scale(int n, int m, const double *wv, double *v,
+                           const double *__restrict__ gm, const int *__restrict__ flags) {
+  if (flags[F_GATE]) return;
+  const double inv = 1.0 / gm[gm_hpass(m) + m];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) v[i] = wv[i] * inv;
+}
+
+// y = H^{-1} g by back substitution over the cycle's jend columns
+__global__ void k_gm_solve(int m, double *__restrict__ gm, const int *__restrict__ flags) {
+  if (flags[F_DONE] || threadIdx.x != 0) return;
+  const int je = (int)gm[gm_slot(m) + GM_JEND];
+  const double *H = gm + gm_h(m), *g = gm + gm_g(m);
+  double *y = gm + gm_y(m);
+  for (int i = je - 1; i >= 0; --i) {
+    double s = g[i];
+    for (int k = i + 1; k < je; ++k) s -= H[(size_t)k * (m + 1) + i] * y[k];
+    const double d = H[(size_t)i * (m + 1) + i];
+    y[i] = d != 0.0 ? s / d : 0.0;
+  }
+}
+
+// u = sum_{j < jend} y_j V_j
+__global__ void k_gm_combine(int n, int m, const double *__restrict__ V, size_t ldv, const double *__restrict__ gm,
+                             double *__restrict__ u, const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int je = (int)gm[gm_slot(m) + GM_JEND];
+  double s = 0.0;
+  for (int j = 0; j < je; ++j) s += gm[gm_y(m) + j] * V[(size_t)j * ldv + i];
+  u[i] = s;
+}
+
+__global__ void k_gm_addx(int n, double *__restrict__ x, const double *__restrict__ z, const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = x[i] + z[i];
+}
+
+// r = b - A x with L1 and L2 partials
+__global__ void k_gm_resid(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+                           const double *__restrict__ anb, const double *__restrict__ b,
+                           const double *__restrict__ x, double *__restrict__ r, double *__restrict__ part,
+                           const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double v = b[i] - ell_ax(i, n, w, nbr, ac, anb, x);
+    r[i] = v;
+    s1 += fabs(v);
+    s2 += v * v;
+  }
+  s1 = block_sum<kBlock>(s1);
+  s2 = block_sum<kBlock>(s2);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1;
+    part[kPartials + blockIdx.x] = s2;
+  }
+}
+
+// start of a cycle: the reference's stop rule on the true L1 residual
+// (CheckConvergenceRoutine.for semantics: fres <= rrf * ires), then
+// beta = ||r||_2, g = (beta, 0, ...), H = 0, and the cycle's L2 target: the
+// L2 residual scaled by the L1 reduction still needed, halved for safety
+__global__ void k_gm_restart(int np, int m, const double *__restrict__ part, double *__restrict__ gm,
+                             double *__restrict__ scal, int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const double l1 = reduce_partials(part, np);
+  const double l2 = sqrt(reduce_partials(part + kPartials, np));
+  if (threadIdx.x != 0) return;
+  scal[S_FRES] = l1;
+  const bool done = !(l1 > scal[S_RRF] * scal[S_IRES] && flags[F_ITER] < flags[F_ITMAX]);
+  flags[F_DONE] = done;
+  flags[F_INNER] = 0;
+  flags[F_GATE] = done;
+  for (size_t k = 0; k < gm_cs(m); ++k) gm[gm_h(m) + k] = 0.0;
+  double *g = gm + gm_g(m);
+  g[0] = l2;
+  for (int k = 1; k <= m; ++k) g[k] = 0.0;
+  gm[gm_slot(m) + GM_BETA] = l2;
+  gm[gm_slot(m) + GM_JEND] = 0;
+  gm[gm_slot(m) + GM_TOL2] = l1 > 0.0 ? 0.5 * l2 * scal[S_RRF] * scal[S_IRES] / l1 : 0.0;
+  if (!(l2 > 0.0)) {
+    flags[F_DONE] = 1;
+    flags[F_GATE] = 1;
+  }
+}
+
+// v_0 = r / beta
+__global__ void k_gm_v0(int n, int m, const double *__restrict__ r, double *__restrict__ v,
+                        const double *__restrict__ gm, const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const double inv = 1.0 / gm[gm_slot(m) + GM_BETA];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) v[i] = r[i] * inv;
 }
 
 // ---------------------------------------------------------------- finalisers
@@ -668,6 +862,8 @@ void greedy_colors(int n, int w, const int *nbr0, std::vector<int> &ptr, std::ve
 }
 
 struct SolveWork {
+  int gm_m = 0;
+  DevBuf<double> gm_V, gm_state, gm_part;
   int wt = 1;
   DevBuf<int> tlin, tsrc;
   std::vector<int> color_ptr;
@@ -960,6 +1156,65 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       hipLaunchKernelGGL(k_cg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, V2, V3, scal, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
+  } else if (solver == FS_GMRES) {
+    // restarted GMRES(m), right-preconditioned: basis V (m+1 vectors), z = M^-1 v_j
+    // (V10), u = V y (V11), r (V0); the cycle ends when its L2 estimate meets the
+    // target, the basis is full or itmax is reached; the stop rule is checked on
+    // the true L1 residual at every restart
+    const bool pre = (lmg == FS_MG_PRECOND);
+    if (pre) amg_prepare(m, ac, anb);
+    const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
+    const int gmm = std::max(1, std::min(64, m->gmres_m));
+    if (sw.gm_m != gmm || sw.gm_V.n < (size_t)(gmm + 1) * n) {
+      sw.gm_V.alloc((size_t)(gmm + 1) * n);
+      sw.gm_state.alloc(gm_size(gmm));
+      sw.gm_part.alloc((size_t)(gmm + 1) * kPartials);
+      sw.gm_m = gmm;
+    }
+    double *V = sw.gm_V.p, *gm = sw.gm_state.p, *gp = sw.gm_part.p;
+    double *Z = m->v[10].p, *U = m->v[11].p;
+    const size_t ldv = (size_t)n;
+    auto restart = [&]() {
+      hipLaunchKernelGGL(k_gm_resid, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, part, flags);
+      hipLaunchKernelGGL(k_gm_restart, dim3(1), dim3(kBlock), 0, s, gr, gmm, part, gm, scal, flags);
+      hipLaunchKernelGGL(k_gm_v0, dim3(ge), dim3(kBlock), 0, s, n, gmm, V0, V, gm, flags);
+    };
+    if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
+    restart();
+    int fl[F_NFLAGS];
+    host_flags(m, fl);
+    m->kclock.solve_first = m->kclock.used;
+    while (!fl[F_DONE]) {
+      for (int j = 0; j < gmm; ++j) {
+        const double *vj = V + (size_t)j * ldv;
+        double *wv = V + (size_t)(j + 1) * ldv;
+        const double *zj = vj;
+        if (pre) {
+          amg_apply(m, vj, Z, flags + F_GATE, alpha);
+          zj = Z;
+        }
+        hipLaunchKernelGGL(k_gm_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, zj, wv, flags);
+        for (int pass = 0; pass < 2; ++pass) {
+          hipLaunchKernelGGL(k_gm_dots, dim3(gr, j + 1), dim3(kBlock), 0, s, n, V, ldv, wv, gp, flags);
+          hipLaunchKernelGGL(k_gm_hfin, dim3(j + 1), dim3(kBlock), 0, s, gr, gmm, j, gp, gm, flags);
+          hipLaunchKernelGGL(k_gm_maxpy, dim3(ge), dim3(kBlock), 0, s, n, j + 1, gmm, V, ldv, gm, wv, flags);
+        }
+        hipLaunchKernelGGL(k_gm_norm2, dim3(gr), dim3(kBlock), 0, s, n, wv, gp, flags);
+        hipLaunchKernelGGL(k_gm_col, dim3(1), dim3(kBlock), 0, s, gr, gmm, j, gp, gm, flags);
+        hipLaunchKernelGGL(k_gm_scale, dim3(ge), dim3(kBlock), 0, s, n, gmm, wv, wv, gm, flags);
+      }
+      hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, s, gmm, gm, flags);
+      hipLaunchKernelGGL(k_gm_combine, dim3(ge), dim3(kBlock), 0, s, n, gmm, V, ldv, gm, U, flags);
+      const double *zu = U;
+      if (pre) {
+        amg_apply(m, U, Z, flags, alpha);
+        zu = Z;
+      }
+      hipLaunchKernelGGL(k_gm_addx, dim3(ge), dim3(kBlock), 0, s, n, dphi, zu, flags);
+      restart();
+      host_flags(m, fl);
+    }
+    iters = fl[F_ITER];
   } else if (solver == FS_ILU) {
     set_error("solver 'ilu' (sequential D-ILU sweeps) is not offered on the device; use pbcg, bicgs or sor");
     return -3;

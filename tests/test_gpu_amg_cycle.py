@@ -61,7 +61,7 @@ def _solve(nbr, nnbr, ac, anb, bc, solver, opts, lmg=2):
     return m.solve_equation("pressc", np.zeros(len(ac)), 1e-8, 400, solver, lmg, ac, anb, bc)
 
 
-@pytest.mark.parametrize("solver", ["bicgs", "cg"])
+@pytest.mark.parametrize("solver", ["bicgs", "cg", "gmres"])
 @pytest.mark.parametrize("opts", VARIANTS, ids=_ids)
 def test_neumann_variants_reach_1e8(neumann, solver, opts):
     asym = opts.get("mg_presweep", 2) != opts.get("mg_postsweep", 2) or \
@@ -76,8 +76,9 @@ def test_neumann_variants_reach_1e8(neumann, solver, opts):
     assert st["iterations"] < 60, st
 
 
-@pytest.mark.parametrize("solver", ["bicgs", "cg"])
-@pytest.mark.parametrize("opts", [{}, {"mg_fp32": 1}, {"mg_tail_cells": 0}, {"mg_maxlevels": 3, "mg_fp32": 1}],
+@pytest.mark.parametrize("solver", ["bicgs", "cg", "gmres"])
+@pytest.mark.parametrize("opts", [{}, {"mg_fp32": 1}, {"mg_tail_cells": 0}, {"mg_maxlevels": 3, "mg_fp32": 1},
+                                  {"gmres_restart": 5}],
                          ids=_ids)
 def test_shifted_variants_match_direct(shifted, solver, opts):
     nbr, nnbr, ac, anb, bc, A, x = shifted
@@ -99,11 +100,23 @@ def test_overcorrection_saves_iterations(neumann):
     assert its[1.7] < its[1.0], its
 
 
-def test_unpreconditioned_cg(shifted):
+@pytest.mark.parametrize("solver", ["cg", "gmres"])
+def test_unpreconditioned_krylov(shifted, solver):
     nbr, nnbr, ac, anb, bc, A, x = shifted
-    rc, fit, dphi, res4, st = _solve(nbr, nnbr, ac, anb, bc, "cg", {}, lmg=0)
+    rc, fit, dphi, res4, st = _solve(nbr, nnbr, ac, anb, bc, solver, {"gmres_restart": 40}, lmg=0)
     assert rc == 0, st
     np.testing.assert_allclose(dphi, x, rtol=0, atol=1e-6 * np.abs(x).max())
+
+
+def test_gmres_on_nonsymmetric_momentum_system():
+    """GMRES + AMG on an upwind convection-diffusion (momentum-like) system."""
+    nbr, nnbr, ac, anb, bc = systems.momentum_like(24, 20, 16, 5)
+    A = systems.sparse(nbr, nnbr, ac, anb)
+    x = spl.spsolve(A.tocsc(), bc)
+    for fp32 in (0, 1):
+        rc, fit, dphi, res4, st = _solve(nbr, nnbr, ac, anb, bc, "gmres", {"mg_fp32": fp32})
+        assert rc == 0, st
+        np.testing.assert_allclose(dphi, x, rtol=0, atol=1e-6 * np.abs(x).max())
 
 
 def test_multigrid_as_solver_with_tail(shifted):

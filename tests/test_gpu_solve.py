@@ -93,7 +93,7 @@ def test_amg_maps_and_galerkin_bitwise(case):
 
 @pytest.mark.parametrize("case", ["pshift", "momentum"])
 @pytest.mark.parametrize("solver,lmg,fp32", [("sor", 0, 0), ("bicgs", 0, 0), ("bicgs", 2, 0), ("bicgs", 2, 1),
-                                             ("sor", 1, 0), ("sor", 1, 1)])
+                                             ("sor", 1, 0), ("sor", 1, 1), ("gmres", 0, 0), ("gmres", 2, 1)])
 def test_converged_solution_matches_direct(case, solver, lmg, fp32):
     g = golden(f"solve_{case}.npz")
     m = _mesh(g)

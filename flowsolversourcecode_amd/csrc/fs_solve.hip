@@ -220,13 +220,15 @@ __global__ void k_pbcg_start(int n, const double *__restrict__ bc, double *__res
 }
 
 // bknum partials: sum (r*dc)*rr (PBCGMethod.for:163-170).
+// z = dc*r (the reference's D-ILU asolve) or, with dc null, the
+// preconditioned vector zp given (one AMG V-cycle)
 __global__ void k_pbcg_bknum(int n, const double *__restrict__ r, const double *__restrict__ rr,
-                             const double *__restrict__ dc, double *__restrict__ part,
-                             const int *__restrict__ flags) {
+                             const double *__restrict__ dc, const double *__restrict__ zp,
+                             double *__restrict__ part, const int *__restrict__ flags) {
   if (flags[F_DONE]) return;
   double s = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-    s += (r[i] * dc[i]) * rr[i];
+    s += (dc ? r[i] * dc[i] : zp[i]) * rr[i];
   s = block_sum<kBlock>(s);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -234,13 +236,14 @@ __global__ void k_pbcg_bknum(int n, const double *__restrict__ r, const double *
 // p = bk*p + r*dc, pp = bk*pp + rr*dc; on the first iteration p = r*dc,
 // pp = rr*dc (PBCGMethod.for:172-192).
 __global__ void k_pbcg_p(int n, const double *__restrict__ r, const double *__restrict__ rr,
-                         const double *__restrict__ dc, double *__restrict__ p,
+                         const double *__restrict__ dc, const double *__restrict__ zp,
+                         const double *__restrict__ zzp, double *__restrict__ p,
                          double *__restrict__ pp, const double *__restrict__ scal,
                          const int *__restrict__ flags) {
   if (flags[F_DONE]) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const double z = r[i] * dc[i], zz = rr[i] * dc[i];
+  const double z = dc ? r[i] * dc[i] : zp[i], zz = dc ? rr[i] * dc[i] : zzp[i];
   if (flags[F_ITER] == 0) {
     p[i] = z;
     pp[i] = zz;
@@ -1067,11 +1070,21 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     hipLaunchKernelGGL(k_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, dphi, V0);
     hipLaunchKernelGGL(k_pbcg_start, dim3(ge), dim3(kBlock), 0, s, n, bc, V0, V1);
     hipLaunchKernelGGL(k_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, V0, V1);
-    // PBCG keeps the reference's D-ILU preconditioner in every multigrid mode
+    // with FS_MG_PRECOND the D-ILU asolve is replaced by one AMG V-cycle for
+    // both r and rr (the transpose preconditioner is the cycle itself: exact
+    // for symmetric operators such as p', an approximation otherwise)
+    const bool pre = (lmg == FS_MG_PRECOND);
+    if (pre) amg_prepare(m, ac, anb);
+    const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
+    const double *dc = pre ? nullptr : sw.dc.p;
     iters = run_batches(m, itmax, [&]() {
-      hipLaunchKernelGGL(k_pbcg_bknum, dim3(gr), dim3(kBlock), 0, s, n, V0, V1, sw.dc.p, part, flags);
+      if (pre) {
+        amg_apply(m, V0, V6, flags, alpha);
+        amg_apply(m, V1, V7, flags, alpha);
+      }
+      hipLaunchKernelGGL(k_pbcg_bknum, dim3(gr), dim3(kBlock), 0, s, n, V0, V1, dc, V6, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_PBCG_BK, gr, n, nf, part, scal, flags);
-      hipLaunchKernelGGL(k_pbcg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V1, sw.dc.p, V2, V3, scal, flags);
+      hipLaunchKernelGGL(k_pbcg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V1, dc, V6, V7, V2, V3, scal, flags);
       hipLaunchKernelGGL(k_pbcg_ap, dim3(gr), dim3(kBlock), 0, s, n, w, sw.wt, nbr, sw.tlin.p,
                          sw.tsrc.p, ac, anb, V2, V3, V4, V5, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_PBCG_AK, gr, n, nf, part, scal, flags);

@@ -136,3 +136,17 @@ def test_bad_alpha_rejected():
     m = fs.Mesh(nbr, nnbr)
     with pytest.raises(fs.FsError):
         m.set_option("mg_alpha", 7.0)
+
+
+def test_pbcg_with_amg_preconditioner(shifted):
+    """PBCG with the V-cycle in place of its D-ILU asolve (for r and rr: the
+    cycle is symmetric, as this operator is) reaches the stop rule and the
+    direct solution; the reference's D-ILU PBCG is still far from it after
+    the same iteration budget.  (The reference's minimum-residual shadow
+    rr = A r keeps BiCG well behind BiCGSTAB or CG with the same cycle.)"""
+    nbr, nnbr, ac, anb, bc, A, x = shifted
+    rc, fit, dphi, res4, st = _solve(nbr, nnbr, ac, anb, bc, "pbcg", {}, lmg=2)
+    assert rc == 0, st
+    np.testing.assert_allclose(dphi, x, rtol=0, atol=1e-6 * np.abs(x).max())
+    rc0, fit0, dphi0, res40, st0 = _solve(nbr, nnbr, ac, anb, bc, "pbcg", {}, lmg=0)
+    assert rc0 == 1 and st0["final_l1"] > 1e3 * st["final_l1"], (st, st0)

@@ -32,7 +32,7 @@ void set_error(const std::string &msg);
   } while (0)
 
 constexpr int kBlock = 256;     // threads per block for streaming kernels
-constexpr int kPartials = 1024; // fixed number of reduction partials
+constexpr int kPartials = 8192; // fixed number of reduction partials (one row per thread up to 2.1 M rows)
 
 // Device buffer owning a hipMalloc allocation.
 template <class T> struct DevBuf {

@@ -42,6 +42,8 @@ enum Flag { F_DONE = 0, F_ITER = 1, F_ITMAX = 2, F_REJECT = 3, F_INNER = 4, F_GA
 // F_GATE = F_DONE | F_INNER, the flag the cycle's kernels (and the V-cycle
 // they call) test
 
+constexpr int kFinBlock = 1024; // threads of the one-workgroup finalise kernels
+
 static inline int grid_for(int n) {
   int g = (n + kBlock - 1) / kBlock;
   return g < 1 ? 1 : g;
@@ -106,7 +108,7 @@ __device__ __forceinline__ double ell_atx(int i, int n, int wt, const int *__res
 __global__ void k_init_norms(int n, const double *__restrict__ ac, const double *__restrict__ bc,
                              const double *__restrict__ fit, double *__restrict__ part) {
   double s1 = 0.0, m1 = -1.0, s2 = 0.0, m2 = -1.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double r = fabs(bc[i]);
     s1 += r;
     m1 = fmax(m1, r);
@@ -132,7 +134,7 @@ __global__ void k_resid_l1(int n, int w, const int *__restrict__ nbr, const doub
                            const int *__restrict__ flags) {
   if (flags && flags[F_DONE]) return;
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     s += fabs(ell_res(i, n, w, nbr, ac, anb, b, x));
   s = block_sum<kBlock>(s);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -176,7 +178,7 @@ __global__ void k_cg_update(int n, double *__restrict__ x, double *__restrict__ 
   if (flags[F_DONE]) return;
   const double al = scal[S_ALPHA];
   double a = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     x[i] = x[i] + al * p[i];
     const double ri = r[i] - al * q[i];
     r[i] = ri;
@@ -191,7 +193,7 @@ __global__ void k_resid_vec(int n, int w, const int *__restrict__ nbr, const dou
                             const double *__restrict__ x, double *__restrict__ r,
                             double *__restrict__ r0, double *__restrict__ part) {
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double v = ell_res(i, n, w, nbr, ac, anb, b, x);
     r[i] = v;
     if (r0) r0[i] = v;
@@ -227,7 +229,7 @@ __global__ void k_pbcg_bknum(int n, const double *__restrict__ r, const double *
                              double *__restrict__ part, const int *__restrict__ flags) {
   if (flags[F_DONE]) return;
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     s += (dc ? r[i] * dc[i] : zp[i]) * rr[i];
   s = block_sum<kBlock>(s);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -263,7 +265,7 @@ __global__ void k_pbcg_ap(int n, int w, int wt, const int *__restrict__ nbr,
                           double *__restrict__ part, const int *__restrict__ flags) {
   if (flags[F_DONE]) return;
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double zi = ell_ax(i, n, w, nbr, ac, anb, p);
     z[i] = zi;
     zz[i] = ell_atx(i, n, wt, tlin, tsrc, ac, anb, pp);
@@ -329,7 +331,7 @@ __global__ void k_dot2(int n, const double *__restrict__ a, const double *__rest
                        double *__restrict__ part, const int *__restrict__ flags) {
   if (flags && flags[F_DONE]) return;
   double s0 = 0.0, s1 = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     s0 += a[i] * b[i];
     if (c) s1 += c[i] * d[i];
   }
@@ -362,7 +364,7 @@ __global__ void k_spmv_dot(int n, int w, const int *__restrict__ nbr, const doub
                            double *__restrict__ part, const int *__restrict__ flags) {
   if (flags[F_DONE]) return;
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double yi = ell_ax(i, n, w, nbr, ac, anb, x);
     y[i] = yi;
     s += yi * d[i];
@@ -388,7 +390,7 @@ __global__ void k_spmv_dot_ts(int n, int w, const int *__restrict__ nbr,
                               const int *__restrict__ flags) {
   if (flags[F_DONE]) return;
   double a0 = 0.0, a1 = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double ti = ell_ax(i, n, w, nbr, ac, anb, x);
     t[i] = ti;
     a0 += ti * s[i];
@@ -411,7 +413,7 @@ __global__ void k_bicg_update(int n, double *__restrict__ x, double *__restrict_
   if (flags[F_DONE]) return;
   const double al = scal[S_ALPHA], om = scal[S_OMEGA];
   double a = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     x[i] = x[i] + al * phat[i] + om * shat[i];
     const double ri = s[i] - om * t[i];
     r[i] = ri;
@@ -451,7 +453,7 @@ __global__ void k_gm_dots(int n, const double *__restrict__ V, size_t ldv, const
   if (flags[F_GATE]) return;
   const double *vj = V + (size_t)blockIdx.y * ldv;
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += vj[i] * wv[i];
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += vj[i] * wv[i];
   s = block_sum<kBlock>(s);
   if (threadIdx.x == 0) part[(size_t)blockIdx.y * kPartials + blockIdx.x] = s;
 }
@@ -486,7 +488,7 @@ __global__ void k_gm_norm2(int n, const double *__restrict__ wv, double *__restr
                            const int *__restrict__ flags) {
   if (flags[F_GATE]) return;
   double s = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += wv[i] * wv[i];
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += wv[i] * wv[i];
   s = block_sum<kBlock>(s);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -576,7 +578,7 @@ __global__ void k_gm_resid(int n, int w, const int *__restrict__ nbr, const doub
                            const int *__restrict__ flags) {
   if (flags[F_DONE]) return;
   double s1 = 0.0, s2 = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double v = b[i] - ell_ax(i, n, w, nbr, ac, anb, x);
     r[i] = v;
     s1 += fabs(v);
@@ -643,15 +645,16 @@ enum FinOp {
   FIN_WARM_SCALE      // gamma = (A x0, b)/(A x0, A x0)
 };
 
+// one workgroup of kFinBlock threads combines up to kPartials partials
 __device__ double reduce_partials(const double *part, int np) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
-  return block_sum<kBlock>(s);
+  for (int i = threadIdx.x; i < np; i += kFinBlock) s += part[i];
+  return block_sum<kFinBlock>(s);
 }
 __device__ double reduce_partials_max(const double *part, int np) {
   double s = -1.0;
-  for (int i = threadIdx.x; i < np; i += kBlock) s = fmax(s, part[i]);
-  return block_max<kBlock>(s);
+  for (int i = threadIdx.x; i < np; i += kFinBlock) s = fmax(s, part[i]);
+  return block_max<kFinBlock>(s);
 }
 
 __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ part,
@@ -1032,7 +1035,7 @@ static void scale_guess(fs_mesh *m, const double *ac, const double *anb, const d
   hipStream_t s = m->stream;
   hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, m->w, m->d_nbr.p, ac, anb, x, v, bc,
                      m->partial.p, m->flags.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_WARM_SCALE, gr, n, 4, m->partial.p, m->scal.p,
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM_SCALE, gr, n, 4, m->partial.p, m->scal.p,
                      m->flags.p);
   hipLaunchKernelGGL(k_warm_scale, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, m->scal.p, m->flags.p);
 }
@@ -1056,7 +1059,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
   FS_HIP(hipMemcpyAsync(flags, h_fl, sizeof(h_fl), hipMemcpyHostToDevice, s));
   if (!warm) FS_HIP(hipMemsetAsync(dphi, 0, sizeof(double) * n, s)); // dphi=0. (:47)
   hipLaunchKernelGGL(k_init_norms, dim3(gr), dim3(kBlock), 0, s, n, ac, bc, fit, part);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_INIT_NORMS, gr, n, nf, part, scal, flags);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_INIT_NORMS, gr, n, nf, part, scal, flags);
 
   int iters = 0;
   double *V0 = m->v[0].p, *V1 = m->v[1].p, *V2 = m->v[2].p, *V3 = m->v[3].p, *V4 = m->v[4].p,
@@ -1083,14 +1086,14 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
         amg_apply(m, V1, V7, flags, alpha);
       }
       hipLaunchKernelGGL(k_pbcg_bknum, dim3(gr), dim3(kBlock), 0, s, n, V0, V1, dc, V6, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_PBCG_BK, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_PBCG_BK, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_pbcg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V1, dc, V6, V7, V2, V3, scal, flags);
       hipLaunchKernelGGL(k_pbcg_ap, dim3(gr), dim3(kBlock), 0, s, n, w, sw.wt, nbr, sw.tlin.p,
                          sw.tsrc.p, ac, anb, V2, V3, V4, V5, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_PBCG_AK, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_PBCG_AK, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_pbcg_axpy, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, V1, V2, V4, V5, scal, flags);
       hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
   } else if (solver == FS_SOR) {
     // multicolour symmetric Gauss-Seidel: colours forward then backward
@@ -1107,7 +1110,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
                            sw.color_cells.p + b, n, w, nbr, ac, anb, bc, dphi, flags);
       }
       hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
   } else if (solver == FS_BICGS) {
     // right-preconditioned BiCGSTAB; r=V0 r0=V1 p=V2 v=V3 s=V4 t=V5 phat=V6 shat=V7
@@ -1119,26 +1122,26 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
     hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, V1, part);
     if (warm) {
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, V1, bc, flags);
     }
     FS_HIP(hipMemsetAsync(V3, 0, sizeof(double) * n, s));
     iters = run_batches(m, itmax, [&]() {
       hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V1, V0, nullptr, nullptr, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_BICG_RHO, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_RHO, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V2, scal, flags);
       if (pre) amg_apply(m, V2, V6, flags, alpha);
       const double *ph = pre ? V6 : V2;
       timed_launch(m, FS_CLOCK_SPMV, spmv_bytes, k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, ph,
                    V3, (const double *)V1, part, (const int *)flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_BICG_ALPHA, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_ALPHA, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_bicg_s, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V4, scal, flags);
       if (pre) amg_apply(m, V4, V7, flags, alpha);
       const double *sh = pre ? V7 : V4;
       hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, sh, V5, V4, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_BICG_OMEGA, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_OMEGA, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, ph, sh, V4, V5, scal, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
   } else if (solver == FS_CG) {
     // preconditioned conjugate gradient; r=V0 z=V1 p=V2 q=V3
@@ -1151,7 +1154,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, nullptr,
                        part);
     if (warm) {
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, (double *)nullptr, bc, flags);
     }
     iters = run_batches(m, itmax, [&]() {
@@ -1161,13 +1164,13 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
         z = V1;
       }
       hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V0, z, nullptr, nullptr, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CG_RHO, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CG_RHO, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_cg_p, dim3(ge), dim3(kBlock), 0, s, n, z, V2, scal, flags);
       timed_launch(m, FS_CLOCK_SPMV, spmv_bytes, k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, V2,
                    V3, (const double *)V2, part, (const int *)flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CG_ALPHA, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CG_ALPHA, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_cg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, V2, V3, scal, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
   } else if (solver == FS_GMRES) {
     // restarted GMRES(m), right-preconditioned: basis V (m+1 vectors), z = M^-1 v_j
@@ -1189,7 +1192,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     const size_t ldv = (size_t)n;
     auto restart = [&]() {
       hipLaunchKernelGGL(k_gm_resid, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, part, flags);
-      hipLaunchKernelGGL(k_gm_restart, dim3(1), dim3(kBlock), 0, s, gr, gmm, part, gm, scal, flags);
+      hipLaunchKernelGGL(k_gm_restart, dim3(1), dim3(kFinBlock), 0, s, gr, gmm, part, gm, scal, flags);
       hipLaunchKernelGGL(k_gm_v0, dim3(ge), dim3(kBlock), 0, s, n, gmm, V0, V, gm, flags);
     };
     if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
@@ -1209,11 +1212,11 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
         hipLaunchKernelGGL(k_gm_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, zj, wv, flags);
         for (int pass = 0; pass < 2; ++pass) {
           hipLaunchKernelGGL(k_gm_dots, dim3(gr, j + 1), dim3(kBlock), 0, s, n, V, ldv, wv, gp, flags);
-          hipLaunchKernelGGL(k_gm_hfin, dim3(j + 1), dim3(kBlock), 0, s, gr, gmm, j, gp, gm, flags);
+          hipLaunchKernelGGL(k_gm_hfin, dim3(j + 1), dim3(kFinBlock), 0, s, gr, gmm, j, gp, gm, flags);
           hipLaunchKernelGGL(k_gm_maxpy, dim3(ge), dim3(kBlock), 0, s, n, j + 1, gmm, V, ldv, gm, wv, flags);
         }
         hipLaunchKernelGGL(k_gm_norm2, dim3(gr), dim3(kBlock), 0, s, n, wv, gp, flags);
-        hipLaunchKernelGGL(k_gm_col, dim3(1), dim3(kBlock), 0, s, gr, gmm, j, gp, gm, flags);
+        hipLaunchKernelGGL(k_gm_col, dim3(1), dim3(kFinBlock), 0, s, gr, gmm, j, gp, gm, flags);
         hipLaunchKernelGGL(k_gm_scale, dim3(ge), dim3(kBlock), 0, s, n, gmm, wv, wv, gm, flags);
       }
       hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, s, gmm, gm, flags);
@@ -1243,7 +1246,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
   int fl[F_NFLAGS];
   host_flags(m, fl);
   hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, nullptr);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, s, (int)FIN_TRUE_RES, gr, n, nf, part, scal, flags);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_TRUE_RES, gr, n, nf, part, scal, flags);
   add_inplace(m, fit, dphi);
   double h_out[S_NSLOTS];
   FS_HIP(hipMemcpyAsync(h_out, scal, sizeof(h_out), hipMemcpyDeviceToHost, s));
@@ -1276,7 +1279,7 @@ __global__ void k_add(int n, double *__restrict__ a, const double *__restrict__ 
 }
 
 void launch_fin_conv(fs_mesh *m, int np) {
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, m->stream, (int)FIN_CONV, np, m->n, 4,
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, m->stream, (int)FIN_CONV, np, m->n, 4,
                      m->partial.p, m->scal.p, m->flags.p);
 }
 

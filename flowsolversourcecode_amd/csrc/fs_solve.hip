@@ -423,6 +423,54 @@ __global__ void k_bicg_update(int n, double *__restrict__ x, double *__restrict_
   if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
+// ---- the reference's ILU iteration (ILUMethod.for:4): rc = bc - A dphi with
+// the diagonal first, then level-scheduled forward and backward substitution
+// with the operation order of the sequential loops
+__global__ void k_ilu_resid(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+                            const double *__restrict__ anb, const double *__restrict__ bc,
+                            const double *__restrict__ dphi, double *__restrict__ rc, const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double r = -ac[i] * dphi[i] + bc[i];
+  for (int j = 0; j < w; ++j) {
+    const int k = nbr[(size_t)j * n + i];
+    if (k >= 0) r = r - anb[(size_t)j * n + i] * dphi[k];
+  }
+  rc[i] = r;
+}
+
+__global__ void k_ilu_fwd(int cnt, const int *__restrict__ cells, const int *__restrict__ ptr,
+                          const int *__restrict__ lin, const int *__restrict__ src, const double *__restrict__ anb,
+                          const double *__restrict__ dc, double *rc, const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= cnt) return;
+  const int j = cells[t];
+  double v = rc[j];
+  for (int q = ptr[j]; q < ptr[j + 1]; ++q) {
+    const int i = src[q];
+    const double p1 = dc[i] * rc[i];
+    v = v - anb[lin[q]] * p1;
+  }
+  rc[j] = v;
+}
+
+__global__ void k_ilu_bwd(int cnt, const int *__restrict__ cells, const int *__restrict__ ptr,
+                          const int *__restrict__ lin, const int *__restrict__ src, const double *__restrict__ anb,
+                          const double *__restrict__ dc, double *rc, double *__restrict__ dphi,
+                          const int *__restrict__ flags) {
+  if (flags[F_DONE]) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = cells[t];
+  double v = rc[i];
+  for (int q = ptr[i]; q < ptr[i + 1]; ++q) v = v - anb[lin[q]] * rc[src[q]];
+  const double p1 = dc[i] * v;
+  rc[i] = p1;
+  dphi[i] = dphi[i] + p1;
+}
+
 // ---- restarted GMRES with right preconditioning (matrixSolverOMP.f90:48 is
 // the reference's GMRES; this one keeps its Krylov basis and Hessenberg
 // matrix on the device).  Classical Gram-Schmidt applied twice, all dot
@@ -867,7 +915,17 @@ void greedy_colors(int n, int w, const int *nbr0, std::vector<int> &ptr, std::ve
   for (int i = 0; i < n; ++i) cells[fill[col[i]]++] = i;
 }
 
+// Level-scheduled triangular sweeps of the reference's ILU iteration
+// (ILUMethod.for:45-140): per cell the contributions in the reference's
+// order, cells grouped in dependency levels.
+struct IluSweep {
+  bool ready = false;
+  std::vector<int> fwd_level_ptr, bwd_level_ptr;
+  DevBuf<int> fwd_cells, fwd_ptr, fwd_lin, fwd_src, bwd_cells, bwd_ptr, bwd_lin, bwd_src;
+};
+
 struct SolveWork {
+  IluSweep ilu;
   int gm_m = 0;
   DevBuf<double> gm_V, gm_state, gm_part;
   int wt = 1;
@@ -880,6 +938,82 @@ struct SolveWork {
 static SolveWork &work(fs_mesh *m) {
   static_assert(sizeof(SolveWork) > 0, "");
   return *reinterpret_cast<SolveWork *>(m->solve_work);
+}
+
+// Lists of the forward substitution (ILUMethod.for:45-93: rc(j) -=
+// anb(j, slot of i) * dc(i) * rc(i) for the lower neighbours i of j, i
+// increasing) and of the backward one (:97-138: rc(i) -= anb(i, j2) * rc(j)
+// for the upper neighbours j of i in slot order), with their levels.
+static void levels_of(int n, const std::vector<int> &ptr, const std::vector<int> &src, bool up,
+                      std::vector<int> &level_ptr, std::vector<int> &cells) {
+  std::vector<int> level(n, 0);
+  int maxl = 0;
+  for (int q = 0; q < n; ++q) {
+    const int i = up ? n - 1 - q : q; // dependencies are processed first
+    int l = 0;
+    for (int t = ptr[i]; t < ptr[i + 1]; ++t) l = std::max(l, level[src[t]] + 1);
+    level[i] = l;
+    maxl = std::max(maxl, l);
+  }
+  level_ptr.assign(maxl + 2, 0);
+  for (int i = 0; i < n; ++i) level_ptr[level[i] + 1]++;
+  for (int l = 0; l <= maxl; ++l) level_ptr[l + 1] += level_ptr[l];
+  cells.assign(n, 0);
+  std::vector<int> fill(level_ptr.begin(), level_ptr.end() - 1);
+  for (int i = 0; i < n; ++i) cells[fill[level[i]]++] = i;
+}
+
+static void precompute_ilu_sweep(fs_mesh *m, IluSweep &S) {
+  const int n = m->n;
+  const std::vector<int> &nb = m->h_nbr, &nn = m->h_nnbr;
+  std::vector<std::vector<int>> flin(n), fsrc(n);
+  for (int i1 = 1; i1 <= n; ++i1) {
+    for (int j2 = 1; j2 <= nn[i1 - 1]; ++j2) {
+      const int j1 = nb[(size_t)(j2 - 1) * n + (i1 - 1)];
+      if (!(j1 != 0 && j1 > i1 && j1 <= n)) continue;
+      const int j3 = nn[j1 - 1];
+      int i2 = 0, k = 0;
+      while (i2 + 1 <= j3 && k != i1) {
+        i2 = i2 + 1;
+        k = nb[(size_t)(i2 - 1) * n + (j1 - 1)];
+      }
+      if (k != 0 && k == i1) {
+        flin[j1 - 1].push_back((i2 - 1) * n + (j1 - 1));
+        fsrc[j1 - 1].push_back(i1 - 1);
+      }
+    }
+  }
+  std::vector<int> fptr(n + 1, 0), fl, fs, bptr(n + 1, 0), bl, bs;
+  for (int j = 0; j < n; ++j) {
+    fptr[j + 1] = fptr[j] + (int)flin[j].size();
+    fl.insert(fl.end(), flin[j].begin(), flin[j].end());
+    fs.insert(fs.end(), fsrc[j].begin(), fsrc[j].end());
+  }
+  for (int i1 = 1; i1 <= n; ++i1) {
+    if (i1 < n)
+      for (int j2 = 1; j2 <= nn[i1 - 1]; ++j2) {
+        const int j = nb[(size_t)(j2 - 1) * n + (i1 - 1)];
+        if (j > i1) {
+          bl.push_back((j2 - 1) * n + (i1 - 1));
+          bs.push_back(j - 1);
+        }
+      }
+    bptr[i1] = (int)bl.size();
+  }
+  std::vector<int> fcells, bcells;
+  levels_of(n, fptr, fs, false, S.fwd_level_ptr, fcells);
+  levels_of(n, bptr, bs, true, S.bwd_level_ptr, bcells);
+  for (auto *v : {&fl, &fs, &bl, &bs})
+    if (v->empty()) v->push_back(0);
+  S.fwd_cells.upload(fcells, m->stream);
+  S.fwd_ptr.upload(fptr, m->stream);
+  S.fwd_lin.upload(fl, m->stream);
+  S.fwd_src.upload(fs, m->stream);
+  S.bwd_cells.upload(bcells, m->stream);
+  S.bwd_ptr.upload(bptr, m->stream);
+  S.bwd_lin.upload(bl, m->stream);
+  S.bwd_src.upload(bs, m->stream);
+  S.ready = true;
 }
 
 int mesh_prepare_solver(fs_mesh *m) {
@@ -1232,8 +1366,29 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     }
     iters = fl[F_ITER];
   } else if (solver == FS_ILU) {
-    set_error("solver 'ilu' (sequential D-ILU sweeps) is not offered on the device; use pbcg, bicgs or sor");
-    return -3;
+    // the reference's ILU iteration: exact D-ILU diagonal, then per iteration
+    // the residual and the two level-scheduled substitutions
+    if (!sw.ilu.ready) precompute_ilu_sweep(m, sw.ilu);
+    ilu_factor(m, ac, anb, sw.dc.p);
+    IluSweep &S = sw.ilu;
+    double *rcv = m->v[0].p;
+    iters = run_batches(m, itmax, [&]() {
+      hipLaunchKernelGGL(k_ilu_resid, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, rcv, flags);
+      for (size_t l = 0; l + 1 < S.fwd_level_ptr.size(); ++l) {
+        const int b0 = S.fwd_level_ptr[l], b1 = S.fwd_level_ptr[l + 1];
+        if (b1 > b0)
+          hipLaunchKernelGGL(k_ilu_fwd, dim3(grid_for(b1 - b0)), dim3(kBlock), 0, s, b1 - b0, S.fwd_cells.p + b0,
+                             S.fwd_ptr.p, S.fwd_lin.p, S.fwd_src.p, anb, sw.dc.p, rcv, flags);
+      }
+      for (size_t l = 0; l + 1 < S.bwd_level_ptr.size(); ++l) {
+        const int b0 = S.bwd_level_ptr[l], b1 = S.bwd_level_ptr[l + 1];
+        if (b1 > b0)
+          hipLaunchKernelGGL(k_ilu_bwd, dim3(grid_for(b1 - b0)), dim3(kBlock), 0, s, b1 - b0, S.bwd_cells.p + b0,
+                             S.bwd_ptr.p, S.bwd_lin.p, S.bwd_src.p, anb, sw.dc.p, rcv, dphi, flags);
+      }
+      hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+    });
   } else {
     set_error("unsupported solver code " + std::to_string(solver));
     return -3;

@@ -59,6 +59,26 @@ def test_pbcg_follows_reference_history(case):
 
 
 @pytest.mark.parametrize("case", CASES + ["pbig"])
+def test_ilu_iterations_bitwise(case):
+    """The reference's ILU iteration (exact D-ILU diagonal, level-scheduled
+    substitutions in the sequential loops' operation order): dphi after a
+    fixed number of iterations is bitwise the oracle's, which is bitwise the
+    reference build's (golden ILU solution below)."""
+    g = golden(f"solve_{case}.npz")
+    m = _mesh(g)
+    for k in (1, 3, 7):
+        rc, fit, dphi, res4, st = m.solve_equation("pressc", g["fit"], 0.0, k, "ilu", 0, g["ac"], g["anb"], g["bc"])
+        assert st["iterations"] == k
+        fo, dphi_o, _, _ = po.solve_equation(g["nbr"], g["nnbr"], g["ac"], g["anb"], g["bc"], g["fit"], 0.0, k, po.ILU)
+        assert np.array_equal(dphi, dphi_o), (k, np.abs(dphi - dphi_o).max())
+    if "ilu_dphi" in g:
+        rc, fit, dphi, res4, st = m.solve_equation("pressc", g["fit"], float(g["rrf"]), len(g["ilu_hist"]), "ilu", 0,
+                                                   g["ac"], g["anb"], g["bc"])
+        assert st["iterations"] == len(g["ilu_hist"])
+        assert np.array_equal(dphi, g["ilu_dphi"])
+
+
+@pytest.mark.parametrize("case", CASES + ["pbig"])
 def test_amg_maps_and_galerkin_bitwise(case):
     g = golden(f"solve_{case}.npz")
     m = _mesh(g)

@@ -104,6 +104,35 @@ def _clock_read(mesh_handle, site):
     return {"launches": int(out[0]), "avg_ms": float(out[1]), "bytes": float(out[2]), "untimed": int(out[3])}
 
 
+def read_polymesh(directory):
+    """Reads an OpenFOAM polyMesh directory (ASCII) with the library's reader
+    (fs_polymesh_read) into a meshgen.PolyMesh."""
+    from .meshgen import PolyMesh
+    L = lib()
+    L.fs_polymesh_read.argtypes = [C.c_char_p, _ip, C.POINTER(C.c_void_p)]
+    L.fs_polymesh_get.argtypes = [C.c_void_p, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _ip, _ip, C.c_char_p, C.c_int]
+    L.fs_polymesh_destroy.argtypes = [C.c_void_p]
+    counts = np.zeros(5, np.int32)
+    h = C.c_void_p()
+    _check(L.fs_polymesh_read(str(directory).encode(), _iptr(counts), C.byref(h)), "fs_polymesh_read")
+    try:
+        npts, nf, nfn, nint, npat = (int(v) for v in counts)
+        x, y, z = (np.zeros(npts) for _ in range(3))
+        fp = np.zeros(nf + 1, np.int32)
+        fn = np.zeros(nfn, np.int32)
+        ow = np.zeros(nf, np.int32)
+        nb = np.zeros(nint, np.int32)
+        ps = np.zeros(npat, np.int32)
+        pn = np.zeros(npat, np.int32)
+        names = C.create_string_buffer(64 * max(npat, 1))
+        _check(L.fs_polymesh_get(h, _dptr(x), _dptr(y), _dptr(z), _iptr(fp), _iptr(fn), _iptr(ow), _iptr(nb),
+                                 _iptr(ps), _iptr(pn), names, 64), "fs_polymesh_get")
+    finally:
+        L.fs_polymesh_destroy(h)
+    patch_names = [names.raw[64 * k:64 * (k + 1)].split(b"\0")[0].decode() for k in range(npat)]
+    return PolyMesh(np.stack([x, y, z], 1), fp, fn, ow, nb, patch_names, pn, None)
+
+
 def time_device_copy(nbytes, reps=3):
     """Average ms of one device-to-device copy of nbytes (streaming kernel)."""
     ms = np.zeros(1)

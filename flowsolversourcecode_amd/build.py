@@ -10,7 +10,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfsamd.so")
-SOURCES = ["fs_api.hip", "fs_solve.hip", "fs_amg.hip", "fs_mesh.hip", "fs_assemble.hip", "fs_simple.hip", "fs_probe.hip", "fs_mpnn.hip"]
+SOURCES = ["fs_api.hip", "fs_solve.hip", "fs_amg.hip", "fs_mesh.hip", "fs_assemble.hip", "fs_simple.hip", "fs_probe.hip", "fs_mpnn.hip", "fs_polymesh.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

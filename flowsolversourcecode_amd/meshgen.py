@@ -133,3 +133,43 @@ def lid_driven_cavity_walls(mesh, lid="ymax", lid_velocity=(1.0, 0.0, 0.0)):
     e -= mesh.n_internal
     bu[s:e], bv[s:e], bw[s:e] = lid_velocity
     return bu, bv, bw
+
+
+def write_polymesh(mesh, directory):
+    """Writes the mesh as an OpenFOAM polyMesh directory in ASCII (the files
+    the reference's Readit reads: points, faces, owner, neighbour, boundary)."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+
+    def header(f, cls, obj):
+        f.write("FoamFile\n{\n    version     2.0;\n    format      ascii;\n"
+                f"    class       {cls};\n    object      {obj};\n}}\n\n")
+
+    with open(os.path.join(directory, "points"), "w") as f:
+        header(f, "vectorField", "points")
+        f.write(f"{len(mesh.points)}\n(\n")
+        for p in mesh.points:
+            f.write(f"({float(p[0])!r} {float(p[1])!r} {float(p[2])!r})\n")
+        f.write(")\n")
+    with open(os.path.join(directory, "faces"), "w") as f:
+        header(f, "faceList", "faces")
+        f.write(f"{mesh.n_faces}\n(\n")
+        for k in range(mesh.n_faces):
+            nodes = mesh.face_nodes[mesh.face_ptr[k]:mesh.face_ptr[k + 1]]
+            f.write(f"{len(nodes)}(" + " ".join(str(int(v)) for v in nodes) + ")\n")
+        f.write(")\n")
+    for name, arr in (("owner", mesh.owner), ("neighbour", mesh.neighbour)):
+        with open(os.path.join(directory, name), "w") as f:
+            header(f, "labelList", name)
+            f.write(f"{len(arr)}\n(\n")
+            f.write("\n".join(str(int(v)) for v in arr))
+            f.write("\n)\n")
+    with open(os.path.join(directory, "boundary"), "w") as f:
+        header(f, "polyBoundaryMesh", "boundary")
+        f.write(f"{len(mesh.patch_names)}\n(\n")
+        start = mesh.n_internal
+        for name, size in zip(mesh.patch_names, mesh.patch_sizes):
+            f.write(f"    {name}\n    {{\n        type            wall;\n        inGroups        List<word> 1(wall);\n"
+                    f"        nFaces          {int(size)};\n        startFace       {start};\n    }}\n")
+            start += int(size)
+        f.write(")\n")

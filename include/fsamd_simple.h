@@ -120,6 +120,19 @@ int fs_geom_dims(fs_geom *g, int *dims);
 int fs_geom_get(fs_geom *g, const char *name, double *out, long long count);
 int fs_geom_topology(fs_geom *g, int which, int *out, long long count);
 
+/* An OpenFOAM polyMesh directory (ASCII points, faces, owner, neighbour,
+ * boundary), the on-disk input of the reference's Readit
+ * (GridAndGeometry/ReadFoamMesh.f90:31-230).  counts = {points, faces,
+ * face-node entries, interior faces, patches}.  fs_polymesh_get copies the
+ * arrays in the layout fs_case_create takes (0-based; face_ptr has
+ * faces + 1 entries) and the patch names, name_len bytes each. */
+typedef struct fs_polymesh fs_polymesh;
+int fs_polymesh_read(const char *dir, int *counts, fs_polymesh **out);
+int fs_polymesh_get(fs_polymesh *m, double *x, double *y, double *z, int *face_ptr, int *face_nodes,
+                    int *owner, int *neighbour, int *patch_start, int *patch_nfaces, char *patch_names,
+                    int name_len);
+int fs_polymesh_destroy(fs_polymesh *m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -266,7 +266,7 @@ class SimpleStats(C.Structure):
 
 
 PHASES = {"momentum_gradients": 0, "assemble_u": 1, "assemble_v": 2, "assemble_w": 3, "solve_u": 4,
-          "solve_v": 5, "solve_w": 6, "pc_assemble": 7, "pc_solve_correct": 8}
+          "solve_v": 5, "solve_w": 6, "pc_assemble": 7, "pc_solve_correct": 8, "update_symmetry": 9}
 TOPOLOGY = {"nef": 0, "gef": 1, "enbr": 2, "iown": 3, "inbr": 4}
 
 
@@ -299,6 +299,7 @@ def _setup_case_symbols(L):
     L.fs_case_set.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_longlong]
     L.fs_case_dims.argtypes = [C.c_void_p, _ip]
     L.fs_case_topology.argtypes = [C.c_void_p, C.c_int, _ip, C.c_longlong]
+    L.fs_case_set_patch_types.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
     L.fs_case_time_spmv.argtypes = [C.c_void_p, C.c_int, _dp]
     L._case_ready = True
 
@@ -336,8 +337,12 @@ class Geometry:
         return out.reshape(self.W, self.N) if name in ("gef", "enbr") else out
 
 
+BC_TYPES = {"wall": 0, "symmetry": 1}
+
+
 class Case:
-    """Device-resident SIMPLE case (fs_case_*) on a polyMesh with no-slip walls."""
+    """Device-resident SIMPLE case (fs_case_*) on a polyMesh bounded by no-slip
+    walls and symmetry planes."""
 
     SIZES = {"u": "N", "v": "N", "w": "N", "p": "N", "pc": "N", "mdot": "Fi", "gu": "3N", "gv": "3N",
              "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bgp": "3Fb",
@@ -418,6 +423,14 @@ class Case:
         a = np.ascontiguousarray(values, dtype=np.float64)
         assert a.size == n
         _check(lib().fs_case_set(self.h, name.encode(), _dptr(a), n), f"fs_case_set({name})")
+
+    def set_patch_types(self, patch_sizes, patch_types):
+        """Boundary condition per patch, the patches in face order: "wall"
+        (no-slip, the velocity given at creation) or "symmetry"."""
+        sizes = np.ascontiguousarray(patch_sizes, dtype=np.int32)
+        codes = np.array([BC_TYPES[t] if isinstance(t, str) else int(t) for t in patch_types], np.int32)
+        _check(lib().fs_case_set_patch_types(self.h, len(sizes), _iptr(sizes), _iptr(codes)),
+               "fs_case_set_patch_types")
 
     def topology(self, name):
         """Readit's topology arrays of the case's mesh (reference numbering):

@@ -1,7 +1,8 @@
 // One steady SIMPLE outer iteration on the device (MainProgram.for:257-348):
 // SolveMomentumX/Y/Z (Momentum/SolveXMomentumEquation.for:175-299 and twins)
 // and SolveContinuity (ContinuityEquation/SolveContinuityEquation.for:37-114)
-// for laminar incompressible constant-property flow bounded by no-slip walls.
+// for laminar incompressible constant-property flow bounded by no-slip walls
+// and symmetry planes.
 //
 // Face loops run face-ordered over structure-of-arrays geometry (coalesced);
 // every face quantity the reference accumulates (FluxCf/FluxFf/FluxTf, the
@@ -125,17 +126,28 @@ __global__ void k_gg_cell(int N, int W, int Fi, const int *__restrict__ code,
   }
 }
 
-// Wall projection of the boundary pressure gradient (CorrectGradients.for,
-// CorrectPresureGradientAtWalls) over all boundary faces (all are walls here).
+// Tangential projection of the boundary pressure gradient: at no-slip walls
+// CorrectPresureGradientAtWalls (CorrectGradients.for:3-60), at symmetry
+// faces CorrectPressureGradientAtSymmetryBoundary (CorrectGradients.for
+// :244-289) with its own operand order and the z-normal it reads from face
+// (i,i) of the patch (bzq).  btype == nullptr: every face is a wall.
 __global__ void k_wall_pgrad(int Fb, int Fi, int N, const int *__restrict__ own,
                              const double *__restrict__ nx, const double *__restrict__ ny,
-                             const double *__restrict__ nz, const double *__restrict__ gp,
+                             const double *__restrict__ nz, const int *__restrict__ btype,
+                             const int *__restrict__ bzq, const double *__restrict__ gp,
                              double *__restrict__ bgp) {
   const int b = blockIdx.x * kBlock + threadIdx.x;
   if (b >= Fb) return;
   const int f = Fi + b, i1 = own[f];
   const double x = nx[f], y = ny[f], z = nz[f];
   const double gx = gp[i1], gy = gp[N + i1], gz = gp[2 * N + i1];
+  if (btype && btype[b] == FS_BC_SYMMETRY) {
+    const double zq = nz[Fi + bzq[b]];
+    bgp[b] = gx * (1. - x * x) - gy * x * y - gz * x * zq;
+    bgp[Fb + b] = gy * (1. - y * y) - gx * y * x - gz * y * zq;
+    bgp[2 * Fb + b] = gz * (1. - zq * zq) - gx * zq * x - gy * zq * y;
+    return;
+  }
   bgp[b] = (1 - x * x) * gx - x * y * gy - x * z * gz;
   bgp[Fb + b] = (1 - y * y) * gy - y * x * gx - y * z * gz;
   bgp[2 * Fb + b] = (1 - z * z) * gz - z * x * gx - z * y * gy;
@@ -178,7 +190,8 @@ struct MomArgs {
   int comp;
   double mu, c23;
   const double *phi, *u, *v, *w;          // current velocity components
-  const double *bu, *bv, *bw;              // wall velocities
+  const double *bu, *bv, *bw;              // boundary velocities
+  const int *btype;                        // per boundary face, nullptr = all walls
   const double *gu, *gv, *gw;              // cell gradients (boundary copies)
   const double *fg;                        // face gradients (12 blocks)
   const double *mdot;
@@ -233,8 +246,9 @@ __global__ void k_mom_face(int Fi, const int *__restrict__ own, const int *__res
   tf[f] = T;
 }
 
-// No-slip wall faces (AssembleIsotropicDiffusion.for:148-219), then the
-// explicit and bulk terms with the owner's gradient as boundary gradient.
+// No-slip wall faces (AssembleIsotropicDiffusion.for:148-219) or symmetry
+// faces (AssembleIsotropicDiffusion.for:562-620), then the explicit and bulk
+// terms with the owner's gradient as boundary gradient.
 __global__ void k_mom_wall(int Fb, int Fi, int N, const int *__restrict__ own,
                            const double *__restrict__ A, const double *__restrict__ nx,
                            const double *__restrict__ ny, const double *__restrict__ nz,
@@ -250,7 +264,18 @@ __global__ void k_mom_wall(int Fb, int Fi, int N, const int *__restrict__ own,
   const double dn = dx[f] * x + dy[f] * y + dz[f] * z;
   const double bphi = c == 0 ? a.bu[b] : c == 1 ? a.bv[b] : a.bw[b];
   double cfl, vfl;
-  if (c == 0) {
+  if (a.btype && a.btype[b] == FS_BC_SYMMETRY) {
+    if (c == 0) {
+      cfl = 2. * gamf * area * (x * x) / dn;
+      vfl = -((-(2. * gamf * area / dn)) * (a.v[i1] * y + a.w[i1] * z) * x);
+    } else if (c == 1) {
+      cfl = 2. * gamf * area * (y * y) / dn;
+      vfl = -((-(2. * gamf * area / dn)) * (a.u[i1] * x + a.w[i1] * z) * y);
+    } else {
+      cfl = 2. * gamf * area * (z * z) / dn;
+      vfl = -((-(2. * gamf * area / dn)) * (a.u[i1] * x + a.v[i1] * y) * z);
+    }
+  } else if (c == 0) {
     cfl = gamf * area * (1. - x * x) / dn;
     vfl = -((gamf * area / dn) * (bphi * (1. - x * x) + (a.v[i1] - a.bv[b]) * x * y +
                                    (a.w[i1] - a.bw[b]) * x * z));
@@ -446,6 +471,24 @@ __global__ void k_wall_p(int Fb, int Fi, const int *__restrict__ own, const doub
   bp[b] = p[own[f]] + bgp[b] * dx[f] + bgp[Fb + b] * dy[f] + bgp[2 * Fb + b] * dz[f];
 }
 
+// UpdateSymmetry (SymmetryAndAxisBoundary.for:27-44): boundary velocity of a
+// symmetry face = the owner's velocity less its normal component.
+__global__ void k_update_symmetry(int Fb, int Fi, const int *__restrict__ own,
+                                  const int *__restrict__ btype, const double *__restrict__ nx,
+                                  const double *__restrict__ ny, const double *__restrict__ nz,
+                                  const double *__restrict__ u, const double *__restrict__ v,
+                                  const double *__restrict__ w, double *__restrict__ bu,
+                                  double *__restrict__ bv, double *__restrict__ bw) {
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= Fb || btype[b] != FS_BC_SYMMETRY) return;
+  const int f = Fi + b, i1 = own[f];
+  const double ax = nx[f], ay = ny[f], az = nz[f];
+  const double amdot = u[i1] * ax + v[i1] * ay + w[i1] * az;
+  bu[b] = u[i1] - amdot * ax;
+  bv[b] = v[i1] - amdot * ay;
+  bw[b] = w[i1] - amdot * az;
+}
+
 // L1 of the cell mass imbalance: per-cell sum of outgoing mdot in slot order
 __global__ void k_mass_l1(int N, int W, const int *__restrict__ code,
                           const double *__restrict__ mdot, double *__restrict__ part) {
@@ -478,6 +521,8 @@ struct SimpleCase {
   DevBuf<double> u, v, w, p, pc, mdot, gu, gv, gw, gp, gpc, bp, bgp, fg, du, dv, dw, us, vs, ws;
   DevBuf<double> fp, fv, cf, ff, tf, ac, anb, bc, dphi, part;
   DevBuf<double> pcprev; // the last converged p' (input of the warm-start network)
+  DevBuf<int> btype, bzq; // boundary condition per boundary face (empty: all walls)
+  bool sym_dirty = false;  // symmetry boundary velocities not yet derived
   double rho = 1.0, mu = 0.01, urf_u = 0.7, urf_p = 0.3;
   int msolver = FS_SOR, mlmg = 0, mitmax = 20, psolver = FS_BICGS, plmg = FS_MG_PRECOND,
       pitmax = 500, refcell = 0;
@@ -523,7 +568,7 @@ static void phase_momentum_gradients(SimpleCase &C) {
   a.grad[0] = C.gu.p; a.grad[1] = C.gv.p; a.grad[2] = C.gw.p; a.grad[3] = C.gp.p;
   gradients<4>(C, a);
   hipLaunchKernelGGL(k_wall_pgrad, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N,
-                     C.own.p, C.nx.p, C.ny.p, C.nz.p, C.gp.p, C.bgp.p);
+                     C.own.p, C.nx.p, C.ny.p, C.nz.p, C.btype.p, C.bzq.p, C.gp.p, C.bgp.p);
   hipLaunchKernelGGL(k_face_grads, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p,
                      C.nei.p, C.g.p, C.d.p, C.ex.p, C.ey.p, C.ez.p, C.gu.p, C.gv.p, C.gw.p, C.gp.p,
                      C.p.p, C.fg.p);
@@ -537,6 +582,7 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
   a.phi = comp == 0 ? C.u.p : comp == 1 ? C.v.p : C.w.p;
   a.u = C.u.p; a.v = C.v.p; a.w = C.w.p;
   a.bu = C.bu.p; a.bv = C.bv.p; a.bw = C.bw.p;
+  a.btype = C.btype.p;
   a.gu = C.gu.p; a.gv = C.gv.p; a.gw = C.gw.p;
   a.fg = C.fg.p;
   a.mdot = C.mdot.p;
@@ -585,6 +631,13 @@ static void phase_pc_assemble(SimpleCase &C) {
                      C.ff.p, C.tf.p, C.ac.p, C.anb.p, C.bc.p);
 }
 
+static void update_symmetry(SimpleCase &C) {
+  C.sym_dirty = false;
+  if (!C.btype.p) return;
+  hipLaunchKernelGGL(k_update_symmetry, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.own.p,
+                     C.btype.p, C.nx.p, C.ny.p, C.nz.p, C.u.p, C.v.p, C.w.p, C.bu.p, C.bv.p, C.bw.p);
+}
+
 static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
   FS_HIP(hipMemsetAsync(C.pc.p, 0, sizeof(double) * C.N, C.s)); // InitializePressureCorrection
   fs_solve_stats ss{};
@@ -625,6 +678,7 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
                      C.cf.p, C.ff.p, C.pc.p, C.mdot.p);
   hipLaunchKernelGGL(k_wall_p, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.own.p, C.dx.p,
                      C.dy.p, C.dz.p, C.p.p, C.bgp.p, C.bp.p);
+  update_symmetry(C); // SolveContinuityEquation.for:96
   return 0;
 }
 
@@ -642,6 +696,7 @@ static double mass_l1(SimpleCase &C) {
 int simple_iterate(SimpleCase &C, int niter, fs_simple_stats *st) {
   auto t0 = std::chrono::steady_clock::now();
   fs_simple_stats loc{};
+  if (C.sym_dirty) update_symmetry(C); // the start-up UpdateSymmetry (MainProgram.for:128)
   for (int it = 0; it < niter; ++it) {
     phase_momentum_gradients(C);
     for (int comp = 0; comp < 3; ++comp) {
@@ -845,6 +900,7 @@ int fs_case_phase(fs_case *cs, int phase) {
   case FS_PHASE_SOLVE_W: { int rc = fs::phase_momentum_solve(C, 2, nullptr); if (rc < 0) return rc; break; }
   case FS_PHASE_PC_ASSEMBLE: fs::phase_pc_assemble(C); break;
   case FS_PHASE_PC_SOLVE_CORRECT: { int rc = fs::phase_pc_solve_correct(C, nullptr); if (rc < 0) return rc; break; }
+  case FS_PHASE_UPDATE_SYMMETRY: fs::update_symmetry(C); break;
   default:
     set_error("fs_case_phase: unknown phase");
     return -1;
@@ -910,6 +966,55 @@ int fs_case_set(fs_case *cs, const char *name, const double *in, long long count
   }
   FS_HIP(hipMemcpyAsync(p, in, sizeof(double) * n, hipMemcpyHostToDevice, cs->c.s));
   FS_HIP(hipStreamSynchronize(cs->c.s));
+  const std::string nm(name);
+  if (nm == "u" || nm == "v" || nm == "w") cs->c.sym_dirty = cs->c.btype.p != nullptr;
+  return 0;
+  CASE_GUARD_END
+}
+
+int fs_case_set_patch_types(fs_case *cs, int npatches, const int *patch_nfaces, const int *patch_type) {
+  if (!cs || npatches <= 0 || !patch_nfaces || !patch_type) {
+    set_error("fs_case_set_patch_types: invalid arguments");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  std::vector<int> type(C.Fb), zq(C.Fb);
+  long long start = 0;
+  bool any = false;
+  for (int q = 0; q < npatches; ++q) {
+    const int n = patch_nfaces[q], t = patch_type[q];
+    if (n < 0 || (t != FS_BC_WALL && t != FS_BC_SYMMETRY) || start + n > C.Fb) {
+      set_error("fs_case_set_patch_types: patch " + std::to_string(q) + " has a bad size or type");
+      return -1;
+    }
+    if (t == FS_BC_SYMMETRY && n < q + 1) {
+      set_error("fs_case_set_patch_types: symmetry patch " + std::to_string(q + 1) + " has fewer than " +
+                std::to_string(q + 1) + " faces");
+      return -1;
+    }
+    for (int j = 0; j < n; ++j) {
+      type[start + j] = t;
+      zq[start + j] = (int)start + q; // face i of patch i
+    }
+    any = any || t == FS_BC_SYMMETRY;
+    start += n;
+  }
+  if (start != C.Fb) {
+    set_error("fs_case_set_patch_types: patches hold " + std::to_string(start) + " faces, the boundary " +
+              std::to_string(C.Fb));
+    return -1;
+  }
+  if (!any) {
+    C.btype.release();
+    C.bzq.release();
+    C.sym_dirty = false;
+    return 0;
+  }
+  C.btype.upload(type, C.s);
+  C.bzq.upload(zq, C.s);
+  FS_HIP(hipStreamSynchronize(C.s));
+  C.sym_dirty = true;
   return 0;
   CASE_GUARD_END
 }

@@ -135,6 +135,61 @@ def lid_driven_cavity_walls(mesh, lid="ymax", lid_velocity=(1.0, 0.0, 0.0)):
     return bu, bv, bw
 
 
+def regroup(mesh, groups):
+    """The mesh with its boundary faces regrouped into new patches: groups is
+    a list of (new name, [old patch names]); each new patch holds the faces of
+    its old patches in the order given.  Interior faces are unchanged."""
+    ni = mesh.n_internal
+    used = [n for _, olds in groups for n in olds]
+    if sorted(used) != sorted(mesh.patch_names):
+        raise ValueError("groups must use every patch exactly once")
+    order, sizes = [], []
+    for _, olds in groups:
+        k = 0
+        for name in olds:
+            s, e = mesh.patch_face_range(name)
+            order.extend(range(s, e))
+            k += e - s
+        sizes.append(k)
+    faces = np.concatenate([np.arange(ni), np.asarray(order, dtype=np.int64)])
+    counts = np.diff(mesh.face_ptr)[faces]
+    ptr = np.zeros(len(faces) + 1, np.int32)
+    ptr[1:] = np.cumsum(counts)
+    nodes = np.concatenate([mesh.face_nodes[mesh.face_ptr[f]:mesh.face_ptr[f + 1]] for f in faces]).astype(np.int32)
+    return PolyMesh(mesh.points.copy(), ptr, nodes, mesh.owner[faces].astype(np.int32), mesh.neighbour.copy(),
+                    [g for g, _ in groups], sizes, mesh.shape)
+
+
+def rotate_x(mesh, angle):
+    """The mesh rotated by angle (radians) about the x axis through the origin."""
+    c, s = np.cos(angle), np.sin(angle)
+    R = np.array([[1.0, 0.0, 0.0], [0.0, c, -s], [0.0, s, c]])
+    return PolyMesh(mesh.points @ R.T, mesh.face_ptr.copy(), mesh.face_nodes.copy(), mesh.owner.copy(),
+                    mesh.neighbour.copy(), mesh.patch_names, mesh.patch_sizes.copy(), mesh.shape)
+
+
+def quarter_turn_x(mesh):
+    """The mesh turned by a right angle about the x axis, exactly:
+    (x, y, z) -> (x, -z, y)."""
+    P = mesh.points
+    pts = np.stack([P[:, 0], -P[:, 2], P[:, 1]], 1)
+    return PolyMesh(pts, mesh.face_ptr.copy(), mesh.face_nodes.copy(), mesh.owner.copy(), mesh.neighbour.copy(),
+                    mesh.patch_names, mesh.patch_sizes.copy(), mesh.shape)
+
+
+def cavity_2d(n, perturb=0.0, seed=0, split_front_back=False):
+    """The two-dimensional lid-driven cavity: n x n x 1 hexahedra on
+    [0,1]^2 x [0,1/n], patches lid (y = 1, moving at u = 1), walls (x = 0,
+    x = 1, y = 0; at rest) and frontBack (z = 0 and z = 1/n, symmetry), or
+    front and back as two symmetry patches.  Returns (mesh, (bu, bv, bw),
+    patch types)."""
+    m = box_hex(n, n, 1, lz=1.0 / n, perturb=perturb, seed=seed)
+    fb = [("front", ["zmin"]), ("back", ["zmax"])] if split_front_back else [("frontBack", ["zmin", "zmax"])]
+    m = regroup(m, [("lid", ["ymax"]), ("walls", ["xmin", "xmax", "ymin"])] + fb)
+    types = ["wall", "wall"] + ["symmetry"] * len(fb)
+    return m, lid_driven_cavity_walls(m, lid="lid"), types
+
+
 def write_polymesh(mesh, directory):
     """Writes the mesh as an OpenFOAM polyMesh directory in ASCII (the files
     the reference's Readit reads: points, faces, owner, neighbour, boundary)."""

@@ -72,7 +72,8 @@ enum fs_phase {
   FS_PHASE_SOLVE_V = 5,
   FS_PHASE_SOLVE_W = 6,
   FS_PHASE_PC_ASSEMBLE = 7,
-  FS_PHASE_PC_SOLVE_CORRECT = 8
+  FS_PHASE_PC_SOLVE_CORRECT = 8,
+  FS_PHASE_UPDATE_SYMMETRY = 9
 };
 
 /* wall_u/v/w: velocity of every boundary face (nfaces - nif entries). */
@@ -82,6 +83,21 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
                    const double *wall_u, const double *wall_v,
                    const double *wall_w, double rho, double mu, fs_case **out);
 int fs_case_destroy(fs_case *c);
+
+/* Boundary condition types (BCType of the reference's case definition). */
+enum fs_bc_type { FS_BC_WALL = 0, FS_BC_SYMMETRY = 1 };
+/* Boundary condition of each patch, the patches covering the boundary faces
+ * in order (npatches sizes summing to NFacesTotal - NIFaces).  A case starts
+ * with every boundary face a no-slip wall of the velocity given at creation.
+ * A symmetry face takes the owner's velocity less its normal component
+ * (UpdateSymmetry, SymmetryAndAxisBoundary.for:4-44; applied before the next
+ * iteration and after every pressure correction), slip diffusion
+ * (AssembleIsotropicDiffusion.for:562-620) and the symmetry projection of the
+ * boundary pressure gradient (CorrectGradients.for:244-289, which reads the
+ * z-normal of face i of patch i: a symmetry patch p, counted from 1, needs
+ * at least p faces).  Fails for sizes that do not cover the boundary. */
+int fs_case_set_patch_types(fs_case *c, int npatches, const int *patch_nfaces,
+                            const int *patch_type);
 /* urfMomentum, urfPressure, ASSolverMomentum/rrFMomentum/ASIterMomentum/
  * LMultigridMomentum and the Continuity equivalents (User0.F90:271-318). */
 int fs_case_configure(fs_case *c, double urf_u, double urf_p, int msolver,
@@ -96,6 +112,8 @@ int fs_case_phase(fs_case *c, int phase);
  * bw bp bgp fg du dv dw us vs ws ac anb bc cf ff tf and geometry vol Sx Sy
  * Sz g gdiff Tx Ty Tz dcf A kx ky kz); count must equal the field length. */
 int fs_case_get(fs_case *c, const char *name, double *out, long long count);
+/* Setting u, v or w re-derives the symmetry boundary velocities before the
+ * next iteration, as the reference's start-up does. */
 int fs_case_set(fs_case *c, const char *name, const double *in, long long count);
 /* Average duration (ms) of the ELL product y = A*x with the last assembled
  * system (the p' operator after an iteration), reps launches timed with HIP

@@ -1,7 +1,7 @@
 /*
  * oracle/or_simple.c -- CPU restatement of one steady SIMPLE outer iteration
  * of the reference (MainProgram.for:257-348) for a laminar incompressible
- * constant-property flow bounded by no-slip walls:
+ * constant-property flow bounded by no-slip walls and symmetry planes:
  *   SolveMomentumX/Y/Z (Momentum/SolveXMomentumEquation.for:175-299 and the
  *   Y/Z twins), SolveContinuity (ContinuityEquation/SolveContinuityEquation.for
  *   :37-114).
@@ -85,7 +85,53 @@ or_case *or_case_new(or_mesh *m, const double *wall_u, const double *wall_v,
   return c;
 }
 
+int or_case_set_patch_types(or_case *c, int npatches, const int *patch_nfaces, const int *patch_type) {
+  int total = 0;
+  for (int q = 0; q < npatches; ++q) {
+    if (patch_nfaces[q] < 0) return -1;
+    if (patch_type[q] != OR_BC_WALL && patch_type[q] != OR_BC_SYMMETRY) return -1;
+    /* CorrectPressureGradientAtSymmetryBoundary reads BFaceAreanz(i,i): face
+     * i of patch i */
+    if (patch_type[q] == OR_BC_SYMMETRY && patch_nfaces[q] < q + 1) return -1;
+    total += patch_nfaces[q];
+  }
+  if (total != c->Fb) return -1;
+  free(c->btype);
+  free(c->bzq);
+  c->btype = (int *)calloc(c->Fb ? c->Fb : 1, sizeof(int));
+  c->bzq = (int *)calloc(c->Fb ? c->Fb : 1, sizeof(int));
+  int start = 0;
+  for (int q = 0; q < npatches; ++q) {
+    for (int j = 0; j < patch_nfaces[q]; ++j) {
+      c->btype[start + j] = patch_type[q];
+      c->bzq[start + j] = start + q;
+    }
+    start += patch_nfaces[q];
+  }
+  c->sym_dirty = 1;
+  return 0;
+}
+
+/* UpdateSymmetry (SymmetryAndAxisBoundary.for:27-44): the boundary velocity
+ * of a symmetry face is the owner's velocity less its normal component. */
+void or_case_update_symmetry(or_case *c) {
+  c->sym_dirty = 0;
+  if (!c->btype) return;
+  const or_mesh *m = c->m;
+  for (int b = 0; b < c->Fb; ++b) {
+    if (c->btype[b] != OR_BC_SYMMETRY) continue;
+    const int f = c->Fi + b, i1 = m->owner[f];
+    const double ax = m->nx[f], ay = m->ny[f], az = m->nz[f];
+    const double amdot = c->u[i1] * ax + c->v[i1] * ay + c->w[i1] * az;
+    c->bu[b] = c->u[i1] - amdot * ax;
+    c->bv[b] = c->v[i1] - amdot * ay;
+    c->bw[b] = c->w[i1] - amdot * az;
+  }
+}
+
 void or_case_delete(or_case *c) {
+  free(c->btype);
+  free(c->bzq);
   double *arrs[] = {c->bu0, c->bv0, c->bw0, c->u, c->v, c->w, c->p, c->pc, c->mdot, c->gu,
                     c->gv, c->gw, c->gp, c->gpc, c->bu, c->bv, c->bw, c->bp, c->bpc, c->bgu,
                     c->bgv, c->bgw, c->bgp, c->bgpc, c->fgu, c->fgv, c->fgw, c->fgp, c->du,
@@ -203,6 +249,15 @@ void or_case_momentum_gradients(or_case *c) {
     const int f = Fi + b, i1 = m->owner[f];
     const double nx = m->nx[f], ny = m->ny[f], nz = m->nz[f];
     const double gx = c->gp[i1], gy = c->gp[N + i1], gz = c->gp[2 * N + i1];
+    if (c->btype && c->btype[b] == OR_BC_SYMMETRY) {
+      /* CorrectPressureGradientAtSymmetryBoundary (CorrectGradients.for
+       * :270-281), its operand order and its z-normal of face (i,i) */
+      const double zq = m->nz[Fi + c->bzq[b]];
+      c->bgp[b] = gx * (1. - nx * nx) - gy * nx * ny - gz * nx * zq;
+      c->bgp[Fb + b] = gy * (1. - ny * ny) - gx * ny * nx - gz * ny * zq;
+      c->bgp[2 * Fb + b] = gz * (1. - zq * zq) - gx * zq * nx - gy * zq * ny;
+      continue;
+    }
     c->bgp[b] = (1 - nx * nx) * gx - nx * ny * gy - nx * nz * gz;
     c->bgp[Fb + b] = (1 - ny * ny) * gy - ny * nx * gx - ny * nz * gz;
     c->bgp[2 * Fb + b] = (1 - nz * nz) * gz - nz * nx * gx - nz * ny * gy;
@@ -283,7 +338,19 @@ void or_case_momentum_assemble(or_case *c, int comp) {
     const double nx = m->nx[f], ny = m->ny[f], nz = m->nz[f], area = m->A[f];
     const double dn = m->dx[f] * nx + m->dy[f] * ny + m->dz[f] * nz;
     double cfl, vfl;
-    if (comp == 0) {
+    if (c->btype && c->btype[b] == OR_BC_SYMMETRY) {
+      /* symmetry faces (AssembleIsotropicDiffusion.for:562-620) */
+      if (comp == 0) {
+        cfl = 2. * gamf * area * (nx * nx) / dn;
+        vfl = -((-(2. * gamf * area / dn)) * (c->v[i1] * ny + c->w[i1] * nz) * nx);
+      } else if (comp == 1) {
+        cfl = 2. * gamf * area * (ny * ny) / dn;
+        vfl = -((-(2. * gamf * area / dn)) * (c->u[i1] * nx + c->w[i1] * nz) * ny);
+      } else {
+        cfl = 2. * gamf * area * (nz * nz) / dn;
+        vfl = -((-(2. * gamf * area / dn)) * (c->u[i1] * nx + c->v[i1] * ny) * nz);
+      }
+    } else if (comp == 0) {
       cfl = gamf * area * (1. - nx * nx) / dn;
       vfl = -((gamf * area / dn) * (bphi[b] * (1. - nx * nx) + (c->v[i1] - c->bv[b]) * nx * ny +
                                      (c->w[i1] - c->bw[b]) * nx * nz));
@@ -491,6 +558,7 @@ void or_case_pc_solve_correct(or_case *c) {
     const int f = Fi + b, i1 = m->owner[f];
     c->bp[b] = c->p[i1] + c->bgp[b] * m->dx[f] + c->bgp[Fb + b] * m->dy[f] + c->bgp[2 * Fb + b] * m->dz[f];
   }
+  or_case_update_symmetry(c); /* SolveContinuityEquation.for:96 */
 }
 
 /* L1 of the cell mass imbalance sum(mdot out). */
@@ -507,6 +575,8 @@ double or_case_mass_imbalance(or_case *c) {
 }
 
 void or_case_iterate(or_case *c, int niter) {
+  /* the UpdateSymmetry of the start-up (MainProgram.for:128) */
+  if (c->sym_dirty) or_case_update_symmetry(c);
   for (int it = 0; it < niter; ++it) {
     or_case_momentum_gradients(c);
     for (int comp = 0; comp < 3; ++comp) {

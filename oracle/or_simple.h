@@ -2,7 +2,7 @@
  * oracle/or_simple.h -- CPU restatement of the reference's mesh topology,
  * geometry and steady SIMPLE iteration for a laminar, incompressible,
  * constant-property flow in a box bounded by no-slip walls (moving or at
- * rest).  TEST INFRASTRUCTURE ONLY.
+ * rest) and symmetry planes.  TEST INFRASTRUCTURE ONLY.
  */
 #ifndef OR_SIMPLE_H
 #define OR_SIMPLE_H
@@ -62,13 +62,25 @@ typedef struct {
   double *ac, *anb, *bc, *dphi;
   double res_mom[3][4], res_pc[4], mass_l1;
   int it_mom[3], it_pc;
+  /* boundary condition per boundary face (NULL = every face a no-slip
+   * wall): OR_BC_WALL or OR_BC_SYMMETRY; bzq[b] = the boundary face whose
+   * z-normal the symmetry pressure-gradient projection reads */
+  int *btype, *bzq;
+  int sym_dirty; /* symmetry boundary velocities not yet derived */
 } or_case;
+
+enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 1 };
 
 or_case *or_case_new(or_mesh *m, const double *wall_u, const double *wall_v,
                      const double *wall_w, double rho, double mu, double urf_u,
                      double urf_p, int msolver, double mrrf, int mitmax,
                      int psolver, double prrf, int pitmax);
 void or_case_delete(or_case *c);
+/* Boundary condition of each patch (patches cover the boundary faces in
+ * order); returns 0, or -1 for sizes that do not cover the boundary or a
+ * symmetry patch p (1-based) with fewer than p faces. */
+int or_case_set_patch_types(or_case *c, int npatches, const int *patch_nfaces, const int *patch_type);
+void or_case_update_symmetry(or_case *c);
 void or_gradient(or_case *c, const double *phi, const double *bphi, double *gr, double *bgr);
 void or_case_momentum_gradients(or_case *c);
 void or_case_momentum_assemble(or_case *c, int comp);

@@ -288,7 +288,8 @@ class _OrCase(C.Structure):
                    "bgu", "bgv", "bgw", "bgp", "bgpc", "fgu", "fgv", "fgw", "fgp", "du", "dv", "dw", "us", "vs",
                    "ws", "visc", "fcf", "fff", "fvf", "ftf", "fce", "fte", "effdiv", "ac", "anb", "bc", "dphi")] + \
                [("res_mom", C.c_double * 12), ("res_pc", C.c_double * 4), ("mass_l1", C.c_double),
-                ("it_mom", C.c_int * 3), ("it_pc", C.c_int)]
+                ("it_mom", C.c_int * 3), ("it_pc", C.c_int), ("btype", C.POINTER(C.c_int)),
+                ("bzq", C.POINTER(C.c_int)), ("sym_dirty", C.c_int)]
 
 
 class OrCase:
@@ -335,6 +336,19 @@ class OrCase:
 
     def set(self, name, values):
         self.view(name)[:] = values
+
+    def set_patch_types(self, patch_sizes, patch_types):
+        """Boundary condition per patch, in patch order: "wall" or "symmetry"."""
+        L = lib()
+        L.or_case_set_patch_types.argtypes = [C.POINTER(_OrCase), C.c_int, _ip, _ip]
+        codes = np.array([{"wall": 0, "symmetry": 1}[t] for t in patch_types], np.int32)
+        sizes = np.ascontiguousarray(patch_sizes, dtype=np.int32)
+        if L.or_case_set_patch_types(self.ptr, len(sizes), _c(sizes, np.int32), _c(codes, np.int32)) != 0:
+            raise ValueError("or_case_set_patch_types: patches do not fit the boundary")
+
+    def update_symmetry(self):
+        lib().or_case_update_symmetry.argtypes = [C.POINTER(_OrCase)]
+        lib().or_case_update_symmetry(self.ptr)
 
     def iterate(self, n):
         lib().or_case_iterate(self.ptr, n)

@@ -53,6 +53,30 @@ def cpu_baseline(args):
                       f"{k} iterations in {el:.1f} s ({rate:.3f} it/s), scaled by cell count {n}^3/{args.n}^3"}
 
 
+def cpu_baseline_cavity2d(args):
+    """The oracle's SIMPLE (one thread) on the two-dimensional cavity of the
+    cavity2d variant with the reference's own p' solver (PBCG with its D-ILU,
+    rrF 1e-8, itmax as the device run): a bounded sample of --cavity2d-cpu
+    seconds after one warm-up iteration."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from flowsolversourcecode_amd import meshgen
+    m, (bu, bv, bw), types = meshgen.cavity_2d(64)
+    oc = po.OrCase(m, bu, bv, bw, rho=1.0, mu=0.01, urf_u=0.9, urf_p=0.1, msolver=po.SOR, mrrf=0.1, mitmax=10,
+                   psolver=po.PBCG, prrf=1e-8, pitmax=args.cavity2d_pitmax)
+    oc.set_patch_types(m.patch_sizes, types)
+    oc.iterate(1)
+    k, t0 = 0, time.perf_counter()
+    while True:
+        oc.iterate(1)
+        k += 1
+        el = time.perf_counter() - t0
+        if el > args.cavity2d_cpu or k >= 500:
+            break
+    return {"value": k / el, "unit": "SIMPLE iter/s", "cores": 1, "kind": "port",
+            "sample": f"oracle, 64x64x1 cavity, p' PBCG (itmax {args.cavity2d_pitmax}), {k} iterations in {el:.1f} s"}
+
+
 COPY_BYTES = 1 << 30
 HBM_PEAK_GBPS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_spmv_dot_cavity128_hbm_traffic.md")
@@ -89,6 +113,40 @@ def traffic_probe(case, args):
                       "copy_ms": copy_ms, "copy_GBps": 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9}), flush=True)
 
 
+def cavity2d(args, fs, meshgen):
+    """The two-dimensional cavity (64 x 64 x 1 hexahedra, frontBack symmetry,
+    Re = 100, urf 0.9 / 0.1, momentum SOR rrF 0.1 / 10 sweeps), --cavity2d-iters
+    outer iterations from rest with three p' solvers to rrF 1e-8: the
+    reference's PBCG (D-ILU), BiCGSTAB + FP64 AMG and BiCGSTAB + FP32 AMG.
+    Reports iter/s, p' ms and iterations per solve, solves stopped at itmax,
+    and the centre-line deviation from the Re = 100 benchmark."""
+    m, (bu, bv, bw), types = meshgen.cavity_2d(64)
+    out = {"workload": "lid-driven cavity 64x64x1 hex, frontBack symmetry, Re=100, "
+                       f"{args.cavity2d_iters} SIMPLE iterations from rest"}
+    for name, ps, lmg, fp32, pit in (("pbcg_reference", "pbcg", 0, 0, args.cavity2d_pitmax),
+                                     ("bicgs_amg_fp64", "bicgs", fs.MG_PRECOND, 0, 300),
+                                     ("bicgs_amg_fp32", "bicgs", fs.MG_PRECOND, 1, 300)):
+        c = fs.Case(m, bu, bv, bw, rho=1.0, mu=0.01)
+        c.set_patch_types(m.patch_sizes, types)
+        c.configure(urf_u=0.9, urf_p=0.1, msolver="sor", mrrf=0.1, mitmax=10, psolver=ps, prrf=1e-8,
+                    pitmax=pit, plmg=lmg)
+        c.set_option("mg_fp32", fp32)
+        st = c.iterate(args.cavity2d_iters)
+        eu, ev = meshgen.centreline_errors(c.get("u"), c.get("v"), 64)
+        k = args.cavity2d_iters
+        out[name] = {"value": k / st["seconds"], "unit": "SIMPLE iter/s", "pc_solve_ms": 1e3 * st["pc_seconds"] / k,
+                     "pc_iterations_per_solve": st["pc_iterations"] / k, "pc_stopped_at_itmax": st["pc_unconverged"],
+                     "pc_itmax": pit, "mass_imbalance_l1": st["mass_l1"],
+                     "centreline_max_dev_u_v": [eu, ev]}
+        print(f"# cavity2d {name}: {k / st['seconds']:.1f} it/s, p' {1e3 * st['pc_seconds'] / k:.3f} ms "
+              f"({st['pc_iterations'] / k:.1f} its, {st['pc_unconverged']} at itmax), Ghia dev {eu:.4f} {ev:.4f}",
+              file=sys.stderr, flush=True)
+        del c
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_cavity2d(args)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +162,9 @@ def main():
                     help="skip the FP64-plain and learned-warm-start runs")
     ap.add_argument("--learn-samples", type=int, default=8)
     ap.add_argument("--learn-steps", type=int, default=160)
+    ap.add_argument("--cavity2d-iters", type=int, default=500, help="outer iterations of the 2D-cavity variant")
+    ap.add_argument("--cavity2d-pitmax", type=int, default=500, help="p' itmax of the 2D cavity's PBCG runs")
+    ap.add_argument("--cavity2d-cpu", type=float, default=6.0, help="seconds of the 2D cavity's CPU sample")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="set up the workload, run one iteration, then only the ELL product (--spmv-reps launches) "
                          "and a 1 GiB device copy; for hardware-counter runs")
@@ -212,6 +273,7 @@ def main():
         print(f"# learned: {args.steps / sl['seconds']:.2f} it/s, p' its {sl['pc_iterations'] / args.steps:.1f} "
               f"(zero start {head_pc:.1f}), train {t_train:.1f} s", file=sys.stderr, flush=True)
         del lc, net
+        variants["cavity2d_64x64x1_re100"] = cavity2d(args, fs, meshgen)
 
     cpu = None if args.no_cpu else cpu_baseline(args)
     out = {

@@ -190,6 +190,32 @@ def cavity_2d(n, perturb=0.0, seed=0, split_front_back=False):
     return m, lid_driven_cavity_walls(m, lid="lid"), types
 
 
+# Ghia, Ghia & Shin, J. Comput. Phys. 48 (1982) 387-411, Tables I and II,
+# Re = 100: (y, u) on the vertical centre line x = 0.5 and (x, v) on the
+# horizontal centre line y = 0.5 of the two-dimensional lid-driven cavity.
+GHIA_RE100_U = np.array([
+    [0.9766, 0.84123], [0.9688, 0.78871], [0.9609, 0.73722], [0.9531, 0.68717], [0.8516, 0.23151],
+    [0.7344, 0.00332], [0.6172, -0.13641], [0.5000, -0.20581], [0.4531, -0.21090], [0.2813, -0.15662],
+    [0.1719, -0.10150], [0.1016, -0.06434], [0.0703, -0.04775], [0.0625, -0.04192], [0.0547, -0.03717]])
+GHIA_RE100_V = np.array([
+    [0.9688, -0.05906], [0.9609, -0.07391], [0.9531, -0.08864], [0.9453, -0.10313], [0.9063, -0.16914],
+    [0.8594, -0.22445], [0.8047, -0.24533], [0.5000, 0.05454], [0.2344, 0.17527], [0.2266, 0.17507],
+    [0.1563, 0.16077], [0.0938, 0.12317], [0.0781, 0.10890], [0.0703, 0.10091], [0.0625, 0.09233]])
+
+
+def centreline_errors(u, v, n):
+    """Largest deviation of the cavity_2d(n) centre-line velocities (cell
+    (i, j) is i + n j; the centre lines fall between cell columns, n even)
+    from the Re = 100 benchmark values."""
+    u, v = np.asarray(u).reshape(n, n), np.asarray(v).reshape(n, n)
+    yc = (np.arange(n) + 0.5) / n
+    uc = 0.5 * (u[:, n // 2 - 1] + u[:, n // 2])
+    vc = 0.5 * (v[n // 2 - 1, :] + v[n // 2, :])
+    eu = np.abs(np.interp(GHIA_RE100_U[:, 0], yc, uc) - GHIA_RE100_U[:, 1]).max()
+    ev = np.abs(np.interp(GHIA_RE100_V[:, 0], yc, vc) - GHIA_RE100_V[:, 1]).max()
+    return float(eu), float(ev)
+
+
 def write_polymesh(mesh, directory):
     """Writes the mesh as an OpenFOAM polyMesh directory in ASCII (the files
     the reference's Readit reads: points, faces, owner, neighbour, boundary)."""

@@ -272,6 +272,27 @@ __device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, const in
                                          const T *__restrict__ anb, const T *x, const int *__restrict__ parent,
                                          const T *__restrict__ xc, T alpha) {
   T s = T(0);
+  if (LPR == 1) {
+    // one lane per row: the row's slots in batches (loads first, then the
+    // terms in slot order)
+    for (int j0 = 0; j0 < w; j0 += kEllBatch) {
+      int k[kEllBatch];
+      T a[kEllBatch], xv[kEllBatch];
+#pragma unroll
+      for (int u = 0; u < kEllBatch; ++u) k[u] = nbr[(size_t)min(j0 + u, w - 1) * n + i];
+#pragma unroll
+      for (int u = 0; u < kEllBatch; ++u) {
+        const int kk = k[u] >= 0 ? k[u] : i;
+        a[u] = anb[(size_t)min(j0 + u, w - 1) * n + i];
+        xv[u] = x[kk];
+        if (PRO) xv[u] = xv[u] + alpha * xc[parent[kk]];
+      }
+#pragma unroll
+      for (int u = 0; u < kEllBatch; ++u)
+        if (j0 + u < w && k[u] >= 0) s += a[u] * xv[u];
+    }
+    return s;
+  }
   for (int j = lane; j < w; j += LPR) {
     const int k = nbr[(size_t)j * n + i];
     if (k >= 0) {
@@ -354,6 +375,70 @@ k_sweep_csr(int n, const int *__restrict__ rp, const int *__restrict__ col, cons
     if (PRO) xi = xi + alpha * xc[parent[i]];
     xn[i] = (TO)(xi + omega * (b[i] - ac[i] * xi - s) / ac[i]);
   }
+}
+
+// ---- large coarse levels in sliced ELL: slices of 64 rows (one wavefront),
+// one row per lane, a slice's entries column-major (entry k of the slice's
+// lane l at ptr[s] + 64 k + l, so a wavefront's loads of entry k coalesce),
+// padded to the slice's longest row with zero values whose column is the row
+// itself.  row[] maps a slot to its row (-1: past the end); rows are sorted by
+// length within windows of 256 rows so that a slice's rows have similar
+// lengths.  Each lane sums its row's entries in CSR order.
+template <class T, bool PRO>
+__device__ __forceinline__ T sell_offdiag(int slot, int lane, const int *__restrict__ ptr, const int *__restrict__ col,
+                                          const T *__restrict__ val, const T *x, const int *__restrict__ parent,
+                                          const T *__restrict__ xc, T alpha) {
+  const int sl = slot >> 6, q0 = ptr[sl] + lane, len = (ptr[sl + 1] - ptr[sl]) >> 6;
+  constexpr int B = 4; // entries per batch: columns first, then values and gathers
+  T s = T(0);
+  for (int k0 = 0; k0 < len; k0 += B) {
+    int c[B];
+    T v[B], xv[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) c[u] = col[q0 + (min(k0 + u, len - 1) << 6)];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      v[u] = val[q0 + (min(k0 + u, len - 1) << 6)];
+      xv[u] = x[c[u]];
+      if (PRO) xv[u] = xv[u] + alpha * xc[parent[c[u]]];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (k0 + u < len) s += v[u] * xv[u];
+  }
+  return s;
+}
+
+template <class T, class TO, bool PRO>
+__global__ void __launch_bounds__(kBlock)
+k_sweep_sell(int nslots, const int *__restrict__ ptr, const int *__restrict__ row, const int *__restrict__ col,
+             const T *__restrict__ val, const T *__restrict__ ac, const T *__restrict__ b, const T *x,
+             const int *__restrict__ parent, const T *__restrict__ xc, T alpha, T omega, TO *xn,
+             const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int t = xcd_block() * kBlock + threadIdx.x;
+  if (t >= nslots) return;
+  const int i = row[t];
+  if (i < 0) return;
+  const T s = sell_offdiag<T, PRO>(t, t & 63, ptr, col, val, x, parent, xc, alpha);
+  T xi = x[i];
+  if (PRO) xi = xi + alpha * xc[parent[i]];
+  xn[i] = (TO)(xi + omega * (b[i] - ac[i] * xi - s) / ac[i]);
+}
+
+// r = b - A x on a sliced-ELL level
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_resid_sell(int nslots, const int *__restrict__ ptr, const int *__restrict__ row, const int *__restrict__ col,
+             const T *__restrict__ val, const T *__restrict__ ac, const T *__restrict__ b, const T *__restrict__ x,
+             T *__restrict__ r, const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int t = xcd_block() * kBlock + threadIdx.x;
+  if (t >= nslots) return;
+  const int i = row[t];
+  if (i < 0) return;
+  const T s = sell_offdiag<T, false>(t, t & 63, ptr, col, val, x, nullptr, nullptr, T(0));
+  r[i] = b[i] - ac[i] * x[i] - s;
 }
 
 // Residual of a coarse level summed straight into the next level's
@@ -448,11 +533,7 @@ __global__ void k_gsc(int cnt, const int *__restrict__ cells, int n, int w, cons
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= cnt) return;
   const int i = cells[t];
-  T s = b[i];
-  for (int j = 0; j < w; ++j) {
-    const int k = nbr[(size_t)j * n + i];
-    if (k >= 0) s = s - anb[(size_t)j * n + i] * x[k];
-  }
+  const T s = ell_fold<kEllBatch>(i, n, w, nbr, anb, (const T *)x, b[i], [](T acc, T a, T xk) { return acc - a * xk; });
   x[i] = s / ac[i];
 }
 
@@ -464,11 +545,8 @@ __global__ void k_amg_res(int n, int w, const int *__restrict__ nbr, const doubl
   if (flags[0]) return;
   double s = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    double v = b[i] - ac[i] * x[i];
-    for (int j = 0; j < w; ++j) {
-      const int k = nbr[(size_t)j * n + i];
-      if (k >= 0) v = v - anb[(size_t)j * n + i] * x[k];
-    }
+    const double v = ell_fold<kEllBatch>(i, n, w, nbr, anb, x, b[i] - ac[i] * x[i],
+                                         [](double acc, double a, double xk) { return acc - a * xk; });
     r[i] = v;
     s += fabs(v);
   }
@@ -716,7 +794,57 @@ struct DLevel {
   DevBuf<int> csr_rp, csr_col, csr_pos;
   DevBuf<double> val64;
   DevBuf<float> val32;
+  // large levels: sliced ELL instead (val64/val32 then hold the sliced
+  // layout and csr_pos points into it)
+  bool sell = false;
+  int sell_slots = 0;
+  DevBuf<int> sell_ptr, sell_row, sell_col;
 };
+
+constexpr int kSellWindow = 256; // rows sorted by length within windows of this size
+
+// Sliced-ELL image of a CSR level: slot order, slice offsets, columns, and
+// the map from CSR entry to sliced entry.
+static void build_sell(int n, const std::vector<int> &rp, const std::vector<int> &col, std::vector<int> &ptr,
+                       std::vector<int> &row, std::vector<int> &scol, std::vector<int> &csr2sell) {
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  for (int w0 = 0; w0 < n; w0 += kSellWindow) {
+    const int w1 = std::min(n, w0 + kSellWindow);
+    std::stable_sort(order.begin() + w0, order.begin() + w1,
+                     [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+  }
+  const int ns = (n + 63) / 64;
+  row.assign((size_t)ns * 64, -1);
+  ptr.assign(ns + 1, 0);
+  for (int s = 0; s < ns; ++s) {
+    int len = 0;
+    for (int l = 0; l < 64 && s * 64 + l < n; ++l) {
+      const int r = order[s * 64 + l];
+      row[(size_t)s * 64 + l] = r;
+      len = std::max(len, rp[r + 1] - rp[r]);
+    }
+    ptr[s + 1] = ptr[s] + 64 * len;
+  }
+  scol.assign(std::max(ptr[ns], 1), 0);
+  csr2sell.assign(std::max<size_t>(col.size(), 1), -1);
+  for (int s = 0; s < ns; ++s) {
+    const int len = (ptr[s + 1] - ptr[s]) / 64;
+    for (int l = 0; l < 64; ++l) {
+      const int r = row[(size_t)s * 64 + l];
+      for (int k = 0; k < len; ++k) {
+        const int q = ptr[s] + 64 * k + l;
+        if (r < 0) scol[q] = 0;
+        else if (k < rp[r + 1] - rp[r]) {
+          scol[q] = col[rp[r] + k];
+          csr2sell[rp[r] + k] = q;
+        } else {
+          scol[q] = r; // padding: zero value times the row's own x
+        }
+      }
+    }
+  }
+}
 
 // The tail of one precision: its first level, descriptor and byte image.
 struct TailImage {
@@ -952,12 +1080,30 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
         rp[i + 1] = (int)col.size();
       }
       D.nnz = (long long)col.size();
+      // a level that runs grid-wide (larger than the tail) and has at least
+      // mg_sell_rows rows keeps its operator in sliced ELL
+      D.sell = m->mg_sell_rows > 0 && h.n >= m->mg_sell_rows && h.n > m->mg_tail_cells;
+      size_t nval = std::max<size_t>(col.size(), 1);
+      if (D.sell) {
+        std::vector<int> sptr, srow, scol, c2s;
+        build_sell(h.n, rp, col, sptr, srow, scol, c2s);
+        for (auto &q : pos)
+          if (q >= 0) q = c2s[q];
+        D.sell_slots = (int)srow.size();
+        D.sell_ptr.upload(sptr, s);
+        D.sell_row.upload(srow, s);
+        D.sell_col.upload(scol, s);
+        nval = scol.size();
+      }
       if (col.empty()) col.push_back(0);
       D.csr_rp.upload(rp, s);
       D.csr_col.upload(col, s);
       D.csr_pos.upload(pos, s);
-      D.val64.alloc(col.size());
-      D.val32.alloc(col.size());
+      D.val64.alloc(nval);
+      D.val32.alloc(nval);
+      // padded sliced entries are never written by the Galerkin kernel
+      FS_HIP_THROW(hipMemsetAsync(D.val64.p, 0, sizeof(double) * nval, s));
+      FS_HIP_THROW(hipMemsetAsync(D.val32.p, 0, sizeof(float) * nval, s));
     }
     if (l + 1 < nl) {
       std::vector<int> par0(h.n);
@@ -1104,6 +1250,12 @@ static void launch_sweep(fs_mesh *m, DLevel &D, int level, const T *b, const T *
                          int nc, T alpha, T om, TO *xn, const int *flags) {
   hipStream_t s = m->stream;
   const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
+  if (level > 0 && D.sell) {
+    hipLaunchKernelGGL((k_sweep_sell<T, TO, PRO>), dim3(gfor(D.sell_slots)), dim3(kBlock), 0, s, D.sell_slots,
+                       D.sell_ptr.p, D.sell_row.p, D.sell_col.p, LV<T>::val(D), ac, b, x, parent, xc, alpha, om, xn,
+                       flags);
+    return;
+  }
   if (level > 0) {
     const int *rp = D.csr_rp.p, *col = D.csr_col.p;
     const T *val = LV<T>::val(D);
@@ -1145,6 +1297,11 @@ template <class T>
 static void launch_resid(fs_mesh *m, DLevel &D, const T *b, const T *x, T *r, const int *flags) {
   hipStream_t s = m->stream;
   const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
+  if (D.sell) {
+    hipLaunchKernelGGL((k_resid_sell<T>), dim3(gfor(D.sell_slots)), dim3(kBlock), 0, s, D.sell_slots, D.sell_ptr.p,
+                       D.sell_row.p, D.sell_col.p, LV<T>::val(D), ac, b, x, r, flags);
+    return;
+  }
   if (D.w <= 8)
     hipLaunchKernelGGL((k_resid<1, T>), dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x, r,
                        flags);
@@ -1288,7 +1445,7 @@ static void vcycle(fs_mesh *m, const double *rhs, double *out, double alpha, con
     if (l + 1 == ts) mode = 0;                                  // the tail kernel takes over
     else if (l + 1 == L - 1) mode = 1;                          // grid-wide coarsest: `coarse` sweeps
     else mode = (!use_gs(m, C) && cpre > 0) ? 1 : 2;
-    if (l > 0 && !use_gs(m, D)) {
+    if (l > 0 && !use_gs(m, D) && !D.sell) {
       launch_resid_restrict<T>(m, D, C, b, res[l], om, mode, flags);
     } else {
       launch_resid<T>(m, D, b, res[l], LV<T>::r(D), flags);

@@ -52,4 +52,36 @@ __device__ __forceinline__ int xcd_block() {
   return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// Fold of the occupied ELL slots of row i (column-major nbr/anb, neighbour
+// k >= 0) in slot order: acc = op(acc, anb_j, x_k) for j = 0, 1, ..., w-1.
+// The slots go in batches of B: the batch's neighbour indices are loaded
+// first, then its coefficients and gathered values, then the terms are
+// combined in slot order.  A row thus keeps B independent loads in flight
+// instead of waiting on each slot's index before fetching its value; slots
+// past w re-read slot w-1 and empty slots read x_i, both discarded, so the
+// result is exactly that of the plain slot loop.
+template <int B, class T, class Op>
+__device__ __forceinline__ T ell_fold(int i, int n, int w, const int *__restrict__ nbr, const T *__restrict__ anb,
+                                      const T *x, T acc, Op op) {
+  for (int j0 = 0; j0 < w; j0 += B) {
+    int k[B];
+    T a[B], xv[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) k[u] = nbr[(size_t)min(j0 + u, w - 1) * n + i];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      a[u] = anb[(size_t)min(j0 + u, w - 1) * n + i];
+      xv[u] = x[k[u] >= 0 ? k[u] : i];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (j0 + u < w && k[u] >= 0) acc = op(acc, a[u], xv[u]);
+  }
+  return acc;
+}
+
+// batch width for a row of w slots: all of a hexahedron's or tetrahedron's
+// slots at once
+constexpr int kEllBatch = 8;
+
 } // namespace fs

@@ -60,12 +60,9 @@ __device__ __forceinline__ double ell_ax(int i, int n, int w, const int *__restr
                                          const double *__restrict__ ac,
                                          const double *__restrict__ anb,
                                          const double *__restrict__ x) {
-  double s = ac[i] * x[i];
-  for (int j = 0; j < w; ++j) {
-    const int k = nbr[(size_t)j * n + i];
-    if (k >= 0) s = s + anb[(size_t)j * n + i] * x[k];
-  }
-  return s;
+  const double s = ac[i] * x[i];
+  return ell_fold<kEllBatch>(i, n, w, nbr, anb, x, s,
+                             [](double acc, double a, double xk) { return acc + a * xk; });
 }
 
 // r = b - A*x in the order of CheckConvergence (CheckConvergenceRoutine.for:
@@ -75,12 +72,9 @@ __device__ __forceinline__ double ell_res(int i, int n, int w, const int *__rest
                                           const double *__restrict__ anb,
                                           const double *__restrict__ b,
                                           const double *__restrict__ x) {
-  double r = b[i] - ac[i] * x[i];
-  for (int j = 0; j < w; ++j) {
-    const int k = nbr[(size_t)j * n + i];
-    if (k >= 0) r = r - anb[(size_t)j * n + i] * x[k];
-  }
-  return r;
+  const double r = b[i] - ac[i] * x[i];
+  return ell_fold<kEllBatch>(i, n, w, nbr, anb, x, r,
+                             [](double acc, double a, double xk) { return acc - a * xk; });
 }
 
 // y = A^T*x in the order of dsprstx (PBCGMethod.for:359-368): diagonal first,
@@ -315,11 +309,8 @@ __global__ void k_gs_color(int cnt, const int *__restrict__ cells, int n, int w,
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= cnt) return;
   const int i = cells[t];
-  double s = bc[i];
-  for (int j = 0; j < w; ++j) {
-    const int k = nbr[(size_t)j * n + i];
-    if (k >= 0) s = s - anb[(size_t)j * n + i] * x[k];
-  }
+  const double s = ell_fold<kEllBatch>(i, n, w, nbr, anb, (const double *)x, bc[i],
+                                       [](double acc, double a, double xk) { return acc - a * xk; });
   x[i] = s / ac[i];
 }
 
@@ -432,12 +423,8 @@ __global__ void k_ilu_resid(int n, int w, const int *__restrict__ nbr, const dou
   if (flags[F_DONE]) return;
   const int i = xcd_block() * kBlock + threadIdx.x;
   if (i >= n) return;
-  double r = -ac[i] * dphi[i] + bc[i];
-  for (int j = 0; j < w; ++j) {
-    const int k = nbr[(size_t)j * n + i];
-    if (k >= 0) r = r - anb[(size_t)j * n + i] * dphi[k];
-  }
-  rc[i] = r;
+  rc[i] = ell_fold<kEllBatch>(i, n, w, nbr, anb, dphi, -ac[i] * dphi[i] + bc[i],
+                              [](double acc, double a, double xk) { return acc - a * xk; });
 }
 
 __global__ void k_ilu_fwd(int cnt, const int *__restrict__ cells, const int *__restrict__ ptr,

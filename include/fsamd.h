@@ -80,7 +80,11 @@ enum fs_option {
                                  solver (the reference), 1.7 as a
                                  preconditioner [0]                           */
   FS_OPT_MG_COARSE_PRESWEEP = 15, /* pre-sweeps on levels >= 1 (-1: as level 0) [-1] */
-  FS_OPT_MG_COARSE_POSTSWEEP = 16 /* post-sweeps on levels >= 1 (-1: as level 0) [-1] */
+  FS_OPT_MG_COARSE_POSTSWEEP = 16, /* post-sweeps on levels >= 1 (-1: as level 0) [-1] */
+  FS_OPT_MG_SELL_ROWS = 17    /* coarse levels with at least this many rows keep
+                                 their operator in sliced ELL (64-row slices,
+                                 rows sorted by length within 256-row windows,
+                                 one row per lane); 0 = CSR everywhere [20000] */
 };
 
 /* Statistics of the last solve. */

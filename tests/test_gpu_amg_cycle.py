@@ -7,7 +7,7 @@ not the solution: every variant must drive the true L1 residual of the system
 below 1e-8 of its initial value, and on a non-singular system reach the sparse
 direct solution to 1e-6 of its largest entry.  Systems of 15k cells give a
 hierarchy with grid-wide levels above the tail threshold and tail levels
-below it.
+below it; mg_sell_rows = 1 puts every grid-wide coarse level in sliced ELL.
 """
 import numpy as np
 import pytest
@@ -47,6 +47,11 @@ VARIANTS = [
     {"mg_presweep": 1, "mg_postsweep": 3},
     {"mg_smoother": 1},
     {"mg_smoother": 1, "mg_fp32": 1},
+    {"mg_sell_rows": 1},                        # grid-wide coarse levels in sliced ELL
+    {"mg_sell_rows": 1, "mg_fp32": 1},
+    {"mg_sell_rows": 1, "mg_tail_cells": 0, "mg_fp32": 1},
+    {"mg_sell_rows": 1, "mg_coarse_presweep": 0, "mg_coarse_postsweep": 2},
+    {"mg_sell_rows": 0},                        # CSR on every coarse level
 ]
 
 
@@ -78,7 +83,8 @@ def test_neumann_variants_reach_1e8(neumann, solver, opts):
 
 @pytest.mark.parametrize("solver", ["bicgs", "cg", "gmres"])
 @pytest.mark.parametrize("opts", [{}, {"mg_fp32": 1}, {"mg_tail_cells": 0}, {"mg_maxlevels": 3, "mg_fp32": 1},
-                                  {"gmres_restart": 5}],
+                                  {"gmres_restart": 5}, {"mg_sell_rows": 1, "mg_fp32": 1},
+                                  {"mg_sell_rows": 1, "mg_tail_cells": 0}],
                          ids=_ids)
 def test_shifted_variants_match_direct(shifted, solver, opts):
     nbr, nnbr, ac, anb, bc, A, x = shifted
@@ -150,3 +156,16 @@ def test_pbcg_with_amg_preconditioner(shifted):
     np.testing.assert_allclose(dphi, x, rtol=0, atol=1e-6 * np.abs(x).max())
     rc0, fit0, dphi0, res40, st0 = _solve(nbr, nnbr, ac, anb, bc, "pbcg", {}, lmg=0)
     assert rc0 == 1 and st0["final_l1"] > 1e3 * st["final_l1"], (st, st0)
+
+
+def test_sliced_ell_cycle_matches_csr(neumann):
+    """The sliced-ELL coarse levels change only the order of each row's sum:
+    the same iteration count (within one) and the same solution."""
+    nbr, nnbr, ac, anb, bc, A = neumann
+    out = {}
+    for rows in (0, 1):
+        rc, fit, dphi, res4, st = _solve(nbr, nnbr, ac, anb, bc, "bicgs", {"mg_sell_rows": rows})
+        assert rc == 0, st
+        out[rows] = (st["iterations"], dphi - dphi.mean())
+    assert abs(out[0][0] - out[1][0]) <= 1
+    np.testing.assert_allclose(out[1][1], out[0][1], rtol=0, atol=1e-6 * np.abs(out[0][1]).max())

@@ -162,6 +162,8 @@ def main():
                     help="skip the FP64-plain and learned-warm-start runs")
     ap.add_argument("--learn-samples", type=int, default=8)
     ap.add_argument("--learn-steps", type=int, default=160)
+    ap.add_argument("--clock-stride", type=int, default=5,
+                    help="time every n-th launch of the roofline kernels with HIP events")
     ap.add_argument("--cavity2d-iters", type=int, default=500, help="outer iterations of the 2D-cavity variant")
     ap.add_argument("--cavity2d-pitmax", type=int, default=500, help="p' itmax of the 2D cavity's PBCG runs")
     ap.add_argument("--cavity2d-cpu", type=float, default=6.0, help="seconds of the 2D cavity's CPU sample")
@@ -202,8 +204,9 @@ def main():
               file=sys.stderr, flush=True)
     # per-launch HIP events (recorded by the dispatches themselves on the
     # case's stream) for the ELL product of the Krylov loop and the sweeps of
-    # the finest AMG level, over the timed region
-    case.kernel_clock_start(["spmv", "fine_sweep"], capacity=16384)
+    # the finest AMG level, over the timed region, on every --clock-stride-th
+    # launch of each (the events cost a few microseconds per timed launch)
+    case.kernel_clock_start(["spmv", "fine_sweep"], capacity=16384, stride=args.clock_stride)
     st = case.iterate(args.steps)  # synchronises before and after
     el = st["seconds"]
     value = args.steps / el
@@ -219,7 +222,7 @@ def main():
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": kernel,
                 "bytes_per_launch": c["bytes"], "bytes_per_cell": per_cell, "avg_ms": c["avg_ms"],
-                "launches": c["launches"], "idle_launches": c["untimed"],
+                "launches": c["launches"], "idle_launches": c["untimed"], "sampled_every": args.clock_stride,
                 "share_of_step": c["launches"] * c["avg_ms"] * 1e-3 / el}
 
     roof_spmv = roof(clk["spmv"], "k_spmv_dot (FP64 ELL y=A*x with (r0,y) partials, BiCGSTAB loop)",

@@ -78,6 +78,7 @@ def lib():
         L.fs_kernel_clock_start.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.fs_kernel_clock_read.argtypes = [C.c_void_p, C.c_int, _dp]
         L.fs_kernel_clock_clear.argtypes = [C.c_void_p]
+        L.fs_kernel_clock_stride.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -89,7 +90,8 @@ def last_error():
 CLOCK_SITES = {"off": 0, "spmv": 1, "fine_sweep": 2}
 
 
-def _clock_start(mesh_handle, sites, capacity):
+def _clock_start(mesh_handle, sites, capacity, stride=1):
+    _check(lib().fs_kernel_clock_stride(mesh_handle, int(stride)), "fs_kernel_clock_stride")
     mask = 0
     for st in ([sites] if isinstance(sites, (str, int)) else sites):
         mask |= 1 << (CLOCK_SITES[st] if isinstance(st, str) else int(st))
@@ -186,9 +188,10 @@ class Mesh:
         k = OPTIONS[key] if isinstance(key, str) else int(key)
         _check(lib().fs_set_option(self.h, k, float(value)), "fs_set_option")
 
-    def kernel_clock_start(self, sites, capacity=4096):
-        """Arm per-launch HIP-event timing of solve-loop kernel sites ("spmv", "fine_sweep")."""
-        _clock_start(self.h, sites, capacity)
+    def kernel_clock_start(self, sites, capacity=4096, stride=1):
+        """Arm per-launch HIP-event timing of solve-loop kernel sites ("spmv",
+        "fine_sweep"), every stride-th launch of each site."""
+        _clock_start(self.h, sites, capacity, stride)
 
     def kernel_clock_read(self, site):
         """{launches, avg_ms, bytes, untimed} of one site since the clock was started."""
@@ -390,9 +393,10 @@ class Case:
         k = OPTIONS[key] if isinstance(key, str) else int(key)
         _check(lib().fs_set_option(lib().fs_case_mesh(self.h), k, float(value)), "fs_set_option")
 
-    def kernel_clock_start(self, sites, capacity=4096):
-        """Arm per-launch HIP-event timing of solve-loop kernel sites ("spmv", "fine_sweep")."""
-        _clock_start(lib().fs_case_mesh(self.h), sites, capacity)
+    def kernel_clock_start(self, sites, capacity=4096, stride=1):
+        """Arm per-launch HIP-event timing of solve-loop kernel sites ("spmv",
+        "fine_sweep"), every stride-th launch of each site."""
+        _clock_start(lib().fs_case_mesh(self.h), sites, capacity, stride)
 
     def kernel_clock_read(self, site):
         """{launches, avg_ms, bytes, untimed} of one site since the clock was started."""

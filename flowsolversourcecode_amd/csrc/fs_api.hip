@@ -51,6 +51,16 @@ int fs_kernel_clock_read(fs_mesh *m, int site, double *out) {
   FS_GUARD_END
 }
 
+int fs_kernel_clock_stride(fs_mesh *m, int stride) {
+  if (!m) {
+    set_error("fs_kernel_clock_stride: mesh is NULL");
+    return -1;
+  }
+  FS_GUARD_BEGIN
+  return fs::kernel_clock_stride(m, stride);
+  FS_GUARD_END
+}
+
 int fs_kernel_clock_clear(fs_mesh *m) {
   if (!m) {
     set_error("fs_kernel_clock_clear: mesh is NULL");

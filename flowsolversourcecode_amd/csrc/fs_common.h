@@ -83,6 +83,8 @@ struct Hierarchy; // fs_amg.hip
 struct KernelClock {
   int sites = 0, cap = 0, used = 0;  // sites: bit (1 << site) per armed site
   int dropped[4] = {0, 0, 0, 0};
+  int stride = 1;                    // time every stride-th launch of a site
+  long long seen[4] = {0, 0, 0, 0};  // launches of each site since the start
   int cur_iter = -1;       // Krylov iteration being launched, -1 outside a loop
   int solve_first = 0;     // first record of the solve in progress
   std::vector<hipEvent_t> ev;  // 2 per record

@@ -14,7 +14,7 @@ template <class K, class... A>
 void timed_launch(fs_mesh *m, int site, double bytes, K kernel, dim3 grid, dim3 block, unsigned lds,
                   hipStream_t s, A... args) {
   KernelClock &c = m->kclock;
-  if (site != 0 && (c.sites >> site & 1)) {
+  if (site != 0 && (c.sites >> site & 1) && c.seen[site & 3]++ % c.stride == 0) {
     if (c.used < c.cap) {
       const int u = c.used++;
       c.iter[u] = c.cur_iter;
@@ -31,6 +31,7 @@ void timed_launch(fs_mesh *m, int site, double bytes, K kernel, dim3 grid, dim3 
 int kernel_clock_start(fs_mesh *m, int site_mask, int capacity);
 int kernel_clock_read(fs_mesh *m, int site, double *out);
 int kernel_clock_clear(fs_mesh *m);
+int kernel_clock_stride(fs_mesh *m, int stride);
 void kernel_clock_release(fs_mesh *m);
 
 int mesh_prepare_solver(fs_mesh *m);

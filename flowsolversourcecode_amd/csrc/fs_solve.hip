@@ -1053,6 +1053,7 @@ int kernel_clock_start(fs_mesh *m, int site_mask, int capacity) {
   c.cap = capacity;
   c.used = 0;
   for (int &d : c.dropped) d = 0;
+  for (long long &k : c.seen) k = 0;
   c.solve_first = 0;
   c.iter.assign(capacity, -1);
   c.site.assign(capacity, 0);
@@ -1094,7 +1095,18 @@ int kernel_clock_clear(fs_mesh *m) {
   FS_HIP(hipStreamSynchronize(m->stream));
   c.used = 0;
   for (int &d : c.dropped) d = 0;
+  for (long long &k : c.seen) k = 0;
   c.solve_first = 0;
+  return 0;
+}
+
+int kernel_clock_stride(fs_mesh *m, int stride) {
+  if (stride < 1 || stride > 1024) {
+    set_error("fs_kernel_clock_stride: stride must lie in [1, 1024]");
+    return -1;
+  }
+  m->kclock.stride = stride;
+  for (long long &k : m->kclock.seen) k = 0;
   return 0;
 }
 

@@ -180,7 +180,9 @@ int fs_time_device_copy(long long bytes, int reps, double *avg_ms);
  * convergence, which return at once, are left out), out[1] their mean
  * duration in ms, out[2] their mean algorithmic bytes, out[3] launches of the
  * site not timed (idle or beyond the capacity).  fs_kernel_clock_clear
- * empties the records; the sites stay armed. */
+ * empties the records; the sites stay armed.  fs_kernel_clock_stride times
+ * only every stride-th launch of each site (default 1: every launch), which
+ * keeps the events' own cost out of the throughput of the timed region. */
 enum fs_clock_site {
   FS_CLOCK_OFF = 0,
   FS_CLOCK_SPMV = 1,      /* k_spmv_dot, the FP64 operator product of the
@@ -190,6 +192,7 @@ enum fs_clock_site {
 int fs_kernel_clock_start(fs_mesh *mesh, int site_mask, int capacity);
 int fs_kernel_clock_read(fs_mesh *mesh, int site, double *out);
 int fs_kernel_clock_clear(fs_mesh *mesh);
+int fs_kernel_clock_stride(fs_mesh *mesh, int stride);
 
 /* Message of the last failing call on this thread. */
 const char *fs_last_error(void);

@@ -125,7 +125,7 @@ struct fs_mesh {
   double mg_omega = 0.8;
   int mg_tail_cells = 1024, mg_coarse_sweeps = 16;
   int mg_cpre = -1, mg_cpost = -1; // sweeps on the coarse levels (-1: as on the finest)
-  int mg_sell_rows = 20000;        // coarse levels at least this large use sliced ELL
+  int mg_sell_rows = 100000;       // coarse levels at least this large use sliced ELL
   double mg_alpha = 0.0;           // over-correction (0: 1 as solver, 1.7 as preconditioner)
   int gmres_m = 30, check_every = 8;
   int ncalls_mg = 0;

@@ -19,6 +19,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "fs_common.h"
 #include "fs_geom.h"
@@ -45,27 +46,41 @@ struct GradArgs { // up to 4 fields at once
 
 // Green-Gauss face value phi_f = phi_C g + phi_F (1-g)
 // (StandardGreenGaussCorrection1Gradient.for:51-53)
+// Face values live over all F faces (field k at fp[k F + f]): boundary faces
+// carry the boundary value (or the owner's own value when the field has
+// none), so the cell sums read one array whatever the face's role.
 template <int NF>
-__global__ void k_gg_face0(int Fi, const int *__restrict__ own, const int *__restrict__ nei,
+__global__ void k_gg_face0(int Fi, int F, const int *__restrict__ own, const int *__restrict__ nei,
                            const double *__restrict__ g, GradArgs a, double *__restrict__ fp) {
   const int f = xcd_block() * kBlock + threadIdx.x;
-  if (f >= Fi) return;
-  const int i = own[f], j = nei[f];
+  if (f >= F) return;
+  const int i = own[f];
+  if (f >= Fi) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) fp[(size_t)k * F + f] = a.bval[k] ? a.bval[k][f - Fi] : a.phi[k][i];
+    return;
+  }
+  const int j = nei[f];
   const double gf = g[f];
 #pragma unroll
-  for (int k = 0; k < NF; ++k) fp[(size_t)k * Fi + f] = a.phi[k][i] * gf + a.phi[k][j] * (1. - gf);
+  for (int k = 0; k < NF; ++k) fp[(size_t)k * F + f] = a.phi[k][i] * gf + a.phi[k][j] * (1. - gf);
 }
 
 // corrected face value at the face centroid (:108-147):
 // phi_f = phi_f' + sum_d (grad_d C g + grad_d F (1-g)) (r_f - r_f')_d
 template <int NF>
-__global__ void k_gg_facecorr(int Fi, int N, const int *__restrict__ own,
+__global__ void k_gg_facecorr(int Fi, int F, int N, const int *__restrict__ own,
                               const int *__restrict__ nei, const double *__restrict__ g,
                               const double *__restrict__ kx, const double *__restrict__ ky,
                               const double *__restrict__ kz, const double *__restrict__ fp,
                               GradArgs a, double *__restrict__ fv) {
   const int f = xcd_block() * kBlock + threadIdx.x;
-  if (f >= Fi) return;
+  if (f >= F) return;
+  if (f >= Fi) { // boundary values are not corrected
+#pragma unroll
+    for (int k = 0; k < NF; ++k) fv[(size_t)k * F + f] = fp[(size_t)k * F + f];
+    return;
+  }
   const int i = own[f], j = nei[f];
   const double gf = g[f], og = 1. - gf;
 #pragma unroll
@@ -77,7 +92,7 @@ __global__ void k_gg_facecorr(int Fi, int N, const int *__restrict__ own,
     cy = cy * ky[f];
     double cz = gr[2 * N + i] * gf + gr[2 * N + j] * og;
     cz = cz * kz[f];
-    fv[(size_t)k * Fi + f] = fp[(size_t)k * Fi + f] + cx + cy + cz;
+    fv[(size_t)k * F + f] = fp[(size_t)k * F + f] + cx + cy + cz;
   }
 }
 
@@ -85,8 +100,30 @@ __global__ void k_gg_facecorr(int Fi, int N, const int *__restrict__ own,
 // division by the volume (:35-64, :148-172).  Interior faces add +phi_f S_f
 // at the owner and -phi_f S_f at the neighbour; boundary faces add the
 // boundary value times S_b.
+// One face's term of the cell sums: +phi_f S_f (owner, boundary) or
+// -phi_f S_f (neighbour).
 template <int NF>
-__global__ void k_gg_cell(int N, int W, int Fi, const int *__restrict__ code,
+__device__ __forceinline__ void gg_term(int role, double sx, double sy, double sz, const double *v, double *gx,
+                                        double *gy, double *gz) {
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const double tx = v[k] * sx, ty = v[k] * sy, tz = v[k] * sz;
+    if (role == ROLE_NEIGH) {
+      gx[k] = gx[k] - tx;
+      gy[k] = gy[k] - ty;
+      gz[k] = gz[k] - tz;
+    } else {
+      gx[k] = gx[k] + tx;
+      gy[k] = gy[k] + ty;
+      gz[k] = gz[k] + tz;
+    }
+  }
+}
+
+// WC > 0: a mesh of WC slots per cell, all slots' loads issued before the
+// sums (codes first, then the faces' S and values); WC = 0: any width.
+template <int NF, int WC>
+__global__ void k_gg_cell(int N, int W, int F, const int *__restrict__ code,
                           const double *__restrict__ Sx, const double *__restrict__ Sy,
                           const double *__restrict__ Sz, const double *__restrict__ vol,
                           const double *__restrict__ fv, GradArgs a) {
@@ -95,26 +132,32 @@ __global__ void k_gg_cell(int N, int W, int Fi, const int *__restrict__ code,
   double gx[NF], gy[NF], gz[NF];
 #pragma unroll
   for (int k = 0; k < NF; ++k) gx[k] = gy[k] = gz[k] = 0.0;
-  for (int s = 0; s < W; ++s) {
-    const int cd = code[(size_t)s * N + c];
-    const int role = cd & 3, f = cd >> 2;
-    if (role == ROLE_NONE) continue;
-    const double sx = Sx[f], sy = Sy[f], sz = Sz[f];
+  if constexpr (WC > 0) {
+    int cd[WC];
+    double sx[WC], sy[WC], sz[WC], v[WC][NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) {
-      double v;
-      if (role == ROLE_BOUND) v = a.bval[k] ? a.bval[k][f - Fi] : a.phi[k][c];
-      else v = fv[(size_t)k * Fi + f];
-      const double tx = v * sx, ty = v * sy, tz = v * sz;
-      if (role == ROLE_NEIGH) {
-        gx[k] = gx[k] - tx;
-        gy[k] = gy[k] - ty;
-        gz[k] = gz[k] - tz;
-      } else {
-        gx[k] = gx[k] + tx;
-        gy[k] = gy[k] + ty;
-        gz[k] = gz[k] + tz;
-      }
+    for (int s = 0; s < WC; ++s) cd[s] = code[(size_t)s * N + c];
+#pragma unroll
+    for (int s = 0; s < WC; ++s) {
+      const int f = cd[s] >> 2;
+      sx[s] = Sx[f];
+      sy[s] = Sy[f];
+      sz[s] = Sz[f];
+#pragma unroll
+      for (int k = 0; k < NF; ++k) v[s][k] = fv[(size_t)k * F + f];
+    }
+#pragma unroll
+    for (int s = 0; s < WC; ++s)
+      if ((cd[s] & 3) != ROLE_NONE) gg_term<NF>(cd[s] & 3, sx[s], sy[s], sz[s], v[s], gx, gy, gz);
+  } else {
+    for (int s = 0; s < W; ++s) {
+      const int cd = code[(size_t)s * N + c];
+      const int role = cd & 3, f = cd >> 2;
+      if (role == ROLE_NONE) continue;
+      double v[NF];
+#pragma unroll
+      for (int k = 0; k < NF; ++k) v[k] = fv[(size_t)k * F + f];
+      gg_term<NF>(role, Sx[f], Sy[f], Sz[f], v, gx, gy, gz);
     }
   }
   const double v = vol[c];
@@ -309,6 +352,30 @@ __global__ void k_mom_wall(int Fb, int Fi, int N, const int *__restrict__ own,
 // (CalculateEffectiveDivergence.for, AssembleConvection.for:169-178), the
 // pressure gradient (AssemblePressureGradient.for), under-relaxation
 // (UnderRelaxation.for) and D = V/a_P, u* = u (StoreTheDValues.for).
+// One slot of the momentum gather (face fluxes cf, ff, tf and mass flux md
+// of the slot's face).
+__device__ __forceinline__ double mom_slot(int role, double vc, double vf, double vt, double md, double &a,
+                                           double &b, double &e) {
+  double an = 0.0;
+  if (role == ROLE_OWNER) {
+    a = a + vc;
+    an = an + vf;
+    b = b - vt;
+    e = e + md;
+  } else if (role == ROLE_NEIGH) {
+    a = a - vf;
+    an = an - vc;
+    b = b + vt;
+    e = e - md;
+  } else if (role == ROLE_BOUND) {
+    a = a + vc;
+    b = b - vt;
+    e = e + 0.;
+  }
+  return an;
+}
+
+template <int WC>
 __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
                            const double *__restrict__ cf, const double *__restrict__ ff,
                            const double *__restrict__ tf, const double *__restrict__ mdot,
@@ -319,26 +386,29 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
   const int c = xcd_block() * kBlock + threadIdx.x;
   if (c >= N) return;
   double a = 0.0, b = 0.0, e = 0.0;
-  for (int s = 0; s < W; ++s) {
-    const int cd = code[(size_t)s * N + c];
-    const int role = cd & 3, f = cd >> 2;
-    double an = 0.0;
-    if (role == ROLE_OWNER) {
-      a = a + cf[f];
-      an = an + ff[f];
-      b = b - tf[f];
-      e = e + mdot[f];
-    } else if (role == ROLE_NEIGH) {
-      a = a - ff[f];
-      an = an - cf[f];
-      b = b + tf[f];
-      e = e - mdot[f];
-    } else if (role == ROLE_BOUND) {
-      a = a + cf[f];
-      b = b - tf[f];
-      e = e + 0.;
+  if constexpr (WC > 0) {
+    // all slots' loads first (the mass flux exists for interior faces only)
+    int cd[WC];
+    double vc[WC], vf[WC], vt[WC], vm[WC];
+#pragma unroll
+    for (int s = 0; s < WC; ++s) cd[s] = code[(size_t)s * N + c];
+#pragma unroll
+    for (int s = 0; s < WC; ++s) {
+      const int f = cd[s] >> 2, role = cd[s] & 3;
+      vc[s] = cf[f];
+      vf[s] = ff[f];
+      vt[s] = tf[f];
+      vm[s] = mdot[(role == ROLE_OWNER || role == ROLE_NEIGH) ? f : 0];
     }
-    anb[(size_t)s * N + c] = an;
+#pragma unroll
+    for (int s = 0; s < WC; ++s) anb[(size_t)s * N + c] = mom_slot(cd[s] & 3, vc[s], vf[s], vt[s], vm[s], a, b, e);
+  } else {
+    for (int s = 0; s < W; ++s) {
+      const int cd = code[(size_t)s * N + c];
+      const int role = cd & 3, f = cd >> 2;
+      const double md = (role == ROLE_OWNER || role == ROLE_NEIGH) ? mdot[f] : 0.0;
+      anb[(size_t)s * N + c] = mom_slot(role, cf[f], ff[f], tf[f], md, a, b, e);
+    }
   }
   const double pc = phi[c];
   const double ep = e > 0.0 ? e : 0.0;
@@ -405,6 +475,21 @@ __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
 }
 
 // p' system gather (walls carry no flux: FluxCf = 0, Bmdot = 0).
+__device__ __forceinline__ double pc_slot(int role, double vc, double vf, double vt, double &a, double &b) {
+  double an = 0.0;
+  if (role == ROLE_OWNER) {
+    a = a + vc;
+    an = an + vf;
+    b = b - vt;
+  } else if (role == ROLE_NEIGH) {
+    a = a - vf;
+    an = an - vc;
+    b = b + vt;
+  }
+  return an;
+}
+
+template <int WC>
 __global__ void k_pc_cell(int N, int W, const int *__restrict__ code,
                           const double *__restrict__ cf, const double *__restrict__ ff,
                           const double *__restrict__ tf, double *__restrict__ ac,
@@ -412,20 +497,26 @@ __global__ void k_pc_cell(int N, int W, const int *__restrict__ code,
   const int c = xcd_block() * kBlock + threadIdx.x;
   if (c >= N) return;
   double a = 0.0, b = 0.0;
-  for (int s = 0; s < W; ++s) {
-    const int cd = code[(size_t)s * N + c];
-    const int role = cd & 3, f = cd >> 2;
-    double an = 0.0;
-    if (role == ROLE_OWNER) {
-      a = a + cf[f];
-      an = an + ff[f];
-      b = b - tf[f];
-    } else if (role == ROLE_NEIGH) {
-      a = a - ff[f];
-      an = an - cf[f];
-      b = b + tf[f];
+  if constexpr (WC > 0) {
+    int cd[WC];
+    double vc[WC], vf[WC], vt[WC];
+#pragma unroll
+    for (int s = 0; s < WC; ++s) cd[s] = code[(size_t)s * N + c];
+#pragma unroll
+    for (int s = 0; s < WC; ++s) {
+      const int f = cd[s] >> 2;
+      vc[s] = cf[f];
+      vf[s] = ff[f];
+      vt[s] = tf[f];
     }
-    anb[(size_t)s * N + c] = an;
+#pragma unroll
+    for (int s = 0; s < WC; ++s) anb[(size_t)s * N + c] = pc_slot(cd[s] & 3, vc[s], vf[s], vt[s], a, b);
+  } else {
+    for (int s = 0; s < W; ++s) {
+      const int cd = code[(size_t)s * N + c];
+      const int f = cd >> 2;
+      anb[(size_t)s * N + c] = pc_slot(cd & 3, cf[f], ff[f], tf[f], a, b);
+    }
   }
   ac[c] = a;
   bc[c] = b;
@@ -547,17 +638,33 @@ static void upload_geometry(SimpleCase &C) {
 
 // Gradient of up to four fields: Green-Gauss plus nIterGradient = 2
 // face-centroid corrections (CalculateGradients.for:212-215).
+// Calls f(std::integral_constant<int, WC>) with WC = the case's slots per cell
+// when the cell kernels have a fixed-width form for it, else WC = 0.
+template <class Fn> static void by_width(int W, Fn &&fn) {
+  switch (W) {
+  case 4: fn(std::integral_constant<int, 4>()); break;
+  case 5: fn(std::integral_constant<int, 5>()); break;
+  case 6: fn(std::integral_constant<int, 6>()); break;
+  case 8: fn(std::integral_constant<int, 8>()); break;
+  default: fn(std::integral_constant<int, 0>());
+  }
+}
+
 template <int NF> static void gradients(SimpleCase &C, GradArgs a) {
-  const int Fi = C.Fi, N = C.N;
-  hipLaunchKernelGGL(k_gg_face0<NF>, dim3(blocks(Fi)), dim3(kBlock), 0, C.s, Fi, C.own.p, C.nei.p,
+  const int Fi = C.Fi, F = C.F, N = C.N;
+  hipLaunchKernelGGL(k_gg_face0<NF>, dim3(blocks(F)), dim3(kBlock), 0, C.s, Fi, F, C.own.p, C.nei.p,
                      C.g.p, a, C.fp.p);
-  hipLaunchKernelGGL(k_gg_cell<NF>, dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.W, Fi, C.code.p,
-                     C.Sx.p, C.Sy.p, C.Sz.p, C.vol.p, C.fp.p, a);
+  auto cell = [&](const double *fv) {
+    by_width(C.W, [&](auto wc) {
+      hipLaunchKernelGGL((k_gg_cell<NF, decltype(wc)::value>), dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.W, F,
+                         C.code.p, C.Sx.p, C.Sy.p, C.Sz.p, C.vol.p, fv, a);
+    });
+  };
+  cell(C.fp.p);
   for (int it = 0; it < 2; ++it) {
-    hipLaunchKernelGGL(k_gg_facecorr<NF>, dim3(blocks(Fi)), dim3(kBlock), 0, C.s, Fi, N, C.own.p,
+    hipLaunchKernelGGL(k_gg_facecorr<NF>, dim3(blocks(F)), dim3(kBlock), 0, C.s, Fi, F, N, C.own.p,
                        C.nei.p, C.g.p, C.kx.p, C.ky.p, C.kz.p, C.fp.p, a, C.fv.p);
-    hipLaunchKernelGGL(k_gg_cell<NF>, dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.W, Fi, C.code.p,
-                       C.Sx.p, C.Sy.p, C.Sz.p, C.vol.p, C.fv.p, a);
+    cell(C.fv.p);
   }
 }
 
@@ -594,9 +701,11 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
                      C.cf.p, C.tf.p);
   double *D = comp == 0 ? C.du.p : comp == 1 ? C.dv.p : C.dw.p;
   double *st = comp == 0 ? C.us.p : comp == 1 ? C.vs.p : C.ws.p;
-  hipLaunchKernelGGL(k_mom_cell, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W, C.code.p,
-                     C.cf.p, C.ff.p, C.tf.p, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
-                     C.urf_u, C.ac.p, C.anb.p, C.bc.p, D, st);
+  by_width(C.W, [&](auto wc) {
+    hipLaunchKernelGGL(k_mom_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
+                       C.code.p, C.cf.p, C.ff.p, C.tf.p, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
+                       C.urf_u, C.ac.p, C.anb.p, C.bc.p, D, st);
+  });
 }
 
 static int phase_momentum_solve(SimpleCase &C, int comp, fs_simple_stats *st) {
@@ -627,8 +736,10 @@ static void phase_pc_assemble(SimpleCase &C) {
   hipLaunchKernelGGL(k_rhie_chow, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p,
                      C.nei.p, C.g.p, C.Sx.p, C.Sy.p, C.Sz.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, a,
                      C.mdot.p, C.cf.p, C.ff.p, C.tf.p);
-  hipLaunchKernelGGL(k_pc_cell, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W, C.code.p, C.cf.p,
-                     C.ff.p, C.tf.p, C.ac.p, C.anb.p, C.bc.p);
+  by_width(C.W, [&](auto wc) {
+    hipLaunchKernelGGL(k_pc_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
+                       C.code.p, C.cf.p, C.ff.p, C.tf.p, C.ac.p, C.anb.p, C.bc.p);
+  });
 }
 
 static void update_symmetry(SimpleCase &C) {
@@ -828,8 +939,8 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
     FS_HIP_THROW(hipMemsetAsync(C.bgp.p, 0, sizeof(double) * 3 * Fb, C.s));
     C.fg.alloc(12 * Fi);
     FS_HIP_THROW(hipMemsetAsync(C.fg.p, 0, sizeof(double) * 12 * Fi, C.s));
-    C.fp.alloc(4 * Fi);
-    C.fv.alloc(4 * Fi);
+    C.fp.alloc(4 * F);
+    C.fv.alloc(4 * F);
     C.cf.alloc(F);
     C.ff.alloc(F);
     C.tf.alloc(F);

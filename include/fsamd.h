@@ -84,7 +84,7 @@ enum fs_option {
   FS_OPT_MG_SELL_ROWS = 17    /* coarse levels with at least this many rows keep
                                  their operator in sliced ELL (64-row slices,
                                  rows sorted by length within 256-row windows,
-                                 one row per lane); 0 = CSR everywhere [20000] */
+                                 one row per lane); 0 = CSR everywhere [100000] */
 };
 
 /* Statistics of the last solve. */

@@ -4,8 +4,9 @@ cavity: 128^3 hexahedra on the unit cube, interior nodes perturbed by
 U(-0.15, 0.15) of the spacing (seed 2026), Re = 1000 (rho = 1, mu = 1e-3,
 lid u = 1), FP64 assembly, BiCGSTAB for p' to 1e-8 (L1, relative) with an
 aggregation-AMG V-cycle preconditioner whose hierarchy runs in FP32
-(``--precision fp64`` keeps it in FP64), momentum by multicolour symmetric
-Gauss-Seidel (rrF 0.1, at most 10 sweeps), urf 0.7 / 0.3.
+(``--precision fp64`` keeps it in FP64; one damped-Jacobi pre-sweep and two
+post-sweeps per level, over-correction 1.9), momentum by multicolour
+symmetric Gauss-Seidel (rrF 0.1, at most 10 sweeps), urf 0.7 / 0.3.
 
 One step = one SIMPLE outer iteration (three momentum assemble+solve passes,
 Rhie-Chow, p' assembly and solve, corrections), everything resident in HBM.
@@ -162,6 +163,9 @@ def main():
                     help="skip the FP64-plain and learned-warm-start runs")
     ap.add_argument("--learn-samples", type=int, default=8)
     ap.add_argument("--learn-steps", type=int, default=160)
+    ap.add_argument("--mg-alpha", type=float, default=1.9, help="over-correction of the headline V-cycle")
+    ap.add_argument("--mg-pre", type=int, default=1, help="pre-sweeps per level of the headline V-cycle")
+    ap.add_argument("--mg-post", type=int, default=2, help="post-sweeps per level of the headline V-cycle")
     ap.add_argument("--clock-stride", type=int, default=5,
                     help="time every n-th launch of the roofline kernels with HIP events")
     ap.add_argument("--cavity2d-iters", type=int, default=500, help="outer iterations of the 2D-cavity variant")
@@ -179,7 +183,7 @@ def main():
     import flowsolversourcecode_amd as fs
     from flowsolversourcecode_amd import meshgen
 
-    def make_case(fp32, alpha):
+    def make_case(fp32, alpha, pre=2, post=2):
         pm = meshgen.box_hex(args.n, args.n, args.n, perturb=0.15, seed=2026)
         bu, bv, bw = meshgen.lid_driven_cavity_walls(pm)
         case = fs.Case(pm, bu, bv, bw, rho=1.0, mu=1e-3)
@@ -187,10 +191,15 @@ def main():
                        pitmax=200, plmg=fs.MG_PRECOND)
         case.set_option("mg_fp32", fp32)
         case.set_option("mg_alpha", alpha)
+        case.set_option("mg_presweep", pre)
+        case.set_option("mg_postsweep", post)
         return case
 
+    # the tuned V-cycle of the headline form: one pre-sweep and two post-sweeps
+    # on every level, over-correction 1.9
+    tuned = dict(alpha=args.mg_alpha, pre=args.mg_pre, post=args.mg_post)
     t0 = time.perf_counter()
-    case = make_case(1 if args.precision == "fp32amg" else 0, 0.0)
+    case = make_case(1 if args.precision == "fp32amg" else 0, **tuned)
     setup_s = time.perf_counter() - t0
     n_cells, n_faces = case.N, case.F
     print(f"# setup {setup_s:.1f} s: N={n_cells} faces={n_faces}", file=sys.stderr, flush=True)
@@ -253,7 +262,7 @@ def main():
         del plain
         # FP32 AMG + learned warm start, trained on the run's own first iterations
         from flowsolversourcecode_amd import learn
-        lc = make_case(1, 0.0)
+        lc = make_case(1, **tuned)
         net = learn.Mpnn.for_case(lc, layers=3, seed=7, capacity=args.learn_samples, use_prev=True)
         learn.attach(lc, sink=net)
         t1 = time.perf_counter()
@@ -294,7 +303,9 @@ def main():
         "data": "synthetic seeded mesh",
         "config": {"workload": f"lid-driven cavity {args.n}^3 hex, +/-15% node perturbation, Re=1000, "
                                f"p' BiCGSTAB+AMG({args.precision}) to 1e-8",
-                   "cells": n_cells, "faces": n_faces, "parallelism": "single GPU"},
+                   "cells": n_cells, "faces": n_faces, "parallelism": "single GPU",
+                   "vcycle": {"presweeps": args.mg_pre, "postsweeps": args.mg_post, "alpha": args.mg_alpha,
+                              "omega": 0.8}},
         "pc_solve_ms": 1e3 * st["pc_seconds"] / args.steps,
         "pc_iterations_per_solve": st["pc_iterations"] / args.steps,
         "momentum_ms": 1e3 * st["momentum_seconds"] / args.steps,

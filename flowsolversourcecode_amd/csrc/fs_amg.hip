@@ -268,7 +268,7 @@ __global__ void k_to(int n, const double *__restrict__ a, T *__restrict__ b) {
 // sum_j anb_ij x'_j over the slots of row i, spread over LPR consecutive
 // lanes and combined with shuffles; x' = x + alpha*xc[parent] when PRO
 template <int LPR, class T, bool PRO>
-__device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, const int *__restrict__ nbr,
+__device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, EllNbr nbr,
                                          const T *__restrict__ anb, const T *x, const int *__restrict__ parent,
                                          const T *__restrict__ xc, T alpha) {
   T s = T(0);
@@ -278,8 +278,7 @@ __device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, const in
     for (int j0 = 0; j0 < w; j0 += kEllBatch) {
       int k[kEllBatch];
       T a[kEllBatch], xv[kEllBatch];
-#pragma unroll
-      for (int u = 0; u < kEllBatch; ++u) k[u] = nbr[(size_t)min(j0 + u, w - 1) * n + i];
+      ell_batch<kEllBatch>(nbr, i, n, w, j0, k);
 #pragma unroll
       for (int u = 0; u < kEllBatch; ++u) {
         const int kk = k[u] >= 0 ? k[u] : i;
@@ -294,7 +293,7 @@ __device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, const in
     return s;
   }
   for (int j = lane; j < w; j += LPR) {
-    const int k = nbr[(size_t)j * n + i];
+    const int k = nbr.i32[(size_t)j * n + i];
     if (k >= 0) {
       T xk = x[k];
       if (PRO) xk = xk + alpha * xc[parent[k]];
@@ -312,7 +311,7 @@ __device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, const in
 // the FP64 result of the cycle directly.
 template <int LPR, class T, class TO, bool PRO>
 __global__ void __launch_bounds__(kBlock)
-k_sweep(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac, const T *__restrict__ anb,
+k_sweep(int n, int w, EllNbr nbr, const T *__restrict__ ac, const T *__restrict__ anb,
         const T *__restrict__ b, const T *x, const int *__restrict__ parent, const T *__restrict__ xc,
         T alpha, T omega, TO *xn, const int *__restrict__ flags) {
   if (flags && flags[0]) return;
@@ -330,7 +329,7 @@ k_sweep(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac, con
 // r = b - A x
 template <int LPR, class T>
 __global__ void __launch_bounds__(kBlock)
-k_resid(int n, int w, const int *__restrict__ nbr, const T *__restrict__ ac, const T *__restrict__ anb,
+k_resid(int n, int w, EllNbr nbr, const T *__restrict__ ac, const T *__restrict__ anb,
         const T *__restrict__ b, const T *__restrict__ x, T *__restrict__ r, const int *__restrict__ flags) {
   if (flags && flags[0]) return;
   const int t = xcd_block() * kBlock + threadIdx.x;
@@ -774,6 +773,7 @@ k_coarse_inverse(int n, int w, const int *__restrict__ nbr, const double *__rest
 struct DLevel {
   int n = 0, w = 0;
   const int *nbr = nullptr;      // device, 0-based/-1
+  const int *nbr16 = nullptr;    // level 0: the mesh's packed 16-bit offsets, if any
   DevBuf<int> nbr_own;
   const double *ac64 = nullptr, *anb64 = nullptr;
   DevBuf<double> ac64_own, anb64_own;
@@ -1049,6 +1049,7 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
       }
     if (l == 0) {
       D.nbr = m->d_nbr.p;
+      D.nbr16 = m->d_nbr16.p;
     } else {
       D.nbr_own.upload(nbr0, s);
       D.nbr = D.nbr_own.p;
@@ -1277,17 +1278,18 @@ static void launch_sweep(fs_mesh *m, DLevel &D, int level, const T *b, const T *
   const int site = level == 0 ? FS_CLOCK_FINE_SWEEP : 0;
   double bytes = 0.0;
   if (site)
-    bytes = (double)D.n * (3.0 * sizeof(T) + sizeof(TO)) + 4.0 * D.n * D.w + (double)sizeof(T) * m->nnz +
-            (PRO ? 4.0 * D.n + (double)sizeof(T) * nc : 0.0);
+    bytes = (double)D.n * (3.0 * sizeof(T) + sizeof(TO)) + (D.nbr16 ? 4.0 * ((D.w + 1) / 2) : 4.0 * D.w) * D.n +
+            (double)sizeof(T) * m->nnz + (PRO ? 4.0 * D.n + (double)sizeof(T) * nc : 0.0);
+  const EllNbr nb(D.nbr, D.nbr16);
   if (D.w <= 8)
-    timed_launch(m, site, bytes, k_sweep<1, T, TO, PRO>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac,
-                 anb, b, x, parent, xc, alpha, om, xn, flags);
+    timed_launch(m, site, bytes, k_sweep<1, T, TO, PRO>, dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, nb, ac, anb,
+                 b, x, parent, xc, alpha, om, xn, flags);
   else if (D.w <= 16)
-    timed_launch(m, site, bytes, k_sweep<4, T, TO, PRO>, dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, D.nbr,
-                 ac, anb, b, x, parent, xc, alpha, om, xn, flags);
+    timed_launch(m, site, bytes, k_sweep<4, T, TO, PRO>, dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, nb, ac,
+                 anb, b, x, parent, xc, alpha, om, xn, flags);
   else if (D.w <= 32)
-    timed_launch(m, site, bytes, k_sweep<8, T, TO, PRO>, dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, D.nbr,
-                 ac, anb, b, x, parent, xc, alpha, om, xn, flags);
+    timed_launch(m, site, bytes, k_sweep<8, T, TO, PRO>, dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, nb, ac,
+                 anb, b, x, parent, xc, alpha, om, xn, flags);
   else
     timed_launch(m, site, bytes, k_sweep<16, T, TO, PRO>, dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, D.w,
                  D.nbr, ac, anb, b, x, parent, xc, alpha, om, xn, flags);
@@ -1302,18 +1304,18 @@ static void launch_resid(fs_mesh *m, DLevel &D, const T *b, const T *x, T *r, co
                        D.sell_row.p, D.sell_col.p, LV<T>::val(D), ac, b, x, r, flags);
     return;
   }
+  const EllNbr nb(D.nbr, D.nbr16);
   if (D.w <= 8)
-    hipLaunchKernelGGL((k_resid<1, T>), dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x, r,
-                       flags);
+    hipLaunchKernelGGL((k_resid<1, T>), dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.w, nb, ac, anb, b, x, r, flags);
   else if (D.w <= 16)
-    hipLaunchKernelGGL((k_resid<4, T>), dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x,
-                       r, flags);
+    hipLaunchKernelGGL((k_resid<4, T>), dim3(gfor(D.n * 4)), dim3(kBlock), 0, s, D.n, D.w, nb, ac, anb, b, x, r,
+                       flags);
   else if (D.w <= 32)
-    hipLaunchKernelGGL((k_resid<8, T>), dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b, x,
-                       r, flags);
+    hipLaunchKernelGGL((k_resid<8, T>), dim3(gfor(D.n * 8)), dim3(kBlock), 0, s, D.n, D.w, nb, ac, anb, b, x, r,
+                       flags);
   else
-    hipLaunchKernelGGL((k_resid<16, T>), dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, D.w, D.nbr, ac, anb, b,
-                       x, r, flags);
+    hipLaunchKernelGGL((k_resid<16, T>), dim3(gfor(D.n * 16)), dim3(kBlock), 0, s, D.n, D.w, nb, ac, anb, b, x, r,
+                       flags);
 }
 
 // multicolour symmetric Gauss-Seidel only where it is cheap: few colours,

@@ -123,6 +123,27 @@ int fs_mesh_create(int n_cells, int width, const int *elem_nbr, const int *n_nbr
         m->h_nbr_dev[t] = (j < n_nbr[i] && k > 0) ? k - 1 : -1;
       }
     m->d_nbr.upload(m->h_nbr_dev, m->stream);
+    {
+      // 16-bit offsets when the mesh's numbering keeps every neighbour within
+      // +-32767 cells (the row loops then read 2 bytes per slot instead of 4)
+      const int wp = (width + 1) / 2;
+      std::vector<int> d16((size_t)wp * n_cells, 0);
+      bool fits = true;
+      for (int i = 0; i < n_cells && fits; ++i)
+        for (int j = 0; j < width; ++j) {
+          const size_t t = (size_t)j * n_cells + i;
+          const int k = m->h_nbr_dev[t];
+          if (k < 0) continue;
+          const int d = k - i;
+          if (d == 0 || d < -32767 || d > 32767) {
+            fits = false;
+            break;
+          }
+          const unsigned h = (unsigned)(unsigned short)(short)d;
+          d16[(size_t)(j / 2) * n_cells + i] |= (int)(j % 2 ? h << 16 : h);
+        }
+      if (fits) m->d_nbr16.upload(d16, m->stream);
+    }
     if (n_ifaces > 0 && owner && neighbour) {
       m->nif = n_ifaces;
       m->h_owner.assign(owner, owner + n_ifaces);

@@ -102,6 +102,8 @@ struct fs_mesh {
   std::vector<int> h_nnbr;
   std::vector<int> h_nbr_dev; // 0-based, -1 = none (the device layout)
   fs::DevBuf<int> d_nbr;    // 0-based, -1 = none (slots past nnbr as well)
+  fs::DevBuf<int> d_nbr16;  // the same as int16 offsets k - i (0 = none), two
+                            // slots per word, when every offset fits
   fs::DevBuf<int> d_mir;    // transpose map, linear anb index or -1
   // exact natural-order D-ILU schedule (FactorizeUsingILU.for:29-75)
   int ilu_nlevels = 0;

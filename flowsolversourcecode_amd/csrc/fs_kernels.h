@@ -60,14 +60,45 @@ __device__ __forceinline__ int xcd_block() {
 // instead of waiting on each slot's index before fetching its value; slots
 // past w re-read slot w-1 and empty slots read x_i, both discarded, so the
 // result is exactly that of the plain slot loop.
+// Neighbours of the ELL rows: the 0-based column-major indices (-1 = none)
+// and, when every offset k - i of the mesh fits 16 bits, the same neighbours
+// as int16 offsets from the row (0 = none: a cell is never its own
+// neighbour) packed two slots to a 32-bit word (slot 2p in the low half,
+// 2p+1 in the high half, word p of cell i at d16[p n + i]): half the index
+// bytes in full-width loads.
+struct EllNbr {
+  const int *i32 = nullptr;
+  const int *d16 = nullptr;
+  EllNbr() = default;
+  __host__ __device__ EllNbr(const int *p, const int *d = nullptr) : i32(p), d16(d) {}
+};
+
+// the neighbours of slots j0 .. j0+B-1 of row i (B even; slots past w
+// repeat the last slot), -1 for none
+template <int B>
+__device__ __forceinline__ void ell_batch(const EllNbr &nb, int i, int n, int w, int j0, int *k) {
+  if (nb.d16) {
+    const int wp = (w + 1) >> 1;
+#pragma unroll
+    for (int u = 0; u < B / 2; ++u) {
+      const int v = nb.d16[(size_t)min((j0 >> 1) + u, wp - 1) * n + i];
+      const int d0 = (short)(v & 0xffff), d1 = v >> 16;
+      k[2 * u] = d0 ? i + d0 : -1;
+      k[2 * u + 1] = d1 ? i + d1 : -1;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < B; ++u) k[u] = nb.i32[(size_t)min(j0 + u, w - 1) * n + i];
+  }
+}
+
 template <int B, class T, class Op>
-__device__ __forceinline__ T ell_fold(int i, int n, int w, const int *__restrict__ nbr, const T *__restrict__ anb,
-                                      const T *x, T acc, Op op) {
+__device__ __forceinline__ T ell_fold(int i, int n, int w, EllNbr nb, const T *__restrict__ anb, const T *x, T acc,
+                                      Op op) {
   for (int j0 = 0; j0 < w; j0 += B) {
     int k[B];
     T a[B], xv[B];
-#pragma unroll
-    for (int u = 0; u < B; ++u) k[u] = nbr[(size_t)min(j0 + u, w - 1) * n + i];
+    ell_batch<B>(nb, i, n, w, j0, k);
 #pragma unroll
     for (int u = 0; u < B; ++u) {
       a[u] = anb[(size_t)min(j0 + u, w - 1) * n + i];

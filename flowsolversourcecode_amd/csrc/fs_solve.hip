@@ -56,7 +56,7 @@ static inline int grid_red(int n) {
 
 // y = A*x in the order of dsprsax (PBCGMethod.for:324-329): diagonal first,
 // then the off-diagonals in slot order.
-__device__ __forceinline__ double ell_ax(int i, int n, int w, const int *__restrict__ nbr,
+__device__ __forceinline__ double ell_ax(int i, int n, int w, EllNbr nbr,
                                          const double *__restrict__ ac,
                                          const double *__restrict__ anb,
                                          const double *__restrict__ x) {
@@ -67,7 +67,7 @@ __device__ __forceinline__ double ell_ax(int i, int n, int w, const int *__restr
 
 // r = b - A*x in the order of CheckConvergence (CheckConvergenceRoutine.for:
 // 23-34): b - ac*x first, then each off-diagonal subtracted in slot order.
-__device__ __forceinline__ double ell_res(int i, int n, int w, const int *__restrict__ nbr,
+__device__ __forceinline__ double ell_res(int i, int n, int w, EllNbr nbr,
                                           const double *__restrict__ ac,
                                           const double *__restrict__ anb,
                                           const double *__restrict__ b,
@@ -122,7 +122,7 @@ __global__ void k_init_norms(int n, const double *__restrict__ ac, const double 
 }
 
 // L1 norm of the true residual bc - A*x (CheckConvergence).
-__global__ void k_resid_l1(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+__global__ void k_resid_l1(int n, int w, EllNbr nbr, const double *__restrict__ ac,
                            const double *__restrict__ anb, const double *__restrict__ b,
                            const double *__restrict__ x, double *__restrict__ part,
                            const int *__restrict__ flags) {
@@ -182,7 +182,7 @@ __global__ void k_cg_update(int n, double *__restrict__ x, double *__restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
-__global__ void k_resid_vec(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+__global__ void k_resid_vec(int n, int w, EllNbr nbr, const double *__restrict__ ac,
                             const double *__restrict__ anb, const double *__restrict__ b,
                             const double *__restrict__ x, double *__restrict__ r,
                             double *__restrict__ r0, double *__restrict__ part) {
@@ -302,7 +302,7 @@ __global__ void k_ilu_level(int cnt, const int *__restrict__ cells, const int *_
 // One colour of a multicolour Gauss-Seidel sweep: dphi_i = (bc_i - sum
 // anb*dphi_nb)/ac_i for the cells of the colour (SORMethod.for:22-36 per cell).
 __global__ void k_gs_color(int cnt, const int *__restrict__ cells, int n, int w,
-                           const int *__restrict__ nbr, const double *__restrict__ ac,
+                           EllNbr nbr, const double *__restrict__ ac,
                            const double *__restrict__ anb, const double *__restrict__ bc,
                            double *__restrict__ x, const int *__restrict__ flags) {
   if (flags && flags[F_DONE]) return;
@@ -349,7 +349,7 @@ __global__ void k_bicg_p(int n, const double *__restrict__ r, const double *__re
 }
 
 // v = A*phat and partial (r0, v)
-__global__ void k_spmv_dot(int n, int w, const int *__restrict__ nbr, const double *__restrict__ ac,
+__global__ void k_spmv_dot(int n, int w, EllNbr nbr, const double *__restrict__ ac,
                            const double *__restrict__ anb, const double *__restrict__ x,
                            double *__restrict__ y, const double *__restrict__ d,
                            double *__restrict__ part, const int *__restrict__ flags) {
@@ -374,7 +374,7 @@ __global__ void k_bicg_s(int n, const double *__restrict__ r, const double *__re
 }
 
 // t = A*shat, partials (t,s) and (t,t)
-__global__ void k_spmv_dot_ts(int n, int w, const int *__restrict__ nbr,
+__global__ void k_spmv_dot_ts(int n, int w, EllNbr nbr,
                               const double *__restrict__ ac, const double *__restrict__ anb,
                               const double *__restrict__ x, double *__restrict__ t,
                               const double *__restrict__ s, double *__restrict__ part,
@@ -1166,8 +1166,8 @@ template <class F> static int run_batches(fs_mesh *m, int itmax, F body) {
 static void scale_guess(fs_mesh *m, const double *ac, const double *anb, const double *bc, double *x, double *v) {
   const int n = m->n, gr = grid_red(n);
   hipStream_t s = m->stream;
-  hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, m->w, m->d_nbr.p, ac, anb, x, v, bc,
-                     m->partial.p, m->flags.p);
+  hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, m->w, EllNbr(m->d_nbr.p, m->d_nbr16.p), ac, anb, x,
+                     v, bc, m->partial.p, m->flags.p);
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM_SCALE, gr, n, 4, m->partial.p, m->scal.p,
                      m->flags.p);
   hipLaunchKernelGGL(k_warm_scale, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, m->scal.p, m->flags.p);
@@ -1183,6 +1183,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
   double *part = m->partial.p, *scal = m->scal.p;
   int *flags = m->flags.p;
   const int *nbr = m->d_nbr.p;
+  const EllNbr nb(m->d_nbr.p, m->d_nbr16.p); // the row loops of the hot kernels
   SolveWork &sw = work(m);
 
   double h_sc[S_NSLOTS] = {0};
@@ -1225,7 +1226,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
                          sw.tsrc.p, ac, anb, V2, V3, V4, V5, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_PBCG_AK, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_pbcg_axpy, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, V1, V2, V4, V5, scal, flags);
-      hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, flags);
+      hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
   } else if (solver == FS_SOR) {
@@ -1235,14 +1236,14 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       for (int c = 0; c < nc; ++c) {
         const int b = sw.color_ptr[c], e = sw.color_ptr[c + 1];
         hipLaunchKernelGGL(k_gs_color, dim3(grid_for(e - b)), dim3(kBlock), 0, s, e - b,
-                           sw.color_cells.p + b, n, w, nbr, ac, anb, bc, dphi, flags);
+                           sw.color_cells.p + b, n, w, nb, ac, anb, bc, dphi, flags);
       }
       for (int c = nc - 1; c >= 0; --c) {
         const int b = sw.color_ptr[c], e = sw.color_ptr[c + 1];
         hipLaunchKernelGGL(k_gs_color, dim3(grid_for(e - b)), dim3(kBlock), 0, s, e - b,
-                           sw.color_cells.p + b, n, w, nbr, ac, anb, bc, dphi, flags);
+                           sw.color_cells.p + b, n, w, nb, ac, anb, bc, dphi, flags);
       }
-      hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, flags);
+      hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
   } else if (solver == FS_BICGS) {
@@ -1250,10 +1251,12 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     const bool pre = (lmg == FS_MG_PRECOND);
     if (pre) amg_prepare(m, ac, anb);
     const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
-    // algorithmic bytes of y = A x with the (r0, y) partials: ac, nbr, anb, x, y, r0
-    const double spmv_bytes = 8.0 * n + 4.0 * n * w + 8.0 * m->nnz + 24.0 * n;
+    // algorithmic bytes of y = A x with the (r0, y) partials: ac, the
+    // neighbour words (4 per slot, or 4 per pair of slots), anb, x, y, r0
+    const double ib = nb.d16 ? 4.0 * ((w + 1) / 2) : 4.0 * w;
+    const double spmv_bytes = 8.0 * n + ib * n + 8.0 * m->nnz + 24.0 * n;
     if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
-    hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, V1, part);
+    hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, V0, V1, part);
     if (warm) {
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, V1, bc, flags);
@@ -1265,13 +1268,13 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V2, scal, flags);
       if (pre) amg_apply(m, V2, V6, flags, alpha);
       const double *ph = pre ? V6 : V2;
-      timed_launch(m, FS_CLOCK_SPMV, spmv_bytes, k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, ph,
+      timed_launch(m, FS_CLOCK_SPMV, spmv_bytes, k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, ph,
                    V3, (const double *)V1, part, (const int *)flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_ALPHA, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_bicg_s, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V4, scal, flags);
       if (pre) amg_apply(m, V4, V7, flags, alpha);
       const double *sh = pre ? V7 : V4;
-      hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, sh, V5, V4, part, flags);
+      hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, sh, V5, V4, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_OMEGA, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, ph, sh, V4, V5, scal, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
@@ -1282,9 +1285,9 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     if (pre) amg_prepare(m, ac, anb);
     const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
     // y = A p with the (p, y) partials: ac, nbr, anb, p, y
-    const double spmv_bytes = 8.0 * n + 4.0 * n * w + 8.0 * m->nnz + 16.0 * n;
+    const double spmv_bytes = 8.0 * n + (nb.d16 ? 4.0 * ((w + 1) / 2) : 4.0 * w) * n + 8.0 * m->nnz + 16.0 * n;
     if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
-    hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, nullptr,
+    hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, V0, nullptr,
                        part);
     if (warm) {
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
@@ -1299,7 +1302,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V0, z, nullptr, nullptr, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CG_RHO, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_cg_p, dim3(ge), dim3(kBlock), 0, s, n, z, V2, scal, flags);
-      timed_launch(m, FS_CLOCK_SPMV, spmv_bytes, k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, V2,
+      timed_launch(m, FS_CLOCK_SPMV, spmv_bytes, k_spmv_dot, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, V2,
                    V3, (const double *)V2, part, (const int *)flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CG_ALPHA, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_cg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, V2, V3, scal, part, flags);
@@ -1385,7 +1388,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
           hipLaunchKernelGGL(k_ilu_bwd, dim3(grid_for(b1 - b0)), dim3(kBlock), 0, s, b1 - b0, S.bwd_cells.p + b0,
                              S.bwd_ptr.p, S.bwd_lin.p, S.bwd_src.p, anb, sw.dc.p, rcv, dphi, flags);
       }
-      hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, flags);
+      hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
   } else {
@@ -1399,7 +1402,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
   // true final residual, fit += dphi (:180)
   int fl[F_NFLAGS];
   host_flags(m, fl);
-  hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, part, nullptr);
+  hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, part, nullptr);
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_TRUE_RES, gr, n, nf, part, scal, flags);
   add_inplace(m, fit, dphi);
   double h_out[S_NSLOTS];
@@ -1455,11 +1458,11 @@ int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, 
   FS_HIP(hipEventCreate(&e1));
   int h_fl[F_NFLAGS] = {0, 0, 1, 0};
   FS_HIP(hipMemcpyAsync(m->flags.p, h_fl, sizeof(h_fl), hipMemcpyHostToDevice, m->stream));
-  hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, m->d_nbr.p, ac, anb, x,
+  hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, EllNbr(m->d_nbr.p, m->d_nbr16.p), ac, anb, x,
                      y, dot_with, m->partial.p, m->flags.p);
   FS_HIP(hipEventRecord(e0, m->stream));
   for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, m->d_nbr.p, ac, anb,
+    hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, EllNbr(m->d_nbr.p, m->d_nbr16.p), ac, anb,
                        x, y, dot_with, m->partial.p, m->flags.p);
   FS_HIP(hipEventRecord(e1, m->stream));
   FS_HIP(hipEventSynchronize(e1));

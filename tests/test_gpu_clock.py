@@ -1,7 +1,8 @@
 """The kernel clock (fs_kernel_clock_*, include/fsamd.h): HIP events recorded
 by the solve loop's own dispatches, for the roofline figure of the bench.
 Counts and bytes are fixed by the loop's structure: one ELL product per
-BiCGSTAB iteration, the algorithmic bytes of one product per launch; a
+BiCGSTAB iteration, the algorithmic bytes of one product per launch (with
+16-bit neighbour offsets); a
 stride of k times every k-th launch."""
 import numpy as np
 import pytest
@@ -32,7 +33,8 @@ def test_clock_counts_and_bytes():
     # one product per iteration that did work; later launches of the last
     # batch return at once and are counted apart
     assert sp["launches"] == st["iterations"]
-    assert sp["bytes"] == pytest.approx(8.0 * n + 4.0 * n * w + 8.0 * nnz + 24.0 * n)
+    # neighbour offsets of this numbering fit 16 bits: 2 bytes per slot
+    assert sp["bytes"] == pytest.approx(8.0 * n + 2.0 * n * w + 8.0 * nnz + 24.0 * n)
     assert sp["avg_ms"] > 0.0
     assert c["fine_sweep"]["launches"] > 0 and c["fine_sweep"]["avg_ms"] > 0.0
 

@@ -148,3 +148,37 @@ def test_invalid_solver_code_reports_error():
     m = _mesh(g)
     with pytest.raises(fs.FsError):
         m.solve_equation("pressc", g["fit"], 1e-8, 10, 3, 0, g["ac"], g["anb"], g["bc"])
+
+
+def _permuted(nbr, nnbr, ac, anb, bc, perm):
+    """The same system with cell c renumbered perm[c]."""
+    w, n = nbr.shape
+    nbr2 = np.zeros_like(nbr)
+    anb2 = np.zeros_like(anb)
+    for s in range(w):
+        k = nbr[s]
+        nbr2[s, perm] = np.where(k > 0, perm[np.maximum(k, 1) - 1] + 1, 0)
+        anb2[s, perm] = anb[s]
+    out = [np.zeros_like(a) for a in (nnbr, ac, bc)]
+    for o, a in zip(out, (nnbr, ac, bc)):
+        o[perm] = a
+    return nbr2, out[0], out[1], anb2, out[2]
+
+
+@pytest.mark.parametrize("solver,lmg,shift", [("bicgs", 2, 0.05), ("cg", 2, 0.05), ("sor", 0, 1.0)])
+def test_wide_numbering_uses_32bit_neighbours(solver, lmg, shift):
+    """Neighbour offsets beyond +-32767 (a shuffled numbering of 38 400
+    cells) take the 32-bit index path of the row loops; the 16-bit path
+    serves the natural numbering.  Both reach the same solution."""
+    nbr, nnbr, ac, anb, bc = systems.pressure_like(40, 40, 24, 21, shift=shift)
+    perm = np.random.default_rng(3).permutation(len(ac))
+    p = _permuted(nbr, nnbr, ac, anb, bc, perm)
+    assert np.abs(np.where(p[0] > 0, p[0] - 1 - np.arange(len(ac))[None, :], 0)).max() > 32767
+    sols = []
+    for sys_ in ((nbr, nnbr, ac, anb, bc), p):
+        m = fs.Mesh(sys_[0], sys_[1])
+        rc, fit, dphi, res4, st = m.solve_equation("pressc", np.zeros(len(ac)), 1e-10, 3000, solver, lmg,
+                                                   sys_[2], sys_[3], sys_[4])
+        assert rc == 0, st
+        sols.append(dphi)
+    np.testing.assert_allclose(sols[1][perm], sols[0], rtol=0, atol=1e-7 * np.abs(sols[0]).max())

@@ -483,6 +483,41 @@ __global__ void k_enter(int n, const T *__restrict__ ac, const double *__restric
   x[i] = sweep ? omega * bi / ac[i] : T(0);
 }
 
+// Entry of the cycle fused with the residual of its first sweep, for one
+// pre-sweep on the finest level: b = (T)rhs, x = omega*b/ac, r = b - A x,
+// where the neighbours' x_j = omega*(T)rhs_j/ac_j are formed from their
+// right-hand sides (x is pointwise in b), so no second pass is needed.  The
+// values are those of k_enter followed by k_resid.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_enter_resid(int n, int w, EllNbr nbr, const T *__restrict__ ac, const T *__restrict__ anb,
+              const double *__restrict__ rhs, T *__restrict__ b, T *__restrict__ x, T *__restrict__ r, T omega,
+              const int *__restrict__ flags) {
+  if (flags && flags[0]) return;
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T bi = (T)rhs[i];
+  if (b) b[i] = bi;
+  const T xi = omega * bi / ac[i];
+  x[i] = xi;
+  T s = T(0);
+  for (int j0 = 0; j0 < w; j0 += kEllBatch) {
+    int k[kEllBatch];
+    T a[kEllBatch], xv[kEllBatch];
+    ell_batch<kEllBatch>(nbr, i, n, w, j0, k);
+#pragma unroll
+    for (int u = 0; u < kEllBatch; ++u) {
+      const int kk = k[u] >= 0 ? k[u] : i;
+      a[u] = anb[(size_t)min(j0 + u, w - 1) * n + i];
+      xv[u] = omega * (T)rhs[kk] / ac[kk];
+    }
+#pragma unroll
+    for (int u = 0; u < kEllBatch; ++u)
+      if (j0 + u < w && k[u] >= 0) s += a[u] * xv[u];
+  }
+  r[i] = bi - ac[i] * xi - s;
+}
+
 // Restriction b_c[p] = sum of the member residuals in increasing fine-cell
 // order; mode 1 also takes the coarse level's first sweep from a zero start
 // (x_c = omega*b_c/ac_c), mode 2 sets x_c = 0, mode 0 leaves x_c alone.
@@ -1432,7 +1467,10 @@ static void vcycle(fs_mesh *m, const double *rhs, double *out, double alpha, con
   std::vector<T *> res(L, nullptr), oth(L, nullptr);
   // levels above the bottom: grid-wide kernels
   const int top = ts < L ? ts : L - 1;
-  {
+  // one Jacobi pre-sweep on the finest level: the entry and the residual of
+  // that sweep in one kernel
+  const bool fuse0 = !use_gs(m, D0) && pre0 == 1 && top > 0;
+  if (!fuse0) {
     const bool gs = use_gs(m, D0);
     hipLaunchKernelGGL(k_enter<T>, dim3(gfor(D0.n)), dim3(kBlock), 0, s, D0.n, LV<T>::ac(D0), rhs,
                        fp64 ? nullptr : LV<T>::b(D0), LV<T>::x(D0), om, (!gs && pre0 > 0) ? 1 : 0, flags);
@@ -1441,12 +1479,22 @@ static void vcycle(fs_mesh *m, const double *rhs, double *out, double alpha, con
     DLevel &D = H.lev[l];
     const T *b = l == 0 ? b0 : LV<T>::b(D);
     const int npre = l == 0 ? pre0 : cpre;
-    res[l] = pre_smooth<T>(m, D, l, b, npre, om, oth[l], flags);
     DLevel &C = H.lev[l + 1];
     int mode;
     if (l + 1 == ts) mode = 0;                                  // the tail kernel takes over
     else if (l + 1 == L - 1) mode = 1;                          // grid-wide coarsest: `coarse` sweeps
     else mode = (!use_gs(m, C) && cpre > 0) ? 1 : 2;
+    if (l == 0 && fuse0) {
+      hipLaunchKernelGGL(k_enter_resid<T>, dim3(gfor(D0.n)), dim3(kBlock), 0, s, D0.n, D0.w,
+                         EllNbr(D0.nbr, D0.nbr16), LV<T>::ac(D0), LV<T>::anb(D0), rhs, fp64 ? nullptr : LV<T>::b(D0),
+                         LV<T>::x(D0), LV<T>::r(D0), om, flags);
+      res[0] = LV<T>::x(D0);
+      oth[0] = LV<T>::t(D0);
+      hipLaunchKernelGGL(k_restrict<T>, dim3(gfor(C.n)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p, LV<T>::r(D0),
+                         LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+      continue;
+    }
+    res[l] = pre_smooth<T>(m, D, l, b, npre, om, oth[l], flags);
     if (l > 0 && !use_gs(m, D) && !D.sell) {
       launch_resid_restrict<T>(m, D, C, b, res[l], om, mode, flags);
     } else {

@@ -396,22 +396,29 @@ __global__ void k_spmv_dot_ts(int n, int w, EllNbr nbr,
 }
 
 // x += alpha*phat + omega*shat; r = s - omega*t; partial L1(r)
+// x += alpha phat + omega shat, r = s - omega t, with the partials of |r|_1
+// and of the next iteration's rho = (r0, r) (the sums k_dot2 would form)
 __global__ void k_bicg_update(int n, double *__restrict__ x, double *__restrict__ r,
                               const double *__restrict__ phat, const double *__restrict__ shat,
                               const double *__restrict__ s, const double *__restrict__ t,
-                              const double *__restrict__ scal, double *__restrict__ part,
-                              const int *__restrict__ flags) {
+                              const double *__restrict__ r0, const double *__restrict__ scal,
+                              double *__restrict__ part, const int *__restrict__ flags) {
   if (flags[F_DONE]) return;
   const double al = scal[S_ALPHA], om = scal[S_OMEGA];
-  double a = 0.0;
+  double a = 0.0, d = 0.0;
   for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     x[i] = x[i] + al * phat[i] + om * shat[i];
     const double ri = s[i] - om * t[i];
     r[i] = ri;
     a += fabs(ri);
+    d += r0[i] * ri;
   }
   a = block_sum<kBlock>(a);
-  if (threadIdx.x == 0) part[blockIdx.x] = a;
+  d = block_sum<kBlock>(d);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[kPartials + blockIdx.x] = d;
+  }
 }
 
 // ---- the reference's ILU iteration (ILUMethod.for:4): rc = bc - A dphi with
@@ -677,7 +684,8 @@ enum FinOp {
   FIN_CG_RHO,         // rz = (r,z), beta
   FIN_CG_ALPHA,       // alpha = rz/(p,q)
   FIN_WARM,           // reject an initial guess whose L1 residual exceeds ires
-  FIN_WARM_SCALE      // gamma = (A x0, b)/(A x0, A x0)
+  FIN_WARM_SCALE,     // gamma = (A x0, b)/(A x0, A x0)
+  FIN_CONV_BICG_RHO   // FIN_CONV, then (unless done) FIN_BICG_RHO of the next iteration
 };
 
 // one workgroup of kFinBlock threads combines up to kPartials partials
@@ -692,6 +700,22 @@ __device__ double reduce_partials_max(const double *part, int np) {
   return block_max<kFinBlock>(s);
 }
 
+// rho = (r0, r) and beta of BiCGSTAB.  Breakdown (rho or the last omega
+// zero, or a non-finite beta, as on a singular system whose right-hand side
+// carries round-off outside the range): stop with the current iterate.
+__device__ void bicg_rho(double a, double *__restrict__ scal, int *__restrict__ flags) {
+  if (flags[F_ITER] > 0) {
+    const double beta = (a / scal[S_RHO]) * (scal[S_ALPHA] / scal[S_OMEGA]);
+    if (scal[S_OMEGA] == 0.0 || !isfinite(beta)) {
+      flags[F_DONE] = 1;
+      return;
+    }
+    scal[S_BETA] = beta;
+  }
+  scal[S_RHO] = a;
+  if (a == 0.0) flags[F_DONE] = 1;
+}
+
 __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ part,
                       double *__restrict__ scal, int *__restrict__ flags) {
   if (op != FIN_INIT_NORMS && op != FIN_TRUE_RES && flags[F_DONE]) return;
@@ -701,7 +725,7 @@ __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ 
     b = reduce_partials_max(part + kPartials, np);
     c = reduce_partials(part + 2 * kPartials, np);
     d = reduce_partials_max(part + 3 * kPartials, np);
-  } else if (op == FIN_BICG_OMEGA || op == FIN_WARM_SCALE) {
+  } else if (op == FIN_BICG_OMEGA || op == FIN_WARM_SCALE || op == FIN_CONV_BICG_RHO) {
     b = reduce_partials(part + kPartials, np);
   }
   if (threadIdx.x != 0) return;
@@ -735,20 +759,15 @@ __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ 
     flags[F_DONE] = !(a > scal[S_RRF] * scal[S_IRES] && it < flags[F_ITMAX]);
     break;
   }
-  case FIN_BICG_RHO: {
-    // breakdown (rho or the last omega zero, or a non-finite beta, as on a
-    // singular system whose right-hand side carries round-off outside the
-    // range): stop with the current iterate
-    if (flags[F_ITER] > 0) {
-      const double beta = (a / scal[S_RHO]) * (scal[S_ALPHA] / scal[S_OMEGA]);
-      if (scal[S_OMEGA] == 0.0 || !isfinite(beta)) {
-        flags[F_DONE] = 1;
-        break;
-      }
-      scal[S_BETA] = beta;
-    }
-    scal[S_RHO] = a;
-    if (a == 0.0) flags[F_DONE] = 1;
+  case FIN_BICG_RHO:
+    bicg_rho(a, scal, flags);
+    break;
+  case FIN_CONV_BICG_RHO: {
+    scal[S_FRES] = a;
+    const int it = flags[F_ITER] + 1;
+    flags[F_ITER] = it;
+    flags[F_DONE] = !(a > scal[S_RRF] * scal[S_IRES] && it < flags[F_ITMAX]);
+    if (!flags[F_DONE]) bicg_rho(b, scal, flags);
     break;
   }
   case FIN_BICG_ALPHA:
@@ -1262,9 +1281,10 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, V1, bc, flags);
     }
     FS_HIP(hipMemsetAsync(V3, 0, sizeof(double) * n, s));
+    // rho of the first iteration; every later rho comes with the update
+    hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V1, V0, nullptr, nullptr, part, flags);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_RHO, gr, n, nf, part, scal, flags);
     iters = run_batches(m, itmax, [&]() {
-      hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V1, V0, nullptr, nullptr, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_RHO, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V2, scal, flags);
       if (pre) amg_apply(m, V2, V6, flags, alpha);
       const double *ph = pre ? V6 : V2;
@@ -1276,8 +1296,10 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       const double *sh = pre ? V7 : V4;
       hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, sh, V5, V4, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_OMEGA, gr, n, nf, part, scal, flags);
-      hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, ph, sh, V4, V5, scal, part, flags);
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
+      hipLaunchKernelGGL(k_bicg_update, dim3(gr), dim3(kBlock), 0, s, n, dphi, V0, ph, sh, V4, V5, (const double *)V1,
+                         scal, part, flags);
+      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV_BICG_RHO, gr, n, nf, part, scal,
+                         flags);
     });
   } else if (solver == FS_CG) {
     // preconditioned conjugate gradient; r=V0 z=V1 p=V2 q=V3

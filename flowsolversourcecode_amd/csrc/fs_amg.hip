@@ -528,8 +528,22 @@ __global__ void k_restrict(int nc, const int *__restrict__ mptr, const int *__re
   if (flags && flags[0]) return;
   const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= nc) return;
+  const int q0 = mptr[p], q1 = mptr[p + 1];
   T s = T(0);
-  for (int t = mptr[p]; t < mptr[p + 1]; ++t) s = s + rf[mem[t]];
+  // members in batches: their indices, then their residuals, then the sum
+  // in member order
+  constexpr int B = 8;
+  for (int q = q0; q < q1; q += B) {
+    int id[B];
+    T v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) id[u] = mem[min(q + u, q1 - 1)];
+#pragma unroll
+    for (int u = 0; u < B; ++u) v[u] = rf[id[u]];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (q + u < q1) s = s + v[u];
+  }
   bc[p] = s;
   if (mode == 1) xc[p] = omega * s / acc[p];
   else if (mode == 2) xc[p] = T(0);

@@ -196,37 +196,30 @@ __global__ void k_wall_pgrad(int Fb, int Fi, int N, const int *__restrict__ own,
   bgp[2 * Fb + b] = (1 - z * z) * gz - z * x * gx - z * y * gy;
 }
 
-// Face gradients (InterpolateGradientFromElementToFace.for:16-93): 'average'
-// for u, v, w and 'averagecorrected' for p.  fg = 12 blocks of Fi:
-// [u x,y,z | v x,y,z | w x,y,z | p x,y,z].
+// Pressure face gradient (InterpolateGradientFromElementToFace.for:16-93,
+// 'averagecorrected') into fg[9 Fi .. 12 Fi) (blocks x, y, z); the velocity
+// face gradients ('average') are formed inside k_mom_face3.
 __global__ void k_face_grads(int Fi, int N, const int *__restrict__ own,
                              const int *__restrict__ nei, const double *__restrict__ g,
                              const double *__restrict__ d, const double *__restrict__ ex,
                              const double *__restrict__ ey, const double *__restrict__ ez,
-                             const double *__restrict__ gu, const double *__restrict__ gv,
-                             const double *__restrict__ gw, const double *__restrict__ gp,
-                             const double *__restrict__ p, double *__restrict__ fg) {
+                             const double *__restrict__ gp, const double *__restrict__ p,
+                             double *__restrict__ fg) {
   const int f = xcd_block() * kBlock + threadIdx.x;
   if (f >= Fi) return;
   const int i = own[f], j = nei[f];
   const double gf = g[f], og = 1. - gf;
-  const double *G[4] = {gu, gv, gw, gp};
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    double fx = gf * G[k][i] + og * G[k][j];
-    double fy = gf * G[k][N + i] + og * G[k][N + j];
-    double fz = gf * G[k][2 * N + i] + og * G[k][2 * N + j];
-    if (k == 3) {
-      double corr = (p[j] - p[i]) / d[f];
-      corr = corr - (fx * ex[f] + fy * ey[f] + fz * ez[f]);
-      fx = fx + corr * ex[f];
-      fy = fy + corr * ey[f];
-      fz = fz + corr * ez[f];
-    }
-    fg[(size_t)(3 * k) * Fi + f] = fx;
-    fg[(size_t)(3 * k + 1) * Fi + f] = fy;
-    fg[(size_t)(3 * k + 2) * Fi + f] = fz;
-  }
+  double fx = gf * gp[i] + og * gp[j];
+  double fy = gf * gp[N + i] + og * gp[N + j];
+  double fz = gf * gp[2 * N + i] + og * gp[2 * N + j];
+  double corr = (p[j] - p[i]) / d[f];
+  corr = corr - (fx * ex[f] + fy * ey[f] + fz * ez[f]);
+  fx = fx + corr * ex[f];
+  fy = fy + corr * ey[f];
+  fz = fz + corr * ez[f];
+  fg[(size_t)9 * Fi + f] = fx;
+  fg[(size_t)10 * Fi + f] = fy;
+  fg[(size_t)11 * Fi + f] = fz;
 }
 
 struct MomArgs {
@@ -236,57 +229,77 @@ struct MomArgs {
   const double *bu, *bv, *bw;              // boundary velocities
   const int *btype;                        // per boundary face, nullptr = all walls
   const double *gu, *gv, *gw;              // cell gradients (boundary copies)
-  const double *fg;                        // face gradients (12 blocks)
   const double *mdot;
 };
 
-// Momentum face fluxes of interior faces in the reference's order:
-// diffusion (AssembleIsotropicDiffusion.for:99-117), explicit diffusion
+// Momentum face fluxes of the interior faces for u, v and w in one pass, each
+// component's in the reference's order: diffusion
+// (AssembleIsotropicDiffusion.for:99-117), explicit diffusion
 // (AssembleExplicitDiffusion.for), bulk viscosity (AssembleBulkViscosity.for),
-// upwind convection (AssembleConvection.for:72-88).
-__global__ void k_mom_face(int Fi, const int *__restrict__ own, const int *__restrict__ nei,
-                           const double *__restrict__ g, const double *__restrict__ gdiff,
-                           const double *__restrict__ Tx, const double *__restrict__ Ty,
-                           const double *__restrict__ Tz, const double *__restrict__ Sx,
-                           const double *__restrict__ Sy, const double *__restrict__ Sz, MomArgs a,
-                           double *__restrict__ cf, double *__restrict__ ff,
-                           double *__restrict__ tf) {
+// upwind convection (AssembleConvection.for:72-88).  FluxCf and FluxFf are the
+// same for the three components (cf, ff); FluxTf of component c goes to
+// tf3[c F + f].  The velocity face gradients, 'average' interpolation
+// (InterpolateGradientFromElementToFace.for:16-93), are formed here from the
+// cell gradients.  Interior fluxes depend on u, v, w only through the
+// component's own field, so those of v and w are the same whether formed
+// before or after the u solve.
+__global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, const int *__restrict__ nei,
+                            const double *__restrict__ g, const double *__restrict__ gdiff,
+                            const double *__restrict__ Tx, const double *__restrict__ Ty,
+                            const double *__restrict__ Tz, const double *__restrict__ Sx,
+                            const double *__restrict__ Sy, const double *__restrict__ Sz, double mu, double c23,
+                            const double *__restrict__ u, const double *__restrict__ v,
+                            const double *__restrict__ w, const double *__restrict__ gu,
+                            const double *__restrict__ gv, const double *__restrict__ gw,
+                            const double *__restrict__ mdot, double *__restrict__ cf, double *__restrict__ ff,
+                            double *__restrict__ tf3) {
   const int f = xcd_block() * kBlock + threadIdx.x;
   if (f >= Fi) return;
   const int i = own[f], j = nei[f];
-  const int c = a.comp;
-  const double gf = g[f];
-  const double gamf = gf * a.mu + (1. - gf) * a.mu;
-  const double pi = a.phi[i], pj = a.phi[j];
-  const double *fgc = a.fg + (size_t)(3 * c) * Fi; // face gradient of this component
-  // diffusion
+  const double gf = g[f], og = 1. - gf;
+  // face gradient of component k in direction d: fgv[k][d]
+  const double *G[3] = {gu, gv, gw};
+  double fgv[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    fgv[k][0] = gf * G[k][i] + og * G[k][j];
+    fgv[k][1] = gf * G[k][N + i] + og * G[k][N + j];
+    fgv[k][2] = gf * G[k][2 * N + i] + og * G[k][2 * N + j];
+  }
+  const double gamf = gf * mu + (1. - gf) * mu;
   const double cfl = gamf * gdiff[f], ffl = -(gamf * gdiff[f]);
-  const double vfl = -(gamf * (fgc[f] * Tx[f] + fgc[Fi + f] * Ty[f] + fgc[2 * Fi + f] * Tz[f]));
-  double C = 0.0 + cfl, Fv = 0.0 + ffl, T = 0.0 + cfl * pi + ffl * pj + vfl;
-  // explicit diffusion: -Gamma_f (grad u_c)^T . S
-  const double ga = a.fg[(size_t)c * Fi + f], gb = a.fg[(size_t)(3 + c) * Fi + f],
-               gc = a.fg[(size_t)(6 + c) * Fi + f];
-  const double v2 = -(gamf * (ga * Sx[f] + gb * Sy[f] + gc * Sz[f]));
-  C = C + 0.;
-  Fv = Fv + 0.;
-  T = T + v2;
-  // bulk viscosity
-  const double dv = a.fg[f] + a.fg[(size_t)4 * Fi + f] + a.fg[(size_t)8 * Fi + f];
-  const double S = c == 0 ? Sx[f] : c == 1 ? Sy[f] : Sz[f];
-  const double v3 = -(a.c23 * gamf * dv * S);
-  C = C + 0.;
-  Fv = Fv + 0.;
-  T = T + v3;
-  // upwind convection
-  const double md = a.mdot[f];
+  const double md = mdot[f];
   const double ccl = md > 0. ? md : 0.;
   const double fcl = -(-md > 0. ? -md : 0.);
+  // cf, ff: diffusion, explicit (+0), bulk (+0), convection
+  double C = 0.0 + cfl, Fv = 0.0 + ffl;
+  C = C + 0.;
+  Fv = Fv + 0.;
+  C = C + 0.;
+  Fv = Fv + 0.;
   C = C + ccl;
   Fv = Fv + fcl;
-  T = T + ccl * pi + fcl * pj;
   cf[f] = C;
   ff[f] = Fv;
-  tf[f] = T;
+  const double tx = Tx[f], ty = Ty[f], tz = Tz[f], sx = Sx[f], sy = Sy[f], sz = Sz[f];
+  const double dv = fgv[0][0] + fgv[1][1] + fgv[2][2];
+  const double *P[3] = {u, v, w};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double pi = P[c][i], pj = P[c][j];
+    const double vfl = -(gamf * (fgv[c][0] * tx + fgv[c][1] * ty + fgv[c][2] * tz));
+    double T = 0.0 + cfl * pi + ffl * pj + vfl;
+    // explicit diffusion: -Gamma_f (grad u_c)^T . S
+    const double v2 = -(gamf * (fgv[0][c] * sx + fgv[1][c] * sy + fgv[2][c] * sz));
+    T = T + v2;
+    // bulk viscosity
+    const double S = c == 0 ? sx : c == 1 ? sy : sz;
+    const double v3 = -(c23 * gamf * dv * S);
+    T = T + v3;
+    // upwind convection
+    T = T + ccl * pi + fcl * pj;
+    tf3[(size_t)c * F + f] = T;
+  }
 }
 
 // No-slip wall faces (AssembleIsotropicDiffusion.for:148-219) or symmetry
@@ -611,6 +624,7 @@ struct SimpleCase {
   DevBuf<double> bu, bv, bw;
   DevBuf<double> u, v, w, p, pc, mdot, gu, gv, gw, gp, gpc, bp, bgp, fg, du, dv, dw, us, vs, ws;
   DevBuf<double> fp, fv, cf, ff, tf, ac, anb, bc, dphi, part;
+  DevBuf<double> tf3; // momentum FluxTf of u, v, w (F each)
   DevBuf<double> pcprev; // the last converged p' (input of the warm-start network)
   DevBuf<int> btype, bzq; // boundary condition per boundary face (empty: all walls)
   bool sym_dirty = false;  // symmetry boundary velocities not yet derived
@@ -677,8 +691,12 @@ static void phase_momentum_gradients(SimpleCase &C) {
   hipLaunchKernelGGL(k_wall_pgrad, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N,
                      C.own.p, C.nx.p, C.ny.p, C.nz.p, C.btype.p, C.bzq.p, C.gp.p, C.bgp.p);
   hipLaunchKernelGGL(k_face_grads, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p,
-                     C.nei.p, C.g.p, C.d.p, C.ex.p, C.ey.p, C.ez.p, C.gu.p, C.gv.p, C.gw.p, C.gp.p,
-                     C.p.p, C.fg.p);
+                     C.nei.p, C.g.p, C.d.p, C.ex.p, C.ey.p, C.ez.p, C.gp.p, C.p.p, C.fg.p);
+  // the interior momentum face fluxes of all three components
+  hipLaunchKernelGGL(k_mom_face3, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.F, C.N, C.own.p, C.nei.p,
+                     C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p, C.Sx.p, C.Sy.p, C.Sz.p, C.mu,
+                     (double)(-2.0f / 3.0f), C.u.p, C.v.p, C.w.p, C.gu.p, C.gv.p, C.gw.p, C.mdot.p, C.cf.p,
+                     C.ff.p, C.tf3.p);
 }
 
 static void phase_momentum_assemble(SimpleCase &C, int comp) {
@@ -691,19 +709,18 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
   a.bu = C.bu.p; a.bv = C.bv.p; a.bw = C.bw.p;
   a.btype = C.btype.p;
   a.gu = C.gu.p; a.gv = C.gv.p; a.gw = C.gw.p;
-  a.fg = C.fg.p;
   a.mdot = C.mdot.p;
-  hipLaunchKernelGGL(k_mom_face, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.own.p, C.nei.p,
-                     C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p, C.Sx.p, C.Sy.p, C.Sz.p, a, C.cf.p,
-                     C.ff.p, C.tf.p);
+  // interior faces: k_mom_face3 (momentum gradient phase); boundary faces of
+  // this component, then the cell gather
+  double *tf = C.tf3.p + (size_t)comp * C.F;
   hipLaunchKernelGGL(k_mom_wall, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N, C.own.p,
                      C.A.p, C.nx.p, C.ny.p, C.nz.p, C.dx.p, C.dy.p, C.dz.p, C.Sx.p, C.Sy.p, C.Sz.p, a,
-                     C.cf.p, C.tf.p);
+                     C.cf.p, tf);
   double *D = comp == 0 ? C.du.p : comp == 1 ? C.dv.p : C.dw.p;
   double *st = comp == 0 ? C.us.p : comp == 1 ? C.vs.p : C.ws.p;
   by_width(C.W, [&](auto wc) {
     hipLaunchKernelGGL(k_mom_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
-                       C.code.p, C.cf.p, C.ff.p, C.tf.p, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
+                       C.code.p, C.cf.p, C.ff.p, tf, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
                        C.urf_u, C.ac.p, C.anb.p, C.bc.p, D, st);
   });
 }
@@ -944,6 +961,7 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
     C.cf.alloc(F);
     C.ff.alloc(F);
     C.tf.alloc(F);
+    C.tf3.alloc(3 * F);
     FS_HIP_THROW(hipMemsetAsync(C.ff.p, 0, sizeof(double) * F, C.s));
     C.anb.alloc(N * C.W);
     C.part.alloc(fs::kPartials);

@@ -348,8 +348,7 @@ class Case:
     walls and symmetry planes."""
 
     SIZES = {"u": "N", "v": "N", "w": "N", "p": "N", "pc": "N", "mdot": "Fi", "gu": "3N", "gv": "3N",
-             "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bgp": "3Fb",
-             "fg": "12Fi", "du": "N", "dv": "N", "dw": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
+             "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bgp": "3Fb", "du": "N", "dv": "N", "dw": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
              "anb": "NW", "bc": "N", "cf": "F", "ff": "F", "tf": "F", "vol": "N", "Sx": "F", "Sy": "F",
              "Sz": "F", "g": "F", "gdiff": "F", "Tx": "F", "Ty": "F", "Tz": "F", "dcf": "F", "A": "F",
              "kx": "Fi", "ky": "Fi", "kz": "Fi"}

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -226,7 +227,7 @@ void galerkin_lists(const HostLevel &f, HostLevel &c, std::vector<int> &ptr,
 // coarse entry e of level c = sum of its contributions (FP64), then stored
 // in both precisions when the hierarchy runs in FP32
 __global__ void k_galerkin(int ne, int nc, const int *__restrict__ ptr,
-                           const long long *__restrict__ src, int nf,
+                           const int *__restrict__ src, int nf,
                            const double *__restrict__ acf, const double *__restrict__ anbf,
                            double *__restrict__ acc_out, double *__restrict__ anbc,
                            float *__restrict__ ac32, float *__restrict__ anb32,
@@ -234,10 +235,20 @@ __global__ void k_galerkin(int ne, int nc, const int *__restrict__ ptr,
                            float *__restrict__ val32) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= ne) return;
+  // contributions in list order, loaded in batches (sources, then values)
+  const int t0 = ptr[e], t1 = ptr[e + 1];
   double a = 0.0;
-  for (int t = ptr[e]; t < ptr[e + 1]; ++t) {
-    const long long s = src[t];
-    a = a + (s < nf ? acf[s] : anbf[s - nf]);
+  constexpr int B = 4;
+  for (int t = t0; t < t1; t += B) {
+    int sv[B];
+    double v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) sv[u] = src[min(t + u, t1 - 1)];
+#pragma unroll
+    for (int u = 0; u < B; ++u) v[u] = sv[u] < nf ? acf[sv[u]] : anbf[sv[u] - nf];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (t + u < t1) a = a + v[u];
   }
   if (e < nc) {
     acc_out[e] = a;
@@ -835,7 +846,7 @@ struct DLevel {
   DevBuf<int> mptr, mem;         // members of this level's aggregates (coarse)
   std::vector<int> h_mptr, h_mem;
   DevBuf<int> gptr;
-  DevBuf<long long> gsrc;
+  DevBuf<int> gsrc;               // 32-bit: fine n + n w stays below 2^31
   int gne = 0;
   // levels >= 1: row-major CSR copy of the off-diagonals for the grid-wide
   // Jacobi kernels (values written by k_galerkin through csr_pos)
@@ -1107,7 +1118,14 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
       D.ac64 = D.ac64_own.p;
       D.anb64 = D.anb64_own.p;
       D.gptr.upload(gptrs[l], s);
-      D.gsrc.upload(gsrcs[l], s);
+      {
+        std::vector<int> g32(gsrcs[l].size());
+        for (size_t t = 0; t < g32.size(); ++t) {
+          if (gsrcs[l][t] > INT32_MAX) throw std::runtime_error("AMG level too large for 32-bit Galerkin sources");
+          g32[t] = (int)gsrcs[l][t];
+        }
+        D.gsrc.upload(g32, s);
+      }
       D.gne = (int)gptrs[l].size() - 1;
       // members in increasing fine order
       const HostLevel &f = H->host[l - 1];

@@ -196,32 +196,6 @@ __global__ void k_wall_pgrad(int Fb, int Fi, int N, const int *__restrict__ own,
   bgp[2 * Fb + b] = (1 - z * z) * gz - z * x * gx - z * y * gy;
 }
 
-// Pressure face gradient (InterpolateGradientFromElementToFace.for:16-93,
-// 'averagecorrected') into fg[9 Fi .. 12 Fi) (blocks x, y, z); the velocity
-// face gradients ('average') are formed inside k_mom_face3.
-__global__ void k_face_grads(int Fi, int N, const int *__restrict__ own,
-                             const int *__restrict__ nei, const double *__restrict__ g,
-                             const double *__restrict__ d, const double *__restrict__ ex,
-                             const double *__restrict__ ey, const double *__restrict__ ez,
-                             const double *__restrict__ gp, const double *__restrict__ p,
-                             double *__restrict__ fg) {
-  const int f = xcd_block() * kBlock + threadIdx.x;
-  if (f >= Fi) return;
-  const int i = own[f], j = nei[f];
-  const double gf = g[f], og = 1. - gf;
-  double fx = gf * gp[i] + og * gp[j];
-  double fy = gf * gp[N + i] + og * gp[N + j];
-  double fz = gf * gp[2 * N + i] + og * gp[2 * N + j];
-  double corr = (p[j] - p[i]) / d[f];
-  corr = corr - (fx * ex[f] + fy * ey[f] + fz * ez[f]);
-  fx = fx + corr * ex[f];
-  fy = fy + corr * ey[f];
-  fz = fz + corr * ez[f];
-  fg[(size_t)9 * Fi + f] = fx;
-  fg[(size_t)10 * Fi + f] = fy;
-  fg[(size_t)11 * Fi + f] = fz;
-}
-
 struct MomArgs {
   int comp;
   double mu, c23;
@@ -439,12 +413,14 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
 
 struct RcArgs {
   double rho, urf_u;
-  const double *u, *v, *w, *us, *vs, *ws, *du, *dv, *dw, *gp, *fgp, *p;
+  const double *u, *v, *w, *us, *vs, *ws, *du, *dv, *dw, *gp, *p;
 };
 
 // Rhie-Chow face mass flux (AssembleMdotUsingRhieChow.for:554-759, steady,
 // LRelaxMomentum) followed by the pressure-correction face coefficients
-// (AssemblePressureCorrectionFluxes.for:85-153).
+// (AssemblePressureCorrectionFluxes.for:85-153).  The pressure face gradient,
+// 'averagecorrected' (InterpolateGradientFromElementToFace.for:16-93), is
+// completed here from the averaged cell gradient the flux already forms.
 __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
                             const int *__restrict__ nei, const double *__restrict__ g,
                             const double *__restrict__ Sx, const double *__restrict__ Sy,
@@ -476,7 +452,10 @@ __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
   const double geo = sqrt(due * due + dve * dve + dwe * dwe) / dcf[f];
   const double dut = dusf - due, dvt = dvsf - dve, dwt = dwsf - dwe;
   const double cfl = rhof * geo, ffl = -(rhof * geo);
-  fv = fv - rhof * (a.fgp[f] * dut + a.fgp[Fi + f] * dvt + a.fgp[2 * Fi + f] * dwt);
+  double corr = (a.p[j] - a.p[i]) / dcf[f];
+  corr = corr - (pbx * ex[f] + pby * ey[f] + pbz * ez[f]);
+  const double fgx = pbx + corr * ex[f], fgy = pby + corr * ey[f], fgz = pbz + corr * ez[f];
+  fv = fv - rhof * (fgx * dut + fgy * dvt + fgz * dwt);
   fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
   const double mstar = rhof * (usf * sx + vsf * sy + wsf * sz);
   fv = fv + (1. - a.urf_u) * (mdot[f] - mstar);
@@ -622,7 +601,7 @@ struct SimpleCase {
   DevBuf<double> Sx, Sy, Sz, A, nx, ny, nz, dx, dy, dz, ex, ey, ez, d, g, gdiff, Tx, Ty, Tz, kx,
       ky, kz, vol;
   DevBuf<double> bu, bv, bw;
-  DevBuf<double> u, v, w, p, pc, mdot, gu, gv, gw, gp, gpc, bp, bgp, fg, du, dv, dw, us, vs, ws;
+  DevBuf<double> u, v, w, p, pc, mdot, gu, gv, gw, gp, gpc, bp, bgp, du, dv, dw, us, vs, ws;
   DevBuf<double> fp, fv, cf, ff, tf, ac, anb, bc, dphi, part;
   DevBuf<double> tf3; // momentum FluxTf of u, v, w (F each)
   DevBuf<double> pcprev; // the last converged p' (input of the warm-start network)
@@ -690,8 +669,6 @@ static void phase_momentum_gradients(SimpleCase &C) {
   gradients<4>(C, a);
   hipLaunchKernelGGL(k_wall_pgrad, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N,
                      C.own.p, C.nx.p, C.ny.p, C.nz.p, C.btype.p, C.bzq.p, C.gp.p, C.bgp.p);
-  hipLaunchKernelGGL(k_face_grads, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p,
-                     C.nei.p, C.g.p, C.d.p, C.ex.p, C.ey.p, C.ez.p, C.gp.p, C.p.p, C.fg.p);
   // the interior momentum face fluxes of all three components
   hipLaunchKernelGGL(k_mom_face3, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.F, C.N, C.own.p, C.nei.p,
                      C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p, C.Sx.p, C.Sy.p, C.Sz.p, C.mu,
@@ -748,7 +725,6 @@ static void phase_pc_assemble(SimpleCase &C) {
   a.us = C.us.p; a.vs = C.vs.p; a.ws = C.ws.p;
   a.du = C.du.p; a.dv = C.dv.p; a.dw = C.dw.p;
   a.gp = C.gp.p;
-  a.fgp = C.fg.p + (size_t)9 * C.Fi;
   a.p = C.p.p;
   hipLaunchKernelGGL(k_rhie_chow, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p,
                      C.nei.p, C.g.p, C.Sx.p, C.Sy.p, C.Sz.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, a,
@@ -954,8 +930,6 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
     FS_HIP_THROW(hipMemsetAsync(C.bp.p, 0, sizeof(double) * Fb, C.s));
     C.bgp.alloc(3 * Fb);
     FS_HIP_THROW(hipMemsetAsync(C.bgp.p, 0, sizeof(double) * 3 * Fb, C.s));
-    C.fg.alloc(12 * Fi);
-    FS_HIP_THROW(hipMemsetAsync(C.fg.p, 0, sizeof(double) * 12 * Fi, C.s));
     C.fp.alloc(4 * F);
     C.fv.alloc(4 * F);
     C.cf.alloc(F);
@@ -1047,7 +1021,7 @@ static bool case_field(fs::SimpleCase &C, const char *name, double **p, size_t *
       {"mdot", {&C.mdot, Fi}}, {"gu", {&C.gu, 3 * N}}, {"gv", {&C.gv, 3 * N}},
       {"gw", {&C.gw, 3 * N}}, {"gp", {&C.gp, 3 * N}}, {"gpc", {&C.gpc, 3 * N}},
       {"bu", {&C.bu, Fb}}, {"bv", {&C.bv, Fb}}, {"bw", {&C.bw, Fb}}, {"bp", {&C.bp, Fb}},
-      {"bgp", {&C.bgp, 3 * Fb}}, {"fg", {&C.fg, 12 * Fi}}, {"du", {&C.du, N}}, {"dv", {&C.dv, N}},
+      {"bgp", {&C.bgp, 3 * Fb}}, {"du", {&C.du, N}}, {"dv", {&C.dv, N}},
       {"dw", {&C.dw, N}}, {"us", {&C.us, N}}, {"vs", {&C.vs, N}}, {"ws", {&C.ws, N}},
       {"ac", {&C.ac, N}}, {"anb", {&C.anb, N * C.W}}, {"bc", {&C.bc, N}}, {"cf", {&C.cf, F}},
       {"ff", {&C.ff, F}}, {"tf", {&C.tf, F}}, {"vol", {&C.vol, N}}, {"Sx", {&C.Sx, F}},

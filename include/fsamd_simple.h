@@ -109,7 +109,7 @@ int fs_case_iterate(fs_case *c, int niter, fs_simple_stats *stats);
 /* One phase of the iteration, for tests (enum fs_phase). */
 int fs_case_phase(fs_case *c, int phase);
 /* Host copies of device fields by name (u v w p pc mdot gu gv gw gp gpc bu bv
- * bw bp bgp fg du dv dw us vs ws ac anb bc cf ff tf and geometry vol Sx Sy
+ * bw bp bgp du dv dw us vs ws ac anb bc cf ff tf and geometry vol Sx Sy
  * Sz g gdiff Tx Ty Tz dcf A kx ky kz); count must equal the field length. */
 int fs_case_get(fs_case *c, const char *name, double *out, long long count);
 /* Setting u, v or w re-derives the symmetry boundary velocities before the

@@ -51,10 +51,8 @@ def test_momentum_gradients_bitwise(seed):
     dc.phase("momentum_gradients")
     for k in ("gu", "gv", "gw", "gp"):
         assert np.array_equal(dc.get(k), oc.get(k)), k
-    # the device keeps the pressure face gradient (blocks 9-11 of fg); the
-    # velocity face gradients are formed inside the momentum face kernel and
-    # checked through the assembled systems
-    assert np.array_equal(dc.get("fg")[9 * dc.Fi:], oc.get("fgp"))
+    # the face gradients are formed inside the face kernels (momentum fluxes,
+    # Rhie-Chow) and checked through the assembled systems
     assert np.array_equal(dc.get("bgp"), oc.get("bgp"))
 
 

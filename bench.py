@@ -99,6 +99,18 @@ def measured_traffic(cells):
     return None
 
 
+def index_bytes(case):
+    """Bytes of the ELL neighbour indices the row loops read: 16-bit offsets
+    packed two slots to a word when every neighbour lies within +-32767
+    cells of its row, 32-bit indices otherwise."""
+    enbr = case.topology("enbr").astype(np.int64)  # (W, N), 1-based, 0 = none
+    rows = np.arange(case.N)[None, :]
+    off = np.where(enbr > 0, enbr - 1 - rows, 0)
+    if np.abs(off).max() <= 32767:
+        return 4 * ((case.W + 1) // 2) * case.N
+    return 4 * case.W * case.N
+
+
 def traffic_probe(case, args):
     """Launch sequence for the FETCH_SIZE / WRITE_SIZE counter runs: one SIMPLE
     iteration (to assemble a real p' system), then the ELL product alone, then
@@ -109,7 +121,7 @@ def traffic_probe(case, args):
     ms = case.time_spmv(args.spmv_reps)
     copy_ms = fs.time_device_copy(COPY_BYTES, 3)
     print(json.dumps({"probe": "traffic", "cells": case.N, "W": case.W, "spmv_ms": ms, "spmv_reps": args.spmv_reps,
-                      "spmv_bytes": 8 * case.N + 4 * case.N * case.W + 8 * 2 * case.Fi + 24 * case.N,
+                      "spmv_bytes": 8 * case.N + index_bytes(case) + 8 * 2 * case.Fi + 24 * case.N,
                       "copy_bytes": COPY_BYTES,
                       "copy_ms": copy_ms, "copy_GBps": 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9}), flush=True)
 

@@ -130,6 +130,7 @@ struct fs_mesh {
   int mg_sell_rows = 100000;       // coarse levels at least this large use sliced ELL
   double mg_alpha = 0.0;           // over-correction (0: 1 as solver, 1.7 as preconditioner)
   int gmres_m = 30, check_every = 8;
+  int iter_pred[256] = {0}; // iterations of the last solve per (variable, solver)
   int ncalls_mg = 0;
   long long nnz = 0;                    // off-diagonal entries of the ELL pattern
   fs::KernelClock kclock;

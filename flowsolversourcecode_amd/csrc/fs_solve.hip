@@ -1156,21 +1156,31 @@ static void host_flags(fs_mesh *m, int *fl) {
   FS_HIP_THROW(hipStreamSynchronize(m->stream));
 }
 
-// Runs `body` in batches of check_every iterations until the device sets done.
-template <class F> static int run_batches(fs_mesh *m, int itmax, F body) {
-  int fl[F_NFLAGS];
-  host_flags(m, fl);
+// Runs `body` in batches until the device sets done.  The first batch is the
+// iteration count of the previous solve of the same variable and solver (the
+// SIMPLE loop solves nearly the same system every outer iteration; at most
+// 4 check_every), later batches grow 2, 4, ... up to check_every; without a
+// previous solve every batch is check_every long.  Iterations launched past
+// convergence return at once, so the batch sizes only trade idle launches
+// against host checks.
+template <class F> static int run_batches(fs_mesh *m, int itmax, int key, F body) {
+  int fl[F_NFLAGS] = {0};
   int launched = 0;
+  int &pred = m->iter_pred[key & 255];
+  int next = pred > 0 ? std::min(pred, 4 * m->check_every) : m->check_every, grow = 2;
   m->kclock.solve_first = m->kclock.used;
   while (!fl[F_DONE] && launched < itmax) {
-    const int batch = std::min(m->check_every, itmax - launched);
+    const int batch = std::min(next, itmax - launched);
     for (int b = 0; b < batch; ++b) {
       m->kclock.cur_iter = launched + b;
       body();
     }
     launched += batch;
     host_flags(m, fl);
+    next = pred > 0 ? std::min(grow, m->check_every) : m->check_every;
+    grow *= 2;
   }
+  pred = fl[F_ITER];
   // launches of iterations past the converged one returned at once
   KernelClock &c = m->kclock;
   for (int u = c.solve_first; u < c.used; ++u)
@@ -1233,7 +1243,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     if (pre) amg_prepare(m, ac, anb);
     const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
     const double *dc = pre ? nullptr : sw.dc.p;
-    iters = run_batches(m, itmax, [&]() {
+    iters = run_batches(m, itmax, nf * 8 + solver, [&]() {
       if (pre) {
         amg_apply(m, V0, V6, flags, alpha);
         amg_apply(m, V1, V7, flags, alpha);
@@ -1250,7 +1260,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     });
   } else if (solver == FS_SOR) {
     // multicolour symmetric Gauss-Seidel: colours forward then backward
-    iters = run_batches(m, itmax, [&]() {
+    iters = run_batches(m, itmax, nf * 8 + solver, [&]() {
       const int nc = (int)sw.color_ptr.size() - 1;
       for (int c = 0; c < nc; ++c) {
         const int b = sw.color_ptr[c], e = sw.color_ptr[c + 1];
@@ -1284,7 +1294,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     // rho of the first iteration; every later rho comes with the update
     hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V1, V0, nullptr, nullptr, part, flags);
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_RHO, gr, n, nf, part, scal, flags);
-    iters = run_batches(m, itmax, [&]() {
+    iters = run_batches(m, itmax, nf * 8 + solver, [&]() {
       hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(kBlock), 0, s, n, V0, V3, V2, scal, flags);
       if (pre) amg_apply(m, V2, V6, flags, alpha);
       const double *ph = pre ? V6 : V2;
@@ -1315,7 +1325,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
       hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, (double *)nullptr, bc, flags);
     }
-    iters = run_batches(m, itmax, [&]() {
+    iters = run_batches(m, itmax, nf * 8 + solver, [&]() {
       const double *z = V0;
       if (pre) {
         amg_apply(m, V0, V1, flags, alpha);
@@ -1396,7 +1406,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     ilu_factor(m, ac, anb, sw.dc.p);
     IluSweep &S = sw.ilu;
     double *rcv = m->v[0].p;
-    iters = run_batches(m, itmax, [&]() {
+    iters = run_batches(m, itmax, nf * 8 + solver, [&]() {
       hipLaunchKernelGGL(k_ilu_resid, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, rcv, flags);
       for (size_t l = 0; l + 1 < S.fwd_level_ptr.size(); ++l) {
         const int b0 = S.fwd_level_ptr[l], b1 = S.fwd_level_ptr[l + 1];

@@ -70,7 +70,11 @@ enum fs_option {
   FS_OPT_MG_SMOOTHER = 9,     /* 0 = damped Jacobi, 1 = multicolour symmetric
                                  Gauss-Seidel on levels with <= 4 colours [0]  */
   FS_OPT_GMRES_RESTART = 10,  /* GMRES restart length [30]                      */
-  FS_OPT_CHECK_EVERY = 11,    /* iterations launched between host checks [8]    */
+  FS_OPT_CHECK_EVERY = 11,    /* iterations launched between host checks [8];
+                                 a solve's first batch is the iteration count
+                                 of the previous solve of the same variable
+                                 and solver (at most 4x this), later batches
+                                 grow 2, 4, ... up to this                     */
   FS_OPT_MG_TAIL_CELLS = 12,  /* levels with at most this many cells run in one
                                  single-workgroup kernel [1024]                 */
   FS_OPT_MG_COARSE_SWEEPS = 13,/* smoothing sweeps on a coarsest level too large

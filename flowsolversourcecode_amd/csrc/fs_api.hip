@@ -170,7 +170,7 @@ int fs_mesh_destroy(fs_mesh *m) {
   (void)hipStreamSynchronize(m->stream);
   m->amg.reset();
   fs::mesh_release_solver(m);
-  hipStream_t s = m->stream;
+  hipStream_t s = m->own_stream ? m->stream : nullptr;
   delete m;
   if (s) (void)hipStreamDestroy(s);
   return 0;

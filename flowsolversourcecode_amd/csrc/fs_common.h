@@ -98,6 +98,7 @@ struct KernelClock {
 struct fs_mesh {
   int n = 0, w = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;   // false: the stream belongs to another context
   std::vector<int> h_nbr;   // reference layout, 1-based, 0 = boundary
   std::vector<int> h_nnbr;
   std::vector<int> h_nbr_dev; // 0-based, -1 = none (the device layout)

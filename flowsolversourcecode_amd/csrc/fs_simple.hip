@@ -367,11 +367,14 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
                            const double *__restrict__ cf, const double *__restrict__ ff,
                            const double *__restrict__ tf, const double *__restrict__ mdot,
                            const double *__restrict__ phi, const double *__restrict__ gpc,
-                           const double *__restrict__ vol, double urf, double *__restrict__ ac,
-                           double *__restrict__ anb, double *__restrict__ bc,
-                           double *__restrict__ D, double *__restrict__ star) {
+                           const double *__restrict__ vol, double urf, const int *__restrict__ perm,
+                           double *__restrict__ ac, double *__restrict__ anb, double *__restrict__ bc,
+                           double *__restrict__ D, double *__restrict__ star, double *__restrict__ xq) {
   const int c = xcd_block() * kBlock + threadIdx.x;
   if (c >= N) return;
+  // the system row of cell c: its colour-major position when perm is given
+  // (the unknown is then copied to that position as well, into xq)
+  const int q = perm ? perm[c] : c;
   double a = 0.0, b = 0.0, e = 0.0;
   if constexpr (WC > 0) {
     // all slots' loads first (the mass flux exists for interior faces only)
@@ -388,13 +391,13 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
       vm[s] = mdot[(role == ROLE_OWNER || role == ROLE_NEIGH) ? f : 0];
     }
 #pragma unroll
-    for (int s = 0; s < WC; ++s) anb[(size_t)s * N + c] = mom_slot(cd[s] & 3, vc[s], vf[s], vt[s], vm[s], a, b, e);
+    for (int s = 0; s < WC; ++s) anb[(size_t)s * N + q] = mom_slot(cd[s] & 3, vc[s], vf[s], vt[s], vm[s], a, b, e);
   } else {
     for (int s = 0; s < W; ++s) {
       const int cd = code[(size_t)s * N + c];
       const int role = cd & 3, f = cd >> 2;
       const double md = (role == ROLE_OWNER || role == ROLE_NEIGH) ? mdot[f] : 0.0;
-      anb[(size_t)s * N + c] = mom_slot(role, cf[f], ff[f], tf[f], md, a, b, e);
+      anb[(size_t)s * N + q] = mom_slot(role, cf[f], ff[f], tf[f], md, a, b, e);
     }
   }
   const double pc = phi[c];
@@ -405,10 +408,18 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
   a = a + fce;
   b = b - fte;
   a = a / urf;
-  ac[c] = a;
-  bc[c] = b;
+  ac[q] = a;
+  bc[q] = b;
   D[c] = vol[c] / a;
   star[c] = pc;
+  if (perm) xq[q] = pc;
+}
+
+// y[c] = x[perm[c]]: a colour-major vector back in cell order
+__global__ void k_unpermute(int N, const int *__restrict__ perm, const double *__restrict__ x,
+                            double *__restrict__ y) {
+  const int c = xcd_block() * kBlock + threadIdx.x;
+  if (c < N) y[c] = x[perm[c]];
 }
 
 struct RcArgs {
@@ -606,6 +617,15 @@ struct SimpleCase {
   DevBuf<double> tf3; // momentum FluxTf of u, v, w (F each)
   DevBuf<double> pcprev; // the last converged p' (input of the warm-start network)
   DevBuf<int> btype, bzq; // boundary condition per boundary face (empty: all walls)
+  // momentum systems in colour-major cell order (the colour classes of the
+  // multicolour Gauss-Seidel contiguous): their solver context on the
+  // renumbered mesh (sharing the stream), the renumbering, the permuted
+  // unknown, and whether ac/anb/bc currently hold such a system
+  fs_mesh *mom_mesh = nullptr;
+  DevBuf<int> mperm;
+  std::vector<int> h_mperm;
+  DevBuf<double> mx;
+  bool sys_perm = false;
   bool sym_dirty = false;  // symmetry boundary velocities not yet derived
   double rho = 1.0, mu = 0.01, urf_u = 0.7, urf_p = 0.3;
   int msolver = FS_SOR, mlmg = 0, mitmax = 20, psolver = FS_BICGS, plmg = FS_MG_PRECOND,
@@ -678,6 +698,7 @@ static void phase_momentum_gradients(SimpleCase &C) {
 
 static void phase_momentum_assemble(SimpleCase &C, int comp) {
   MomArgs a{};
+  C.sys_perm = C.mom_mesh != nullptr;
   a.comp = comp;
   a.mu = C.mu;
   a.c23 = (double)(-2.0f / 3.0f); // REAL(4) literal -2./3. (AssembleBulkViscosity.for:49)
@@ -698,17 +719,52 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
   by_width(C.W, [&](auto wc) {
     hipLaunchKernelGGL(k_mom_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
                        C.code.p, C.cf.p, C.ff.p, tf, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
-                       C.urf_u, C.ac.p, C.anb.p, C.bc.p, D, st);
+                       C.urf_u, C.mom_mesh ? C.mperm.p : nullptr, C.ac.p, C.anb.p, C.bc.p, D, st, C.mx.p);
   });
+}
+
+// The momentum context follows the solver options set on the case's context.
+static void mirror_options(fs_mesh *d, const fs_mesh *s) {
+  const bool structural = d->mg_maxlevels != s->mg_maxlevels || d->mg_minparents != s->mg_minparents ||
+                          d->mg_tail_cells != s->mg_tail_cells || d->mg_sell_rows != s->mg_sell_rows;
+  d->mg_pre = s->mg_pre;
+  d->mg_post = s->mg_post;
+  d->mg_maxcycles = s->mg_maxcycles;
+  d->mg_maxlevels = s->mg_maxlevels;
+  d->mg_minparents = s->mg_minparents;
+  d->mg_reagg = s->mg_reagg;
+  d->mg_fp32 = s->mg_fp32;
+  d->mg_smoother = s->mg_smoother;
+  d->mg_omega = s->mg_omega;
+  d->mg_tail_cells = s->mg_tail_cells;
+  d->mg_coarse_sweeps = s->mg_coarse_sweeps;
+  d->mg_cpre = s->mg_cpre;
+  d->mg_cpost = s->mg_cpost;
+  d->mg_sell_rows = s->mg_sell_rows;
+  d->mg_alpha = s->mg_alpha;
+  d->gmres_m = s->gmres_m;
+  d->check_every = s->check_every;
+  if (structural) d->amg.reset();
 }
 
 static int phase_momentum_solve(SimpleCase &C, int comp, fs_simple_stats *st) {
   double *phi = comp == 0 ? C.u.p : comp == 1 ? C.v.p : C.w.p;
   fs_solve_stats ss{};
   double res4[4];
-  const int rc = solve_device(C.mesh, comp + 1, phi, C.mrrf, C.mitmax, C.msolver, C.mlmg, C.ac.p,
-                              C.anb.p, C.bc.p, C.dphi.p, res4, &ss);
-  if (rc < 0) return rc;
+  int rc;
+  if (C.mom_mesh) {
+    // the system is in colour-major order, and so is the unknown the
+    // assembly copied into mx: solve for it, then bring it back
+    mirror_options(C.mom_mesh, C.mesh);
+    rc = solve_device(C.mom_mesh, comp + 1, C.mx.p, C.mrrf, C.mitmax, C.msolver, C.mlmg, C.ac.p, C.anb.p, C.bc.p,
+                      C.dphi.p, res4, &ss);
+    if (rc < 0) return rc;
+    hipLaunchKernelGGL(k_unpermute, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.mperm.p, C.mx.p, phi);
+  } else {
+    rc = solve_device(C.mesh, comp + 1, phi, C.mrrf, C.mitmax, C.msolver, C.mlmg, C.ac.p, C.anb.p, C.bc.p,
+                      C.dphi.p, res4, &ss);
+    if (rc < 0) return rc;
+  }
   if (st) {
     st->momentum_iterations += ss.iterations;
     st->momentum_seconds += ss.seconds;
@@ -719,6 +775,7 @@ static int phase_momentum_solve(SimpleCase &C, int comp, fs_simple_stats *st) {
 
 static void phase_pc_assemble(SimpleCase &C) {
   RcArgs a{};
+  C.sys_perm = false;
   a.rho = C.rho;
   a.urf_u = C.urf_u;
   a.u = C.u.p; a.v = C.v.p; a.w = C.w.p;
@@ -902,6 +959,41 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
       return rc;
     }
     C.s = C.mesh->stream;
+    {
+      // colour-major renumbering of the cells for the momentum systems: the
+      // greedy colouring of the mesh (the Gauss-Seidel colours), colour by
+      // colour in increasing cell order.  Greedy colouring of the renumbered
+      // mesh gives the same classes, now contiguous.
+      std::vector<int> ptr, cells;
+      fs::greedy_colors(C.N, C.W, C.mesh->h_nbr_dev.data(), ptr, cells);
+      C.h_mperm.assign(C.N, 0);
+      for (int q = 0; q < C.N; ++q) C.h_mperm[cells[q]] = q;
+      std::vector<int> enbr_p((size_t)C.N * C.W, 0), nef_p(C.N, 0);
+      for (int c = 0; c < C.N; ++c) {
+        const int q = C.h_mperm[c];
+        nef_p[q] = C.geom.nef[c];
+        for (int j = 0; j < C.W; ++j) {
+          const int k = C.geom.enbr[(size_t)j * C.N + c];
+          enbr_p[(size_t)j * C.N + q] = k > 0 ? C.h_mperm[k - 1] + 1 : 0;
+        }
+      }
+      rc = fs_mesh_create(C.N, C.W, enbr_p.data(), nef_p.data(), 0, nullptr, nullptr, nullptr, nullptr,
+                          &C.mom_mesh);
+      if (rc != 0) {
+        std::string msg = fs_last_error();
+        fs_mesh_destroy(C.mesh);
+        delete cs;
+        set_error("fs_case_create: " + msg);
+        return rc;
+      }
+      // the momentum context works on the case's stream
+      FS_HIP_THROW(hipStreamSynchronize(C.mom_mesh->stream));
+      FS_HIP_THROW(hipStreamDestroy(C.mom_mesh->stream));
+      C.mom_mesh->stream = C.s;
+      C.mom_mesh->own_stream = false;
+      C.mperm.upload(C.h_mperm, C.s);
+      C.mx.alloc(C.N);
+    }
     C.own.upload(owner, nfaces, C.s);
     C.nei.upload(neighbour, nif, C.s);
     // the per-slot face code of the mesh context
@@ -941,6 +1033,7 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
     C.part.alloc(fs::kPartials);
     FS_HIP_THROW(hipStreamSynchronize(C.s));
   } catch (...) {
+    if (C.mom_mesh) fs_mesh_destroy(C.mom_mesh);
     if (C.mesh) fs_mesh_destroy(C.mesh);
     delete cs;
     throw;
@@ -952,9 +1045,10 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
 
 int fs_case_destroy(fs_case *cs) {
   if (!cs) return 0;
-  fs_mesh *m = cs->c.mesh;
+  fs_mesh *m = cs->c.mesh, *mm = cs->c.mom_mesh;
   if (m) (void)hipStreamSynchronize(m->stream);
-  delete cs; // device buffers first, the mesh (and its stream) last
+  delete cs; // device buffers first, the meshes last, the stream's owner after the other
+  if (mm) fs_mesh_destroy(mm);
   if (m) fs_mesh_destroy(m);
   return 0;
 }
@@ -1050,6 +1144,15 @@ int fs_case_get(fs_case *cs, const char *name, double *out, long long count) {
   }
   FS_HIP(hipMemcpyAsync(out, p, sizeof(double) * n, hipMemcpyDeviceToHost, cs->c.s));
   FS_HIP(hipStreamSynchronize(cs->c.s));
+  const std::string nm(name);
+  auto &C = cs->c;
+  if (C.sys_perm && (nm == "ac" || nm == "bc" || nm == "anb")) {
+    // a momentum system is held in colour-major order: return cell order
+    std::vector<double> tmp(out, out + n);
+    const size_t N = C.N, blocks_ = n / N;
+    for (size_t j = 0; j < blocks_; ++j)
+      for (size_t c = 0; c < N; ++c) out[j * N + c] = tmp[j * N + C.h_mperm[c]];
+  }
   return 0;
   CASE_GUARD_END
 }

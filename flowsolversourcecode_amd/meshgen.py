@@ -123,6 +123,87 @@ def box_hex(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, perturb=0.0, seed=0):
                     PATCHES, sizes, (nx, ny, nz))
 
 
+# Kuhn split of a hexahedron into 6 tetrahedra along its diagonal v0-v6
+# (local vertices v0 (i,j,k), v1 (i+1,j,k), v2 (i+1,j+1,k), v3 (i,j+1,k) and
+# v4..v7 the same at k+1): the 6 monotone edge paths from v0 to v6.  Every
+# hexahedron uses the same diagonal direction, so the split is conforming.
+_KUHN = ((0, 1, 2, 6), (0, 1, 5, 6), (0, 4, 5, 6), (0, 4, 7, 6), (0, 3, 7, 6), (0, 3, 2, 6))
+
+
+def box_tet(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, perturb=0.0, seed=0):
+    """nx*ny*nz hexahedra of box_hex (same points, same perturbation) each
+    split into 6 tetrahedra (Kuhn), as a polyMesh with box_hex's conventions:
+    interior faces ordered by owner then neighbour with the normal from owner
+    to neighbour, boundary faces grouped by patch (xmin, xmax, ymin, ymax,
+    zmin, zmax), each patch ordered by owner, normals outward.  Cell 6 h + t
+    is tetrahedron t of hexahedron h (h = i + nx (j + ny k))."""
+    hexm = box_hex(nx, ny, nz, lx, ly, lz, perturb, seed)
+    pts = hexm.points
+    ix = np.arange(nx + 1)
+    I, J, K = np.meshgrid(ix, np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    I, J, K = (a.transpose(2, 1, 0).ravel() for a in (I, J, K))
+
+    def P(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci, cj, ck = (a.transpose(2, 1, 0).ravel() for a in (ci, cj, ck))  # hex order, i fastest
+    corner = np.stack([P(ci, cj, ck), P(ci + 1, cj, ck), P(ci + 1, cj + 1, ck), P(ci, cj + 1, ck),
+                       P(ci, cj, ck + 1), P(ci + 1, cj, ck + 1), P(ci + 1, cj + 1, ck + 1),
+                       P(ci, cj + 1, ck + 1)], 1)
+    nh = len(ci)
+    tets = np.stack([corner[:, list(t)] for t in _KUHN], 1).reshape(6 * nh, 4)
+    ncell = 6 * nh
+    cent = pts[tets].mean(axis=1)
+    # the 4 faces of every tetrahedron (the vertex opposite each one left out)
+    tri = np.concatenate([tets[:, [1, 2, 3]], tets[:, [0, 2, 3]], tets[:, [0, 1, 3]], tets[:, [0, 1, 2]]])
+    cell = np.tile(np.arange(ncell), 4)
+    key = np.sort(tri, axis=1)
+    order = np.lexsort((key[:, 2], key[:, 1], key[:, 0]))
+    key, tri, cell = key[order], tri[order], cell[order]
+    same = np.all(key[1:] == key[:-1], axis=1)
+    first = np.nonzero(same)[0]                  # pairs (first, first + 1)
+    paired = np.zeros(len(key), bool)
+    paired[first] = paired[first + 1] = True
+    c0, c1 = cell[first], cell[first + 1]
+    own = np.minimum(c0, c1)
+    nei = np.maximum(c0, c1)
+    ftri = tri[first]
+
+    def orient(t, frm, to):
+        a, b, c = pts[t[:, 0]], pts[t[:, 1]], pts[t[:, 2]]
+        nrm = np.cross(b - a, c - a)
+        flip = (nrm * (to - frm)).sum(1) < 0
+        t = t.copy()
+        t[flip, 1], t[flip, 2] = t[flip, 2], t[flip, 1].copy()
+        return t
+
+    ftri = orient(ftri, cent[own], cent[nei])
+    o = np.lexsort((nei, own))
+    ftri, own, nei = ftri[o], own[o], nei[o]
+    # boundary faces: outward normals, then patches
+    btri, bcell = tri[~paired], cell[~paired]
+    fc = pts[btri].mean(axis=1)
+    btri = orient(btri, cent[bcell], fc)
+    bI, bJ, bK = I[btri], J[btri], K[btri]
+    sides = [np.all(bI == 0, 1), np.all(bI == nx, 1), np.all(bJ == 0, 1), np.all(bJ == ny, 1),
+             np.all(bK == 0, 1), np.all(bK == nz, 1)]
+    if not np.all(np.sum(sides, axis=0) == 1):
+        raise RuntimeError("a boundary face lies on no single box side")
+    bq, bo, sizes = [], [], []
+    for m in sides:
+        idx = np.nonzero(m)[0]
+        idx = idx[np.argsort(bcell[idx], kind="stable")]
+        bq.append(btri[idx])
+        bo.append(bcell[idx])
+        sizes.append(len(idx))
+    faces = np.concatenate([ftri] + bq).astype(np.int32)
+    owner = np.concatenate([own] + bo).astype(np.int32)
+    face_ptr = (np.arange(len(faces) + 1) * 3).astype(np.int32)
+    return PolyMesh(pts.copy(), face_ptr, faces.ravel().copy(), owner, nei.astype(np.int32), PATCHES, sizes,
+                    (nx, ny, nz))
+
+
 def lid_driven_cavity_walls(mesh, lid="ymax", lid_velocity=(1.0, 0.0, 0.0)):
     """Wall velocity of every boundary face: the lid patch moves, the other
     walls are at rest.  Returns (bu, bv, bw), one entry per boundary face."""

@@ -160,6 +160,31 @@ def cavity2d(args, fs, meshgen):
     return out
 
 
+def tet_cavity(args, fs, meshgen):
+    """The lid-driven cavity on tetrahedra: --tet-n^3 hexahedra of the unit
+    cube (interior nodes perturbed +-15 %, seed 2026) each split into 6
+    tetrahedra (Kuhn), Re = 1000, the headline form's settings and cycle."""
+    n = args.tet_n
+    pm = meshgen.box_tet(n, n, n, perturb=0.15, seed=2026)
+    bu, bv, bw = meshgen.lid_driven_cavity_walls(pm)
+    c = fs.Case(pm, bu, bv, bw, rho=1.0, mu=1e-3)
+    c.configure(urf_u=0.7, urf_p=0.3, msolver="sor", mrrf=0.1, mitmax=10, psolver="bicgs", prrf=1e-8, pitmax=200,
+                plmg=fs.MG_PRECOND)
+    for k, v in (("mg_fp32", 1), ("mg_alpha", args.mg_alpha), ("mg_presweep", args.mg_pre),
+                 ("mg_postsweep", args.mg_post), ("mg_omega", args.mg_omega)):
+        c.set_option(k, v)
+    c.iterate(args.warmup)
+    st = c.iterate(args.steps)
+    out = {"workload": f"lid-driven cavity {n}^3 hex x 6 tetrahedra (Kuhn), +/-15% node perturbation, Re=1000",
+           "cells": c.N, "faces": c.F, "value": args.steps / st["seconds"], "unit": "SIMPLE iter/s",
+           "ms_per_step": 1e3 * st["seconds"] / args.steps, "pc_solve_ms": 1e3 * st["pc_seconds"] / args.steps,
+           "pc_iterations_per_solve": st["pc_iterations"] / args.steps,
+           "momentum_ms": 1e3 * st["momentum_seconds"] / args.steps, "mass_imbalance_l1": st["mass_l1"]}
+    print(f"# tet cavity {c.N} cells: {out['value']:.1f} it/s, p' {out['pc_solve_ms']:.2f} ms "
+          f"({out['pc_iterations_per_solve']:.1f} its)", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +204,7 @@ def main():
     ap.add_argument("--mg-pre", type=int, default=1, help="pre-sweeps per level of the headline V-cycle")
     ap.add_argument("--mg-post", type=int, default=2, help="post-sweeps per level of the headline V-cycle")
     ap.add_argument("--mg-omega", type=float, default=0.85, help="Jacobi damping of the headline V-cycle")
+    ap.add_argument("--tet-n", type=int, default=64, help="hexahedra per side of the tetrahedral cavity variant")
     ap.add_argument("--clock-stride", type=int, default=5,
                     help="time every n-th launch of the roofline kernels with HIP events")
     ap.add_argument("--cavity2d-iters", type=int, default=500, help="outer iterations of the 2D-cavity variant")
@@ -300,6 +326,7 @@ def main():
               f"(zero start {head_pc:.1f}), train {t_train:.1f} s", file=sys.stderr, flush=True)
         del lc, net
         variants["cavity2d_64x64x1_re100"] = cavity2d(args, fs, meshgen)
+        variants["tet_cavity"] = tet_cavity(args, fs, meshgen)
 
     cpu = None if args.no_cpu else cpu_baseline(args)
     out = {

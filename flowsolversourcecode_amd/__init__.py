@@ -303,6 +303,8 @@ def _setup_case_symbols(L):
     L.fs_case_dims.argtypes = [C.c_void_p, _ip]
     L.fs_case_topology.argtypes = [C.c_void_p, C.c_int, _ip, C.c_longlong]
     L.fs_case_set_patch_types.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
+    L.fs_case_set_convection.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]
+    L.fs_case_set_gradient_limiters.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.fs_case_time_spmv.argtypes = [C.c_void_p, C.c_int, _dp]
     L._case_ready = True
 
@@ -341,6 +343,10 @@ class Geometry:
 
 
 BC_TYPES = {"wall": 0, "symmetry": 1}
+HR_FRAMEWORKS = {"none": 0, "tvd": 1, "nvf": 2}
+CONV_SCHEMES = {name: i for i, name in enumerate(
+    ("upwind", "downwind", "fromm", "sou", "cd", "quick", "superbee", "minmod", "osher", "vanleer", "muscl",
+     "smart", "hquick", "umist", "charm", "boundedcd", "stoic", "gamma"))}
 
 
 class Case:
@@ -434,6 +440,18 @@ class Case:
         codes = np.array([BC_TYPES[t] if isinstance(t, str) else int(t) for t in patch_types], np.int32)
         _check(lib().fs_case_set_patch_types(self.h, len(sizes), _iptr(sizes), _iptr(codes)),
                "fs_case_set_patch_types")
+
+    def set_convection(self, framework, scheme, bleed=0.0, start_iter=0):
+        """HRFrameworkMomentum ("none", "tvd", "nvf"), ConvectionSchemeMomentum
+        (e.g. "sou", "minmod"), BleedMomentum and nIterStartApplyingHR."""
+        _check(lib().fs_case_set_convection(self.h, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], bleed,
+                                            start_iter), "fs_case_set_convection")
+
+    def set_gradient_limiters(self, momentum=0, continuity=0):
+        """LimitGradientMomentumMethod / LimitGradientContinuityMethod: 0 off,
+        1 or 2 (LimitGradients.for)."""
+        _check(lib().fs_case_set_gradient_limiters(self.h, momentum, continuity),
+               "fs_case_set_gradient_limiters")
 
     def topology(self, name):
         """Readit's topology arrays of the case's mesh (reference numbering):

@@ -23,6 +23,7 @@
 
 #include "fs_common.h"
 #include "fs_geom.h"
+#include "fs_hr.h"
 #include "fs_kernels.h"
 #include "fs_solve.h"
 #include "fs_mpnn.h"
@@ -196,6 +197,155 @@ __global__ void k_wall_pgrad(int Fb, int Fi, int N, const int *__restrict__ own,
   bgp[2 * Fb + b] = (1 - z * z) * gz - z * x * gx - z * y * gy;
 }
 
+// Gradient limiting (LimitGradients.for:3): fields k of the preceding
+// gradient launch whose method is 1 or 2; their boundary values are that
+// launch's boundary face values fp[k F + f].
+struct LimArgs {
+  const double *phi[4];
+  double *grad[4];
+  int method[4];      // 0: leave the field, 1 or 2
+  const double *eps2; // per field, (1e-3 (max - min))^2 of the method-2 fields
+};
+
+// Range of each method-2 field over its cells and boundary values
+// (PhiFieldMinMax, LimitGradients.for:440-476): per-block min and max into
+// part[(2k) nb + b] and part[(2k+1) nb + b].
+template <int NF>
+__global__ void k_lim_range(int N, int Fi, int F, const double *__restrict__ fp, LimArgs a,
+                            double *__restrict__ part) {
+  double lo[NF], hi[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    lo[k] = 1.e30;
+    hi[k] = -1.e30;
+  }
+  const int Fb = F - Fi;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < N + Fb; t += gridDim.x * kBlock)
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      if (a.method[k] != 2) continue;
+      const double v = t < N ? a.phi[k][t] : fp[(size_t)k * F + Fi + (t - N)];
+      lo[k] = min2(lo[k], v);
+      hi[k] = max2(hi[k], v);
+    }
+  __shared__ double s_lo[NF][kBlock], s_hi[NF][kBlock];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    s_lo[k][threadIdx.x] = lo[k];
+    s_hi[k][threadIdx.x] = hi[k];
+  }
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        s_lo[k][threadIdx.x] = min2(s_lo[k][threadIdx.x], s_lo[k][threadIdx.x + h]);
+        s_hi[k][threadIdx.x] = max2(s_hi[k][threadIdx.x], s_hi[k][threadIdx.x + h]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      part[(size_t)(2 * k) * gridDim.x + blockIdx.x] = s_lo[k][0];
+      part[(size_t)(2 * k + 1) * gridDim.x + blockIdx.x] = s_hi[k][0];
+    }
+}
+
+// eps2 = (epsilon (phiMax - phiMin))^2, epsilon = 1e-3 (Constants1.F90,
+// LimitGradients.for:307), from nb per-block ranges; one block.
+template <int NF>
+__global__ void k_lim_eps(int nb, const double *__restrict__ part, double *__restrict__ eps2) {
+  __shared__ double s_lo[NF][kBlock], s_hi[NF][kBlock];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    double lo = 1.e30, hi = -1.e30;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+      lo = min2(lo, part[(size_t)(2 * k) * nb + b]);
+      hi = max2(hi, part[(size_t)(2 * k + 1) * nb + b]);
+    }
+    s_lo[k][threadIdx.x] = lo;
+    s_hi[k][threadIdx.x] = hi;
+  }
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        s_lo[k][threadIdx.x] = min2(s_lo[k][threadIdx.x], s_lo[k][threadIdx.x + h]);
+        s_hi[k][threadIdx.x] = max2(s_hi[k][threadIdx.x], s_hi[k][threadIdx.x + h]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const double e = 1.e-3 * (s_hi[k][0] - s_lo[k][0]);
+      eps2[k] = e * e;
+    }
+}
+
+// One factor per cell scales its gradient (LimitGradientMethod1,
+// LimitGradients.for:68-262 with rk = 1, and LimitGradientMethod2, :263-437):
+// the min/max over the cell, its neighbours and its boundary values, then
+// each face's extrapolation d = grad . (r_f - r_C), interior faces in slot
+// order before boundary faces as the reference's two face loops visit them.
+// Every cell reads only its own gradient, so the update is in place; the
+// boundary gradients are the owners' (read from the cell arrays).
+template <int NF>
+__global__ void k_limit(int N, int W, int F, const int *__restrict__ code, const int *__restrict__ nbr,
+                        const double *__restrict__ fcx, const double *__restrict__ fcy,
+                        const double *__restrict__ fcz, const double *__restrict__ xc,
+                        const double *__restrict__ yc, const double *__restrict__ zc,
+                        const double *__restrict__ fp, LimArgs a) {
+  const int c = xcd_block() * kBlock + threadIdx.x;
+  if (c >= N) return;
+  const double tiny = 1.e-15;
+  const double cx = xc[c], cy = yc[c], cz = zc[c];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const int meth = a.method[k];
+    if (meth == 0) continue;
+    const double *phi = a.phi[k];
+    const double pc = phi[c];
+    double lo = min2(1.e30, pc), hi = max2(-1.e30, pc);
+    for (int pass = 0; pass < 2; ++pass)
+      for (int s = 0; s < W; ++s) {
+        const int cd = code[(size_t)s * N + c], role = cd & 3;
+        if (pass == 0 ? (role != ROLE_OWNER && role != ROLE_NEIGH) : role != ROLE_BOUND) continue;
+        const double v = pass == 0 ? phi[nbr[(size_t)s * N + c]] : fp[(size_t)k * F + (cd >> 2)];
+        lo = min2(lo, v);
+        hi = max2(hi, v);
+      }
+    double *gr = a.grad[k];
+    const double gx = gr[c], gy = gr[N + c], gz = gr[2 * N + c];
+    const double eps2 = meth == 2 ? a.eps2[k] : 0.;
+    const double dmax = hi - pc, dmin = lo - pc;
+    double lim = meth == 2 ? (double)1.e30f : 1.;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int s = 0; s < W; ++s) {
+        const int cd = code[(size_t)s * N + c], role = cd & 3, f = cd >> 2;
+        if (pass == 0 ? (role != ROLE_OWNER && role != ROLE_NEIGH) : role != ROLE_BOUND) continue;
+        const double d = gx * (fcx[f] - cx) + gy * (fcy[f] - cy) + gz * (fcz[f] - cz);
+        if (meth == 1) {
+          if (d > dmax + tiny) lim = min2(lim, dmax / (d + tiny));
+          else if (d < dmin - tiny) lim = min2(lim, dmin / (d + tiny));
+        } else {
+          const double base = pass == 0 ? pc : fp[(size_t)k * F + f];
+          const double dp = d >= 0. ? hi - base : lo - base;
+          const double dp2 = dp * dp, dm2 = d * d;
+          const double part1 = (dp2 + eps2) * d + 2. * dm2 * dp;
+          const double part2 = dp2 + 2. * dm2 + dp * d + eps2;
+          const double r = part1 / (copysign(1., d) * (fabs(d) + tiny) * part2);
+          lim = min2(min2(1., r), lim);
+        }
+      }
+    gr[c] = lim * gx;
+    gr[N + c] = lim * gy;
+    gr[2 * N + c] = lim * gz;
+  }
+}
+
 struct MomArgs {
   int comp;
   double mu, c23;
@@ -210,13 +360,16 @@ struct MomArgs {
 // component's in the reference's order: diffusion
 // (AssembleIsotropicDiffusion.for:99-117), explicit diffusion
 // (AssembleExplicitDiffusion.for), bulk viscosity (AssembleBulkViscosity.for),
-// upwind convection (AssembleConvection.for:72-88).  FluxCf and FluxFf are the
-// same for the three components (cf, ff); FluxTf of component c goes to
-// tf3[c F + f].  The velocity face gradients, 'average' interpolation
-// (InterpolateGradientFromElementToFace.for:16-93), are formed here from the
-// cell gradients.  Interior fluxes depend on u, v, w only through the
-// component's own field, so those of v and w are the same whether formed
-// before or after the u solve.
+// upwind convection (AssembleConvection.for:72-88) and, for HRF = FS_HR_TVD
+// or FS_HR_NVF, the deferred high-resolution correction
+// (AssembleHRSchemeCorrection.for:39-133) from the upwind cell's gradient.
+// FluxCf and FluxFf are the same for the three components (cf, ff); FluxTf
+// of component c goes to tf3[c F + f].  The velocity face gradients,
+// 'average' interpolation (InterpolateGradientFromElementToFace.for:16-93),
+// are formed here from the cell gradients.  Interior fluxes depend on u, v,
+// w only through the component's own field and gradient, so those of v and w
+// are the same whether formed before or after the u solve.
+template <int HRF>
 __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, const int *__restrict__ nei,
                             const double *__restrict__ g, const double *__restrict__ gdiff,
                             const double *__restrict__ Tx, const double *__restrict__ Ty,
@@ -225,21 +378,28 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
                             const double *__restrict__ u, const double *__restrict__ v,
                             const double *__restrict__ w, const double *__restrict__ gu,
                             const double *__restrict__ gv, const double *__restrict__ gw,
-                            const double *__restrict__ mdot, double *__restrict__ cf, double *__restrict__ ff,
+                            const double *__restrict__ mdot, const double *__restrict__ dcx,
+                            const double *__restrict__ dcy, const double *__restrict__ dcz, int scheme,
+                            double bleed, double *__restrict__ cf, double *__restrict__ ff,
                             double *__restrict__ tf3) {
   const int f = xcd_block() * kBlock + threadIdx.x;
   if (f >= Fi) return;
   const int i = own[f], j = nei[f];
   const double gf = g[f], og = 1. - gf;
-  // face gradient of component k in direction d: fgv[k][d]
+  // cell gradients of component k in direction d at the owner and neighbour
   const double *G[3] = {gu, gv, gw};
-  double fgv[3][3];
+  double gi[3][3], gj[3][3], fgv[3][3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    fgv[k][0] = gf * G[k][i] + og * G[k][j];
-    fgv[k][1] = gf * G[k][N + i] + og * G[k][N + j];
-    fgv[k][2] = gf * G[k][2 * N + i] + og * G[k][2 * N + j];
-  }
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      gi[k][d] = G[k][(size_t)d * N + i];
+      gj[k][d] = G[k][(size_t)d * N + j];
+    }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) fgv[k][d] = gf * gi[k][d] + og * gj[k][d];
   const double gamf = gf * mu + (1. - gf) * mu;
   const double cfl = gamf * gdiff[f], ffl = -(gamf * gdiff[f]);
   const double md = mdot[f];
@@ -258,6 +418,14 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
   const double tx = Tx[f], ty = Ty[f], tz = Tz[f], sx = Sx[f], sy = Sy[f], sz = Sz[f];
   const double dv = fgv[0][0] + fgv[1][1] + fgv[2][2];
   const double *P[3] = {u, v, w};
+  // the upwind side: the owner for mdot > 0, else the neighbour
+  const bool up = md > 0.;
+  double ddx = 0., ddy = 0., ddz = 0.;
+  if constexpr (HRF != FS_HR_NONE) {
+    ddx = up ? dcx[f] : -dcx[f];
+    ddy = up ? dcy[f] : -dcy[f];
+    ddz = up ? dcz[f] : -dcz[f];
+  }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const double pi = P[c][i], pj = P[c][j];
@@ -272,6 +440,11 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
     T = T + v3;
     // upwind convection
     T = T + ccl * pi + fcl * pj;
+    if constexpr (HRF != FS_HR_NONE) {
+      const double dphi = hr_increment<HRF>(scheme, bleed, up ? pi : pj, up ? pj : pi, up ? gi[c][0] : gj[c][0],
+                                            up ? gi[c][1] : gj[c][1], up ? gi[c][2] : gj[c][2], ddx, ddy, ddz);
+      T = T + md * dphi * 1.;
+    }
     tf3[(size_t)c * F + f] = T;
   }
 }
@@ -617,6 +790,12 @@ struct SimpleCase {
   DevBuf<double> tf3; // momentum FluxTf of u, v, w (F each)
   DevBuf<double> pcprev; // the last converged p' (input of the warm-start network)
   DevBuf<int> btype, bzq; // boundary condition per boundary face (empty: all walls)
+  // high-resolution convection and gradient limiting; iter = nIter, the
+  // number of the outer iteration in progress (0 before the first)
+  int hr_fw = FS_HR_NONE, hr_scheme = FS_CONV_UPWIND, hr_start = 0, lim_mom = 0, lim_cont = 0, iter = 0;
+  double bleed = 0.0;
+  DevBuf<double> fcx, fcy, fcz, xc, yc, zc; // centroids, uploaded with the first limiter
+  DevBuf<double> lim_part, lim_eps;
   // momentum systems in colour-major cell order (the colour classes of the
   // multicolour Gauss-Seidel contiguous): their solver context on the
   // renumbered mesh (sharing the stream), the renumbering, the permuted
@@ -681,19 +860,58 @@ template <int NF> static void gradients(SimpleCase &C, GradArgs a) {
   }
 }
 
+// The limiters of the fields of the gradient launch just made (its boundary
+// face values are still in fp), methods[k] per field.
+template <int NF> static void limit_gradients(SimpleCase &C, const GradArgs &g, const int *methods) {
+  LimArgs a{};
+  bool any = false, range = false;
+  for (int k = 0; k < NF; ++k) {
+    a.phi[k] = g.phi[k];
+    a.grad[k] = g.grad[k];
+    a.method[k] = methods[k];
+    any = any || methods[k] != 0;
+    range = range || methods[k] == 2;
+  }
+  if (!any) return;
+  if (range) {
+    const int nb = std::min(256, blocks(C.N + C.Fb));
+    C.lim_part.alloc((size_t)2 * NF * nb);
+    C.lim_eps.alloc(4);
+    hipLaunchKernelGGL(k_lim_range<NF>, dim3(nb), dim3(kBlock), 0, C.s, C.N, C.Fi, C.F, C.fp.p, a,
+                       C.lim_part.p);
+    hipLaunchKernelGGL(k_lim_eps<NF>, dim3(1), dim3(kBlock), 0, C.s, nb, C.lim_part.p, C.lim_eps.p);
+    a.eps2 = C.lim_eps.p;
+  }
+  hipLaunchKernelGGL(k_limit<NF>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W, C.F, C.code.p,
+                     C.mesh->d_nbr.p, C.fcx.p, C.fcy.p, C.fcz.p, C.xc.p, C.yc.p, C.zc.p, C.fp.p, a);
+}
+
 static void phase_momentum_gradients(SimpleCase &C) {
   GradArgs a{};
   a.phi[0] = C.u.p; a.phi[1] = C.v.p; a.phi[2] = C.w.p; a.phi[3] = C.p.p;
   a.bval[0] = C.bu.p; a.bval[1] = C.bv.p; a.bval[2] = C.bw.p; a.bval[3] = C.bp.p;
   a.grad[0] = C.gu.p; a.grad[1] = C.gv.p; a.grad[2] = C.gw.p; a.grad[3] = C.gp.p;
   gradients<4>(C, a);
+  // LimitGradientVariable before the boundary corrections
+  // (CalculateGradients.for:252-272)
+  const int methods[4] = {C.lim_mom, C.lim_mom, C.lim_mom, C.lim_cont};
+  limit_gradients<4>(C, a, methods);
   hipLaunchKernelGGL(k_wall_pgrad, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N,
                      C.own.p, C.nx.p, C.ny.p, C.nz.p, C.btype.p, C.bzq.p, C.gp.p, C.bgp.p);
-  // the interior momentum face fluxes of all three components
-  hipLaunchKernelGGL(k_mom_face3, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.F, C.N, C.own.p, C.nei.p,
-                     C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p, C.Sx.p, C.Sy.p, C.Sz.p, C.mu,
-                     (double)(-2.0f / 3.0f), C.u.p, C.v.p, C.w.p, C.gu.p, C.gv.p, C.gw.p, C.mdot.p, C.cf.p,
-                     C.ff.p, C.tf3.p);
+  // the interior momentum face fluxes of all three components, with the
+  // high-resolution correction from iteration nIterStartApplyingHR on
+  // (AssembleConvection.for:156-165)
+  const int hrf = (C.hr_fw != FS_HR_NONE && C.hr_scheme != FS_CONV_UPWIND && C.iter >= C.hr_start)
+                      ? C.hr_fw : FS_HR_NONE;
+  auto face3 = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.F, C.N, C.own.p, C.nei.p,
+                       C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p, C.Sx.p, C.Sy.p, C.Sz.p, C.mu,
+                       (double)(-2.0f / 3.0f), C.u.p, C.v.p, C.w.p, C.gu.p, C.gv.p, C.gw.p, C.mdot.p, C.dx.p,
+                       C.dy.p, C.dz.p, C.hr_scheme, C.bleed, C.cf.p, C.ff.p, C.tf3.p);
+  };
+  if (hrf == FS_HR_TVD) face3(k_mom_face3<FS_HR_TVD>);
+  else if (hrf == FS_HR_NVF) face3(k_mom_face3<FS_HR_NVF>);
+  else face3(k_mom_face3<FS_HR_NONE>);
 }
 
 static void phase_momentum_assemble(SimpleCase &C, int comp) {
@@ -833,6 +1051,8 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
   a.bval[0] = nullptr; // walls: BPressureC = PressureC(owner)
   a.grad[0] = C.gpc.p;
   gradients<1>(C, a);
+  const int methods[1] = {C.lim_cont};
+  limit_gradients<1>(C, a, methods);
   hipLaunchKernelGGL(k_correct_u, dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.gpc.p, C.du.p, C.dv.p,
                      C.dw.p, C.u.p, C.v.p, C.w.p);
   hipLaunchKernelGGL(k_correct_mdot, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.own.p, C.nei.p,
@@ -859,6 +1079,7 @@ int simple_iterate(SimpleCase &C, int niter, fs_simple_stats *st) {
   fs_simple_stats loc{};
   if (C.sym_dirty) update_symmetry(C); // the start-up UpdateSymmetry (MainProgram.for:128)
   for (int it = 0; it < niter; ++it) {
+    C.iter = C.iter + 1; // nIter (MainProgram.for:257)
     phase_momentum_gradients(C);
     for (int comp = 0; comp < 3; ++comp) {
       phase_momentum_assemble(C, comp);
@@ -1174,6 +1395,47 @@ int fs_case_set(fs_case *cs, const char *name, const double *in, long long count
   FS_HIP(hipStreamSynchronize(cs->c.s));
   const std::string nm(name);
   if (nm == "u" || nm == "v" || nm == "w") cs->c.sym_dirty = cs->c.btype.p != nullptr;
+  return 0;
+  CASE_GUARD_END
+}
+
+int fs_case_set_convection(fs_case *cs, int framework, int scheme, double bleed, int start_iter) {
+  if (!cs) {
+    set_error("fs_case_set_convection: null case");
+    return -1;
+  }
+  if (!fs::hr_scheme_defined(framework, scheme)) {
+    set_error("fs_case_set_convection: scheme " + std::to_string(scheme) + " is not defined in framework " +
+              std::to_string(framework));
+    return -1;
+  }
+  auto &C = cs->c;
+  C.hr_fw = framework;
+  C.hr_scheme = scheme;
+  C.bleed = bleed;
+  C.hr_start = start_iter;
+  return 0;
+}
+
+int fs_case_set_gradient_limiters(fs_case *cs, int momentum_method, int continuity_method) {
+  if (!cs || momentum_method < 0 || momentum_method > 2 || continuity_method < 0 || continuity_method > 2) {
+    set_error("fs_case_set_gradient_limiters: methods are 0 (off), 1 or 2");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  if ((momentum_method || continuity_method) && !C.xc.p) {
+    const auto &F = C.geom.face;
+    C.fcx.upload(F.cx, C.s);
+    C.fcy.upload(F.cy, C.s);
+    C.fcz.upload(F.cz, C.s);
+    C.xc.upload(C.geom.xc, C.s);
+    C.yc.upload(C.geom.yc, C.s);
+    C.zc.upload(C.geom.zc, C.s);
+    FS_HIP(hipStreamSynchronize(C.s));
+  }
+  C.lim_mom = momentum_method;
+  C.lim_cont = continuity_method;
   return 0;
   CASE_GUARD_END
 }

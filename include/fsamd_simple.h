@@ -98,6 +98,34 @@ enum fs_bc_type { FS_BC_WALL = 0, FS_BC_SYMMETRY = 1 };
  * at least p faces).  Fails for sizes that do not cover the boundary. */
 int fs_case_set_patch_types(fs_case *c, int npatches, const int *patch_nfaces,
                             const int *patch_type);
+/* High-resolution convection of the momentum equations: HRFrameworkMomentum
+ * ('none', 'tvd', 'nvf'), ConvectionSchemeMomentum, BleedMomentum and
+ * nIterStartApplyingHR (User0.F90:367-410).  Upwind fluxes stay implicit and
+ * mdot (phi_HR - phi_C) of every interior face joins FluxTf as a deferred
+ * correction (AssembleConvection.for:156-165,
+ * AssembleHRSchemeCorrection.for:4-133) from the outer iteration numbered
+ * start_iter on (iterations count from 1 over the case's life).  A case
+ * starts with upwind convection.  Fails for a scheme the framework does not
+ * define: TVD (LimitersTVDType.for:4) defines upwind .. charm, NVF
+ * (LimitersNVFType.for:4) upwind, smart, minmod, boundedcd, cd, osher, stoic,
+ * muscl, superbee, downwind and gamma; the volume-fraction schemes stacs,
+ * hric and cicsam are not provided. */
+enum fs_hr_framework { FS_HR_NONE = 0, FS_HR_TVD = 1, FS_HR_NVF = 2 };
+enum fs_conv_scheme {
+  FS_CONV_UPWIND = 0, FS_CONV_DOWNWIND = 1, FS_CONV_FROMM = 2, FS_CONV_SOU = 3, FS_CONV_CD = 4,
+  FS_CONV_QUICK = 5, FS_CONV_SUPERBEE = 6, FS_CONV_MINMOD = 7, FS_CONV_OSHER = 8, FS_CONV_VANLEER = 9,
+  FS_CONV_MUSCL = 10, FS_CONV_SMART = 11, FS_CONV_HQUICK = 12, FS_CONV_UMIST = 13, FS_CONV_CHARM = 14,
+  FS_CONV_BOUNDEDCD = 15, FS_CONV_STOIC = 16, FS_CONV_GAMMA = 17
+};
+int fs_case_set_convection(fs_case *c, int framework, int scheme, double bleed, int start_iter);
+/* Gradient limiters (LimitGradientVariable, LimitGradients.for:3):
+ * momentum_method limits the u, v, w gradients, continuity_method those of
+ * p and p' (LimitGradientMomentum/ContinuityMethod, User0.F90:132-149);
+ * 0 = off (the reference's default), 1 = min/max limiter with rk = 1
+ * (:68-262), 2 = the smooth limiter with eps = 1e-3 of the field's range
+ * (:263-437).  The limited gradient replaces the Green-Gauss one before the
+ * boundary corrections, and the boundary gradients become the owners'. */
+int fs_case_set_gradient_limiters(fs_case *c, int momentum_method, int continuity_method);
 /* urfMomentum, urfPressure, ASSolverMomentum/rrFMomentum/ASIterMomentum/
  * LMultigridMomentum and the Continuity equivalents (User0.F90:271-318). */
 int fs_case_configure(fs_case *c, double urf_u, double urf_p, int msolver,

@@ -29,6 +29,21 @@ $FC $FFLAGS -c ref_solve_driver.f90 -o _ref/ref_solve_driver.o
 $FC -fopenmp -o _ref/ref_solve.x _ref/ref_solve_driver.o "${OBJS[@]}"
 echo "built _ref/ref_solve.x"
 
+# The gradient limiter and the high-resolution convection correction with
+# their limiter functions, linked with ref_conv_driver.f90.
+CONV_OBJS=()
+for f in "$REF"/SolveEquations/GradientFiles/LimitGradients.for \
+         "$REF"/SolveEquations/Convection/AssembleHRSchemeCorrection.for \
+         "$REF"/SolveEquations/Convection/LimitersTVDType.for \
+         "$REF"/SolveEquations/Convection/LimitersNVFType.for; do
+  b=$(basename "$f"); o="_ref/obj/${b%.*}.o"
+  if [ ! -f "$o" ] || [ "$f" -nt "$o" ]; then $FC $FFLAGS -c "$f" -o "$o"; fi
+  CONV_OBJS+=("$o")
+done
+$FC $FFLAGS -c ref_conv_driver.f90 -o _ref/ref_conv_driver.o
+$FC -fopenmp -o _ref/ref_conv.x _ref/ref_conv_driver.o "${CONV_OBJS[@]}" "${OBJS[@]}"
+echo "built _ref/ref_conv.x"
+
 # The same driver linked with the device drop-in SolveEquation (the Fortran
 # shim of the product) instead of SolveAlegbraicEquations.for.
 LIBDIR="$(cd .. && pwd)/flowsolversourcecode_amd"

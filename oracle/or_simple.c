@@ -235,6 +235,211 @@ static void face_gradient(or_case *c, const double *phi, const double *gr, doubl
   }
 }
 
+/* MIN and MAX as the reference build evaluates them: left to right, a NaN
+ * operand after the first is passed over. */
+static double mn2(double a, double b) { return (a < b || b != b) ? a : b; }
+static double mx2(double a, double b) { return (a > b || b != b) ? a : b; }
+
+/* TVDLimiter (LimitersTVDType.for:4-115): psi(r) of the TVD form. */
+double or_tvd_psi(int scheme, double r) {
+  switch (scheme) {
+  case OR_CONV_UPWIND: return 0.;
+  case OR_CONV_DOWNWIND: return 2.;
+  case OR_CONV_FROMM: return 0.5 * (1. + r);
+  case OR_CONV_SOU: return r;
+  case OR_CONV_CD: return 1.;
+  case OR_CONV_QUICK: return (3. + r) / 4.;
+  case OR_CONV_SUPERBEE: return mx2(mx2(0., mn2(1., 2. * r)), mn2(2., r));
+  case OR_CONV_MINMOD: return mx2(0., mn2(1., r));
+  case OR_CONV_OSHER: return mx2(0., mn2(2., r));
+  case OR_CONV_VANLEER: return (r + fabs(r)) / (1. + fabs(r));
+  case OR_CONV_MUSCL: return mx2(0., mn2(mn2(2. * r, 0.5 * (1. + r)), 2.));
+  case OR_CONV_SMART: return mx2(0., mn2(mn2(2. * r, 0.75 * r + 0.25), 4.));
+  case OR_CONV_HQUICK: return 2. * (r + fabs(r)) / (r + 3.);
+  case OR_CONV_UMIST: return mx2(0., mn2(mn2(mn2(2. * r, 0.25 + 0.75 * r), 0.75 + 0.25 * r), 2.));
+  case OR_CONV_CHARM: return r <= 0. ? 0. : r * (3. * r + 1.) / ((r + 1.) * (r + 1.));
+  default: return NAN;
+  }
+}
+
+/* NVFLimiter (LimitersNVFType.for:4-112): the normalised face value of the
+ * NVF form; BettaConvection is the REAL(4) literal 0.1 (User0.F90:408). */
+double or_nvf_phif(int scheme, double t) {
+  const int in01 = t >= 0. && t <= 1.;
+  switch (scheme) {
+  case OR_CONV_UPWIND: return t;
+  case OR_CONV_SMART: return in01 ? mn2(mn2(3. * t, 0.375 + 0.75 * t), (2. + t) / 3.) : t;
+  case OR_CONV_MINMOD: return in01 ? mn2(1.5 * t, 0.5 * t + 0.5) : t;
+  case OR_CONV_BOUNDEDCD: return in01 ? 0.5 * t + 0.5 : t;
+  case OR_CONV_CD: return 0.5 * t + 0.5;
+  case OR_CONV_OSHER: return in01 ? mn2(1.5 * t, 1.) : t;
+  case OR_CONV_STOIC: return in01 ? mn2(mn2(0.5 * t + 0.5, 0.375 + 0.75 * t), (2. + t) / 3.) : t;
+  case OR_CONV_MUSCL: return in01 ? mn2(mn2(2. * t, 0.25 + t), 1.) : t;
+  case OR_CONV_SUPERBEE: return in01 ? mn2(mn2(mn2(2. * t, 0.5 * t + 0.5), 1.5 * t), 1.) : t;
+  case OR_CONV_DOWNWIND: return 1.;
+  case OR_CONV_GAMMA: {
+    const double beta = (double)0.1f;
+    if (t > 0. && t < beta) {
+      const double factor = 0.5 / beta;
+      return (-factor * t + (1. + factor)) * t;
+    }
+    if (t >= beta && t < 1.) return 0.5 + 0.5 * t;
+    return t;
+  }
+  default: return NAN;
+  }
+}
+
+static int tvd_defines(int s) { return s >= OR_CONV_UPWIND && s <= OR_CONV_CHARM; }
+static int nvf_defines(int s) {
+  switch (s) {
+  case OR_CONV_UPWIND: case OR_CONV_SMART: case OR_CONV_MINMOD: case OR_CONV_BOUNDEDCD: case OR_CONV_CD:
+  case OR_CONV_OSHER: case OR_CONV_STOIC: case OR_CONV_MUSCL: case OR_CONV_SUPERBEE: case OR_CONV_DOWNWIND:
+  case OR_CONV_GAMMA: return 1;
+  default: return 0;
+  }
+}
+
+int or_case_set_convection(or_case *c, int framework, int scheme, double bleed, int start_iter) {
+  if (framework == OR_HR_TVD ? !tvd_defines(scheme) : framework == OR_HR_NVF ? !nvf_defines(scheme)
+                                                                               : framework != OR_HR_NONE)
+    return -1;
+  c->hr_fw = framework;
+  c->hr_scheme = scheme;
+  c->bleed = bleed;
+  c->hr_start = start_iter;
+  return 0;
+}
+
+int or_case_set_gradient_limiters(or_case *c, int momentum_method, int continuity_method) {
+  if (momentum_method < 0 || momentum_method > 2 || continuity_method < 0 || continuity_method > 2) return -1;
+  c->lim_mom = momentum_method;
+  c->lim_cont = continuity_method;
+  return 0;
+}
+
+/* LimitGradientMethod1 (LimitGradients.for:68-262, rk = 1) and
+ * LimitGradientMethod2 (:263-437 with PhiFieldMinMax :440-476): one factor
+ * per cell from the min/max of the cell, its neighbours and its boundary
+ * values and the gradient's extrapolation to each face centroid; the
+ * boundary gradients become the owners' limited gradients. */
+void or_limit_gradient(or_case *c, int method, const double *phi, const double *bphi, double *gr, double *bgr) {
+  const or_mesh *m = c->m;
+  const int N = c->N, Fi = c->Fi, Fb = c->Fb, W = c->W;
+  const double big = 1.e30, tiny = 1.e-15;
+  double *lo = dz(N), *hi = dz(N), *lim = dz(N);
+  double eps2 = 0.;
+  if (method == 2) {
+    double pmin = big, pmax = -big;
+    for (int i = 0; i < N; ++i) { pmin = mn2(pmin, phi[i]); pmax = mx2(pmax, phi[i]); }
+    for (int b = 0; b < Fb; ++b) { pmin = mn2(pmin, bphi[b]); pmax = mx2(pmax, bphi[b]); }
+    const double e = 1.e-3 * (pmax - pmin);
+    eps2 = e * e;
+  }
+  for (int i = 0; i < N; ++i) {
+    lo[i] = mn2(big, phi[i]);
+    hi[i] = mx2(-big, phi[i]);
+    lim[i] = method == 2 ? (double)1.e30f : 1.;
+  }
+  for (int i = 0; i < N; ++i)
+    for (int s = 0; s < m->nef[i]; ++s) {
+      const int ke = m->enbr[(size_t)s * N + i];
+      if (ke != 0) { lo[i] = mn2(lo[i], phi[ke - 1]); hi[i] = mx2(hi[i], phi[ke - 1]); }
+    }
+  for (int b = 0; b < Fb; ++b) {
+    const int k = m->owner[Fi + b];
+    lo[k] = mn2(lo[k], bphi[b]);
+    hi[k] = mx2(hi[k], bphi[b]);
+  }
+  /* one face's term: the face centroid f seen from cell i, with base the
+   * value the method-2 increment starts from */
+  for (int pass = 0; pass < 2; ++pass) {
+    const int count = pass == 0 ? N : Fb;
+    for (int q = 0; q < count; ++q) {
+      const int ns = pass == 0 ? m->nef[q] : 1;
+      for (int s = 0; s < ns; ++s) {
+        int i, f;
+        double base;
+        if (pass == 0) {
+          if (m->enbr[(size_t)s * N + q] == 0) continue;
+          i = q;
+          f = m->gef[(size_t)s * N + q] - 1;
+          base = phi[i];
+        } else {
+          f = Fi + q;
+          i = m->owner[f];
+          base = bphi[q];
+        }
+        const double gx = gr[i], gy = gr[N + i], gz = gr[2 * N + i];
+        const double d = gx * (m->fcx[f] - m->xc[i]) + gy * (m->fcy[f] - m->yc[i]) + gz * (m->fcz[f] - m->zc[i]);
+        if (method == 1) {
+          const double dmax = hi[i] - phi[i], dmin = lo[i] - phi[i];
+          if (d > dmax + tiny) lim[i] = mn2(lim[i], dmax / (d + tiny));
+          else if (d < dmin - tiny) lim[i] = mn2(lim[i], dmin / (d + tiny));
+        } else {
+          const double dp = d >= 0. ? hi[i] - base : lo[i] - base;
+          const double dp2 = dp * dp, dm2 = d * d;
+          const double part1 = (dp2 + eps2) * d + 2. * dm2 * dp;
+          const double part2 = dp2 + 2. * dm2 + dp * d + eps2;
+          const double r = part1 / (copysign(1., d) * (fabs(d) + tiny) * part2);
+          lim[i] = mn2(mn2(1., r), lim[i]);
+        }
+      }
+    }
+  }
+  (void)W;
+  for (int i = 0; i < N; ++i) {
+    gr[i] = lim[i] * gr[i];
+    gr[N + i] = lim[i] * gr[N + i];
+    gr[2 * N + i] = lim[i] * gr[2 * N + i];
+  }
+  for (int b = 0; b < Fb; ++b) {
+    const int k = m->owner[Fi + b];
+    bgr[b] = gr[k];
+    bgr[Fb + b] = gr[N + k];
+    bgr[2 * Fb + b] = gr[2 * N + k];
+  }
+  free(lo);
+  free(hi);
+  free(lim);
+}
+
+/* HRSchemesCorrections (AssembleHRSchemeCorrection.for:31-133) for a velocity
+ * component: the deferred correction mdot (phi_HR - phi_C) of every interior
+ * face added to FluxTf, phi_U extrapolated from the upwind cell's gradient. */
+void or_hr_correction(or_case *c, const double *phi, const double *gr, double *ftf) {
+  if (c->hr_fw == OR_HR_NONE || c->hr_scheme == OR_CONV_UPWIND) return;
+  const or_mesh *m = c->m;
+  const int N = c->N;
+  const double small = (double)1.e-9f;
+  for (int k = 0; k < c->Fi; ++k) {
+    const int i = m->owner[k], j = m->neigh[k];
+    int iu;
+    double phic, phid, dcx, dcy, dcz;
+    if (c->mdot[k] > 0.) {
+      iu = i; phic = phi[i]; phid = phi[j];
+      dcx = m->dx[k]; dcy = m->dy[k]; dcz = m->dz[k];
+    } else {
+      iu = j; phic = phi[j]; phid = phi[i];
+      dcx = -m->dx[k]; dcy = -m->dy[k]; dcz = -m->dz[k];
+    }
+    const double phiu = phid - 2. * (gr[iu] * dcx + gr[N + iu] * dcy + gr[2 * N + iu] * dcz);
+    double phihr;
+    if (c->hr_fw == OR_HR_NVF) {
+      const double diff = phid - phiu;
+      const double t = fabs(diff) < small ? 0. : (phic - phiu) / diff;
+      phihr = or_nvf_phif(c->hr_scheme, t) * diff + phiu;
+      if (c->hr_scheme != OR_CONV_GAMMA) phihr = (1. - c->bleed) * phihr + c->bleed * phic;
+    } else {
+      const double diff = phid - phic;
+      const double r = fabs(diff) < small ? 0. : (phic - phiu) / diff;
+      phihr = phic + 0.5 * or_tvd_psi(c->hr_scheme, r) * diff;
+      phihr = (1. - c->bleed) * phihr + c->bleed * phic;
+    }
+    ftf[k] = ftf[k] + c->mdot[k] * (phihr - phic) * 1.;
+  }
+}
+
 /* The gradient block at the head of SolveMomentumX (:182-222), with the
  * tangential projection of the boundary pressure gradient at walls
  * (CorrectGradients.for, CorrectPresureGradientAtWalls). */
@@ -243,6 +448,14 @@ void or_case_momentum_gradients(or_case *c) {
   or_gradient(c, c->v, c->bv, c->gv, c->bgv);
   or_gradient(c, c->w, c->bw, c->gw, c->bgw);
   or_gradient(c, c->p, c->bp, c->gp, c->bgp);
+  /* LimitGradientVariable after the Green-Gauss sweeps, before the boundary
+   * corrections (CalculateGradients.for:252-272) */
+  if (c->lim_mom) {
+    or_limit_gradient(c, c->lim_mom, c->u, c->bu, c->gu, c->bgu);
+    or_limit_gradient(c, c->lim_mom, c->v, c->bv, c->gv, c->bgv);
+    or_limit_gradient(c, c->lim_mom, c->w, c->bw, c->gw, c->bgw);
+  }
+  if (c->lim_cont) or_limit_gradient(c, c->lim_cont, c->p, c->bp, c->gp, c->bgp);
   const or_mesh *m = c->m;
   const int Fi = c->Fi, Fb = c->Fb, N = c->N;
   for (int b = 0; b < Fb; ++b) {
@@ -422,6 +635,9 @@ void or_case_momentum_assemble(or_case *c, int comp) {
     c->fff[k] = c->fff[k] + ffl;
     c->ftf[k] = c->ftf[k] + cfl * phi[i] + ffl * phi[j];
   }
+  /* deferred high-resolution correction (AssembleConvection.for:156-165) */
+  if (c->hr_fw != OR_HR_NONE && c->iter >= c->hr_start)
+    or_hr_correction(c, phi, comp == 0 ? c->gu : comp == 1 ? c->gv : c->gw, c->ftf);
   /* effective divergence (CalculateEffectiveDivergence.for:16-28); the
    * boundary mass fluxes of walls are zero */
   memset(c->effdiv, 0, sizeof(double) * N);
@@ -545,6 +761,7 @@ void or_case_pc_solve_correct(or_case *c) {
   const double rpc = c->refcell >= 0 ? c->pc[c->refcell] : 0.0;
   for (int i = 0; i < N; ++i) c->p[i] = c->p[i] + c->urf_p * (c->pc[i] - rpc);
   or_gradient(c, c->pc, c->bpc, c->gpc, c->bgpc);
+  if (c->lim_cont) or_limit_gradient(c, c->lim_cont, c->pc, c->bpc, c->gpc, c->bgpc);
   for (int i = 0; i < N; ++i) {
     c->u[i] = c->u[i] - c->du[i] * c->gpc[i];
     c->v[i] = c->v[i] - c->dv[i] * c->gpc[N + i];
@@ -578,6 +795,7 @@ void or_case_iterate(or_case *c, int niter) {
   /* the UpdateSymmetry of the start-up (MainProgram.for:128) */
   if (c->sym_dirty) or_case_update_symmetry(c);
   for (int it = 0; it < niter; ++it) {
+    c->iter = c->iter + 1; /* nIter of MainProgram.for:257 */
     or_case_momentum_gradients(c);
     for (int comp = 0; comp < 3; ++comp) {
       or_case_momentum_assemble(c, comp);

@@ -67,9 +67,21 @@ typedef struct {
    * z-normal the symmetry pressure-gradient projection reads */
   int *btype, *bzq;
   int sym_dirty; /* symmetry boundary velocities not yet derived */
+  /* high-resolution convection of the momentum equations (HRFramework,
+   * ConvectionScheme, Bleed, nIterStartApplyingHR) and the gradient limiters
+   * (LimitGradientMomentum/ContinuityMethod, 0 = off); iter = nIter, the
+   * number of the outer iteration in progress */
+  int hr_fw, hr_scheme, hr_start, lim_mom, lim_cont, iter;
+  double bleed;
 } or_case;
 
 enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 1 };
+enum { OR_HR_NONE = 0, OR_HR_TVD = 1, OR_HR_NVF = 2 };
+enum {
+  OR_CONV_UPWIND = 0, OR_CONV_DOWNWIND, OR_CONV_FROMM, OR_CONV_SOU, OR_CONV_CD, OR_CONV_QUICK,
+  OR_CONV_SUPERBEE, OR_CONV_MINMOD, OR_CONV_OSHER, OR_CONV_VANLEER, OR_CONV_MUSCL, OR_CONV_SMART,
+  OR_CONV_HQUICK, OR_CONV_UMIST, OR_CONV_CHARM, OR_CONV_BOUNDEDCD, OR_CONV_STOIC, OR_CONV_GAMMA
+};
 
 or_case *or_case_new(or_mesh *m, const double *wall_u, const double *wall_v,
                      const double *wall_w, double rho, double mu, double urf_u,
@@ -81,6 +93,14 @@ void or_case_delete(or_case *c);
  * symmetry patch p (1-based) with fewer than p faces. */
 int or_case_set_patch_types(or_case *c, int npatches, const int *patch_nfaces, const int *patch_type);
 void or_case_update_symmetry(or_case *c);
+/* Returns -1 for a scheme the framework does not define (or a steady
+ * single-phase case cannot use) and for an unknown framework. */
+int or_case_set_convection(or_case *c, int framework, int scheme, double bleed, int start_iter);
+int or_case_set_gradient_limiters(or_case *c, int momentum_method, int continuity_method);
+double or_tvd_psi(int scheme, double r);
+double or_nvf_phif(int scheme, double phic_tilde);
+void or_limit_gradient(or_case *c, int method, const double *phi, const double *bphi, double *gr, double *bgr);
+void or_hr_correction(or_case *c, const double *phi, const double *gr, double *ftf);
 void or_gradient(or_case *c, const double *phi, const double *bphi, double *gr, double *bgr);
 void or_case_momentum_gradients(or_case *c);
 void or_case_momentum_assemble(or_case *c, int comp);

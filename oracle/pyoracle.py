@@ -135,6 +135,87 @@ def run_reference(nbr, nnbr, ac, anb, bc, fit, rrf, itmax, solver, mode=0, lmg=0
     return out
 
 
+REF_CONV = os.path.join(HERE, "_ref", "ref_conv.x")
+
+
+def ref_conv_available():
+    return os.path.exists(REF_CONV)
+
+
+def _patch_matrix(vals, patch_sizes, nbfmax, dtype):
+    """Boundary-face values (patch by patch) as the reference's (NumberOfBCSets,
+    NBFacesMax) array, in Fortran memory order."""
+    out = np.zeros((len(patch_sizes), nbfmax), dtype)
+    o = 0
+    for q, n in enumerate(patch_sizes):
+        out[q, :n] = vals[o:o + n]
+        o += n
+    return np.asfortranarray(out)
+
+
+def _patch_unmatrix(a, patch_sizes):
+    return np.concatenate([a[q, :n] for q, n in enumerate(patch_sizes)]) if len(patch_sizes) else a[:0, 0]
+
+
+def run_reference_conv(pm, mesh, mode, phi, grad, bphi, mdot=None, ftf=None, method=1, scheme="upwind",
+                       framework="tvd", bleed=0.0):
+    """Run the reference's LimitGradientVariable (mode 1, ``method`` 1 or 2) or
+    HRSchemesCorrections for 'velx' (mode 2) on a mesh slice of ``mesh`` (an
+    OrMesh of the polyMesh ``pm``).  grad = [gx | gy | gz] (3N); the boundary
+    gradients are the owners' (the Green-Gauss boundary copies).  Returns the
+    limited (grad, bgrad) for mode 1 and FluxTf (Fi) for mode 2."""
+    N, W, Fi = mesh.N, mesh.W, mesh.nif
+    sizes = [int(x) for x in pm.patch_sizes]
+    nbc, nbfmax = len(sizes), max(sizes)
+    own = np.asarray(pm.owner, np.int64)
+    bown = own[Fi:]
+    grad = np.asarray(grad, np.float64)
+    fc = {k: mesh.face[k] for k in ("fcx", "fcy", "fcz")}
+    if mdot is None:
+        mdot = np.zeros(Fi)
+    if ftf is None:
+        ftf = np.zeros(Fi)
+    with tempfile.TemporaryDirectory(dir=os.path.join(HERE, "_ref")) as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.array([N, W, Fi, nbc, nbfmax, mode, method], np.int32).tobytes())
+            f.write(scheme.ljust(20).encode() + framework.ljust(4).encode())
+            f.write(np.array([bleed], np.float64).tobytes())
+            f.write(np.array(sizes, np.int32).tobytes())
+            f.write(_patch_matrix(bown + 1, sizes, nbfmax, np.int32).tobytes(order="F"))
+            f.write(_c(mesh.nef, np.int32).tobytes())
+            f.write(_c(mesh.enbr, np.int32).tobytes())
+            f.write(_c(mesh.gef, np.int32).tobytes())
+            f.write(_c(own[:Fi] + 1, np.int32).tobytes())
+            f.write(_c(np.asarray(pm.neighbour, np.int64)[:Fi] + 1, np.int32).tobytes())
+            for k in ("xc", "yc", "zc"):
+                f.write(_c(mesh.cell[k], np.float64).tobytes())
+            for k in ("fcx", "fcy", "fcz"):
+                f.write(_c(fc[k][:Fi], np.float64).tobytes())
+            for k in ("fcx", "fcy", "fcz"):
+                f.write(_patch_matrix(fc[k][Fi:], sizes, nbfmax, np.float64).tobytes(order="F"))
+            for k in ("g", "dx", "dy", "dz"):
+                f.write(_c(mesh.face[k][:Fi], np.float64).tobytes())
+            f.write(_c(mdot, np.float64).tobytes())
+            f.write(_c(ftf, np.float64).tobytes())
+            f.write(_c(phi, np.float64).tobytes())
+            for d in range(3):
+                f.write(_c(grad[d * N:(d + 1) * N], np.float64).tobytes())
+            f.write(_patch_matrix(bphi, sizes, nbfmax, np.float64).tobytes(order="F"))
+            for d in range(3):
+                f.write(_patch_matrix(grad[d * N:(d + 1) * N][bown], sizes, nbfmax, np.float64).tobytes(order="F"))
+        r = subprocess.run([REF_CONV, fin, fout], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference run failed: {r.stderr[-2000:]}")
+        d = np.fromfile(fout, dtype=np.float64)
+    if mode == 2:
+        return d[:Fi].copy()
+    g = d[:3 * N].copy()
+    bm = d[3 * N:].reshape(3, nbfmax, nbc)  # three Fortran (nbc, nbfmax) blocks
+    bg = np.concatenate([_patch_unmatrix(bm[k].T, sizes) for k in range(3)])
+    return g, bg
+
+
 def _unflatten_mg(L, ne, ije, ijn, parents, nchild, eparent, nnbr_mg, acmg, enbr, children, anbmg):
     """Turn the reference's flat MultiGrid1 arrays into per-level maps:
     parent[l] (level l cells -> 1-based level l+1 cell), seed/children of level
@@ -289,7 +370,15 @@ class _OrCase(C.Structure):
                    "ws", "visc", "fcf", "fff", "fvf", "ftf", "fce", "fte", "effdiv", "ac", "anb", "bc", "dphi")] + \
                [("res_mom", C.c_double * 12), ("res_pc", C.c_double * 4), ("mass_l1", C.c_double),
                 ("it_mom", C.c_int * 3), ("it_pc", C.c_int), ("btype", C.POINTER(C.c_int)),
-                ("bzq", C.POINTER(C.c_int)), ("sym_dirty", C.c_int)]
+                ("bzq", C.POINTER(C.c_int)), ("sym_dirty", C.c_int)] + \
+               [(n, C.c_int) for n in ("hr_fw", "hr_scheme", "hr_start", "lim_mom", "lim_cont", "iter")] + \
+               [("bleed", C.c_double)]
+
+
+HR_FRAMEWORKS = {"none": 0, "tvd": 1, "nvf": 2}
+CONV_SCHEMES = {name: i for i, name in enumerate(
+    ("upwind", "downwind", "fromm", "sou", "cd", "quick", "superbee", "minmod", "osher", "vanleer", "muscl",
+     "smart", "hquick", "umist", "charm", "boundedcd", "stoic", "gamma"))}
 
 
 class OrCase:
@@ -345,6 +434,40 @@ class OrCase:
         sizes = np.ascontiguousarray(patch_sizes, dtype=np.int32)
         if L.or_case_set_patch_types(self.ptr, len(sizes), _c(sizes, np.int32), _c(codes, np.int32)) != 0:
             raise ValueError("or_case_set_patch_types: patches do not fit the boundary")
+
+    def set_convection(self, framework, scheme, bleed=0.0, start_iter=0):
+        """HRFramework ('none', 'tvd', 'nvf'), ConvectionScheme, Bleed and
+        nIterStartApplyingHR of the momentum equations."""
+        L = lib()
+        L.or_case_set_convection.argtypes = [C.POINTER(_OrCase), C.c_int, C.c_int, C.c_double, C.c_int]
+        if L.or_case_set_convection(self.ptr, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], bleed,
+                                    start_iter) != 0:
+            raise ValueError(f"convection scheme {scheme!r} not defined in framework {framework!r}")
+
+    def set_gradient_limiters(self, momentum=0, continuity=0):
+        """LimitGradientMomentumMethod / LimitGradientContinuityMethod (0 = off)."""
+        L = lib()
+        L.or_case_set_gradient_limiters.argtypes = [C.POINTER(_OrCase), C.c_int, C.c_int]
+        if L.or_case_set_gradient_limiters(self.ptr, momentum, continuity) != 0:
+            raise ValueError("limiter method must be 0, 1 or 2")
+
+    def limit_gradient(self, method, phi, bphi, grad):
+        """or_limit_gradient on copies: returns the limited (grad, bgrad)."""
+        L = lib()
+        L.or_limit_gradient.argtypes = [C.POINTER(_OrCase), C.c_int, _dp, _dp, _dp, _dp]
+        g = np.array(grad, np.float64)
+        bg = np.zeros(3 * self.dims["Fb"])
+        L.or_limit_gradient(self.ptr, method, _c(phi, np.float64), _c(bphi, np.float64), g, bg)
+        return g, bg
+
+    def hr_correction(self, phi, grad, ftf):
+        """or_hr_correction with the case's mdot and convection settings on a
+        copy of ftf (Fi or more entries); returns it."""
+        L = lib()
+        L.or_hr_correction.argtypes = [C.POINTER(_OrCase), _dp, _dp, _dp]
+        out = np.array(ftf, np.float64)
+        L.or_hr_correction(self.ptr, _c(phi, np.float64), _c(grad, np.float64), out)
+        return out
 
     def update_symmetry(self):
         lib().or_case_update_symmetry.argtypes = [C.POINTER(_OrCase)]

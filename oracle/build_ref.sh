@@ -44,6 +44,36 @@ $FC $FFLAGS -c ref_conv_driver.f90 -o _ref/ref_conv_driver.o
 $FC -fopenmp -o _ref/ref_conv.x _ref/ref_conv_driver.o "${CONV_OBJS[@]}" "${OBJS[@]}"
 echo "built _ref/ref_conv.x"
 
+# The reference's physics sources (geometry processing, assembly, gradients,
+# interpolation, continuity, boundary updates, turbulence support) in an
+# archive, so that the linker takes only what ref_simple_driver.f90 reaches.
+# The mesh reader and the storage allocator are left out (they need source
+# patches); the driver installs their results itself.
+mkdir -p _ref/simple_obj
+AR_SRCS=()
+AR_OBJS=()
+for d in Assemble Buoyancy ContinuityEquation Convection Diffusion Energy FalseTransient GradientFiles \
+         InitializeFluxesAndCoefficients Interpolation Momentum Periodicity PhysicalProperties \
+         SetBoundaryPatches SurfaceTension Transient UnderRelaxVariables UpdateVariablesAtBoundaries \
+         UpdateVariablesFromBoundaryConditions NormalAndTangentialVelocities Tensors FarFieldVariables; do
+  for f in "$REF/SolveEquations/$d"/*; do AR_SRCS+=("$f"); done
+done
+AR_SRCS+=("$REF/SolveEquations/GridAndGeometry/ProcessGeometryRoutine.for")
+while IFS= read -r f; do AR_SRCS+=("$f"); done < <(find "$REF/Turbulence" -type f \( -name '*.for' -o -name '*.f90' -o -name '*.F90' \) | sort)
+for f in "${AR_SRCS[@]}"; do
+  b=$(basename "$f"); o="_ref/simple_obj/${b%.*}.o"
+  if [ -f "_ref/obj/${b%.*}.o" ]; then continue; fi
+  if [ ! -f "$o" ] || [ "$f" -nt "$o" ]; then
+    $FC $FFLAGS -c "$f" -o "$o" 2> "_ref/simple_obj/${b%.*}.err" || { rm -f "$o"; continue; }
+  fi
+  AR_OBJS+=("$o")
+done
+rm -f _ref/libref_simple.a
+ar rcs _ref/libref_simple.a "${AR_OBJS[@]}" "${CONV_OBJS[@]}"
+$FC $FFLAGS -c ref_simple_driver.f90 -o _ref/ref_simple_driver.o
+$FC -fopenmp -o _ref/ref_simple.x _ref/ref_simple_driver.o "${OBJS[@]}" _ref/libref_simple.a
+echo "built _ref/ref_simple.x"
+
 # The same driver linked with the device drop-in SolveEquation (the Fortran
 # shim of the product) instead of SolveAlegbraicEquations.for.
 LIBDIR="$(cd .. && pwd)/flowsolversourcecode_amd"

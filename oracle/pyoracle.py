@@ -216,6 +216,113 @@ def run_reference_conv(pm, mesh, mode, phi, grad, bphi, mdot=None, ftf=None, met
     return g, bg
 
 
+REF_SIMPLE = os.path.join(HERE, "_ref", "ref_simple.x")
+
+REF_GEOM_FACE = ("Sx", "Sy", "Sz", "A", "fcx", "fcy", "fcz", "nx", "ny", "nz", "dcf", "dx", "dy", "dz", "ex",
+                 "ey", "ez", "Ex", "Ey", "Ez", "E", "gdiff", "Tx", "Ty", "Tz", "T")
+
+
+def ref_simple_available():
+    return os.path.exists(REF_SIMPLE)
+
+
+def _simple_input(f, pm, mesh, mode, extra_ints=(), extra_doubles=(), tail=()):
+    """The polyMesh and the Readit topology of ``mesh`` (an OrMesh of ``pm``)
+    in ref_simple_driver.f90's input layout."""
+    s = mesh.ptr.contents
+    N, W, nf, nif = mesh.N, mesh.W, mesh.nf, mesh.nif
+    lenw = s.len_w
+    fptr = np.asarray(pm.face_ptr, np.int64)
+    fnn = np.diff(fptr).astype(np.int32)
+    maxfn = int(fnn.max())
+    fn = np.zeros((nf, maxfn), np.int32)
+    nodes = np.asarray(pm.face_nodes, np.int64)
+    for k in range(nf):
+        fn[k, :fnn[k]] = nodes[fptr[k]:fptr[k + 1]] + 1
+    pts = np.ascontiguousarray(pm.points, np.float64)
+    nen = np.ctypeslib.as_array(s.nen, shape=(N,)).astype(np.int32)
+    lst = np.ctypeslib.as_array(s.len, shape=(N * lenw,)).astype(np.int32).reshape(lenw, N).T
+    lst = np.where(np.arange(lenw)[None, :] < nen[:, None], lst + 1, 0).astype(np.int32)
+    f.write(np.array([len(pts), nf, nif, N, len(pm.patch_sizes), W, lenw, maxfn, mode] + list(extra_ints),
+                     np.int32).tobytes())
+    if extra_doubles:
+        f.write(np.array(extra_doubles, np.float64).tobytes())
+    for d in range(3):
+        f.write(_c(pts[:, d], np.float64).tobytes())
+    f.write(fnn.tobytes())
+    f.write(np.asfortranarray(fn).tobytes(order="F"))
+    f.write(_c(np.asarray(pm.owner) + 1, np.int32).tobytes())
+    f.write(_c(np.asarray(pm.neighbour)[:nif] + 1, np.int32).tobytes())
+    f.write(_c(pm.patch_sizes, np.int32).tobytes())
+    for a in tail:
+        f.write(a)
+    f.write(_c(mesh.nef, np.int32).tobytes())
+    f.write(_c(mesh.gef, np.int32).tobytes())
+    f.write(_c(mesh.enbr, np.int32).tobytes())
+    f.write(_c(mesh.iown, np.int32).tobytes())
+    f.write(_c(mesh.inbr, np.int32).tobytes())
+    f.write(nen.tobytes())
+    f.write(np.asfortranarray(lst).tobytes(order="F"))
+
+
+def run_reference_geometry(pm, mesh):
+    """The reference's ProcessGeometry on the polyMesh ``pm`` with the Readit
+    topology of ``mesh``: a dict of face fields over all faces (interior,
+    then boundary) and cell fields, named as OrMesh names them, plus "g"
+    (interior faces)."""
+    nf, nif, N = mesh.nf, mesh.nif, mesh.N
+    with tempfile.TemporaryDirectory(dir=os.path.join(HERE, "_ref")) as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            _simple_input(f, pm, mesh, 1, tail=(_c(np.zeros(len(pm.patch_sizes)), np.int32).tobytes(),))
+        r = subprocess.run([REF_SIMPLE, fin, fout], capture_output=True, text=True, cwd=td)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference run failed: {r.stdout[-1000:]} {r.stderr[-2000:]}")
+        d = np.fromfile(fout, dtype=np.float64)
+    out, o = {}, 0
+    for k in REF_GEOM_FACE:
+        out[k] = d[o:o + nf].copy()
+        o += nf
+    out["g"] = d[o:o + nif].copy()
+    o += nif
+    for k in ("vol", "xc", "yc", "zc"):
+        out[k] = d[o:o + N].copy()
+        o += N
+    return out
+
+
+def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, niter=1, rho=1.0, mu=0.01,
+                         urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1, mitmax=20, psolver=SOR, prrf=0.01,
+                         pitmax=200, framework="none", scheme="upwind", bleed=0.0, start_iter=0, lim_mom=0,
+                         lim_cont=0):
+    """niter outer iterations of the reference's own SIMPLE (SolveMomentumX/Y/Z,
+    SolveContinuity) from rest on the polyMesh ``pm`` with the Readit topology of
+    ``mesh``; patch_types per patch "wall"/"symmetry" (default all walls).
+    Returns a dict u, v, w, p (N) and mdot (Fi)."""
+    nf, nif, N = mesh.nf, mesh.nif, mesh.N
+    codes = np.array([{"wall": 0, "symmetry": 1}[t] for t in (patch_types or ["wall"] * len(pm.patch_sizes))],
+                     np.int32)
+    ints = (msolver, mitmax, psolver, pitmax, niter, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], lim_mom,
+            lim_cont, start_iter)
+    dbl = (rho, mu, urf_u, urf_p, mrrf, prrf, bleed)
+    with tempfile.TemporaryDirectory(dir=os.path.join(HERE, "_ref")) as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            _simple_input(f, pm, mesh, 2, extra_ints=ints, extra_doubles=dbl, tail=(codes.tobytes(),))
+            for a in (wall_u, wall_v, wall_w):
+                f.write(_c(a, np.float64).tobytes())
+        r = subprocess.run([REF_SIMPLE, fin, fout], capture_output=True, text=True, cwd=td)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference run failed ({r.returncode}): {r.stdout[-1500:]} {r.stderr[-1500:]}")
+        d = np.fromfile(fout, dtype=np.float64)
+    o = len(REF_GEOM_FACE) * nf + nif + 4 * N
+    out = {}
+    for k, n in (("u", N), ("v", N), ("w", N), ("p", N), ("mdot", nif)):
+        out[k] = d[o:o + n].copy()
+        o += n
+    return out
+
+
 def _unflatten_mg(L, ne, ije, ijn, parents, nchild, eparent, nnbr_mg, acmg, enbr, children, anbmg):
     """Turn the reference's flat MultiGrid1 arrays into per-level maps:
     parent[l] (level l cells -> 1-based level l+1 cell), seed/children of level

@@ -15,6 +15,7 @@ import os
 import struct
 import subprocess
 import tempfile
+import time
 
 import numpy as np
 
@@ -294,16 +295,18 @@ def run_reference_geometry(pm, mesh):
 def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, niter=1, rho=1.0, mu=0.01,
                          urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1, mitmax=20, psolver=SOR, prrf=0.01,
                          pitmax=200, framework="none", scheme="upwind", bleed=0.0, start_iter=0, lim_mom=0,
-                         lim_cont=0):
+                         lim_cont=0, plmg=0, seconds=False):
     """niter outer iterations of the reference's own SIMPLE (SolveMomentumX/Y/Z,
     SolveContinuity) from rest on the polyMesh ``pm`` with the Readit topology of
-    ``mesh``; patch_types per patch "wall"/"symmetry" (default all walls).
-    Returns a dict u, v, w, p (N) and mdot (Fi)."""
+    ``mesh``; patch_types per patch "wall"/"symmetry" (default all walls);
+    plmg = 1 solves p' with the reference's algebraic multigrid
+    (LMultigridContinuity).  Returns a dict u, v, w, p (N) and mdot (Fi), and
+    with seconds=True also the run's wall time ("seconds")."""
     nf, nif, N = mesh.nf, mesh.nif, mesh.N
     codes = np.array([{"wall": 0, "symmetry": 1}[t] for t in (patch_types or ["wall"] * len(pm.patch_sizes))],
                      np.int32)
     ints = (msolver, mitmax, psolver, pitmax, niter, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], lim_mom,
-            lim_cont, start_iter)
+            lim_cont, start_iter, plmg)
     dbl = (rho, mu, urf_u, urf_p, mrrf, prrf, bleed)
     with tempfile.TemporaryDirectory(dir=os.path.join(HERE, "_ref")) as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
@@ -311,7 +314,10 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
             _simple_input(f, pm, mesh, 2, extra_ints=ints, extra_doubles=dbl, tail=(codes.tobytes(),))
             for a in (wall_u, wall_v, wall_w):
                 f.write(_c(a, np.float64).tobytes())
-        r = subprocess.run([REF_SIMPLE, fin, fout], capture_output=True, text=True, cwd=td)
+        t0 = time.perf_counter()
+        r = subprocess.run([REF_SIMPLE, fin, fout], capture_output=True, text=True, cwd=td,
+                           env=dict(os.environ, OMP_NUM_THREADS="1"))
+        el = time.perf_counter() - t0
         if r.returncode != 0:
             raise RuntimeError(f"reference run failed ({r.returncode}): {r.stdout[-1500:]} {r.stderr[-1500:]}")
         d = np.fromfile(fout, dtype=np.float64)
@@ -320,6 +326,8 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
     for k, n in (("u", N), ("v", N), ("w", N), ("p", N), ("mdot", nif)):
         out[k] = d[o:o + n].copy()
         o += n
+    if seconds:
+        out["seconds"] = el
     return out
 
 

@@ -20,7 +20,7 @@
 ! checker only and never part of the product.
 program ref_simple_driver
   use User0
-  use MultiGrid2, only: nIter
+  use MultiGrid2, only: nIter, nIterMG
   use BoundaryConditions1
   use Variables1
   use Variables2
@@ -28,6 +28,7 @@ program ref_simple_driver
   use Variables4
   use PhysicalProperties1
   use Residuals1
+  use MultiGrid1
   use Geometry1, only: NumberOfNodes, NumberOfElements, NumberOfBCSets, NumbOfElementNodes, &
                        ListOfElementNodes, NumbOfElementFaces, x, y, z
   use Geometry3, only: NIFaces, NFacesTotal, NIFaceOwner, NIFaceNeighbor, NBFaces, NBFaceOwner, &
@@ -39,7 +40,7 @@ program ref_simple_driver
   character(len=1024) :: fin, fout
   integer :: npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode
   integer :: i, j, k, f, q, nbfmax, nb, it
-  integer :: msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart
+  integer :: msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg
   double precision :: rho, mu, urfu, urfp, mrrf, prrf, bleed
   integer, allocatable :: fnn(:), fnodes(:,:), owner(:), neigh(:), psize(:), ptype(:)
   double precision, allocatable :: wallu(:), wallv(:), wallw(:)
@@ -54,7 +55,7 @@ program ref_simple_driver
   open(unit=21, file=trim(fin), access='stream', form='unformatted', status='old')
   read(21) npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode
   if (mode == 2) then
-    read(21) msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart
+    read(21) msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg
     read(21) rho, mu, urfu, urfp, mrrf, prrf, bleed
   end if
   allocate(fnn(nfaces), fnodes(nfaces, maxfn), owner(nfaces), neigh(nif), psize(npatches), ptype(npatches))
@@ -185,7 +186,14 @@ contains
     ASSolverContinuity = solvers(psolver)
     rrFContinuity = prrf
     ASIterContinuity = pitmax
-    LMultigridContinuity = .false.
+    LMultigridContinuity = plmg /= 0
+    ! the algebraic agglomeration (the geometric ones read node connectivity
+    ! the mesh reader builds and this driver does not install)
+    MGType = 'algebraic'
+    MGVariable = 'pressurecorrection'
+    LTestMultiGrid = .false.
+    LPrintMultiGridResiduals = .false.
+    nIterStartApplyingMG = 1
     HRFrameworkMomentum = frameworks(hrfw)
     ConvectionSchemeMomentum = schemes(hrscheme)
     BleedMomentum = bleed
@@ -236,6 +244,7 @@ contains
     call CalculateInitialmdot
     do it = 1, nouter
       nIter = it
+      nIterMG = it
       call CalculatePhysicalProperties
       call SolveMomentumX
       call SolveMomentumY
@@ -283,6 +292,13 @@ contains
     allocate(FluxCE(n), FluxVE(n), FluxTE(n), Sc(n), Sb(n))
     allocate(ResorAbs(18), ResorMax(18), ResorRMS(18), ResorScaled(18))
     allocate(eDiffCoefficient(n), BeDiffCoefficient(npatches, nb2))
+    ! the multigrid arrays exist before the first agglomeration, which
+    ! deallocates and sizes them (ResetAllocationForMultiGrid.for:4)
+    allocate(acMG(1), bcMG(1), anbMG(1), maxanb(1), Residuals(1), dphiMG(1))
+    allocate(NumberOfElementsMG(1), NElementFacesMG(1), NBChildrenMaxMG(1), ijbeginE(1), ijbeginN(1))
+    allocate(ijbeginNOld(1), ElementNeighborMG(1), NumberofElementNeighborsMG(1), Parents(1))
+    allocate(Children(1), NumberOfChildren(1), NeighborOfParent(1), NumberOfNeighborOfParent(1))
+    allocate(ElementParent(1))
     uVelocity = 0.; vVelocity = 0.; wVelocity = 0.
     BuVelocity = 0.; BvVelocity = 0.; BwVelocity = 0.
     uVelGradx = 0.; uVelGrady = 0.; uVelGradz = 0.; vVelGradx = 0.; vVelGrady = 0.; vVelGradz = 0.

@@ -276,6 +276,12 @@ __global__ void k_to(int n, const double *__restrict__ a, T *__restrict__ b) {
 // once when the Krylov loop has set its done flag, so the batches launched
 // after convergence cost only their launch.
 
+// The level kernels load their matrix as ordinary (cached) data: a level's
+// matrix is re-read by every sweep of a cycle and partly stays in the
+// Infinity Cache between them (non-temporal loads here took the 128^3 fine
+// sweep from 4.7 to 4.1 TB/s).
+constexpr bool kStreamLevel = false;
+
 // sum_j anb_ij x'_j over the slots of row i, spread over LPR consecutive
 // lanes and combined with shuffles; x' = x + alpha*xc[parent] when PRO
 template <int LPR, class T, bool PRO>
@@ -289,11 +295,11 @@ __device__ __forceinline__ T row_offdiag(int i, int lane, int n, int w, EllNbr n
     for (int j0 = 0; j0 < w; j0 += kEllBatch) {
       int k[kEllBatch];
       T a[kEllBatch], xv[kEllBatch];
-      ell_batch<kEllBatch>(nbr, i, n, w, j0, k);
+      ell_batch<kEllBatch, kStreamLevel>(nbr, i, n, w, j0, k);
 #pragma unroll
       for (int u = 0; u < kEllBatch; ++u) {
         const int kk = k[u] >= 0 ? k[u] : i;
-        a[u] = anb[(size_t)min(j0 + u, w - 1) * n + i];
+        a[u] = ld_mat<kStreamLevel>(&anb[(size_t)min(j0 + u, w - 1) * n + i]);
         xv[u] = x[kk];
         if (PRO) xv[u] = xv[u] + alpha * xc[parent[kk]];
       }
@@ -515,11 +521,11 @@ k_enter_resid(int n, int w, EllNbr nbr, const T *__restrict__ ac, const T *__res
   for (int j0 = 0; j0 < w; j0 += kEllBatch) {
     int k[kEllBatch];
     T a[kEllBatch], xv[kEllBatch];
-    ell_batch<kEllBatch>(nbr, i, n, w, j0, k);
+    ell_batch<kEllBatch, kStreamLevel>(nbr, i, n, w, j0, k);
 #pragma unroll
     for (int u = 0; u < kEllBatch; ++u) {
       const int kk = k[u] >= 0 ? k[u] : i;
-      a[u] = anb[(size_t)min(j0 + u, w - 1) * n + i];
+      a[u] = ld_mat<kStreamLevel>(&anb[(size_t)min(j0 + u, w - 1) * n + i]);
       xv[u] = omega * (T)rhs[kk] / ac[kk];
     }
 #pragma unroll

@@ -73,35 +73,43 @@ struct EllNbr {
   __host__ __device__ EllNbr(const int *p, const int *d = nullptr) : i32(p), d16(d) {}
 };
 
+// A load of matrix data (coefficients, neighbour indices) that a solve
+// streams once per use: with NT it is marked non-temporal, so that it does
+// not displace the gathered vectors from L2 and the Infinity Cache.
+template <bool NT, class T> __device__ __forceinline__ T ld_mat(const T *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // the neighbours of slots j0 .. j0+B-1 of row i (B even; slots past w
 // repeat the last slot), -1 for none
-template <int B>
+template <int B, bool NT = false>
 __device__ __forceinline__ void ell_batch(const EllNbr &nb, int i, int n, int w, int j0, int *k) {
   if (nb.d16) {
     const int wp = (w + 1) >> 1;
 #pragma unroll
     for (int u = 0; u < B / 2; ++u) {
-      const int v = nb.d16[(size_t)min((j0 >> 1) + u, wp - 1) * n + i];
+      const int v = ld_mat<NT>(&nb.d16[(size_t)min((j0 >> 1) + u, wp - 1) * n + i]);
       const int d0 = (short)(v & 0xffff), d1 = v >> 16;
       k[2 * u] = d0 ? i + d0 : -1;
       k[2 * u + 1] = d1 ? i + d1 : -1;
     }
   } else {
 #pragma unroll
-    for (int u = 0; u < B; ++u) k[u] = nb.i32[(size_t)min(j0 + u, w - 1) * n + i];
+    for (int u = 0; u < B; ++u) k[u] = ld_mat<NT>(&nb.i32[(size_t)min(j0 + u, w - 1) * n + i]);
   }
 }
 
-template <int B, class T, class Op>
+template <int B, bool NT = false, class T, class Op>
 __device__ __forceinline__ T ell_fold(int i, int n, int w, EllNbr nb, const T *__restrict__ anb, const T *x, T acc,
                                       Op op) {
   for (int j0 = 0; j0 < w; j0 += B) {
     int k[B];
     T a[B], xv[B];
-    ell_batch<B>(nb, i, n, w, j0, k);
+    ell_batch<B, NT>(nb, i, n, w, j0, k);
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      a[u] = anb[(size_t)min(j0 + u, w - 1) * n + i];
+      a[u] = ld_mat<NT>(&anb[(size_t)min(j0 + u, w - 1) * n + i]);
       xv[u] = x[k[u] >= 0 ? k[u] : i];
     }
 #pragma unroll

@@ -9,10 +9,13 @@ namespace fs {
 void set_error(const std::string &msg);
 
 namespace {
+// One 16-byte element per thread in address order: the streaming form that
+// reaches the highest bandwidth (a grid-stride loop over the same bytes runs
+// about a quarter slower).
 __global__ void __launch_bounds__(256) k_copy16(long long n16, const double2 *__restrict__ src,
                                                 double2 *__restrict__ dst) {
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n16) dst[i] = src[i];
 }
 } // namespace
 } // namespace fs
@@ -36,9 +39,11 @@ extern "C" int fs_time_device_copy(long long bytes, int reps, double *avg_ms) {
   else if ((e = hipStreamCreate(&s)) != hipSuccess) fail(e, "hipStreamCreate");
   else if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
     fail(e, "hipEventCreate");
+  const long long n16 = bytes / 16;
+  const long long nblk = (n16 + 255) / 256;
+  if (rc == 0 && nblk > 0x7fffffffLL) fail(hipErrorInvalidValue, "size");
   if (rc == 0) {
-    const long long n16 = bytes / 16;
-    const int grid = 8192;
+    const int grid = (int)nblk;
     (void)hipMemsetAsync(src, 0, bytes, s);
     hipLaunchKernelGGL(fs::k_copy16, dim3(grid), dim3(256), 0, s, n16, (const double2 *)src, (double2 *)dst);
     (void)hipEventRecord(e0, s);

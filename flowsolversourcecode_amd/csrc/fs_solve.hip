@@ -54,14 +54,18 @@ static inline int grid_red(int n) {
   return g < 1 ? 1 : g;
 }
 
+// The Krylov products stream the FP64 matrix (read twice per iteration, too
+// large to stay cached beside the vectors): non-temporal matrix loads.
+constexpr bool kStreamMatrix = true;
+
 // y = A*x in the order of dsprsax (PBCGMethod.for:324-329): diagonal first,
 // then the off-diagonals in slot order.
 __device__ __forceinline__ double ell_ax(int i, int n, int w, EllNbr nbr,
                                          const double *__restrict__ ac,
                                          const double *__restrict__ anb,
                                          const double *__restrict__ x) {
-  const double s = ac[i] * x[i];
-  return ell_fold<kEllBatch>(i, n, w, nbr, anb, x, s,
+  const double s = ld_mat<kStreamMatrix>(&ac[i]) * x[i];
+  return ell_fold<kEllBatch, kStreamMatrix>(i, n, w, nbr, anb, x, s,
                              [](double acc, double a, double xk) { return acc + a * xk; });
 }
 
@@ -72,8 +76,8 @@ __device__ __forceinline__ double ell_res(int i, int n, int w, EllNbr nbr,
                                           const double *__restrict__ anb,
                                           const double *__restrict__ b,
                                           const double *__restrict__ x) {
-  const double r = b[i] - ac[i] * x[i];
-  return ell_fold<kEllBatch>(i, n, w, nbr, anb, x, r,
+  const double r = b[i] - ld_mat<kStreamMatrix>(&ac[i]) * x[i];
+  return ell_fold<kEllBatch, kStreamMatrix>(i, n, w, nbr, anb, x, r,
                              [](double acc, double a, double xk) { return acc - a * xk; });
 }
 

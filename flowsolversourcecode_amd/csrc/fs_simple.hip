@@ -45,6 +45,10 @@ struct GradArgs { // up to 4 fields at once
   const double *fv[4];   // face values (interior)
 };
 
+// Face kernels stream the per-face geometry once per pass (non-temporal loads)
+// and gather per-cell values, which the hint leaves in the caches.
+constexpr bool kStreamFaces = true;
+
 // Green-Gauss face value phi_f = phi_C g + phi_F (1-g)
 // (StandardGreenGaussCorrection1Gradient.for:51-53)
 // Face values live over all F faces (field k at fp[k F + f]): boundary faces
@@ -82,18 +86,20 @@ __global__ void k_gg_facecorr(int Fi, int F, int N, const int *__restrict__ own,
     for (int k = 0; k < NF; ++k) fv[(size_t)k * F + f] = fp[(size_t)k * F + f];
     return;
   }
-  const int i = own[f], j = nei[f];
-  const double gf = g[f], og = 1. - gf;
+  const int i = ld_mat<kStreamFaces>(&own[f]), j = ld_mat<kStreamFaces>(&nei[f]);
+  const double gf = ld_mat<kStreamFaces>(&g[f]), og = 1. - gf;
+  const double rx = ld_mat<kStreamFaces>(&kx[f]), ry = ld_mat<kStreamFaces>(&ky[f]),
+               rz = ld_mat<kStreamFaces>(&kz[f]);
 #pragma unroll
   for (int k = 0; k < NF; ++k) {
     const double *gr = a.grad[k];
     double cx = gr[i] * gf + gr[j] * og;
-    cx = cx * kx[f];
+    cx = cx * rx;
     double cy = gr[N + i] * gf + gr[N + j] * og;
-    cy = cy * ky[f];
+    cy = cy * ry;
     double cz = gr[2 * N + i] * gf + gr[2 * N + j] * og;
-    cz = cz * kz[f];
-    fv[(size_t)k * F + f] = fp[(size_t)k * F + f] + cx + cy + cz;
+    cz = cz * rz;
+    fv[(size_t)k * F + f] = ld_mat<kStreamFaces>(&fp[(size_t)k * F + f]) + cx + cy + cz;
   }
 }
 
@@ -384,8 +390,8 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
                             double *__restrict__ tf3) {
   const int f = xcd_block() * kBlock + threadIdx.x;
   if (f >= Fi) return;
-  const int i = own[f], j = nei[f];
-  const double gf = g[f], og = 1. - gf;
+  const int i = ld_mat<kStreamFaces>(&own[f]), j = ld_mat<kStreamFaces>(&nei[f]);
+  const double gf = ld_mat<kStreamFaces>(&g[f]), og = 1. - gf;
   // cell gradients of component k in direction d at the owner and neighbour
   const double *G[3] = {gu, gv, gw};
   double gi[3][3], gj[3][3], fgv[3][3];
@@ -401,8 +407,9 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
 #pragma unroll
     for (int d = 0; d < 3; ++d) fgv[k][d] = gf * gi[k][d] + og * gj[k][d];
   const double gamf = gf * mu + (1. - gf) * mu;
-  const double cfl = gamf * gdiff[f], ffl = -(gamf * gdiff[f]);
-  const double md = mdot[f];
+  const double gd = ld_mat<kStreamFaces>(&gdiff[f]);
+  const double cfl = gamf * gd, ffl = -(gamf * gd);
+  const double md = ld_mat<kStreamFaces>(&mdot[f]);
   const double ccl = md > 0. ? md : 0.;
   const double fcl = -(-md > 0. ? -md : 0.);
   // cf, ff: diffusion, explicit (+0), bulk (+0), convection
@@ -415,7 +422,9 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
   Fv = Fv + fcl;
   cf[f] = C;
   ff[f] = Fv;
-  const double tx = Tx[f], ty = Ty[f], tz = Tz[f], sx = Sx[f], sy = Sy[f], sz = Sz[f];
+  const double tx = ld_mat<kStreamFaces>(&Tx[f]), ty = ld_mat<kStreamFaces>(&Ty[f]),
+               tz = ld_mat<kStreamFaces>(&Tz[f]), sx = ld_mat<kStreamFaces>(&Sx[f]),
+               sy = ld_mat<kStreamFaces>(&Sy[f]), sz = ld_mat<kStreamFaces>(&Sz[f]);
   const double dv = fgv[0][0] + fgv[1][1] + fgv[2][2];
   const double *P[3] = {u, v, w};
   // the upwind side: the owner for mdot > 0, else the neighbour

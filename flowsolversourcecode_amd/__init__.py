@@ -305,6 +305,7 @@ def _setup_case_symbols(L):
     L.fs_case_set_patch_types.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
     L.fs_case_set_convection.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]
     L.fs_case_set_gradient_limiters.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.fs_case_set_algorithm.argtypes = [C.c_void_p, C.c_int]
     L.fs_case_time_spmv.argtypes = [C.c_void_p, C.c_int, _dp]
     L._case_ready = True
 
@@ -354,7 +355,7 @@ class Case:
     walls and symmetry planes."""
 
     SIZES = {"u": "N", "v": "N", "w": "N", "p": "N", "pc": "N", "mdot": "Fi", "gu": "3N", "gv": "3N",
-             "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bgp": "3Fb", "du": "N", "dv": "N", "dw": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
+             "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bgp": "3Fb", "du": "N", "dv": "N", "dw": "N", "du2": "N", "dv2": "N", "dw2": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
              "anb": "NW", "bc": "N", "cf": "F", "ff": "F", "tf": "F", "vol": "N", "Sx": "F", "Sy": "F",
              "Sz": "F", "g": "F", "gdiff": "F", "Tx": "F", "Ty": "F", "Tz": "F", "dcf": "F", "A": "F",
              "kx": "Fi", "ky": "Fi", "kz": "Fi"}
@@ -446,6 +447,12 @@ class Case:
         (e.g. "sou", "minmod"), BleedMomentum and nIterStartApplyingHR."""
         _check(lib().fs_case_set_convection(self.h, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], bleed,
                                             start_iter), "fs_case_set_convection")
+
+    def set_algorithm(self, algorithm):
+        """Algorithm: "simple" (default) or "simplec"."""
+        _check(lib().fs_case_set_algorithm(self.h, {"simple": 0, "simplec": 1}[algorithm]),
+               "fs_case_set_algorithm")
+        self._simplec = algorithm == "simplec"
 
     def set_gradient_limiters(self, momentum=0, continuity=0):
         """LimitGradientMomentumMethod / LimitGradientContinuityMethod: 0 off,

@@ -551,7 +551,8 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
                            const double *__restrict__ phi, const double *__restrict__ gpc,
                            const double *__restrict__ vol, double urf, const int *__restrict__ perm,
                            double *__restrict__ ac, double *__restrict__ anb, double *__restrict__ bc,
-                           double *__restrict__ D, double *__restrict__ star, double *__restrict__ xq) {
+                           double *__restrict__ D, double *__restrict__ star, double *__restrict__ xq,
+                           double *__restrict__ D2) {
   const int c = xcd_block() * kBlock + threadIdx.x;
   if (c >= N) return;
   // the system row of cell c: its colour-major position when perm is given
@@ -595,6 +596,11 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
   D[c] = vol[c] / a;
   star[c] = pc;
   if (perm) xq[q] = pc;
+  if (D2) { // SIMPLEC: V / (a_P + the row's anb in slot order)
+    double t = a;
+    for (int s = 0; s < W; ++s) t = t + anb[(size_t)s * N + q];
+    D2[c] = vol[c] / t;
+  }
 }
 
 // y[c] = x[perm[c]]: a colour-major vector back in cell order
@@ -607,13 +613,27 @@ __global__ void k_unpermute(int N, const int *__restrict__ perm, const double *_
 struct RcArgs {
   double rho, urf_u;
   const double *u, *v, *w, *us, *vs, *ws, *du, *dv, *dw, *gp, *p;
+  const double *du2, *dv2, *dw2; // SIMPLEC's D2 (the p' coefficients)
 };
+
+// geoDiff = |D_E| / d_CF of the face's D S split (MethodDecomposeS = 3,
+// AssembleMdotUsingRhieChow.for:575-615)
+__device__ __forceinline__ double geo_of(double dusf, double dvsf, double dwsf, double ex, double ey, double ez,
+                                         double dcf) {
+  const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
+  const double eu = dusf / mag, ev = dvsf / mag, ew = dwsf / mag;
+  const double dot = ex * eu + ey * ev + ez * ew;
+  const double due = mag * ex / dot, dve = mag * ey / dot, dwe = mag * ez / dot;
+  return sqrt(due * due + dve * dve + dwe * dwe) / dcf;
+}
 
 // Rhie-Chow face mass flux (AssembleMdotUsingRhieChow.for:554-759, steady,
 // LRelaxMomentum) followed by the pressure-correction face coefficients
 // (AssemblePressureCorrectionFluxes.for:85-153).  The pressure face gradient,
 // 'averagecorrected' (InterpolateGradientFromElementToFace.for:16-93), is
 // completed here from the averaged cell gradient the flux already forms.
+// SIMPLEC: the p' coefficients take geoDiff of D2 instead of D1.
+template <bool SIMPLEC>
 __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
                             const int *__restrict__ nei, const double *__restrict__ g,
                             const double *__restrict__ Sx, const double *__restrict__ Sy,
@@ -654,8 +674,16 @@ __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
   fv = fv + (1. - a.urf_u) * (mdot[f] - mstar);
   const double m = cfl * a.p[i] + ffl * a.p[j] + fv;
   mdot[f] = m;
-  cf[f] = cfl;
-  ff[f] = ffl;
+  if constexpr (SIMPLEC) {
+    const double d2u = gf * a.du2[i] + og * a.du2[j], d2v = gf * a.dv2[i] + og * a.dv2[j],
+                 d2w = gf * a.dw2[i] + og * a.dw2[j];
+    const double geo2 = geo_of(d2u * sx, d2v * sy, d2w * sz, ex[f], ey[f], ez[f], dcf[f]);
+    cf[f] = rhof * geo2;
+    ff[f] = -(rhof * geo2);
+  } else {
+    cf[f] = cfl;
+    ff[f] = ffl;
+  }
   tf[f] = m;
 }
 
@@ -805,6 +833,8 @@ struct SimpleCase {
   double bleed = 0.0;
   DevBuf<double> fcx, fcy, fcz, xc, yc, zc; // centroids, uploaded with the first limiter
   DevBuf<double> lim_part, lim_eps;
+  bool simplec = false;            // Algorithm 'simplec': D2 in du2/dv2/dw2
+  DevBuf<double> du2, dv2, dw2;
   // momentum systems in colour-major cell order (the colour classes of the
   // multicolour Gauss-Seidel contiguous): their solver context on the
   // renumbered mesh (sharing the stream), the renumbering, the permuted
@@ -946,7 +976,8 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
   by_width(C.W, [&](auto wc) {
     hipLaunchKernelGGL(k_mom_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
                        C.code.p, C.cf.p, C.ff.p, tf, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
-                       C.urf_u, C.mom_mesh ? C.mperm.p : nullptr, C.ac.p, C.anb.p, C.bc.p, D, st, C.mx.p);
+                       C.urf_u, C.mom_mesh ? C.mperm.p : nullptr, C.ac.p, C.anb.p, C.bc.p, D, st, C.mx.p,
+                       C.simplec ? (comp == 0 ? C.du2.p : comp == 1 ? C.dv2.p : C.dw2.p) : nullptr);
   });
 }
 
@@ -1010,9 +1041,14 @@ static void phase_pc_assemble(SimpleCase &C) {
   a.du = C.du.p; a.dv = C.dv.p; a.dw = C.dw.p;
   a.gp = C.gp.p;
   a.p = C.p.p;
-  hipLaunchKernelGGL(k_rhie_chow, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p,
-                     C.nei.p, C.g.p, C.Sx.p, C.Sy.p, C.Sz.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, a,
-                     C.mdot.p, C.cf.p, C.ff.p, C.tf.p);
+  a.du2 = C.du2.p; a.dv2 = C.dv2.p; a.dw2 = C.dw2.p;
+  auto rc = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p, C.nei.p, C.g.p,
+                       C.Sx.p, C.Sy.p, C.Sz.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, a, C.mdot.p, C.cf.p, C.ff.p,
+                       C.tf.p);
+  };
+  if (C.simplec) rc(k_rhie_chow<true>);
+  else rc(k_rhie_chow<false>);
   by_width(C.W, [&](auto wc) {
     hipLaunchKernelGGL(k_pc_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
                        C.code.p, C.cf.p, C.ff.p, C.tf.p, C.ac.p, C.anb.p, C.bc.p);
@@ -1062,8 +1098,10 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
   gradients<1>(C, a);
   const int methods[1] = {C.lim_cont};
   limit_gradients<1>(C, a, methods);
-  hipLaunchKernelGGL(k_correct_u, dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.gpc.p, C.du.p, C.dv.p,
-                     C.dw.p, C.u.p, C.v.p, C.w.p);
+  // CorrectVelocity with D2 (= D1 for SIMPLE)
+  hipLaunchKernelGGL(k_correct_u, dim3(blocks(N)), dim3(kBlock), 0, C.s, N, C.gpc.p,
+                     C.simplec ? C.du2.p : C.du.p, C.simplec ? C.dv2.p : C.dv.p, C.simplec ? C.dw2.p : C.dw.p,
+                     C.u.p, C.v.p, C.w.p);
   hipLaunchKernelGGL(k_correct_mdot, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.own.p, C.nei.p,
                      C.cf.p, C.ff.p, C.pc.p, C.mdot.p);
   hipLaunchKernelGGL(k_wall_p, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.own.p, C.dx.p,
@@ -1346,7 +1384,8 @@ static bool case_field(fs::SimpleCase &C, const char *name, double **p, size_t *
       {"gw", {&C.gw, 3 * N}}, {"gp", {&C.gp, 3 * N}}, {"gpc", {&C.gpc, 3 * N}},
       {"bu", {&C.bu, Fb}}, {"bv", {&C.bv, Fb}}, {"bw", {&C.bw, Fb}}, {"bp", {&C.bp, Fb}},
       {"bgp", {&C.bgp, 3 * Fb}}, {"du", {&C.du, N}}, {"dv", {&C.dv, N}},
-      {"dw", {&C.dw, N}}, {"us", {&C.us, N}}, {"vs", {&C.vs, N}}, {"ws", {&C.ws, N}},
+      {"dw", {&C.dw, N}}, {"du2", {&C.du2, C.simplec ? N : 0}}, {"dv2", {&C.dv2, C.simplec ? N : 0}},
+      {"dw2", {&C.dw2, C.simplec ? N : 0}}, {"us", {&C.us, N}}, {"vs", {&C.vs, N}}, {"ws", {&C.ws, N}},
       {"ac", {&C.ac, N}}, {"anb", {&C.anb, N * C.W}}, {"bc", {&C.bc, N}}, {"cf", {&C.cf, F}},
       {"ff", {&C.ff, F}}, {"tf", {&C.tf, F}}, {"vol", {&C.vol, N}}, {"Sx", {&C.Sx, F}},
       {"Sy", {&C.Sy, F}}, {"Sz", {&C.Sz, F}}, {"g", {&C.g, F}}, {"gdiff", {&C.gdiff, F}},
@@ -1424,6 +1463,25 @@ int fs_case_set_convection(fs_case *cs, int framework, int scheme, double bleed,
   C.bleed = bleed;
   C.hr_start = start_iter;
   return 0;
+}
+
+int fs_case_set_algorithm(fs_case *cs, int algorithm) {
+  if (!cs || (algorithm != FS_ALGORITHM_SIMPLE && algorithm != FS_ALGORITHM_SIMPLEC)) {
+    set_error("fs_case_set_algorithm: algorithm is FS_ALGORITHM_SIMPLE or FS_ALGORITHM_SIMPLEC");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  C.simplec = algorithm == FS_ALGORITHM_SIMPLEC;
+  if (C.simplec && !C.du2.p) {
+    for (auto *b : {&C.du2, &C.dv2, &C.dw2}) {
+      b->alloc(C.N);
+      FS_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * C.N, C.s));
+    }
+    FS_HIP(hipStreamSynchronize(C.s));
+  }
+  return 0;
+  CASE_GUARD_END
 }
 
 int fs_case_set_gradient_limiters(fs_case *cs, int momentum_method, int continuity_method) {

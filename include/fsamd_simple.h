@@ -28,8 +28,10 @@
  *               AssemblePressureCorrectionFluxes.for:4, CorrectPressure.for:4,
  *               CorrectMdotandVelocity.for:4,722)
  * with the reference's settings MethodCalcGradient = 2, nIterGradient = 2,
- * MethodDecomposeS = 3, Algorithm 'simple', LRelaxMomentum, upwind
- * convection (User0.F90).
+ * MethodDecomposeS = 3, LRelaxMomentum (User0.F90), and by default
+ * Algorithm 'simple' and upwind convection; fs_case_set_algorithm,
+ * fs_case_set_convection and fs_case_set_gradient_limiters select SIMPLEC,
+ * the TVD/NVF high-resolution schemes and the gradient limiters.
  *
  * The mesh is a polyMesh in memory (0-based): face_ptr/face_nodes list each
  * face's nodes, faces 0..nif-1 are interior (owner < neighbour), the rest are
@@ -126,6 +128,13 @@ int fs_case_set_convection(fs_case *c, int framework, int scheme, double bleed, 
  * (:263-437).  The limited gradient replaces the Green-Gauss one before the
  * boundary corrections, and the boundary gradients become the owners'. */
 int fs_case_set_gradient_limiters(fs_case *c, int momentum_method, int continuity_method);
+/* Algorithm (User0.F90:33): SIMPLE (the default) or SIMPLEC, whose p'
+ * coefficients and velocity correction take D2 = V/(a_P + sum anb) while the
+ * Rhie-Chow flux keeps D1 = V/a_P (StoreTheDValues.for:20-39,
+ * AssemblePressureCorrectionFluxes.for:81,
+ * CorrectMdotandVelocity.for:774-778).  Fields "du2" "dv2" "dw2" hold D2. */
+enum fs_algorithm { FS_ALGORITHM_SIMPLE = 0, FS_ALGORITHM_SIMPLEC = 1 };
+int fs_case_set_algorithm(fs_case *c, int algorithm);
 /* urfMomentum, urfPressure, ASSolverMomentum/rrFMomentum/ASIterMomentum/
  * LMultigridMomentum and the Continuity equivalents (User0.F90:271-318). */
 int fs_case_configure(fs_case *c, double urf_u, double urf_p, int msolver,

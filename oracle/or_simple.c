@@ -130,6 +130,9 @@ void or_case_update_symmetry(or_case *c) {
 }
 
 void or_case_delete(or_case *c) {
+  free(c->du2);
+  free(c->dv2);
+  free(c->dw2);
   free(c->btype);
   free(c->bzq);
   double *arrs[] = {c->bu0, c->bv0, c->bw0, c->u, c->v, c->w, c->p, c->pc, c->mdot, c->gu,
@@ -667,6 +670,28 @@ void or_case_momentum_assemble(or_case *c, int comp) {
     d[i] = m->vol[i] / c->ac[i];
     st[i] = phi[i];
   }
+  if (c->du2) { /* SIMPLEC: V/(a_P + sum over the slots of anb) */
+    double *d2 = comp == 0 ? c->du2 : comp == 1 ? c->dv2 : c->dw2;
+    for (int i = 0; i < N; ++i) {
+      double a = c->ac[i];
+      for (int s = 0; s < m->nef[i]; ++s) a = a + c->anb[(size_t)s * N + i];
+      d2[i] = m->vol[i] / a;
+    }
+  }
+}
+
+int or_case_set_algorithm(or_case *c, int simplec) {
+  if (simplec != 0 && simplec != 1) return -1;
+  free(c->du2);
+  free(c->dv2);
+  free(c->dw2);
+  c->du2 = c->dv2 = c->dw2 = NULL;
+  if (simplec) {
+    c->du2 = dz(c->N);
+    c->dv2 = dz(c->N);
+    c->dw2 = dz(c->N);
+  }
+  return 0;
 }
 
 void or_case_momentum_solve(or_case *c, int comp) {
@@ -733,8 +758,9 @@ void or_case_pc_assemble(or_case *c) {
   for (int k = 0; k < Fi; ++k) {
     const int i = m->owner[k], j = m->neigh[k];
     const double gf = m->g[k], og = 1. - gf;
-    const double geo = geo_diff(m, k, gf * c->du[i] + og * c->du[j], gf * c->dv[i] + og * c->dv[j],
-                                gf * c->dw[i] + og * c->dw[j]);
+    const double *d2u = c->du2 ? c->du2 : c->du, *d2v = c->du2 ? c->dv2 : c->dv, *d2w = c->du2 ? c->dw2 : c->dw;
+    const double geo = geo_diff(m, k, gf * d2u[i] + og * d2u[j], gf * d2v[i] + og * d2v[j],
+                                gf * d2w[i] + og * d2w[j]);
     c->fcf[k] = rhof * geo;
     c->fff[k] = -(rhof * geo);
     c->ftf[k] = c->mdot[k];
@@ -763,9 +789,11 @@ void or_case_pc_solve_correct(or_case *c) {
   or_gradient(c, c->pc, c->bpc, c->gpc, c->bgpc);
   if (c->lim_cont) or_limit_gradient(c, c->lim_cont, c->pc, c->bpc, c->gpc, c->bgpc);
   for (int i = 0; i < N; ++i) {
-    c->u[i] = c->u[i] - c->du[i] * c->gpc[i];
-    c->v[i] = c->v[i] - c->dv[i] * c->gpc[N + i];
-    c->w[i] = c->w[i] - c->dw[i] * c->gpc[2 * N + i];
+    const double d2u = c->du2 ? c->du2[i] : c->du[i], d2v = c->du2 ? c->dv2[i] : c->dv[i],
+                 d2w = c->du2 ? c->dw2[i] : c->dw[i];
+    c->u[i] = c->u[i] - d2u * c->gpc[i];
+    c->v[i] = c->v[i] - d2v * c->gpc[N + i];
+    c->w[i] = c->w[i] - d2w * c->gpc[2 * N + i];
   }
   for (int k = 0; k < Fi; ++k) {
     const int i = m->owner[k], j = m->neigh[k];

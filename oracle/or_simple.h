@@ -73,6 +73,10 @@ typedef struct {
    * number of the outer iteration in progress */
   int hr_fw, hr_scheme, hr_start, lim_mom, lim_cont, iter;
   double bleed;
+  /* Algorithm 'simplec' (StoreTheDValues.for:30-39): D2 = V/(a_P + sum anb)
+   * in the p' coefficients and the velocity correction; NULL for 'simple'
+   * (D2 = D1) */
+  double *du2, *dv2, *dw2;
 } or_case;
 
 enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 1 };
@@ -97,6 +101,8 @@ void or_case_update_symmetry(or_case *c);
  * single-phase case cannot use) and for an unknown framework. */
 int or_case_set_convection(or_case *c, int framework, int scheme, double bleed, int start_iter);
 int or_case_set_gradient_limiters(or_case *c, int momentum_method, int continuity_method);
+/* 0 'simple', 1 'simplec' */
+int or_case_set_algorithm(or_case *c, int simplec);
 double or_tvd_psi(int scheme, double r);
 double or_nvf_phif(int scheme, double phic_tilde);
 void or_limit_gradient(or_case *c, int method, const double *phi, const double *bphi, double *gr, double *bgr);

@@ -38,11 +38,13 @@ SIMPLE = {
     "hex_nvf_late": (("box_hex", (6, 5, 4), 0.15, 3),
                      dict(niter=8, framework="nvf", scheme="smart", bleed=0.2, start_iter=3, lim_mom=2)),
     "tet_sor": (("box_tet", (3, 3, 2), 0.15, 4), dict(niter=8)),
+    "hex_simplec": (("box_hex", (6, 5, 4), 0.15, 3), dict(niter=8, algorithm="simplec")),
     # converged states (the device is compared with these)
     "hex_converged": (("box_hex", (8, 7, 6), 0.15, 3), dict(niter=400)),
     "tet_converged": (("box_tet", (4, 4, 3), 0.15, 4), dict(niter=400)),
     "hex_sou_converged": (("box_hex", (8, 7, 6), 0.15, 3),
                           dict(niter=400, framework="tvd", scheme="sou", lim_mom=1, lim_cont=1)),
+    "tet_simplec_converged": (("box_tet", (4, 4, 3), 0.15, 4), dict(niter=400, algorithm="simplec", urf_p=1.0)),
     # the two-dimensional cavity of the reference's test cases at Re = 100,
     # run to its converged state
     "cavity2d_64": (("cavity_2d", 64), dict(niter=600, urf_u=0.9, urf_p=0.1, mrrf=0.1, mitmax=10, prrf=0.01,

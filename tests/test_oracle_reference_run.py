@@ -9,7 +9,7 @@ with the mesh, the storage and the case installed by
 oracle/ref_simple_driver.f90.  The restatement reproduces them bitwise:
 geometry of perturbed hexahedra and tetrahedra, whole outer iterations with
 SOR, ILU and PBCG solves, second-order upwind with both gradient limiters,
-NVF convection with bleed and a late start, tetrahedra, and the converged
+NVF convection with bleed and a late start, SIMPLEC, tetrahedra, and the converged
 64 x 64 x 1 cavity with symmetry planes after 600 iterations.
 """
 import numpy as np
@@ -56,6 +56,7 @@ def _oracle_case(name):
     if kw.get("framework", "none") != "none":
         oc.set_convection(kw["framework"], kw["scheme"], kw.get("bleed", 0.0), kw.get("start_iter", 0))
     oc.set_gradient_limiters(kw.get("lim_mom", 0), kw.get("lim_cont", 0))
+    oc.set_algorithm(kw.get("algorithm", "simple"))
     return oc, niter
 
 

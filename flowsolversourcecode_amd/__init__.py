@@ -306,6 +306,7 @@ def _setup_case_symbols(L):
     L.fs_case_set_convection.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]
     L.fs_case_set_gradient_limiters.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.fs_case_set_algorithm.argtypes = [C.c_void_p, C.c_int]
+    L.fs_case_set_gradient_method.argtypes = [C.c_void_p] + [C.c_int] * 4
     L.fs_case_time_spmv.argtypes = [C.c_void_p, C.c_int, _dp]
     L._case_ready = True
 
@@ -447,6 +448,10 @@ class Case:
         (e.g. "sou", "minmod"), BleedMomentum and nIterStartApplyingHR."""
         _check(lib().fs_case_set_convection(self.h, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], bleed,
                                             start_iter), "fs_case_set_convection")
+
+    def set_gradient_method(self, momentum=(2, 2), continuity=(2, 2)):
+        """(MethodCalcGradient, nIterGradient) of u, v, w and of p, p'."""
+        _check(lib().fs_case_set_gradient_method(self.h, *momentum, *continuity), "fs_case_set_gradient_method")
 
     def set_algorithm(self, algorithm):
         """Algorithm: "simple" (default) or "simplec"."""

@@ -833,6 +833,7 @@ struct SimpleCase {
   double bleed = 0.0;
   DevBuf<double> fcx, fcy, fcz, xc, yc, zc; // centroids, uploaded with the first limiter
   DevBuf<double> lim_part, lim_eps;
+  int ngrad_mom = 2, ngrad_cont = 2; // correction sweeps (nIterGradient) of u,v,w and of p, p'
   bool simplec = false;            // Algorithm 'simplec': D2 in du2/dv2/dw2
   DevBuf<double> du2, dv2, dw2;
   // momentum systems in colour-major cell order (the colour classes of the
@@ -881,7 +882,7 @@ template <class Fn> static void by_width(int W, Fn &&fn) {
   }
 }
 
-template <int NF> static void gradients(SimpleCase &C, GradArgs a) {
+template <int NF> static void gradients(SimpleCase &C, GradArgs a, int ncorr) {
   const int Fi = C.Fi, F = C.F, N = C.N;
   hipLaunchKernelGGL(k_gg_face0<NF>, dim3(blocks(F)), dim3(kBlock), 0, C.s, Fi, F, C.own.p, C.nei.p,
                      C.g.p, a, C.fp.p);
@@ -892,7 +893,7 @@ template <int NF> static void gradients(SimpleCase &C, GradArgs a) {
     });
   };
   cell(C.fp.p);
-  for (int it = 0; it < 2; ++it) {
+  for (int it = 0; it < ncorr; ++it) {
     hipLaunchKernelGGL(k_gg_facecorr<NF>, dim3(blocks(F)), dim3(kBlock), 0, C.s, Fi, F, N, C.own.p,
                        C.nei.p, C.g.p, C.kx.p, C.ky.p, C.kz.p, C.fp.p, a, C.fv.p);
     cell(C.fv.p);
@@ -930,11 +931,23 @@ static void phase_momentum_gradients(SimpleCase &C) {
   a.phi[0] = C.u.p; a.phi[1] = C.v.p; a.phi[2] = C.w.p; a.phi[3] = C.p.p;
   a.bval[0] = C.bu.p; a.bval[1] = C.bv.p; a.bval[2] = C.bw.p; a.bval[3] = C.bp.p;
   a.grad[0] = C.gu.p; a.grad[1] = C.gv.p; a.grad[2] = C.gw.p; a.grad[3] = C.gp.p;
-  gradients<4>(C, a);
-  // LimitGradientVariable before the boundary corrections
-  // (CalculateGradients.for:252-272)
-  const int methods[4] = {C.lim_mom, C.lim_mom, C.lim_mom, C.lim_cont};
-  limit_gradients<4>(C, a, methods);
+  // LimitGradientVariable after each gradient, before the boundary
+  // corrections (CalculateGradients.for:252-272); u, v, w and p in one pass
+  // when they take the same number of correction sweeps
+  if (C.ngrad_mom == C.ngrad_cont) {
+    gradients<4>(C, a, C.ngrad_mom);
+    const int methods[4] = {C.lim_mom, C.lim_mom, C.lim_mom, C.lim_cont};
+    limit_gradients<4>(C, a, methods);
+  } else {
+    gradients<3>(C, a, C.ngrad_mom);
+    const int mm[3] = {C.lim_mom, C.lim_mom, C.lim_mom};
+    limit_gradients<3>(C, a, mm);
+    GradArgs b{};
+    b.phi[0] = C.p.p; b.bval[0] = C.bp.p; b.grad[0] = C.gp.p;
+    gradients<1>(C, b, C.ngrad_cont);
+    const int mc[1] = {C.lim_cont};
+    limit_gradients<1>(C, b, mc);
+  }
   hipLaunchKernelGGL(k_wall_pgrad, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N,
                      C.own.p, C.nx.p, C.ny.p, C.nz.p, C.btype.p, C.bzq.p, C.gp.p, C.bgp.p);
   // the interior momentum face fluxes of all three components, with the
@@ -1095,7 +1108,7 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
   a.phi[0] = C.pc.p;
   a.bval[0] = nullptr; // walls: BPressureC = PressureC(owner)
   a.grad[0] = C.gpc.p;
-  gradients<1>(C, a);
+  gradients<1>(C, a, C.ngrad_cont);
   const int methods[1] = {C.lim_cont};
   limit_gradients<1>(C, a, methods);
   // CorrectVelocity with D2 (= D1 for SIMPLE)
@@ -1462,6 +1475,19 @@ int fs_case_set_convection(fs_case *cs, int framework, int scheme, double bleed,
   C.hr_scheme = scheme;
   C.bleed = bleed;
   C.hr_start = start_iter;
+  return 0;
+}
+
+int fs_case_set_gradient_method(fs_case *cs, int momentum_method, int momentum_iters, int continuity_method,
+                                int continuity_iters) {
+  if (!cs || momentum_method < 1 || momentum_method > 2 || continuity_method < 1 || continuity_method > 2 ||
+      momentum_iters < 0 || continuity_iters < 0) {
+    set_error("fs_case_set_gradient_method: methods are 1 or 2, iterations >= 0");
+    return -1;
+  }
+  auto &C = cs->c;
+  C.ngrad_mom = momentum_method == 1 ? 0 : momentum_iters;
+  C.ngrad_cont = continuity_method == 1 ? 0 : continuity_iters;
   return 0;
 }
 

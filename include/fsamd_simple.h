@@ -134,6 +134,14 @@ int fs_case_set_gradient_limiters(fs_case *c, int momentum_method, int continuit
  * AssemblePressureCorrectionFluxes.for:81,
  * CorrectMdotandVelocity.for:774-778).  Fields "du2" "dv2" "dw2" hold D2. */
 enum fs_algorithm { FS_ALGORITHM_SIMPLE = 0, FS_ALGORITHM_SIMPLEC = 1 };
+/* MethodCalcGradientMomentum/Continuity and nIterGradientMomentum/Continuity
+ * (User0.F90:98-115; CalculateGradients.for:212-235): method 1 is plain
+ * Green-Gauss (StandardGreenGaussGradient.for:3), method 2 adds iters
+ * face-centroid correction sweeps (StandardGreenGaussCorrection1Gradient.for
+ * :3); the momentum pair applies to u, v, w, the continuity pair to p and
+ * p'.  A case starts with 2, 2 for both.  Methods 3-6 are not provided. */
+int fs_case_set_gradient_method(fs_case *c, int momentum_method, int momentum_iters, int continuity_method,
+                                int continuity_iters);
 int fs_case_set_algorithm(fs_case *c, int algorithm);
 /* urfMomentum, urfPressure, ASSolverMomentum/rrFMomentum/ASIterMomentum/
  * LMultigridMomentum and the Continuity equivalents (User0.F90:271-318). */

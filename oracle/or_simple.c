@@ -60,6 +60,7 @@ or_case *or_case_new(or_mesh *m, const double *wall_u, const double *wall_v,
   c->prrf = prrf;
   c->pitmax = pitmax;
   c->ngrad = 2;
+  c->ngrad_cont = 2;
   c->refcell = 0; /* ReferencePressureLocation = 1 (User0.F90:50) */
   c->u = dz(N); c->v = dz(N); c->w = dz(N); c->p = dz(N); c->pc = dz(N);
   c->mdot = dz(Fi);
@@ -148,7 +149,15 @@ void or_case_delete(or_case *c) {
  * :35-186): Green-Gauss with the face value re-evaluated at the face
  * centroid, nIterGradient correction sweeps, boundary copies of the owner's
  * gradient.  gr = [gx | gy | gz] (3N), bgr = [bgx | bgy | bgz] (3Fb). */
+static void gradient_n(or_case *c, const double *phi, const double *bphi, double *gr, double *bgr, int ngrad);
+
 void or_gradient(or_case *c, const double *phi, const double *bphi, double *gr, double *bgr) {
+  gradient_n(c, phi, bphi, gr, bgr, c->ngrad);
+}
+
+/* MethodCalcGradient 1 is the first pass alone (StandardGreenGaussGradient.for
+ * :35-96 forms the same sums): ngrad = 0 */
+static void gradient_n(or_case *c, const double *phi, const double *bphi, double *gr, double *bgr, int ngrad) {
   const or_mesh *m = c->m;
   const int N = c->N, Fi = c->Fi, Fb = c->Fb;
   double *gx = gr, *gy = gr + N, *gz = gr + 2 * N;
@@ -169,7 +178,7 @@ void or_gradient(or_case *c, const double *phi, const double *bphi, double *gr, 
   for (int i = 0; i < N; ++i) {
     gx[i] = gx[i] / m->vol[i]; gy[i] = gy[i] / m->vol[i]; gz[i] = gz[i] / m->vol[i];
   }
-  for (int it = 0; it < c->ngrad; ++it) {
+  for (int it = 0; it < ngrad; ++it) {
     memset(fx, 0, sizeof(double) * N);
     memset(fy, 0, sizeof(double) * N);
     memset(fz, 0, sizeof(double) * N);
@@ -450,7 +459,7 @@ void or_case_momentum_gradients(or_case *c) {
   or_gradient(c, c->u, c->bu, c->gu, c->bgu);
   or_gradient(c, c->v, c->bv, c->gv, c->bgv);
   or_gradient(c, c->w, c->bw, c->gw, c->bgw);
-  or_gradient(c, c->p, c->bp, c->gp, c->bgp);
+  gradient_n(c, c->p, c->bp, c->gp, c->bgp, c->ngrad_cont);
   /* LimitGradientVariable after the Green-Gauss sweeps, before the boundary
    * corrections (CalculateGradients.for:252-272) */
   if (c->lim_mom) {
@@ -680,6 +689,14 @@ void or_case_momentum_assemble(or_case *c, int comp) {
   }
 }
 
+int or_case_set_gradient_method(or_case *c, int mom_method, int mom_iters, int cont_method, int cont_iters) {
+  if (mom_method < 1 || mom_method > 2 || cont_method < 1 || cont_method > 2 || mom_iters < 0 || cont_iters < 0)
+    return -1;
+  c->ngrad = mom_method == 1 ? 0 : mom_iters;
+  c->ngrad_cont = cont_method == 1 ? 0 : cont_iters;
+  return 0;
+}
+
 int or_case_set_algorithm(or_case *c, int simplec) {
   if (simplec != 0 && simplec != 1) return -1;
   free(c->du2);
@@ -786,7 +803,7 @@ void or_case_pc_solve_correct(or_case *c) {
   for (int b = 0; b < Fb; ++b) c->bpc[b] = c->pc[m->owner[Fi + b]];
   const double rpc = c->refcell >= 0 ? c->pc[c->refcell] : 0.0;
   for (int i = 0; i < N; ++i) c->p[i] = c->p[i] + c->urf_p * (c->pc[i] - rpc);
-  or_gradient(c, c->pc, c->bpc, c->gpc, c->bgpc);
+  gradient_n(c, c->pc, c->bpc, c->gpc, c->bgpc, c->ngrad_cont);
   if (c->lim_cont) or_limit_gradient(c, c->lim_cont, c->pc, c->bpc, c->gpc, c->bgpc);
   for (int i = 0; i < N; ++i) {
     const double d2u = c->du2 ? c->du2[i] : c->du[i], d2v = c->du2 ? c->dv2[i] : c->dv[i],

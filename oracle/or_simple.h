@@ -77,6 +77,9 @@ typedef struct {
    * in the p' coefficients and the velocity correction; NULL for 'simple'
    * (D2 = D1) */
   double *du2, *dv2, *dw2;
+  /* Green-Gauss correction sweeps of p and p' (nIterGradientContinuity;
+   * ngrad above is nIterGradientMomentum); 0 = MethodCalcGradient 1 */
+  int ngrad_cont;
 } or_case;
 
 enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 1 };
@@ -101,6 +104,9 @@ void or_case_update_symmetry(or_case *c);
  * single-phase case cannot use) and for an unknown framework. */
 int or_case_set_convection(or_case *c, int framework, int scheme, double bleed, int start_iter);
 int or_case_set_gradient_limiters(or_case *c, int momentum_method, int continuity_method);
+/* MethodCalcGradient 1 (plain Green-Gauss) or 2 (with nIterGradient
+ * face-centroid corrections) for the velocities and for p and p' */
+int or_case_set_gradient_method(or_case *c, int mom_method, int mom_iters, int cont_method, int cont_iters);
 /* 0 'simple', 1 'simplec' */
 int or_case_set_algorithm(or_case *c, int simplec);
 double or_tvd_psi(int scheme, double r);

@@ -41,6 +41,7 @@ program ref_simple_driver
   integer :: npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode
   integer :: i, j, k, f, q, nbfmax, nb, it
   integer :: msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec
+  integer :: gmm, gmi, gcm, gci
   double precision :: rho, mu, urfu, urfp, mrrf, prrf, bleed
   integer, allocatable :: fnn(:), fnodes(:,:), owner(:), neigh(:), psize(:), ptype(:)
   double precision, allocatable :: wallu(:), wallv(:), wallw(:)
@@ -55,7 +56,8 @@ program ref_simple_driver
   open(unit=21, file=trim(fin), access='stream', form='unformatted', status='old')
   read(21) npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode
   if (mode == 2) then
-    read(21) msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec
+    read(21) msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec, &
+             gmm, gmi, gcm, gci
     read(21) rho, mu, urfu, urfp, mrrf, prrf, bleed
   end if
   allocate(fnn(nfaces), fnodes(nfaces, maxfn), owner(nfaces), neigh(nif), psize(npatches), ptype(npatches))
@@ -174,6 +176,10 @@ contains
     NumberOfScalarsToSolve = 0
     NumberOfrFieldsToSolve = 0
     LRelaxMomentum = .true.
+    MethodCalcGradientMomentum = gmm
+    nIterGradientMomentum = gmi
+    MethodCalcGradientContinuity = gcm
+    nIterGradientContinuity = gci
     if (isimplec /= 0) then
       Algorithm = 'simplec'
     else

@@ -39,6 +39,8 @@ SIMPLE = {
                      dict(niter=8, framework="nvf", scheme="smart", bleed=0.2, start_iter=3, lim_mom=2)),
     "tet_sor": (("box_tet", (3, 3, 2), 0.15, 4), dict(niter=8)),
     "hex_simplec": (("box_hex", (6, 5, 4), 0.15, 3), dict(niter=8, algorithm="simplec")),
+    "hex_gradient_methods": (("box_hex", (6, 5, 4), 0.15, 3), dict(niter=8, grad_mom=(1, 2), grad_cont=(2, 3))),
+    "tet_gradient_methods": (("box_tet", (3, 3, 2), 0.15, 4), dict(niter=8, grad_mom=(2, 1), grad_cont=(1, 0))),
     # converged states (the device is compared with these)
     "hex_converged": (("box_hex", (8, 7, 6), 0.15, 3), dict(niter=400)),
     "tet_converged": (("box_tet", (4, 4, 3), 0.15, 4), dict(niter=400)),

@@ -9,7 +9,8 @@ with the mesh, the storage and the case installed by
 oracle/ref_simple_driver.f90.  The restatement reproduces them bitwise:
 geometry of perturbed hexahedra and tetrahedra, whole outer iterations with
 SOR, ILU and PBCG solves, second-order upwind with both gradient limiters,
-NVF convection with bleed and a late start, SIMPLEC, tetrahedra, and the converged
+NVF convection with bleed and a late start, SIMPLEC, plain and corrected
+Green-Gauss gradients with other sweep counts, tetrahedra, and the converged
 64 x 64 x 1 cavity with symmetry planes after 600 iterations.
 """
 import numpy as np
@@ -57,6 +58,7 @@ def _oracle_case(name):
         oc.set_convection(kw["framework"], kw["scheme"], kw.get("bleed", 0.0), kw.get("start_iter", 0))
     oc.set_gradient_limiters(kw.get("lim_mom", 0), kw.get("lim_cont", 0))
     oc.set_algorithm(kw.get("algorithm", "simple"))
+    oc.set_gradient_method(kw.get("grad_mom", (2, 2)), kw.get("grad_cont", (2, 2)))
     return oc, niter
 
 

@@ -307,6 +307,7 @@ def _setup_case_symbols(L):
     L.fs_case_set_gradient_limiters.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.fs_case_set_algorithm.argtypes = [C.c_void_p, C.c_int]
     L.fs_case_set_gradient_method.argtypes = [C.c_void_p] + [C.c_int] * 4
+    L.fs_case_set_decompose.argtypes = [C.c_void_p, C.c_int]
     L.fs_case_time_spmv.argtypes = [C.c_void_p, C.c_int, _dp]
     L._case_ready = True
 
@@ -452,6 +453,10 @@ class Case:
     def set_gradient_method(self, momentum=(2, 2), continuity=(2, 2)):
         """(MethodCalcGradient, nIterGradient) of u, v, w and of p, p'."""
         _check(lib().fs_case_set_gradient_method(self.h, *momentum, *continuity), "fs_case_set_gradient_method")
+
+    def set_decompose(self, method):
+        """MethodDecomposeS 1, 2 or 3 (default 3)."""
+        _check(lib().fs_case_set_decompose(self.h, method), "fs_case_set_decompose")
 
     def set_algorithm(self, algorithm):
         """Algorithm: "simple" (default) or "simplec"."""

@@ -33,5 +33,8 @@ struct HostGeom {
 
 void build_topology(const PolyInput &in, HostGeom &g);
 void build_geometry(const PolyInput &in, HostGeom &g);
+// the E/T split (MethodDecomposeS 1, 2 or 3) and the gradient-correction
+// offsets of every face, recomputed in place; owner: 0-based owner per face
+void decompose_faces(HostGeom &g, const int *owner, int method);
 
 } // namespace fs

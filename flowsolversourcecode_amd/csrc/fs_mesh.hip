@@ -195,27 +195,50 @@ void build_geometry(const PolyInput &in, HostGeom &g) {
     orient(f);
     F.g[f] = 1.0;
   }
-  for (int f = 0; f < nf; ++f) { // E/T split, over-relaxed
-    const double dot = F.ex[f] * F.Sx[f] + F.ey[f] * F.Sy[f] + F.ez[f] * F.Sz[f];
-    const double s = F.A[f] * F.A[f] / dot;
-    F.Ex[f] = s * F.ex[f];
-    F.Ey[f] = s * F.ey[f];
-    F.Ez[f] = s * F.ez[f];
-    F.E[f] = std::sqrt(F.Ex[f] * F.Ex[f] + F.Ey[f] * F.Ey[f] + F.Ez[f] * F.Ez[f]);
+  decompose_faces(g, in.owner, 3);
+}
+
+// The E/T split of every face's area vector (ProcessGeometryRoutine.for
+// :442-500 interior, :519-575 boundary) by MethodDecomposeS: 1 minimum
+// correction E = (e.S) e, 2 orthogonal correction E = |S| e, 3 over-relaxed
+// E = |S|^2/(e.S) e; T = S - E, gDiff = |E|/d.  Then the point f' on the
+// centroid line used by the face-centroid correction of the gradient
+// (StandardGreenGaussCorrection1Gradient.for:111-119), kept as the offset
+// r_f - r_f' of every interior face (it is formed with E).
+void decompose_faces(HostGeom &g, const int *owner, int method) {
+  const int nf = g.F, nif = g.Fi;
+  auto &F = g.face;
+  for (int f = 0; f < nf; ++f) {
+    if (method == 1) {
+      const double dot = F.ex[f] * F.Sx[f] + F.ey[f] * F.Sy[f] + F.ez[f] * F.Sz[f];
+      F.Ex[f] = dot * F.ex[f];
+      F.Ey[f] = dot * F.ey[f];
+      F.Ez[f] = dot * F.ez[f];
+      F.E[f] = std::fabs(dot);
+    } else if (method == 2) {
+      F.Ex[f] = F.A[f] * F.ex[f];
+      F.Ey[f] = F.A[f] * F.ey[f];
+      F.Ez[f] = F.A[f] * F.ez[f];
+      F.E[f] = F.A[f];
+    } else {
+      const double dot = F.ex[f] * F.Sx[f] + F.ey[f] * F.Sy[f] + F.ez[f] * F.Sz[f];
+      const double s = F.A[f] * F.A[f] / dot;
+      F.Ex[f] = s * F.ex[f];
+      F.Ey[f] = s * F.ey[f];
+      F.Ez[f] = s * F.ez[f];
+      F.E[f] = std::sqrt(F.Ex[f] * F.Ex[f] + F.Ey[f] * F.Ey[f] + F.Ez[f] * F.Ez[f]);
+    }
     F.gdiff[f] = F.E[f] / F.d[f];
     F.Tx[f] = F.Sx[f] - F.Ex[f];
     F.Ty[f] = F.Sy[f] - F.Ey[f];
     F.Tz[f] = F.Sz[f] - F.Ez[f];
     F.T[f] = std::sqrt(F.Tx[f] * F.Tx[f] + F.Ty[f] * F.Ty[f] + F.Tz[f] * F.Tz[f]);
   }
-  // the point f' on the centroid line used by the face-centroid correction of
-  // the gradient (StandardGreenGaussCorrection1Gradient.for:111-119), kept as
-  // the offset r_f - r_f' of every interior face
   F.kx.assign(nif, 0.0);
   F.ky.assign(nif, 0.0);
   F.kz.assign(nif, 0.0);
   for (int f = 0; f < nif; ++f) {
-    const int j = in.owner[f];
+    const int j = owner[f];
     const double edn = F.Ex[f] * F.nx[f] + F.Ey[f] * F.ny[f] + F.Ez[f] * F.nz[f];
     const double rdn = (F.cx[f] - g.xc[j]) * F.nx[f] + (F.cy[f] - g.yc[j]) * F.ny[f] +
                        (F.cz[f] - g.zc[j]) * F.nz[f];

@@ -616,14 +616,31 @@ struct RcArgs {
   const double *du2, *dv2, *dw2; // SIMPLEC's D2 (the p' coefficients)
 };
 
-// geoDiff = |D_E| / d_CF of the face's D S split (MethodDecomposeS = 3,
-// AssembleMdotUsingRhieChow.for:575-615)
-__device__ __forceinline__ double geo_of(double dusf, double dvsf, double dwsf, double ex, double ey, double ez,
-                                         double dcf) {
-  const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
-  const double eu = dusf / mag, ev = dvsf / mag, ew = dwsf / mag;
-  const double dot = ex * eu + ey * ev + ez * ew;
-  const double due = mag * ex / dot, dve = mag * ey / dot, dwe = mag * ez / dot;
+// geoDiff = |D_E| / d_CF of the face's D S split by MethodDecomposeS DEC
+// (AssembleMdotUsingRhieChow.for:575-615, AssemblePressureCorrectionFluxes.for
+// :99-141): 1 D_E = (e.DS) e, 2 D_E = |DS| e, 3 D_E = |DS|^2/(e.DS) e (the
+// last in the reference's operand order); the E part of D S goes to due..dwe.
+template <int DEC>
+__device__ __forceinline__ double geo_split(double dusf, double dvsf, double dwsf, double ex, double ey,
+                                            double ez, double dcf, double &due, double &dve, double &dwe) {
+  if constexpr (DEC == 1) {
+    const double dot = ex * dusf + ey * dvsf + ez * dwsf;
+    due = dot * ex;
+    dve = dot * ey;
+    dwe = dot * ez;
+  } else if constexpr (DEC == 2) {
+    const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
+    due = ex * mag;
+    dve = ey * mag;
+    dwe = ez * mag;
+  } else {
+    const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
+    const double eu = dusf / mag, ev = dvsf / mag, ew = dwsf / mag;
+    const double dot = ex * eu + ey * ev + ez * ew;
+    due = mag * ex / dot;
+    dve = mag * ey / dot;
+    dwe = mag * ez / dot;
+  }
   return sqrt(due * due + dve * dve + dwe * dwe) / dcf;
 }
 
@@ -633,7 +650,7 @@ __device__ __forceinline__ double geo_of(double dusf, double dvsf, double dwsf, 
 // 'averagecorrected' (InterpolateGradientFromElementToFace.for:16-93), is
 // completed here from the averaged cell gradient the flux already forms.
 // SIMPLEC: the p' coefficients take geoDiff of D2 instead of D1.
-template <bool SIMPLEC>
+template <bool SIMPLEC, int DEC>
 __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
                             const int *__restrict__ nei, const double *__restrict__ g,
                             const double *__restrict__ Sx, const double *__restrict__ Sy,
@@ -658,11 +675,8 @@ __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
   const double rhof = a.rho;
   double fv = rhof * (ub * sx + vb * sy + wb * sz);
   const double dusf = du * sx, dvsf = dv * sy, dwsf = dw * sz;
-  const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
-  const double eu = dusf / mag, ev = dvsf / mag, ew = dwsf / mag;
-  const double dot = ex[f] * eu + ey[f] * ev + ez[f] * ew;
-  const double due = mag * ex[f] / dot, dve = mag * ey[f] / dot, dwe = mag * ez[f] / dot;
-  const double geo = sqrt(due * due + dve * dve + dwe * dwe) / dcf[f];
+  double due, dve, dwe;
+  const double geo = geo_split<DEC>(dusf, dvsf, dwsf, ex[f], ey[f], ez[f], dcf[f], due, dve, dwe);
   const double dut = dusf - due, dvt = dvsf - dve, dwt = dwsf - dwe;
   const double cfl = rhof * geo, ffl = -(rhof * geo);
   double corr = (a.p[j] - a.p[i]) / dcf[f];
@@ -677,7 +691,8 @@ __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
   if constexpr (SIMPLEC) {
     const double d2u = gf * a.du2[i] + og * a.du2[j], d2v = gf * a.dv2[i] + og * a.dv2[j],
                  d2w = gf * a.dw2[i] + og * a.dw2[j];
-    const double geo2 = geo_of(d2u * sx, d2v * sy, d2w * sz, ex[f], ey[f], ez[f], dcf[f]);
+    double e2u, e2v, e2w;
+    const double geo2 = geo_split<DEC>(d2u * sx, d2v * sy, d2w * sz, ex[f], ey[f], ez[f], dcf[f], e2u, e2v, e2w);
     cf[f] = rhof * geo2;
     ff[f] = -(rhof * geo2);
   } else {
@@ -834,6 +849,8 @@ struct SimpleCase {
   DevBuf<double> fcx, fcy, fcz, xc, yc, zc; // centroids, uploaded with the first limiter
   DevBuf<double> lim_part, lim_eps;
   int ngrad_mom = 2, ngrad_cont = 2; // correction sweeps (nIterGradient) of u,v,w and of p, p'
+  int decomp = 3;                  // MethodDecomposeS of the E/T split and of Rhie-Chow
+  std::vector<int> h_own;          // owner of every face (host copy, 0-based)
   bool simplec = false;            // Algorithm 'simplec': D2 in du2/dv2/dw2
   DevBuf<double> du2, dv2, dw2;
   // momentum systems in colour-major cell order (the colour classes of the
@@ -1060,8 +1077,14 @@ static void phase_pc_assemble(SimpleCase &C) {
                        C.Sx.p, C.Sy.p, C.Sz.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, a, C.mdot.p, C.cf.p, C.ff.p,
                        C.tf.p);
   };
-  if (C.simplec) rc(k_rhie_chow<true>);
-  else rc(k_rhie_chow<false>);
+  switch (C.decomp * 2 + (C.simplec ? 1 : 0)) {
+  case 2: rc(k_rhie_chow<false, 1>); break;
+  case 3: rc(k_rhie_chow<true, 1>); break;
+  case 4: rc(k_rhie_chow<false, 2>); break;
+  case 5: rc(k_rhie_chow<true, 2>); break;
+  case 7: rc(k_rhie_chow<true, 3>); break;
+  default: rc(k_rhie_chow<false, 3>);
+  }
   by_width(C.W, [&](auto wc) {
     hipLaunchKernelGGL(k_pc_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
                        C.code.p, C.cf.p, C.ff.p, C.tf.p, C.ac.p, C.anb.p, C.bc.p);
@@ -1276,6 +1299,7 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
       C.mx.alloc(C.N);
     }
     C.own.upload(owner, nfaces, C.s);
+    C.h_own.assign(owner, owner + nfaces);
     C.nei.upload(neighbour, nif, C.s);
     // the per-slot face code of the mesh context
     C.code.alloc((size_t)C.N * C.W);
@@ -1489,6 +1513,28 @@ int fs_case_set_gradient_method(fs_case *cs, int momentum_method, int momentum_i
   C.ngrad_mom = momentum_method == 1 ? 0 : momentum_iters;
   C.ngrad_cont = continuity_method == 1 ? 0 : continuity_iters;
   return 0;
+}
+
+int fs_case_set_decompose(fs_case *cs, int method) {
+  if (!cs || method < 1 || method > 3) {
+    set_error("fs_case_set_decompose: MethodDecomposeS is 1, 2 or 3");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  fs::decompose_faces(C.geom, C.h_own.data(), method);
+  const auto &F = C.geom.face;
+  C.gdiff.upload(F.gdiff, C.s);
+  C.Tx.upload(F.Tx, C.s);
+  C.Ty.upload(F.Ty, C.s);
+  C.Tz.upload(F.Tz, C.s);
+  C.kx.upload(F.kx, C.s);
+  C.ky.upload(F.ky, C.s);
+  C.kz.upload(F.kz, C.s);
+  FS_HIP(hipStreamSynchronize(C.s));
+  C.decomp = method;
+  return 0;
+  CASE_GUARD_END
 }
 
 int fs_case_set_algorithm(fs_case *cs, int algorithm) {

@@ -142,6 +142,13 @@ enum fs_algorithm { FS_ALGORITHM_SIMPLE = 0, FS_ALGORITHM_SIMPLEC = 1 };
  * p'.  A case starts with 2, 2 for both.  Methods 3-6 are not provided. */
 int fs_case_set_gradient_method(fs_case *c, int momentum_method, int momentum_iters, int continuity_method,
                                 int continuity_iters);
+/* MethodDecomposeS (User0.F90:70): the split of each face's area vector into
+ * E (along the centroid line) and T, 1 minimum correction, 2 orthogonal
+ * correction, 3 over-relaxed (the default); the geometry's E, T and gDiff
+ * are recomputed (ProcessGeometryRoutine.for:428-575) and the Rhie-Chow and
+ * p' coefficients split D S the same way (AssembleMdotUsingRhieChow.for
+ * :575-615, AssemblePressureCorrectionFluxes.for:99-141). */
+int fs_case_set_decompose(fs_case *c, int method);
 int fs_case_set_algorithm(fs_case *c, int algorithm);
 /* urfMomentum, urfPressure, ASSolverMomentum/rrFMomentum/ASIterMomentum/
  * LMultigridMomentum and the Continuity equivalents (User0.F90:271-318). */

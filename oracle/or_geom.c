@@ -282,21 +282,7 @@ int or_mesh_geometry(or_mesh *m) {
     m->nz[f] = m->Sz[f] / m->A[f];
     m->g[f] = 1.0;
   }
-  /* over-relaxed E/T split, MethodDecomposeS = 3 (:467-481, boundary
-   * :536-556) */
-  for (int f = 0; f < nf; ++f) {
-    const double dot = m->ex[f] * m->Sx[f] + m->ey[f] * m->Sy[f] + m->ez[f] * m->Sz[f];
-    const double s = m->A[f] * m->A[f] / dot;
-    m->Ex[f] = s * m->ex[f];
-    m->Ey[f] = s * m->ey[f];
-    m->Ez[f] = s * m->ez[f];
-    m->E[f] = sqrt(m->Ex[f] * m->Ex[f] + m->Ey[f] * m->Ey[f] + m->Ez[f] * m->Ez[f]);
-    m->gdiff[f] = m->E[f] / m->dcf[f];
-    m->Tx[f] = m->Sx[f] - m->Ex[f];
-    m->Ty[f] = m->Sy[f] - m->Ey[f];
-    m->Tz[f] = m->Sz[f] - m->Ez[f];
-    m->T[f] = sqrt(m->Tx[f] * m->Tx[f] + m->Ty[f] * m->Ty[f] + m->Tz[f] * m->Tz[f]);
-  }
+  or_mesh_decompose(m, 3);
   return 0;
 }
 
@@ -339,4 +325,35 @@ or_mesh *or_mesh_new(int npoints, const double *x, const double *y, const double
 void or_mesh_delete(or_mesh *m) {
   or_mesh_free(m);
   free(m);
+}
+
+/* The E/T split by MethodDecomposeS (ProcessGeometryRoutine.for:442-500,
+ * boundary :519-575): 1 E = (e.S) e, 2 E = |S| e, 3 E = |S|^2/(e.S) e. */
+void or_mesh_decompose(or_mesh *m, int method) {
+  for (int f = 0; f < m->nfaces; ++f) {
+    if (method == 1) {
+      const double dot = m->ex[f] * m->Sx[f] + m->ey[f] * m->Sy[f] + m->ez[f] * m->Sz[f];
+      m->Ex[f] = dot * m->ex[f];
+      m->Ey[f] = dot * m->ey[f];
+      m->Ez[f] = dot * m->ez[f];
+      m->E[f] = fabs(dot);
+    } else if (method == 2) {
+      m->Ex[f] = m->A[f] * m->ex[f];
+      m->Ey[f] = m->A[f] * m->ey[f];
+      m->Ez[f] = m->A[f] * m->ez[f];
+      m->E[f] = m->A[f];
+    } else {
+      const double dot = m->ex[f] * m->Sx[f] + m->ey[f] * m->Sy[f] + m->ez[f] * m->Sz[f];
+      const double s = m->A[f] * m->A[f] / dot;
+      m->Ex[f] = s * m->ex[f];
+      m->Ey[f] = s * m->ey[f];
+      m->Ez[f] = s * m->ez[f];
+      m->E[f] = sqrt(m->Ex[f] * m->Ex[f] + m->Ey[f] * m->Ey[f] + m->Ez[f] * m->Ez[f]);
+    }
+    m->gdiff[f] = m->E[f] / m->dcf[f];
+    m->Tx[f] = m->Sx[f] - m->Ex[f];
+    m->Ty[f] = m->Sy[f] - m->Ey[f];
+    m->Tz[f] = m->Sz[f] - m->Ez[f];
+    m->T[f] = sqrt(m->Tx[f] * m->Tx[f] + m->Ty[f] * m->Ty[f] + m->Tz[f] * m->Tz[f]);
+  }
 }

@@ -61,6 +61,7 @@ or_case *or_case_new(or_mesh *m, const double *wall_u, const double *wall_v,
   c->pitmax = pitmax;
   c->ngrad = 2;
   c->ngrad_cont = 2;
+  c->decomp = 3;
   c->refcell = 0; /* ReferencePressureLocation = 1 (User0.F90:50) */
   c->u = dz(N); c->v = dz(N); c->w = dz(N); c->p = dz(N); c->pc = dz(N);
   c->mdot = dz(Fi);
@@ -697,6 +698,13 @@ int or_case_set_gradient_method(or_case *c, int mom_method, int mom_iters, int c
   return 0;
 }
 
+int or_case_set_decompose(or_case *c, int method) {
+  if (method < 1 || method > 3) return -1;
+  or_mesh_decompose(c->m, method);
+  c->decomp = method;
+  return 0;
+}
+
 int or_case_set_algorithm(or_case *c, int simplec) {
   if (simplec != 0 && simplec != 1) return -1;
   free(c->du2);
@@ -724,13 +732,33 @@ void or_case_momentum_solve(or_case *c, int comp) {
  * :554-759 face loop, steady with LRelaxMomentum) followed by
  * AssemblePressureCorrection (AssemblePressureCorrectionFluxes.for:81-164)
  * and the scatter. */
-static double geo_diff(const or_mesh *m, int k, double du, double dv, double dw) {
-  const double dusf = du * m->Sx[k], dvsf = dv * m->Sy[k], dwsf = dw * m->Sz[k];
-  const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
-  const double eu = dusf / mag, ev = dvsf / mag, ew = dwsf / mag;
-  const double dot = m->ex[k] * eu + m->ey[k] * ev + m->ez[k] * ew;
-  const double due = mag * m->ex[k] / dot, dve = mag * m->ey[k] / dot, dwe = mag * m->ez[k] / dot;
-  return sqrt(due * due + dve * dve + dwe * dwe) / m->dcf[k];
+/* geoDiff of the D S split by MethodDecomposeS (AssembleMdotUsingRhieChow.for
+ * :575-615, AssemblePressureCorrectionFluxes.for:99-141); the E part of D S
+ * goes to de[3]. */
+static double geo_split(int method, double ex, double ey, double ez, double cf, double dusf, double dvsf,
+                        double dwsf, double *de) {
+  double due, dve, dwe;
+  if (method == 1) {
+    const double dot = ex * dusf + ey * dvsf + ez * dwsf;
+    due = dot * ex; dve = dot * ey; dwe = dot * ez;
+  } else if (method == 2) {
+    const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
+    due = ex * mag; dve = ey * mag; dwe = ez * mag;
+  } else {
+    const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
+    const double eu = dusf / mag, ev = dvsf / mag, ew = dwsf / mag;
+    const double dot = ex * eu + ey * ev + ez * ew;
+    due = mag * ex / dot; dve = mag * ey / dot; dwe = mag * ez / dot;
+  }
+  de[0] = due; de[1] = dve; de[2] = dwe;
+  return sqrt(due * due + dve * dve + dwe * dwe) / cf;
+}
+
+static double geo_diff(const or_case *c, int k, double du, double dv, double dw) {
+  const or_mesh *m = c->m;
+  double de[3];
+  return geo_split(c->decomp, m->ex[k], m->ey[k], m->ez[k], m->dcf[k], du * m->Sx[k], dv * m->Sy[k],
+                   dw * m->Sz[k], de);
 }
 
 void or_case_pc_assemble(or_case *c) {
@@ -759,12 +787,9 @@ void or_case_pc_assemble(or_case *c) {
     const double mbar = ub * sx + vb * sy + wb * sz;
     double fv = rhof * mbar;
     const double dusf = du * sx, dvsf = dv * sy, dwsf = dw * sz;
-    const double mag = sqrt(dusf * dusf + dvsf * dvsf + dwsf * dwsf);
-    const double eu = dusf / mag, ev = dvsf / mag, ew = dwsf / mag;
-    const double dot = m->ex[k] * eu + m->ey[k] * ev + m->ez[k] * ew;
-    const double due = mag * m->ex[k] / dot, dve = mag * m->ey[k] / dot, dwe = mag * m->ez[k] / dot;
-    const double geo = sqrt(due * due + dve * dve + dwe * dwe) / m->dcf[k];
-    const double dut = dusf - due, dvt = dvsf - dve, dwt = dwsf - dwe;
+    double de[3];
+    const double geo = geo_split(c->decomp, m->ex[k], m->ey[k], m->ez[k], m->dcf[k], dusf, dvsf, dwsf, de);
+    const double dut = dusf - de[0], dvt = dvsf - de[1], dwt = dwsf - de[2];
     const double cfl = rhof * geo, ffl = -(rhof * geo);
     fv = fv - rhof * (c->fgp[k] * dut + c->fgp[Fi + k] * dvt + c->fgp[2 * Fi + k] * dwt);
     fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
@@ -776,7 +801,7 @@ void or_case_pc_assemble(or_case *c) {
     const int i = m->owner[k], j = m->neigh[k];
     const double gf = m->g[k], og = 1. - gf;
     const double *d2u = c->du2 ? c->du2 : c->du, *d2v = c->du2 ? c->dv2 : c->dv, *d2w = c->du2 ? c->dw2 : c->dw;
-    const double geo = geo_diff(m, k, gf * d2u[i] + og * d2u[j], gf * d2v[i] + og * d2v[j],
+    const double geo = geo_diff(c, k, gf * d2u[i] + og * d2u[j], gf * d2v[i] + og * d2v[j],
                                 gf * d2w[i] + og * d2w[j]);
     c->fcf[k] = rhof * geo;
     c->fff[k] = -(rhof * geo);

@@ -37,6 +37,7 @@ void or_mesh_delete(or_mesh *m);
 void or_mesh_free(or_mesh *m);
 int or_mesh_topology(or_mesh *m);
 int or_mesh_geometry(or_mesh *m);
+void or_mesh_decompose(or_mesh *m, int method);
 
 /* one SIMPLE case on a mesh */
 typedef struct {
@@ -80,6 +81,7 @@ typedef struct {
   /* Green-Gauss correction sweeps of p and p' (nIterGradientContinuity;
    * ngrad above is nIterGradientMomentum); 0 = MethodCalcGradient 1 */
   int ngrad_cont;
+  int decomp; /* MethodDecomposeS of the mesh's E/T split and of Rhie-Chow */
 } or_case;
 
 enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 1 };
@@ -107,6 +109,8 @@ int or_case_set_gradient_limiters(or_case *c, int momentum_method, int continuit
 /* MethodCalcGradient 1 (plain Green-Gauss) or 2 (with nIterGradient
  * face-centroid corrections) for the velocities and for p and p' */
 int or_case_set_gradient_method(or_case *c, int mom_method, int mom_iters, int cont_method, int cont_iters);
+/* MethodDecomposeS 1, 2 or 3: recomputes the mesh's E/T split */
+int or_case_set_decompose(or_case *c, int method);
 /* 0 'simple', 1 'simplec' */
 int or_case_set_algorithm(or_case *c, int simplec);
 double or_tvd_psi(int scheme, double r);

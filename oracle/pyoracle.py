@@ -227,7 +227,7 @@ def ref_simple_available():
     return os.path.exists(REF_SIMPLE)
 
 
-def _simple_input(f, pm, mesh, mode, extra_ints=(), extra_doubles=(), tail=()):
+def _simple_input(f, pm, mesh, mode, extra_ints=(), extra_doubles=(), tail=(), decomp=3):
     """The polyMesh and the Readit topology of ``mesh`` (an OrMesh of ``pm``)
     in ref_simple_driver.f90's input layout."""
     s = mesh.ptr.contents
@@ -244,7 +244,7 @@ def _simple_input(f, pm, mesh, mode, extra_ints=(), extra_doubles=(), tail=()):
     nen = np.ctypeslib.as_array(s.nen, shape=(N,)).astype(np.int32)
     lst = np.ctypeslib.as_array(s.len, shape=(N * lenw,)).astype(np.int32).reshape(lenw, N).T
     lst = np.where(np.arange(lenw)[None, :] < nen[:, None], lst + 1, 0).astype(np.int32)
-    f.write(np.array([len(pts), nf, nif, N, len(pm.patch_sizes), W, lenw, maxfn, mode] + list(extra_ints),
+    f.write(np.array([len(pts), nf, nif, N, len(pm.patch_sizes), W, lenw, maxfn, mode, decomp] + list(extra_ints),
                      np.int32).tobytes())
     if extra_doubles:
         f.write(np.array(extra_doubles, np.float64).tobytes())
@@ -266,7 +266,7 @@ def _simple_input(f, pm, mesh, mode, extra_ints=(), extra_doubles=(), tail=()):
     f.write(np.asfortranarray(lst).tobytes(order="F"))
 
 
-def run_reference_geometry(pm, mesh):
+def run_reference_geometry(pm, mesh, decomp=3):
     """The reference's ProcessGeometry on the polyMesh ``pm`` with the Readit
     topology of ``mesh``: a dict of face fields over all faces (interior,
     then boundary) and cell fields, named as OrMesh names them, plus "g"
@@ -275,7 +275,8 @@ def run_reference_geometry(pm, mesh):
     with tempfile.TemporaryDirectory(dir=os.path.join(HERE, "_ref")) as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
-            _simple_input(f, pm, mesh, 1, tail=(_c(np.zeros(len(pm.patch_sizes)), np.int32).tobytes(),))
+            _simple_input(f, pm, mesh, 1, tail=(_c(np.zeros(len(pm.patch_sizes)), np.int32).tobytes(),),
+                          decomp=decomp)
         r = subprocess.run([REF_SIMPLE, fin, fout], capture_output=True, text=True, cwd=td)
         if r.returncode != 0:
             raise RuntimeError(f"reference run failed: {r.stdout[-1000:]} {r.stderr[-2000:]}")
@@ -296,7 +297,7 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
                          urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1, mitmax=20, psolver=SOR, prrf=0.01,
                          pitmax=200, framework="none", scheme="upwind", bleed=0.0, start_iter=0, lim_mom=0,
                          lim_cont=0, plmg=0, algorithm="simple", grad_mom=(2, 2), grad_cont=(2, 2),
-                         seconds=False):
+                         decomp=3, seconds=False):
     """niter outer iterations of the reference's own SIMPLE (SolveMomentumX/Y/Z,
     SolveContinuity) from rest on the polyMesh ``pm`` with the Readit topology of
     ``mesh``; patch_types per patch "wall"/"symmetry" (default all walls);
@@ -312,7 +313,8 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
     with tempfile.TemporaryDirectory(dir=os.path.join(HERE, "_ref")) as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
-            _simple_input(f, pm, mesh, 2, extra_ints=ints, extra_doubles=dbl, tail=(codes.tobytes(),))
+            _simple_input(f, pm, mesh, 2, extra_ints=ints, extra_doubles=dbl, tail=(codes.tobytes(),),
+                          decomp=decomp)
             for a in (wall_u, wall_v, wall_w):
                 f.write(_c(a, np.float64).tobytes())
         t0 = time.perf_counter()
@@ -490,7 +492,7 @@ class _OrCase(C.Structure):
                [(n, C.c_int) for n in ("hr_fw", "hr_scheme", "hr_start", "lim_mom", "lim_cont", "iter")] + \
                [("bleed", C.c_double)] + \
                [(n, C.POINTER(C.c_double)) for n in ("du2", "dv2", "dw2")] + \
-               [("ngrad_cont", C.c_int)]
+               [("ngrad_cont", C.c_int), ("decomp", C.c_int)]
 
 
 HR_FRAMEWORKS = {"none": 0, "tvd": 1, "nvf": 2}
@@ -575,6 +577,16 @@ class OrCase:
         L.or_case_set_gradient_method.argtypes = [C.POINTER(_OrCase)] + [C.c_int] * 4
         if L.or_case_set_gradient_method(self.ptr, *momentum, *continuity) != 0:
             raise ValueError("gradient method must be 1 or 2 with iterations >= 0")
+
+    def set_decompose(self, method):
+        """MethodDecomposeS 1, 2 or 3 (recomputes the mesh's E/T split)."""
+        L = lib()
+        L.or_case_set_decompose.argtypes = [C.POINTER(_OrCase), C.c_int]
+        if L.or_case_set_decompose(self.ptr, method) != 0:
+            raise ValueError(method)
+        s = self.mesh.ptr.contents
+        for k in ("Ex", "Ey", "Ez", "E", "gdiff", "Tx", "Ty", "Tz", "T"):
+            self.mesh.face[k] = np.ctypeslib.as_array(getattr(s, k), shape=(self.mesh.nf,)).copy()
 
     def set_algorithm(self, algorithm):
         """Algorithm: "simple" or "simplec"."""

@@ -38,7 +38,7 @@ program ref_simple_driver
   use Geometry4
   implicit none
   character(len=1024) :: fin, fout
-  integer :: npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode
+  integer :: npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode, idecomp
   integer :: i, j, k, f, q, nbfmax, nb, it
   integer :: msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec
   integer :: gmm, gmi, gcm, gci
@@ -54,7 +54,7 @@ program ref_simple_driver
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   open(unit=21, file=trim(fin), access='stream', form='unformatted', status='old')
-  read(21) npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode
+  read(21) npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode, idecomp
   if (mode == 2) then
     read(21) msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec, &
              gmm, gmi, gcm, gci
@@ -115,7 +115,7 @@ program ref_simple_driver
     end do
   end do
 
-  MethodDecomposeS = 3
+  MethodDecomposeS = idecomp
   open(unit=11, file=trim(fout)//'.log', status='replace')
   call ProcessGeometry
   close(11)

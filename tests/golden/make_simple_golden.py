@@ -26,6 +26,7 @@ GEOMETRY = {
     "hex": ("box_hex", (5, 4, 3), 0.2, 3),
     "tet": ("box_tet", (2, 3, 2), 0.2, 4),
 }
+DECOMPOSE = (1, 2, 3)  # MethodDecomposeS of the geometry fixtures
 
 # name: (mesh, settings); meshes are ("box_hex"|"box_tet", n, perturb, seed) with
 # lid-driven cavity walls, or ("cavity_2d", n) with its symmetry planes
@@ -41,6 +42,8 @@ SIMPLE = {
     "hex_simplec": (("box_hex", (6, 5, 4), 0.15, 3), dict(niter=8, algorithm="simplec")),
     "hex_gradient_methods": (("box_hex", (6, 5, 4), 0.15, 3), dict(niter=8, grad_mom=(1, 2), grad_cont=(2, 3))),
     "tet_gradient_methods": (("box_tet", (3, 3, 2), 0.15, 4), dict(niter=8, grad_mom=(2, 1), grad_cont=(1, 0))),
+    "hex_decompose1": (("box_hex", (6, 5, 4), 0.2, 3), dict(niter=8, decomp=1)),
+    "tet_decompose2": (("box_tet", (3, 3, 2), 0.2, 4), dict(niter=8, decomp=2, algorithm="simplec")),
     # converged states (the device is compared with these)
     "hex_converged": (("box_hex", (8, 7, 6), 0.15, 3), dict(niter=400)),
     "tet_converged": (("box_tet", (4, 4, 3), 0.15, 4), dict(niter=400)),
@@ -69,9 +72,10 @@ def main():
     out = {}
     for name, (fn, n, perturb, seed) in GEOMETRY.items():
         pm = getattr(meshgen, fn)(*n, perturb=perturb, seed=seed)
-        g = po.run_reference_geometry(pm, po.OrMesh(pm))
-        for k, v in g.items():
-            out[f"{name}_{k}"] = v
+        for dec in DECOMPOSE:
+            g = po.run_reference_geometry(pm, po.OrMesh(pm), decomp=dec)
+            for k, v in g.items():
+                out[f"{name}_{k}" if dec == 3 else f"{name}_dec{dec}_{k}"] = v
     np.savez_compressed(os.path.join(HERE, "geometry_ref.npz"), **out)
     print("wrote geometry_ref.npz")
     for name, (spec, kw) in SIMPLE.items():

@@ -10,7 +10,8 @@ oracle/ref_simple_driver.f90.  The restatement reproduces them bitwise:
 geometry of perturbed hexahedra and tetrahedra, whole outer iterations with
 SOR, ILU and PBCG solves, second-order upwind with both gradient limiters,
 NVF convection with bleed and a late start, SIMPLEC, plain and corrected
-Green-Gauss gradients with other sweep counts, tetrahedra, and the converged
+Green-Gauss gradients with other sweep counts, the minimum- and
+orthogonal-correction splits (MethodDecomposeS 1, 2), tetrahedra, and the converged
 64 x 64 x 1 cavity with symmetry planes after 600 iterations.
 """
 import numpy as np
@@ -29,16 +30,22 @@ msg = importlib.util.module_from_spec(_spec)
 _spec.loader.exec_module(msg)
 
 
+@pytest.mark.parametrize("dec", msg.DECOMPOSE)
 @pytest.mark.parametrize("name", sorted(msg.GEOMETRY))
-def test_geometry_matches_the_reference_build(name):
+def test_geometry_matches_the_reference_build(name, dec):
+    """ProcessGeometry with MethodDecomposeS 1, 2 and 3."""
     g = golden("geometry_ref.npz")
     fn, n, perturb, seed = msg.GEOMETRY[name]
-    m = po.OrMesh(getattr(meshgen, fn)(*n, perturb=perturb, seed=seed))
+    pm = getattr(meshgen, fn)(*n, perturb=perturb, seed=seed)
+    oc = po.OrCase(pm, *meshgen.lid_driven_cavity_walls(pm))
+    oc.set_decompose(dec)
+    m = oc.mesh
+    pre = name if dec == 3 else f"{name}_dec{dec}"
     for k in po.REF_GEOM_FACE:
-        assert np.array_equal(m.face[k], g[f"{name}_{k}"]), k
-    assert np.array_equal(m.face["g"][:m.nif], g[f"{name}_g"])
+        assert np.array_equal(m.face[k], g[f"{pre}_{k}"]), k
+    assert np.array_equal(m.face["g"][:m.nif], g[f"{pre}_g"])
     for k in ("vol", "xc", "yc", "zc"):
-        assert np.array_equal(m.cell[k], g[f"{name}_{k}"]), k
+        assert np.array_equal(m.cell[k], g[f"{pre}_{k}"]), k
 
 
 def _oracle_case(name):
@@ -59,6 +66,7 @@ def _oracle_case(name):
     oc.set_gradient_limiters(kw.get("lim_mom", 0), kw.get("lim_cont", 0))
     oc.set_algorithm(kw.get("algorithm", "simple"))
     oc.set_gradient_method(kw.get("grad_mom", (2, 2)), kw.get("grad_cont", (2, 2)))
+    oc.set_decompose(kw.get("decomp", 3))
     return oc, niter
 
 

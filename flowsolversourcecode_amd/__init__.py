@@ -308,6 +308,7 @@ def _setup_case_symbols(L):
     L.fs_case_set_algorithm.argtypes = [C.c_void_p, C.c_int]
     L.fs_case_set_gradient_method.argtypes = [C.c_void_p] + [C.c_int] * 4
     L.fs_case_set_decompose.argtypes = [C.c_void_p, C.c_int]
+    L.fs_case_set_gradient_interpolation.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.fs_case_time_spmv.argtypes = [C.c_void_p, C.c_int, _dp]
     L._case_ready = True
 
@@ -453,6 +454,13 @@ class Case:
     def set_gradient_method(self, momentum=(2, 2), continuity=(2, 2)):
         """(MethodCalcGradient, nIterGradient) of u, v, w and of p, p'."""
         _check(lib().fs_case_set_gradient_method(self.h, *momentum, *continuity), "fs_case_set_gradient_method")
+
+    def set_gradient_interpolation(self, momentum="average", continuity="averagecorrected"):
+        """GradientInterpolationSchemeMomentum / Continuity: "average", "upwind",
+        "downwind", "averagecorrected" or "harmonic"."""
+        codes = {"average": 0, "upwind": 1, "downwind": 2, "averagecorrected": 3, "harmonic": 4}
+        _check(lib().fs_case_set_gradient_interpolation(self.h, codes[momentum], codes[continuity]),
+               "fs_case_set_gradient_interpolation")
 
     def set_decompose(self, method):
         """MethodDecomposeS 1, 2 or 3 (default 3)."""

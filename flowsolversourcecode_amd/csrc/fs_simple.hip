@@ -362,6 +362,36 @@ struct MomArgs {
   const double *mdot;
 };
 
+// One field's face gradient by GradientInterpolationScheme
+// (InterpolateGradientFromElementToFace.for:16-108): the owner/neighbour
+// weight is g ('average', 'averagecorrected', 'harmonic') or follows the
+// mass flux ('upwind': 1 for mdot > 0, 'downwind' the opposite);
+// 'averagecorrected' adds the centroid-line correction, 'harmonic' weighs by
+// the (constant) density.
+__device__ __forceinline__ void face_grad(int scheme, double gf, double md, double rho, double gix, double giy,
+                                          double giz, double gjx, double gjy, double gjz, double pi, double pj,
+                                          double dcf, double ex, double ey, double ez, double &fx, double &fy,
+                                          double &fz) {
+  if (scheme == FS_GRADINT_UPWIND) gf = md > 0. ? 1. : 0.;
+  else if (scheme == FS_GRADINT_DOWNWIND) gf = md > 0. ? 0. : 1.;
+  if (scheme == FS_GRADINT_HARMONIC) {
+    fx = rho * (gf * gix / rho + (1. - gf) * gjx / rho);
+    fy = rho * (gf * giy / rho + (1. - gf) * gjy / rho);
+    fz = rho * (gf * giz / rho + (1. - gf) * gjz / rho);
+  } else {
+    fx = gf * gix + (1. - gf) * gjx;
+    fy = gf * giy + (1. - gf) * gjy;
+    fz = gf * giz + (1. - gf) * gjz;
+  }
+  if (scheme == FS_GRADINT_AVERAGECORRECTED) {
+    double corr = (pj - pi) / dcf;
+    corr = corr - (fx * ex + fy * ey + fz * ez);
+    fx = fx + corr * ex;
+    fy = fy + corr * ey;
+    fz = fz + corr * ez;
+  }
+}
+
 // Momentum face fluxes of the interior faces for u, v and w in one pass, each
 // component's in the reference's order: diffusion
 // (AssembleIsotropicDiffusion.for:99-117), explicit diffusion
@@ -386,7 +416,9 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
                             const double *__restrict__ gv, const double *__restrict__ gw,
                             const double *__restrict__ mdot, const double *__restrict__ dcx,
                             const double *__restrict__ dcy, const double *__restrict__ dcz, int scheme,
-                            double bleed, double *__restrict__ cf, double *__restrict__ ff,
+                            double bleed, int gint, double rho, const double *__restrict__ ex,
+                            const double *__restrict__ ey, const double *__restrict__ ez,
+                            const double *__restrict__ dcf, double *__restrict__ cf, double *__restrict__ ff,
                             double *__restrict__ tf3) {
   const int f = xcd_block() * kBlock + threadIdx.x;
   if (f >= Fi) return;
@@ -402,10 +434,21 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
       gi[k][d] = G[k][(size_t)d * N + i];
       gj[k][d] = G[k][(size_t)d * N + j];
     }
+  const double *P[3] = {u, v, w};
+  if (gint == FS_GRADINT_AVERAGE) {
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < 3; ++k)
 #pragma unroll
-    for (int d = 0; d < 3; ++d) fgv[k][d] = gf * gi[k][d] + og * gj[k][d];
+      for (int d = 0; d < 3; ++d) fgv[k][d] = gf * gi[k][d] + og * gj[k][d];
+  } else {
+    const double mdf = mdot[f];
+    const bool corr = gint == FS_GRADINT_AVERAGECORRECTED;
+    const double e0 = corr ? ex[f] : 0., e1 = corr ? ey[f] : 0., e2 = corr ? ez[f] : 0., dd = corr ? dcf[f] : 1.;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      face_grad(gint, gf, mdf, rho, gi[k][0], gi[k][1], gi[k][2], gj[k][0], gj[k][1], gj[k][2], P[k][i], P[k][j],
+                dd, e0, e1, e2, fgv[k][0], fgv[k][1], fgv[k][2]);
+  }
   const double gamf = gf * mu + (1. - gf) * mu;
   const double gd = ld_mat<kStreamFaces>(&gdiff[f]);
   const double cfl = gamf * gd, ffl = -(gamf * gd);
@@ -426,7 +469,6 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
                tz = ld_mat<kStreamFaces>(&Tz[f]), sx = ld_mat<kStreamFaces>(&Sx[f]),
                sy = ld_mat<kStreamFaces>(&Sy[f]), sz = ld_mat<kStreamFaces>(&Sz[f]);
   const double dv = fgv[0][0] + fgv[1][1] + fgv[2][2];
-  const double *P[3] = {u, v, w};
   // the upwind side: the owner for mdot > 0, else the neighbour
   const bool up = md > 0.;
   double ddx = 0., ddy = 0., ddz = 0.;
@@ -614,6 +656,7 @@ struct RcArgs {
   double rho, urf_u;
   const double *u, *v, *w, *us, *vs, *ws, *du, *dv, *dw, *gp, *p;
   const double *du2, *dv2, *dw2; // SIMPLEC's D2 (the p' coefficients)
+  int gint;                      // GradientInterpolationSchemeContinuity
 };
 
 // geoDiff = |D_E| / d_CF of the face's D S split by MethodDecomposeS DEC
@@ -679,9 +722,19 @@ __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
   const double geo = geo_split<DEC>(dusf, dvsf, dwsf, ex[f], ey[f], ez[f], dcf[f], due, dve, dwe);
   const double dut = dusf - due, dvt = dvsf - dve, dwt = dwsf - dwe;
   const double cfl = rhof * geo, ffl = -(rhof * geo);
-  double corr = (a.p[j] - a.p[i]) / dcf[f];
-  corr = corr - (pbx * ex[f] + pby * ey[f] + pbz * ez[f]);
-  const double fgx = pbx + corr * ex[f], fgy = pby + corr * ey[f], fgz = pbz + corr * ez[f];
+  // the pressure face gradient by GradientInterpolationSchemeContinuity (the
+  // averaged cell gradient above also stands in the D_S term)
+  double fgx, fgy, fgz;
+  if (a.gint == FS_GRADINT_AVERAGECORRECTED) {
+    double corr = (a.p[j] - a.p[i]) / dcf[f];
+    corr = corr - (pbx * ex[f] + pby * ey[f] + pbz * ez[f]);
+    fgx = pbx + corr * ex[f];
+    fgy = pby + corr * ey[f];
+    fgz = pbz + corr * ez[f];
+  } else {
+    face_grad(a.gint, gf, mdot[f], a.rho, a.gp[i], a.gp[N + i], a.gp[2 * N + i], a.gp[j], a.gp[N + j],
+              a.gp[2 * N + j], a.p[i], a.p[j], dcf[f], ex[f], ey[f], ez[f], fgx, fgy, fgz);
+  }
   fv = fv - rhof * (fgx * dut + fgy * dvt + fgz * dwt);
   fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
   const double mstar = rhof * (usf * sx + vsf * sy + wsf * sz);
@@ -850,6 +903,7 @@ struct SimpleCase {
   DevBuf<double> lim_part, lim_eps;
   int ngrad_mom = 2, ngrad_cont = 2; // correction sweeps (nIterGradient) of u,v,w and of p, p'
   int decomp = 3;                  // MethodDecomposeS of the E/T split and of Rhie-Chow
+  int gi_mom = FS_GRADINT_AVERAGE, gi_cont = FS_GRADINT_AVERAGECORRECTED; // face-gradient schemes
   std::vector<int> h_own;          // owner of every face (host copy, 0-based)
   bool simplec = false;            // Algorithm 'simplec': D2 in du2/dv2/dw2
   DevBuf<double> du2, dv2, dw2;
@@ -976,7 +1030,8 @@ static void phase_momentum_gradients(SimpleCase &C) {
     hipLaunchKernelGGL(kernel, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.F, C.N, C.own.p, C.nei.p,
                        C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p, C.Sx.p, C.Sy.p, C.Sz.p, C.mu,
                        (double)(-2.0f / 3.0f), C.u.p, C.v.p, C.w.p, C.gu.p, C.gv.p, C.gw.p, C.mdot.p, C.dx.p,
-                       C.dy.p, C.dz.p, C.hr_scheme, C.bleed, C.cf.p, C.ff.p, C.tf3.p);
+                       C.dy.p, C.dz.p, C.hr_scheme, C.bleed, C.gi_mom, C.rho, C.ex.p, C.ey.p, C.ez.p, C.d.p,
+                       C.cf.p, C.ff.p, C.tf3.p);
   };
   if (hrf == FS_HR_TVD) face3(k_mom_face3<FS_HR_TVD>);
   else if (hrf == FS_HR_NVF) face3(k_mom_face3<FS_HR_NVF>);
@@ -1072,6 +1127,7 @@ static void phase_pc_assemble(SimpleCase &C) {
   a.gp = C.gp.p;
   a.p = C.p.p;
   a.du2 = C.du2.p; a.dv2 = C.dv2.p; a.dw2 = C.dw2.p;
+  a.gint = C.gi_cont;
   auto rc = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p, C.nei.p, C.g.p,
                        C.Sx.p, C.Sy.p, C.Sz.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, a, C.mdot.p, C.cf.p, C.ff.p,
@@ -1512,6 +1568,17 @@ int fs_case_set_gradient_method(fs_case *cs, int momentum_method, int momentum_i
   auto &C = cs->c;
   C.ngrad_mom = momentum_method == 1 ? 0 : momentum_iters;
   C.ngrad_cont = continuity_method == 1 ? 0 : continuity_iters;
+  return 0;
+}
+
+int fs_case_set_gradient_interpolation(fs_case *cs, int momentum, int continuity) {
+  if (!cs || momentum < FS_GRADINT_AVERAGE || momentum > FS_GRADINT_HARMONIC || continuity < FS_GRADINT_AVERAGE ||
+      continuity > FS_GRADINT_HARMONIC) {
+    set_error("fs_case_set_gradient_interpolation: schemes are FS_GRADINT_*");
+    return -1;
+  }
+  cs->c.gi_mom = momentum;
+  cs->c.gi_cont = continuity;
   return 0;
 }
 

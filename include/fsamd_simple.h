@@ -149,6 +149,15 @@ int fs_case_set_gradient_method(fs_case *c, int momentum_method, int momentum_it
  * p' coefficients split D S the same way (AssembleMdotUsingRhieChow.for
  * :575-615, AssemblePressureCorrectionFluxes.for:99-141). */
 int fs_case_set_decompose(fs_case *c, int method);
+/* GradientInterpolationSchemeMomentum / Continuity (User0.F90:419-420): the
+ * face gradients of u, v, w (diffusion, explicit and bulk terms) and of p
+ * (Rhie-Chow), InterpolateGradientFromElementToFace.for:16-108.  A case
+ * starts with average / averagecorrected. */
+enum fs_gradient_interpolation {
+  FS_GRADINT_AVERAGE = 0, FS_GRADINT_UPWIND = 1, FS_GRADINT_DOWNWIND = 2, FS_GRADINT_AVERAGECORRECTED = 3,
+  FS_GRADINT_HARMONIC = 4
+};
+int fs_case_set_gradient_interpolation(fs_case *c, int momentum, int continuity);
 int fs_case_set_algorithm(fs_case *c, int algorithm);
 /* urfMomentum, urfPressure, ASSolverMomentum/rrFMomentum/ASIterMomentum/
  * LMultigridMomentum and the Continuity equivalents (User0.F90:271-318). */

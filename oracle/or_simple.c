@@ -62,6 +62,8 @@ or_case *or_case_new(or_mesh *m, const double *wall_u, const double *wall_v,
   c->ngrad = 2;
   c->ngrad_cont = 2;
   c->decomp = 3;
+  c->gi_mom = OR_GI_AVERAGE;          /* User0.F90:419-420 */
+  c->gi_cont = OR_GI_AVERAGECORRECTED;
   c->refcell = 0; /* ReferencePressureLocation = 1 (User0.F90:50) */
   c->u = dz(N); c->v = dz(N); c->w = dz(N); c->p = dz(N); c->pc = dz(N);
   c->mdot = dz(Fi);
@@ -224,18 +226,30 @@ static void gradient_n(or_case *c, const double *phi, const double *bphi, double
   free(fz);
 }
 
-/* InterpolateGradientToFace (InterpolateGradientFromElementToFace.for:16-93):
- * 'average' and, when corrected != 0, 'averagecorrected'. */
-static void face_gradient(or_case *c, const double *phi, const double *gr, double *fg, int corrected) {
+/* InterpolateGradientToFace (InterpolateGradientFromElementToFace.for:16-108):
+ * 'average', 'upwind' and 'downwind' (the weight follows mdot > 0),
+ * 'averagecorrected' (the average plus the centroid-line correction) and
+ * 'harmonic' (density-weighted; the density is constant here). */
+static void face_gradient(or_case *c, const double *phi, const double *gr, double *fg, int scheme) {
   const or_mesh *m = c->m;
   const int N = c->N, Fi = c->Fi;
   for (int k = 0; k < Fi; ++k) {
     const int i = m->owner[k], j = m->neigh[k];
-    const double gf = m->g[k];
-    double fx = gf * gr[i] + (1. - gf) * gr[j];
-    double fy = gf * gr[N + i] + (1. - gf) * gr[N + j];
-    double fz = gf * gr[2 * N + i] + (1. - gf) * gr[2 * N + j];
-    if (corrected) {
+    double gf = m->g[k];
+    if (scheme == OR_GI_UPWIND) gf = c->mdot[k] > 0. ? 1. : 0.;
+    if (scheme == OR_GI_DOWNWIND) gf = c->mdot[k] > 0. ? 0. : 1.;
+    double fx, fy, fz;
+    if (scheme == OR_GI_HARMONIC) {
+      const double r = c->rho; /* Density(i) = Density(j) = Densityf(k) */
+      fx = r * (gf * gr[i] / r + (1. - gf) * gr[j] / r);
+      fy = r * (gf * gr[N + i] / r + (1. - gf) * gr[N + j] / r);
+      fz = r * (gf * gr[2 * N + i] / r + (1. - gf) * gr[2 * N + j] / r);
+    } else {
+      fx = gf * gr[i] + (1. - gf) * gr[j];
+      fy = gf * gr[N + i] + (1. - gf) * gr[N + j];
+      fz = gf * gr[2 * N + i] + (1. - gf) * gr[2 * N + j];
+    }
+    if (scheme == OR_GI_AVERAGECORRECTED) {
       double corr = (phi[j] - phi[i]) / m->dcf[k];
       corr = corr - (fx * m->ex[k] + fy * m->ey[k] + fz * m->ez[k]);
       fx = fx + corr * m->ex[k];
@@ -246,6 +260,13 @@ static void face_gradient(or_case *c, const double *phi, const double *gr, doubl
     fg[Fi + k] = fy;
     fg[2 * Fi + k] = fz;
   }
+}
+
+int or_case_set_gradient_interpolation(or_case *c, int momentum, int continuity) {
+  if (momentum < 0 || momentum > 4 || continuity < 0 || continuity > 4) return -1;
+  c->gi_mom = momentum;
+  c->gi_cont = continuity;
+  return 0;
 }
 
 /* MIN and MAX as the reference build evaluates them: left to right, a NaN
@@ -488,10 +509,10 @@ void or_case_momentum_gradients(or_case *c) {
     c->bgp[Fb + b] = (1 - ny * ny) * gy - ny * nx * gx - ny * nz * gz;
     c->bgp[2 * Fb + b] = (1 - nz * nz) * gz - nz * nx * gx - nz * ny * gy;
   }
-  face_gradient(c, c->u, c->gu, c->fgu, 0);
-  face_gradient(c, c->v, c->gv, c->fgv, 0);
-  face_gradient(c, c->w, c->gw, c->fgw, 0);
-  face_gradient(c, c->p, c->gp, c->fgp, 1);
+  face_gradient(c, c->u, c->gu, c->fgu, c->gi_mom);
+  face_gradient(c, c->v, c->gv, c->fgv, c->gi_mom);
+  face_gradient(c, c->w, c->gw, c->fgw, c->gi_mom);
+  face_gradient(c, c->p, c->gp, c->fgp, c->gi_cont);
 }
 
 /* AssembleGlobalMatrixFaceFluxes + ElementFluxes

@@ -82,7 +82,11 @@ typedef struct {
    * ngrad above is nIterGradientMomentum); 0 = MethodCalcGradient 1 */
   int ngrad_cont;
   int decomp; /* MethodDecomposeS of the mesh's E/T split and of Rhie-Chow */
+  /* GradientInterpolationSchemeMomentum / Continuity (OR_GI_*) */
+  int gi_mom, gi_cont;
 } or_case;
+
+enum { OR_GI_AVERAGE = 0, OR_GI_UPWIND = 1, OR_GI_DOWNWIND = 2, OR_GI_AVERAGECORRECTED = 3, OR_GI_HARMONIC = 4 };
 
 enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 1 };
 enum { OR_HR_NONE = 0, OR_HR_TVD = 1, OR_HR_NVF = 2 };
@@ -109,6 +113,7 @@ int or_case_set_gradient_limiters(or_case *c, int momentum_method, int continuit
 /* MethodCalcGradient 1 (plain Green-Gauss) or 2 (with nIterGradient
  * face-centroid corrections) for the velocities and for p and p' */
 int or_case_set_gradient_method(or_case *c, int mom_method, int mom_iters, int cont_method, int cont_iters);
+int or_case_set_gradient_interpolation(or_case *c, int momentum, int continuity);
 /* MethodDecomposeS 1, 2 or 3: recomputes the mesh's E/T split */
 int or_case_set_decompose(or_case *c, int method);
 /* 0 'simple', 1 'simplec' */

@@ -297,7 +297,7 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
                          urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1, mitmax=20, psolver=SOR, prrf=0.01,
                          pitmax=200, framework="none", scheme="upwind", bleed=0.0, start_iter=0, lim_mom=0,
                          lim_cont=0, plmg=0, algorithm="simple", grad_mom=(2, 2), grad_cont=(2, 2),
-                         decomp=3, seconds=False):
+                         decomp=3, gi_mom="average", gi_cont="averagecorrected", seconds=False):
     """niter outer iterations of the reference's own SIMPLE (SolveMomentumX/Y/Z,
     SolveContinuity) from rest on the polyMesh ``pm`` with the Readit topology of
     ``mesh``; patch_types per patch "wall"/"symmetry" (default all walls);
@@ -308,7 +308,8 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
     codes = np.array([{"wall": 0, "symmetry": 1}[t] for t in (patch_types or ["wall"] * len(pm.patch_sizes))],
                      np.int32)
     ints = (msolver, mitmax, psolver, pitmax, niter, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], lim_mom,
-            lim_cont, start_iter, plmg, {"simple": 0, "simplec": 1}[algorithm], *grad_mom, *grad_cont)
+            lim_cont, start_iter, plmg, {"simple": 0, "simplec": 1}[algorithm], *grad_mom, *grad_cont,
+            GRADIENT_INTERPOLATION[gi_mom], GRADIENT_INTERPOLATION[gi_cont])
     dbl = (rho, mu, urf_u, urf_p, mrrf, prrf, bleed)
     with tempfile.TemporaryDirectory(dir=os.path.join(HERE, "_ref")) as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
@@ -492,7 +493,10 @@ class _OrCase(C.Structure):
                [(n, C.c_int) for n in ("hr_fw", "hr_scheme", "hr_start", "lim_mom", "lim_cont", "iter")] + \
                [("bleed", C.c_double)] + \
                [(n, C.POINTER(C.c_double)) for n in ("du2", "dv2", "dw2")] + \
-               [("ngrad_cont", C.c_int), ("decomp", C.c_int)]
+               [("ngrad_cont", C.c_int), ("decomp", C.c_int), ("gi_mom", C.c_int), ("gi_cont", C.c_int)]
+
+
+GRADIENT_INTERPOLATION = {"average": 0, "upwind": 1, "downwind": 2, "averagecorrected": 3, "harmonic": 4}
 
 
 HR_FRAMEWORKS = {"none": 0, "tvd": 1, "nvf": 2}
@@ -577,6 +581,14 @@ class OrCase:
         L.or_case_set_gradient_method.argtypes = [C.POINTER(_OrCase)] + [C.c_int] * 4
         if L.or_case_set_gradient_method(self.ptr, *momentum, *continuity) != 0:
             raise ValueError("gradient method must be 1 or 2 with iterations >= 0")
+
+    def set_gradient_interpolation(self, momentum="average", continuity="averagecorrected"):
+        """GradientInterpolationSchemeMomentum / Continuity."""
+        L = lib()
+        L.or_case_set_gradient_interpolation.argtypes = [C.POINTER(_OrCase), C.c_int, C.c_int]
+        if L.or_case_set_gradient_interpolation(self.ptr, GRADIENT_INTERPOLATION[momentum],
+                                                GRADIENT_INTERPOLATION[continuity]) != 0:
+            raise ValueError((momentum, continuity))
 
     def set_decompose(self, method):
         """MethodDecomposeS 1, 2 or 3 (recomputes the mesh's E/T split)."""

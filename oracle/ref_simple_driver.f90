@@ -41,7 +41,9 @@ program ref_simple_driver
   integer :: npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode, idecomp
   integer :: i, j, k, f, q, nbfmax, nb, it
   integer :: msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec
-  integer :: gmm, gmi, gcm, gci
+  integer :: gmm, gmi, gcm, gci, gim, gic
+  character(len=16), parameter :: gradinterp(0:4) = (/ 'average         ', 'upwind          ', &
+      'downwind        ', 'averagecorrected', 'harmonic        ' /)
   double precision :: rho, mu, urfu, urfp, mrrf, prrf, bleed
   integer, allocatable :: fnn(:), fnodes(:,:), owner(:), neigh(:), psize(:), ptype(:)
   double precision, allocatable :: wallu(:), wallv(:), wallw(:)
@@ -57,7 +59,7 @@ program ref_simple_driver
   read(21) npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode, idecomp
   if (mode == 2) then
     read(21) msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec, &
-             gmm, gmi, gcm, gci
+             gmm, gmi, gcm, gci, gim, gic
     read(21) rho, mu, urfu, urfp, mrrf, prrf, bleed
   end if
   allocate(fnn(nfaces), fnodes(nfaces, maxfn), owner(nfaces), neigh(nif), psize(npatches), ptype(npatches))
@@ -176,6 +178,8 @@ contains
     NumberOfScalarsToSolve = 0
     NumberOfrFieldsToSolve = 0
     LRelaxMomentum = .true.
+    GradientInterpolationSchemeMomentum = gradinterp(gim)
+    GradientInterpolationSchemeContinuity = gradinterp(gic)
     MethodCalcGradientMomentum = gmm
     nIterGradientMomentum = gmi
     MethodCalcGradientContinuity = gcm

@@ -43,6 +43,9 @@ SIMPLE = {
     "hex_gradient_methods": (("box_hex", (6, 5, 4), 0.15, 3), dict(niter=8, grad_mom=(1, 2), grad_cont=(2, 3))),
     "tet_gradient_methods": (("box_tet", (3, 3, 2), 0.15, 4), dict(niter=8, grad_mom=(2, 1), grad_cont=(1, 0))),
     "hex_decompose1": (("box_hex", (6, 5, 4), 0.2, 3), dict(niter=8, decomp=1)),
+    "hex_face_gradients": (("box_hex", (6, 5, 4), 0.2, 3), dict(niter=8, gi_mom="upwind", gi_cont="harmonic")),
+    "tet_face_gradients": (("box_tet", (3, 3, 2), 0.2, 4),
+                           dict(niter=8, gi_mom="averagecorrected", gi_cont="downwind")),
     "tet_decompose2": (("box_tet", (3, 3, 2), 0.2, 4), dict(niter=8, decomp=2, algorithm="simplec")),
     # converged states (the device is compared with these)
     "hex_converged": (("box_hex", (8, 7, 6), 0.15, 3), dict(niter=400)),

@@ -11,7 +11,8 @@ geometry of perturbed hexahedra and tetrahedra, whole outer iterations with
 SOR, ILU and PBCG solves, second-order upwind with both gradient limiters,
 NVF convection with bleed and a late start, SIMPLEC, plain and corrected
 Green-Gauss gradients with other sweep counts, the minimum- and
-orthogonal-correction splits (MethodDecomposeS 1, 2), tetrahedra, and the converged
+orthogonal-correction splits (MethodDecomposeS 1, 2), other face-gradient
+interpolations, tetrahedra, and the converged
 64 x 64 x 1 cavity with symmetry planes after 600 iterations.
 """
 import numpy as np
@@ -67,6 +68,7 @@ def _oracle_case(name):
     oc.set_algorithm(kw.get("algorithm", "simple"))
     oc.set_gradient_method(kw.get("grad_mom", (2, 2)), kw.get("grad_cont", (2, 2)))
     oc.set_decompose(kw.get("decomp", 3))
+    oc.set_gradient_interpolation(kw.get("gi_mom", "average"), kw.get("gi_cont", "averagecorrected"))
     return oc, niter
 
 

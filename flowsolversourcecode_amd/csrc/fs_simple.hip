@@ -1038,9 +1038,16 @@ static void phase_momentum_gradients(SimpleCase &C) {
   else face3(k_mom_face3<FS_HR_NONE>);
 }
 
+// The momentum systems go to colour-major order only for the multicolour
+// Gauss-Seidel ('sor') solve, whose result depends on the colour classes
+// alone; every other solver sees the reference's numbering (ILU factors and
+// Krylov sums depend on it).
+static bool mom_permuted(const SimpleCase &C) { return C.mom_mesh && C.msolver == FS_SOR && !C.mlmg; }
+
 static void phase_momentum_assemble(SimpleCase &C, int comp) {
   MomArgs a{};
-  C.sys_perm = C.mom_mesh != nullptr;
+  const bool perm = mom_permuted(C);
+  C.sys_perm = perm;
   a.comp = comp;
   a.mu = C.mu;
   a.c23 = (double)(-2.0f / 3.0f); // REAL(4) literal -2./3. (AssembleBulkViscosity.for:49)
@@ -1061,7 +1068,7 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
   by_width(C.W, [&](auto wc) {
     hipLaunchKernelGGL(k_mom_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
                        C.code.p, C.cf.p, C.ff.p, tf, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
-                       C.urf_u, C.mom_mesh ? C.mperm.p : nullptr, C.ac.p, C.anb.p, C.bc.p, D, st, C.mx.p,
+                       C.urf_u, perm ? C.mperm.p : nullptr, C.ac.p, C.anb.p, C.bc.p, D, st, C.mx.p,
                        C.simplec ? (comp == 0 ? C.du2.p : comp == 1 ? C.dv2.p : C.dw2.p) : nullptr);
   });
 }
@@ -1095,7 +1102,7 @@ static int phase_momentum_solve(SimpleCase &C, int comp, fs_simple_stats *st) {
   fs_solve_stats ss{};
   double res4[4];
   int rc;
-  if (C.mom_mesh) {
+  if (mom_permuted(C)) {
     // the system is in colour-major order, and so is the unknown the
     // assembly copied into mx: solve for it, then bring it back
     mirror_options(C.mom_mesh, C.mesh);

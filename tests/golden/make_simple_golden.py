@@ -47,6 +47,16 @@ SIMPLE = {
     "tet_face_gradients": (("box_tet", (3, 3, 2), 0.2, 4),
                            dict(niter=8, gi_mom="averagecorrected", gi_cont="downwind")),
     "tet_decompose2": (("box_tet", (3, 3, 2), 0.2, 4), dict(niter=8, decomp=2, algorithm="simplec")),
+    # the reference's ILU for both solves: the device reproduces these bitwise
+    "hex_ilu": (("box_hex", (6, 5, 4), 0.15, 3), dict(niter=8, msolver=po.ILU, psolver=po.ILU)),
+    "tet_ilu_sou_simplec": (("box_tet", (3, 3, 2), 0.15, 4),
+                            dict(niter=8, msolver=po.ILU, psolver=po.ILU, framework="tvd", scheme="sou",
+                                 lim_mom=1, lim_cont=2, algorithm="simplec")),
+    "cavity2d_8_ilu": (("cavity_2d", 8), dict(niter=8, msolver=po.ILU, psolver=po.ILU, urf_u=0.9, urf_p=0.1)),
+    "hex_ilu_options": (("box_hex", (6, 5, 4), 0.2, 5),
+                        dict(niter=8, msolver=po.ILU, psolver=po.ILU, framework="nvf", scheme="gamma",
+                             decomp=1, grad_mom=(1, 2), grad_cont=(2, 3), gi_mom="averagecorrected",
+                             gi_cont="harmonic")),
     # converged states (the device is compared with these)
     "hex_converged": (("box_hex", (8, 7, 6), 0.15, 3), dict(niter=400)),
     "tet_converged": (("box_tet", (4, 4, 3), 0.15, 4), dict(niter=400)),

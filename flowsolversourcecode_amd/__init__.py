@@ -309,6 +309,8 @@ def _setup_case_symbols(L):
     L.fs_case_set_gradient_method.argtypes = [C.c_void_p] + [C.c_int] * 4
     L.fs_case_set_decompose.argtypes = [C.c_void_p, C.c_int]
     L.fs_case_set_gradient_interpolation.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.fs_case_set_restart_capture.argtypes = [C.c_void_p, C.c_int]
+    L.fs_case_write_solution.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _ip]
     L.fs_case_time_spmv.argtypes = [C.c_void_p, C.c_int, _dp]
     L._case_ready = True
 
@@ -461,6 +463,17 @@ class Case:
         codes = {"average": 0, "upwind": 1, "downwind": 2, "averagecorrected": 3, "harmonic": 4}
         _check(lib().fs_case_set_gradient_interpolation(self.h, codes[momentum], codes[continuity]),
                "fs_case_set_gradient_interpolation")
+
+    def set_restart_capture(self, on=True):
+        """Keep the state the restart file holds beyond the fields (on for
+        the iteration before write_solution)."""
+        _check(lib().fs_case_set_restart_capture(self.h, 1 if on else 0), "fs_case_set_restart_capture")
+
+    def write_solution(self, path, patch_sizes):
+        """The reference's restart file (writeSolution) of the current state."""
+        sizes = np.ascontiguousarray(patch_sizes, dtype=np.int32)
+        _check(lib().fs_case_write_solution(self.h, os.fsencode(path), len(sizes), _iptr(sizes)),
+               "fs_case_write_solution")
 
     def set_decompose(self, method):
         """MethodDecomposeS 1, 2 or 3 (default 3)."""

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdint>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -419,7 +421,7 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
                             double bleed, int gint, double rho, const double *__restrict__ ex,
                             const double *__restrict__ ey, const double *__restrict__ ez,
                             const double *__restrict__ dcf, double *__restrict__ cf, double *__restrict__ ff,
-                            double *__restrict__ tf3) {
+                            double *__restrict__ tf3, double *__restrict__ fg_out) {
   const int f = xcd_block() * kBlock + threadIdx.x;
   if (f >= Fi) return;
   const int i = ld_mat<kStreamFaces>(&own[f]), j = ld_mat<kStreamFaces>(&nei[f]);
@@ -449,6 +451,11 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
       face_grad(gint, gf, mdf, rho, gi[k][0], gi[k][1], gi[k][2], gj[k][0], gj[k][1], gj[k][2], P[k][i], P[k][j],
                 dd, e0, e1, e2, fgv[k][0], fgv[k][1], fgv[k][2]);
   }
+  if (fg_out) // the face gradients of the restart file, component-major
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fg_out[(size_t)(3 * k + d) * Fi + f] = fgv[k][d];
   const double gamf = gf * mu + (1. - gf) * mu;
   const double gd = ld_mat<kStreamFaces>(&gdiff[f]);
   const double cfl = gamf * gd, ffl = -(gamf * gd);
@@ -594,7 +601,7 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
                            const double *__restrict__ vol, double urf, const int *__restrict__ perm,
                            double *__restrict__ ac, double *__restrict__ anb, double *__restrict__ bc,
                            double *__restrict__ D, double *__restrict__ star, double *__restrict__ xq,
-                           double *__restrict__ D2) {
+                           double *__restrict__ D2, double *__restrict__ eff_out) {
   const int c = xcd_block() * kBlock + threadIdx.x;
   if (c >= N) return;
   // the system row of cell c: its colour-major position when perm is given
@@ -625,6 +632,7 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
       anb[(size_t)s * N + q] = mom_slot(role, cf[f], ff[f], tf[f], md, a, b, e);
     }
   }
+  if (eff_out) eff_out[c] = e; // effdiv of the restart file
   const double pc = phi[c];
   const double ep = e > 0.0 ? e : 0.0;
   const double fce = 0.0 + (ep - e);
@@ -657,6 +665,7 @@ struct RcArgs {
   const double *u, *v, *w, *us, *vs, *ws, *du, *dv, *dw, *gp, *p;
   const double *du2, *dv2, *dw2; // SIMPLEC's D2 (the p' coefficients)
   int gint;                      // GradientInterpolationSchemeContinuity
+  double *fgp_out;               // the pressure face gradient of the restart file, or nullptr
 };
 
 // geoDiff = |D_E| / d_CF of the face's D S split by MethodDecomposeS DEC
@@ -737,6 +746,11 @@ __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
   }
   fv = fv - rhof * (fgx * dut + fgy * dvt + fgz * dwt);
   fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
+  if (a.fgp_out) {
+    a.fgp_out[f] = fgx;
+    a.fgp_out[(size_t)Fi + f] = fgy;
+    a.fgp_out[2 * (size_t)Fi + f] = fgz;
+  }
   const double mstar = rhof * (usf * sx + vsf * sy + wsf * sz);
   fv = fv + (1. - a.urf_u) * (mdot[f] - mstar);
   const double m = cfl * a.p[i] + ffl * a.p[j] + fv;
@@ -906,6 +920,11 @@ struct SimpleCase {
   int gi_mom = FS_GRADINT_AVERAGE, gi_cont = FS_GRADINT_AVERAGECORRECTED; // face-gradient schemes
   std::vector<int> h_own;          // owner of every face (host copy, 0-based)
   bool simplec = false;            // Algorithm 'simplec': D2 in du2/dv2/dw2
+  // restart-file state the iteration forms but does not keep: the face
+  // gradients of u, v, w and p and the effective divergence
+  bool capture = false;
+  int captured_iter = -1;          // iteration whose state the buffers hold
+  DevBuf<double> cap_fgm, cap_fgp, cap_eff;
   DevBuf<double> du2, dv2, dw2;
   // momentum systems in colour-major cell order (the colour classes of the
   // multicolour Gauss-Seidel contiguous): their solver context on the
@@ -1031,7 +1050,7 @@ static void phase_momentum_gradients(SimpleCase &C) {
                        C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p, C.Sx.p, C.Sy.p, C.Sz.p, C.mu,
                        (double)(-2.0f / 3.0f), C.u.p, C.v.p, C.w.p, C.gu.p, C.gv.p, C.gw.p, C.mdot.p, C.dx.p,
                        C.dy.p, C.dz.p, C.hr_scheme, C.bleed, C.gi_mom, C.rho, C.ex.p, C.ey.p, C.ez.p, C.d.p,
-                       C.cf.p, C.ff.p, C.tf3.p);
+                       C.cf.p, C.ff.p, C.tf3.p, C.capture ? C.cap_fgm.p : nullptr);
   };
   if (hrf == FS_HR_TVD) face3(k_mom_face3<FS_HR_TVD>);
   else if (hrf == FS_HR_NVF) face3(k_mom_face3<FS_HR_NVF>);
@@ -1069,7 +1088,8 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
     hipLaunchKernelGGL(k_mom_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
                        C.code.p, C.cf.p, C.ff.p, tf, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
                        C.urf_u, perm ? C.mperm.p : nullptr, C.ac.p, C.anb.p, C.bc.p, D, st, C.mx.p,
-                       C.simplec ? (comp == 0 ? C.du2.p : comp == 1 ? C.dv2.p : C.dw2.p) : nullptr);
+                       C.simplec ? (comp == 0 ? C.du2.p : comp == 1 ? C.dv2.p : C.dw2.p) : nullptr,
+                       C.capture ? C.cap_eff.p : nullptr);
   });
 }
 
@@ -1135,6 +1155,7 @@ static void phase_pc_assemble(SimpleCase &C) {
   a.p = C.p.p;
   a.du2 = C.du2.p; a.dv2 = C.dv2.p; a.dw2 = C.dw2.p;
   a.gint = C.gi_cont;
+  a.fgp_out = C.capture ? C.cap_fgp.p : nullptr;
   auto rc = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p, C.nei.p, C.g.p,
                        C.Sx.p, C.Sy.p, C.Sz.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, a, C.mdot.p, C.cf.p, C.ff.p,
@@ -1226,6 +1247,7 @@ int simple_iterate(SimpleCase &C, int niter, fs_simple_stats *st) {
   if (C.sym_dirty) update_symmetry(C); // the start-up UpdateSymmetry (MainProgram.for:128)
   for (int it = 0; it < niter; ++it) {
     C.iter = C.iter + 1; // nIter (MainProgram.for:257)
+    if (C.capture) C.captured_iter = C.iter;
     phase_momentum_gradients(C);
     for (int comp = 0; comp < 3; ++comp) {
       phase_momentum_assemble(C, comp);
@@ -1576,6 +1598,196 @@ int fs_case_set_gradient_method(fs_case *cs, int momentum_method, int momentum_i
   C.ngrad_mom = momentum_method == 1 ? 0 : momentum_iters;
   C.ngrad_cont = continuity_method == 1 ? 0 : continuity_iters;
   return 0;
+}
+
+int fs_case_set_restart_capture(fs_case *cs, int on) {
+  if (!cs) {
+    set_error("fs_case_set_restart_capture: null case");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  C.capture = on != 0;
+  C.captured_iter = -1;
+  if (C.capture) {
+    C.cap_fgm.alloc((size_t)9 * C.Fi);
+    C.cap_fgp.alloc((size_t)3 * C.Fi);
+    C.cap_eff.alloc(C.N);
+  }
+  return 0;
+  CASE_GUARD_END
+}
+
+namespace {
+// A Fortran unformatted sequential file: each record framed by its byte
+// length as a 4-byte integer before and after.
+struct RecordWriter {
+  std::vector<char> out;
+  void rec(const std::vector<const std::vector<double> *> &parts) {
+    size_t n = 0;
+    for (auto *p : parts) n += p->size() * sizeof(double);
+    if (n > 0x7fffffffULL) throw std::runtime_error("restart record above 2 GiB");
+    const int32_t len = (int32_t)n;
+    const char *lp = reinterpret_cast<const char *>(&len);
+    out.insert(out.end(), lp, lp + 4);
+    for (auto *p : parts) {
+      const char *d = reinterpret_cast<const char *>(p->data());
+      out.insert(out.end(), d, d + p->size() * sizeof(double));
+    }
+    out.insert(out.end(), lp, lp + 4);
+  }
+};
+} // namespace
+
+int fs_case_write_solution(fs_case *cs, const char *path, int npatches, const int *patch_nfaces) {
+  if (!cs || !path || npatches <= 0 || !patch_nfaces) {
+    set_error("fs_case_write_solution: invalid arguments");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  if (!C.capture || C.captured_iter != C.iter || C.iter < 1) {
+    set_error("fs_case_write_solution: the last iteration must run with fs_case_set_restart_capture on");
+    return -1;
+  }
+  const size_t N = C.N, Fi = C.Fi, Fb = C.Fb;
+  long long tot = 0;
+  int nbfmax = 0;
+  for (int q = 0; q < npatches; ++q) {
+    if (patch_nfaces[q] < 0) {
+      set_error("fs_case_write_solution: negative patch size");
+      return -1;
+    }
+    tot += patch_nfaces[q];
+    nbfmax = std::max(nbfmax, patch_nfaces[q]);
+  }
+  if (tot != (long long)Fb) {
+    set_error("fs_case_write_solution: patches hold " + std::to_string(tot) + " faces, the boundary " +
+              std::to_string(Fb));
+    return -1;
+  }
+  auto get = [&](const fs::DevBuf<double> &b, size_t n) {
+    std::vector<double> h(n);
+    if (n) FS_HIP_THROW(hipMemcpyAsync(h.data(), b.p, sizeof(double) * n, hipMemcpyDeviceToHost, C.s));
+    return h;
+  };
+  // the constant-density gradient of the start-up's CalculateFaceDensity
+  // (ComputeFaceDensity.for: Gradient with MethodCalcGradientDensity = 2,
+  // nIterGradientDensity = 2, no limiter)
+  fs::DevBuf<double> rc, rb, rg;
+  rc.upload(std::vector<double>(N, C.rho), C.s);
+  rb.upload(std::vector<double>(Fb, C.rho), C.s);
+  rg.alloc(3 * N);
+  {
+    fs::GradArgs a{};
+    a.phi[0] = rc.p;
+    a.bval[0] = rb.p;
+    a.grad[0] = rg.p;
+    fs::gradients<1>(C, a, 2);
+  }
+  const auto u = get(C.u, N), v = get(C.v, N), w = get(C.w, N), p = get(C.p, N), mdot = get(C.mdot, Fi);
+  const auto gu = get(C.gu, 3 * N), gv = get(C.gv, 3 * N), gw = get(C.gw, 3 * N), gp = get(C.gp, 3 * N);
+  const auto bu = get(C.bu, Fb), bv = get(C.bv, Fb), bw = get(C.bw, Fb), bp = get(C.bp, Fb), bgp = get(C.bgp, 3 * Fb);
+  const auto du = get(C.du, N), dv = get(C.dv, N), dw = get(C.dw, N);
+  const auto us = get(C.us, N), vs = get(C.vs, N), ws = get(C.ws, N);
+  const auto du2 = C.simplec ? get(C.du2, N) : du, dv2 = C.simplec ? get(C.dv2, N) : dv,
+             dw2 = C.simplec ? get(C.dw2, N) : dw;
+  const auto fgm = get(C.cap_fgm, 9 * Fi), fgp = get(C.cap_fgp, 3 * Fi), eff = get(C.cap_eff, N);
+  const auto dg = get(rg, 3 * N);
+  FS_HIP(hipStreamSynchronize(C.s));
+  const size_t nbc = npatches, nbm = nbfmax, P = nbc * nbm;
+  // boundary face b of patch q at record j: (q, j) of a (NumberOfBCSets,
+  // NBFacesMax) array, column-major; unused entries 0
+  std::vector<int> bq(Fb), bj(Fb);
+  {
+    size_t b = 0;
+    for (int q = 0; q < npatches; ++q)
+      for (int j = 0; j < patch_nfaces[q]; ++j, ++b) {
+        bq[b] = q;
+        bj[b] = j;
+      }
+  }
+  auto pad = [&](auto value) {
+    std::vector<double> a(P, 0.0);
+    for (size_t b = 0; b < Fb; ++b) a[bq[b] + (size_t)bj[b] * nbc] = value(b);
+    return a;
+  };
+  auto slice = [](const std::vector<double> &a, size_t off, size_t n) {
+    return std::vector<double>(a.begin() + off, a.begin() + off + n);
+  };
+  const std::vector<int> &own = C.h_own;
+  auto owner_copy = [&](const std::vector<double> &g, size_t d) {
+    return pad([&](size_t b) { return g[d * N + own[Fi + b]]; });
+  };
+  const std::vector<double> zN(N, 0.0), zP(P, 0.0), z3nbc(3 * nbc, 0.0), muN(N, C.mu), rhoN(N, C.rho),
+      rhoFi(Fi, C.rho);
+  // BViscosity holds mu on the boundary faces only; BDensity is
+  // ConstantDensity over the whole array, padding included (it is filled
+  // whole at allocation, AllocateStorageRoutine.for:1929-1930)
+  const auto muP = pad([&](size_t) { return C.mu; });
+  const std::vector<double> rhoP(P, C.rho);
+  RecordWriter R;
+  // LSolveMomentum (WriteSolutionToContinue.for:24-46)
+  R.rec({&u, &v, &w});
+  R.rec({&gu});
+  R.rec({&gv});
+  R.rec({&gw});
+  const auto pu = pad([&](size_t b) { return bu[b]; }), pv = pad([&](size_t b) { return bv[b]; }),
+             pw = pad([&](size_t b) { return bw[b]; });
+  R.rec({&pu, &pv, &pw});
+  for (const auto *g : {&gu, &gv, &gw}) {
+    const auto x = owner_copy(*g, 0), y = owner_copy(*g, 1), z = owner_copy(*g, 2);
+    R.rec({&x, &y, &z});
+  }
+  for (int k = 0; k < 3; ++k) {
+    const auto f = slice(fgm, (size_t)3 * k * Fi, 3 * Fi);
+    R.rec({&f});
+  }
+  R.rec({&zP, &zP, &zP});           // x/y/zVeldirection
+  R.rec({&muN});                    // Viscosity
+  R.rec({&muP});                    // BViscosity
+  for (int k = 0; k < 3; ++k) R.rec({&zN, &zN}); // ScMomentum, SbMomentum
+  R.rec({&mdot, &zP, &eff});        // mdot, Bmdot (walls and symmetry planes carry none), effdiv
+  R.rec({&du, &du2, &us});
+  R.rec({&dv, &dv2, &vs});
+  R.rec({&dw, &dw2, &ws});
+  R.rec({&zN, &zP});                // MachNumber, BMachNumber
+  // LSolveContinuity (:61-66)
+  R.rec({&p});
+  const auto pp = pad([&](size_t b) { return bp[b]; });
+  R.rec({&pp});
+  R.rec({&gp});
+  {
+    const auto x = pad([&](size_t b) { return bgp[b]; }), y = pad([&](size_t b) { return bgp[Fb + b]; }),
+               z = pad([&](size_t b) { return bgp[2 * Fb + b]; });
+    R.rec({&x, &y, &z});
+  }
+  R.rec({&fgp});
+  // density (:173-175), effective diffusion coefficient (:207-208)
+  {
+    const auto gx = slice(dg, 0, N), gy = slice(dg, N, N), gz = slice(dg, 2 * N, N);
+    R.rec({&rhoN, &gx, &gy, &gz});
+    const auto bx = owner_copy(dg, 0), by = owner_copy(dg, 1), bz = owner_copy(dg, 2);
+    R.rec({&rhoP, &bx, &by, &bz});
+  }
+  R.rec({&rhoFi});                  // Densityf
+  R.rec({&muN});                    // eDiffCoefficient
+  R.rec({&muP});                    // BeDiffCoefficient
+  R.rec({&z3nbc});                  // MassFlowRate, Old, OldOld
+  R.rec({&z3nbc});                  // OutletPressure, Old, OldOld
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) {
+    set_error(std::string("fs_case_write_solution: cannot open ") + path);
+    return -1;
+  }
+  const size_t wr = std::fwrite(R.out.data(), 1, R.out.size(), fp);
+  const int cl = std::fclose(fp);
+  if (wr != R.out.size() || cl != 0) {
+    set_error(std::string("fs_case_write_solution: write failed for ") + path);
+    return -1;
+  }
+  return 0;
+  CASE_GUARD_END
 }
 
 int fs_case_set_gradient_interpolation(fs_case *cs, int momentum, int continuity) {

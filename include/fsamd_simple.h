@@ -158,6 +158,16 @@ enum fs_gradient_interpolation {
   FS_GRADINT_HARMONIC = 4
 };
 int fs_case_set_gradient_interpolation(fs_case *c, int momentum, int continuity);
+/* The reference's restart file (writeSolution, WriteSolutionToContinue.for:3,
+ * the records a steady, laminar, incompressible flow writes): a Fortran
+ * unformatted sequential file of the state after the last iteration, its
+ * boundary arrays laid out (NumberOfBCSets, NBFacesMax) for the patch sizes
+ * given.  It holds face gradients and the effective divergence that an
+ * iteration forms but does not keep: fs_case_set_restart_capture(c, 1)
+ * keeps them (9 + 3 face arrays and one cell array), and the last iteration
+ * before fs_case_write_solution must run with it on. */
+int fs_case_set_restart_capture(fs_case *c, int on);
+int fs_case_write_solution(fs_case *c, const char *path, int npatches, const int *patch_nfaces);
 int fs_case_set_algorithm(fs_case *c, int algorithm);
 /* urfMomentum, urfPressure, ASSolverMomentum/rrFMomentum/ASIterMomentum/
  * LMultigridMomentum and the Continuity equivalents (User0.F90:271-318). */

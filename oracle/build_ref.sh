@@ -55,7 +55,8 @@ AR_OBJS=()
 for d in Assemble Buoyancy ContinuityEquation Convection Diffusion Energy FalseTransient GradientFiles \
          InitializeFluxesAndCoefficients Interpolation Momentum Periodicity PhysicalProperties \
          SetBoundaryPatches SurfaceTension Transient UnderRelaxVariables UpdateVariablesAtBoundaries \
-         UpdateVariablesFromBoundaryConditions NormalAndTangentialVelocities Tensors FarFieldVariables; do
+         UpdateVariablesFromBoundaryConditions NormalAndTangentialVelocities Tensors FarFieldVariables \
+         WriteAndReadSolution; do
   for f in "$REF/SolveEquations/$d"/*; do AR_SRCS+=("$f"); done
 done
 AR_SRCS+=("$REF/SolveEquations/GridAndGeometry/ProcessGeometryRoutine.for")

@@ -297,7 +297,7 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
                          urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1, mitmax=20, psolver=SOR, prrf=0.01,
                          pitmax=200, framework="none", scheme="upwind", bleed=0.0, start_iter=0, lim_mom=0,
                          lim_cont=0, plmg=0, algorithm="simple", grad_mom=(2, 2), grad_cont=(2, 2),
-                         decomp=3, gi_mom="average", gi_cont="averagecorrected", seconds=False):
+                         decomp=3, gi_mom="average", gi_cont="averagecorrected", restart=False, seconds=False):
     """niter outer iterations of the reference's own SIMPLE (SolveMomentumX/Y/Z,
     SolveContinuity) from rest on the polyMesh ``pm`` with the Readit topology of
     ``mesh``; patch_types per patch "wall"/"symmetry" (default all walls);
@@ -325,14 +325,28 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
         if r.returncode != 0:
             raise RuntimeError(f"reference run failed ({r.returncode}): {r.stdout[-1500:]} {r.stderr[-1500:]}")
         d = np.fromfile(fout, dtype=np.float64)
-    o = len(REF_GEOM_FACE) * nf + nif + 4 * N
-    out = {}
-    for k, n in (("u", N), ("v", N), ("w", N), ("p", N), ("mdot", nif)):
-        out[k] = d[o:o + n].copy()
-        o += n
+        o = len(REF_GEOM_FACE) * nf + nif + 4 * N
+        out = {}
+        for k, n in (("u", N), ("v", N), ("w", N), ("p", N), ("mdot", nif)):
+            out[k] = d[o:o + n].copy()
+            o += n
+        if restart:
+            out["restart"] = open(fout + ".restart", "rb").read()
     if seconds:
         out["seconds"] = el
     return out
+
+
+def fortran_records(blob):
+    """The records of a Fortran unformatted sequential file (4-byte length
+    markers before and after each record)."""
+    recs, o = [], 0
+    while o < len(blob):
+        n = int(np.frombuffer(blob, np.int32, 1, o)[0])
+        recs.append(blob[o + 4:o + 4 + n])
+        assert int(np.frombuffer(blob, np.int32, 1, o + 4 + n)[0]) == n
+        o += n + 8
+    return recs
 
 
 def _unflatten_mg(L, ne, ije, ijn, parents, nchild, eparent, nnbr_mg, acmg, enbr, children, anbmg):

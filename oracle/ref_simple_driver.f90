@@ -155,6 +155,11 @@ program ref_simple_driver
   if (mode == 2) then
     call run_simple
     write(22) uVelocity, vVelocity, wVelocity, Pressure, mdot
+    ! the reference's restart file (WriteSolutionToContinue.for:3) of the
+    ! final state, on unit 4 as MainProgram.for:466 opens it
+    open(unit=4, file=trim(fout)//'.restart', form='unformatted', status='replace')
+    call writeSolution
+    close(4)
   end if
   close(22)
 

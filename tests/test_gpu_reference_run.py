@@ -216,6 +216,22 @@ def test_device_iterations_bitwise_the_reference_run(name):
     dc.set_gradient_method(kw.get("grad_mom", (2, 2)), kw.get("grad_cont", (2, 2)))
     dc.set_decompose(kw.get("decomp", 3))
     dc.set_gradient_interpolation(kw.get("gi_mom", "average"), kw.get("gi_cont", "averagecorrected"))
-    dc.iterate(kw["niter"])
+    dc.iterate(kw["niter"] - 1)
+    dc.set_restart_capture(True)
+    dc.iterate(1)
     for k in ("u", "v", "w", "p", "mdot"):
         assert np.array_equal(dc.get(k), g[k]), (name, k, np.abs(dc.get(k) - g[k]).max())
+    # the restart file of the final state, byte for byte the reference's
+    # writeSolution (WriteSolutionToContinue.for:3)
+    import tempfile
+    import pyoracle as po
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "restart.bin")
+        dc.write_solution(path, pm.patch_sizes)
+        mine = open(path, "rb").read()
+    ref = g["restart"].tobytes()
+    rm, rr = po.fortran_records(mine), po.fortran_records(ref)
+    assert len(rm) == len(rr), (len(rm), len(rr))
+    bad = [k for k in range(len(rr)) if rm[k] != rr[k]]
+    assert not bad, (name, bad)
+    assert mine == ref

@@ -311,6 +311,7 @@ def _setup_case_symbols(L):
     L.fs_case_set_gradient_interpolation.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.fs_case_set_restart_capture.argtypes = [C.c_void_p, C.c_int]
     L.fs_case_write_solution.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _ip]
+    L.fs_case_read_solution.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _ip]
     L.fs_case_time_spmv.argtypes = [C.c_void_p, C.c_int, _dp]
     L._case_ready = True
 
@@ -474,6 +475,12 @@ class Case:
         sizes = np.ascontiguousarray(patch_sizes, dtype=np.int32)
         _check(lib().fs_case_write_solution(self.h, os.fsencode(path), len(sizes), _iptr(sizes)),
                "fs_case_write_solution")
+
+    def read_solution(self, path, patch_sizes):
+        """Resume from a restart file in the reference's layout."""
+        sizes = np.ascontiguousarray(patch_sizes, dtype=np.int32)
+        _check(lib().fs_case_read_solution(self.h, os.fsencode(path), len(sizes), _iptr(sizes)),
+               "fs_case_read_solution")
 
     def set_decompose(self, method):
         """MethodDecomposeS 1, 2 or 3 (default 3)."""

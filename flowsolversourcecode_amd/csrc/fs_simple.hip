@@ -1790,6 +1790,109 @@ int fs_case_write_solution(fs_case *cs, const char *path, int npatches, const in
   CASE_GUARD_END
 }
 
+int fs_case_read_solution(fs_case *cs, const char *path, int npatches, const int *patch_nfaces) {
+  if (!cs || !path || npatches <= 0 || !patch_nfaces) {
+    set_error("fs_case_read_solution: invalid arguments");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  const size_t N = C.N, Fi = C.Fi, Fb = C.Fb;
+  long long tot = 0;
+  int nbfmax = 0;
+  for (int q = 0; q < npatches; ++q) {
+    if (patch_nfaces[q] < 0) {
+      set_error("fs_case_read_solution: negative patch size");
+      return -1;
+    }
+    tot += patch_nfaces[q];
+    nbfmax = std::max(nbfmax, patch_nfaces[q]);
+  }
+  if (tot != (long long)Fb) {
+    set_error("fs_case_read_solution: patches do not cover the boundary");
+    return -1;
+  }
+  FILE *fp = std::fopen(path, "rb");
+  if (!fp) {
+    set_error(std::string("fs_case_read_solution: cannot open ") + path);
+    return -1;
+  }
+  std::vector<char> blob;
+  {
+    char buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, fp)) > 0) blob.insert(blob.end(), buf, buf + n);
+    std::fclose(fp);
+  }
+  // the records, each framed by its 4-byte length
+  std::vector<std::pair<size_t, size_t>> recs; // offset, bytes
+  for (size_t o = 0; o < blob.size();) {
+    int32_t n0, n1;
+    if (o + 4 > blob.size()) break;
+    std::memcpy(&n0, &blob[o], 4);
+    if (n0 < 0 || o + 8 + (size_t)n0 > blob.size()) {
+      set_error("fs_case_read_solution: truncated record");
+      return -1;
+    }
+    std::memcpy(&n1, &blob[o + 4 + n0], 4);
+    if (n1 != n0) {
+      set_error("fs_case_read_solution: record markers disagree");
+      return -1;
+    }
+    recs.push_back({o + 4, (size_t)n0});
+    o += 8 + (size_t)n0;
+  }
+  const size_t P = (size_t)npatches * nbfmax;
+  // the laminar steady layout of fs_case_write_solution
+  const size_t expect[] = {3 * N, 3 * N, 3 * N, 3 * N, 3 * P, 3 * P, 3 * P, 3 * P, 3 * Fi, 3 * Fi, 3 * Fi, 3 * P,
+                           N, P, 2 * N, 2 * N, 2 * N, Fi + P + N, 3 * N, 3 * N, 3 * N, N + P, N, P, 3 * N,
+                           3 * P, 3 * Fi, 4 * N, 4 * P, Fi, N, P, 3 * (size_t)npatches, 3 * (size_t)npatches};
+  const size_t nrec = sizeof(expect) / sizeof(expect[0]);
+  if (recs.size() != nrec) {
+    set_error("fs_case_read_solution: " + std::to_string(recs.size()) + " records, a steady laminar flow has " +
+              std::to_string(nrec));
+    return -1;
+  }
+  for (size_t k = 0; k < nrec; ++k)
+    if (recs[k].second != expect[k] * sizeof(double)) {
+      set_error("fs_case_read_solution: record " + std::to_string(k) + " does not fit this mesh and patches");
+      return -1;
+    }
+  auto at = [&](size_t k, size_t off) { return reinterpret_cast<const double *>(&blob[recs[k].first]) + off; };
+  std::vector<int> bq(Fb), bj(Fb);
+  {
+    size_t b = 0;
+    for (int q = 0; q < npatches; ++q)
+      for (int j = 0; j < patch_nfaces[q]; ++j, ++b) {
+        bq[b] = q;
+        bj[b] = j;
+      }
+  }
+  auto unpad = [&](const double *a) {
+    std::vector<double> v(Fb);
+    for (size_t b = 0; b < Fb; ++b) v[b] = a[bq[b] + (size_t)bj[b] * npatches];
+    return v;
+  };
+  auto put = [&](fs::DevBuf<double> &d, const double *h, size_t n) {
+    FS_HIP_THROW(hipMemcpyAsync(d.p, h, sizeof(double) * n, hipMemcpyHostToDevice, C.s));
+  };
+  put(C.u, at(0, 0), N);
+  put(C.v, at(0, N), N);
+  put(C.w, at(0, 2 * N), N);
+  const auto bu = unpad(at(4, 0)), bv = unpad(at(4, P)), bw = unpad(at(4, 2 * P));
+  put(C.bu, bu.data(), Fb);
+  put(C.bv, bv.data(), Fb);
+  put(C.bw, bw.data(), Fb);
+  put(C.mdot, at(17, 0), Fi);
+  put(C.p, at(22, 0), N);
+  const auto bp = unpad(at(23, 0));
+  put(C.bp, bp.data(), Fb);
+  FS_HIP(hipStreamSynchronize(C.s));
+  C.sym_dirty = false; // the boundary velocities come from the file
+  return 0;
+  CASE_GUARD_END
+}
+
 int fs_case_set_gradient_interpolation(fs_case *cs, int momentum, int continuity) {
   if (!cs || momentum < FS_GRADINT_AVERAGE || momentum > FS_GRADINT_HARMONIC || continuity < FS_GRADINT_AVERAGE ||
       continuity > FS_GRADINT_HARMONIC) {

@@ -168,6 +168,11 @@ int fs_case_set_gradient_interpolation(fs_case *c, int momentum, int continuity)
  * before fs_case_write_solution must run with it on. */
 int fs_case_set_restart_capture(fs_case *c, int on);
 int fs_case_write_solution(fs_case *c, const char *path, int npatches, const int *patch_nfaces);
+/* Resume from such a file (readSolution, WriteSolutionToContinue.for): the
+ * velocities, pressure, mass fluxes and boundary values of u, v, w and p;
+ * everything else is formed again by the next iteration.  Fails unless the
+ * file has the steady laminar layout for this mesh and these patches. */
+int fs_case_read_solution(fs_case *c, const char *path, int npatches, const int *patch_nfaces);
 int fs_case_set_algorithm(fs_case *c, int algorithm);
 /* urfMomentum, urfPressure, ASSolverMomentum/rrFMomentum/ASIterMomentum/
  * LMultigridMomentum and the Continuity equivalents (User0.F90:271-318). */

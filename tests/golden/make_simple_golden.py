@@ -49,6 +49,7 @@ SIMPLE = {
     "tet_decompose2": (("box_tet", (3, 3, 2), 0.2, 4), dict(niter=8, decomp=2, algorithm="simplec")),
     # the reference's ILU for both solves: the device reproduces these bitwise
     "hex_ilu": (("box_hex", (6, 5, 4), 0.15, 3), dict(niter=8, msolver=po.ILU, psolver=po.ILU)),
+    "hex_ilu_half": (("box_hex", (6, 5, 4), 0.15, 3), dict(niter=4, msolver=po.ILU, psolver=po.ILU)),
     "tet_ilu_sou_simplec": (("box_tet", (3, 3, 2), 0.15, 4),
                             dict(niter=8, msolver=po.ILU, psolver=po.ILU, framework="tvd", scheme="sou",
                                  lim_mom=1, lim_cont=2, algorithm="simplec")),

@@ -1402,13 +1402,16 @@ static void gs_sweeps(fs_mesh *m, DLevel &D, const T *b, T *x, int sweeps, const
   hipStream_t s = m->stream;
   const T *ac = LV<T>::ac(D), *anb = LV<T>::anb(D);
   const int nc = (int)D.color_ptr.size() - 1;
+  // symmetric sweeps; a colour is an independent set, so a colour swept
+  // twice in a row (the last of a forward pass, the first of the next
+  // forward pass) would recompute the same values and is swept once
   for (int sw = 0; sw < sweeps; ++sw) {
-    for (int c = 0; c < nc; ++c) {
+    for (int c = sw == 0 ? 0 : 1; c < nc; ++c) {
       const int bb = D.color_ptr[c], e = D.color_ptr[c + 1];
       hipLaunchKernelGGL(k_gsc<T>, dim3(gfor(e - bb)), dim3(kBlock), 0, s, e - bb, D.color_cells.p + bb, D.n,
                          D.w, D.nbr, ac, anb, b, x, flags);
     }
-    for (int c = nc - 1; c >= 0; --c) {
+    for (int c = nc - 2; c >= 0; --c) {
       const int bb = D.color_ptr[c], e = D.color_ptr[c + 1];
       hipLaunchKernelGGL(k_gsc<T>, dim3(gfor(e - bb)), dim3(kBlock), 0, s, e - bb, D.color_cells.p + bb, D.n,
                          D.w, D.nbr, ac, anb, b, x, flags);

@@ -1263,19 +1263,26 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });
   } else if (solver == FS_SOR) {
-    // multicolour symmetric Gauss-Seidel: colours forward then backward
+    // multicolour symmetric Gauss-Seidel: colours forward then backward.  A
+    // colour is an independent set, so its update reads only the other
+    // colours, and a colour swept twice in a row would recompute the same
+    // values bitwise: the backward pass starts at colour nc-2, and every
+    // forward pass after the first at colour 1 (the residual check between
+    // iterations leaves x alone).
+    bool first = true;
     iters = run_batches(m, itmax, nf * 8 + solver, [&]() {
       const int nc = (int)sw.color_ptr.size() - 1;
-      for (int c = 0; c < nc; ++c) {
+      for (int c = first ? 0 : 1; c < nc; ++c) {
         const int b = sw.color_ptr[c], e = sw.color_ptr[c + 1];
         hipLaunchKernelGGL(k_gs_color, dim3(grid_for(e - b)), dim3(kBlock), 0, s, e - b,
                            sw.color_cells.p + b, n, w, nb, ac, anb, bc, dphi, flags);
       }
-      for (int c = nc - 1; c >= 0; --c) {
+      for (int c = nc - 2; c >= 0; --c) {
         const int b = sw.color_ptr[c], e = sw.color_ptr[c + 1];
         hipLaunchKernelGGL(k_gs_color, dim3(grid_for(e - b)), dim3(kBlock), 0, s, e - b,
                            sw.color_cells.p + b, n, w, nb, ac, anb, bc, dphi, flags);
       }
+      first = false;
       hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
     });

@@ -1828,7 +1828,10 @@ int fs_case_read_solution(fs_case *cs, const char *path, int npatches, const int
   std::vector<std::pair<size_t, size_t>> recs; // offset, bytes
   for (size_t o = 0; o < blob.size();) {
     int32_t n0, n1;
-    if (o + 4 > blob.size()) break;
+    if (o + 4 > blob.size()) {
+      set_error("fs_case_read_solution: truncated record marker");
+      return -1;
+    }
     std::memcpy(&n0, &blob[o], 4);
     if (n0 < 0 || o + 8 + (size_t)n0 > blob.size()) {
       set_error("fs_case_read_solution: truncated record");

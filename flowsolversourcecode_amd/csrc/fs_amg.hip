@@ -458,28 +458,36 @@ k_resid_sell(int nslots, const int *__restrict__ ptr, const int *__restrict__ ro
 }
 
 // Residual of a coarse level summed straight into the next level's
-// right-hand side: b_c[p] = sum over the members i of p (in increasing
-// order) of (b_i - A_i x); the coarse level's first sweep as k_restrict
-// (mode 1: x_c = omega b_c/ac_c, 2: x_c = 0, 0: untouched).  LPR lanes per
-// coarse cell work through each member row together.
-template <int LPR, class T>
+// right-hand side: b_c[p] = sum over the members i of p of (b_i - A_i x);
+// the coarse level's first sweep as k_restrict (mode 1: x_c = omega
+// b_c/ac_c, 2: x_c = 0, 0: untouched).  Each coarse cell has MPR groups of
+// LPR lanes (all in one wavefront): group g works through members g, g+MPR,
+// ... in increasing order, its LPR lanes through each member row together,
+// and the groups' sums are combined in a fixed butterfly, so the result is
+// deterministic and a coarse cell waits on ceil(members / MPR) member rows
+// instead of all of them.
+template <int LPR, int MPR, class T>
 __global__ void __launch_bounds__(kBlock)
 k_resid_restrict_csr(int nc, const int *__restrict__ mptr, const int *__restrict__ mem, const int *__restrict__ rp,
                      const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ ac,
                      const T *__restrict__ b, const T *__restrict__ x, T *__restrict__ bc, const T *__restrict__ acc,
                      T *__restrict__ xc, T omega, int mode, const int *__restrict__ flags) {
+  constexpr int G = LPR * MPR;
+  static_assert(64 % G == 0 && kBlock % 64 == 0, "a coarse cell's lanes must share a wavefront");
   if (flags && flags[0]) return;
   const int t = xcd_block() * kBlock + threadIdx.x;
-  const int p = t / LPR, lane = t % LPR;
+  const int p = t / G, lane = t % LPR, grp = (t % G) / LPR;
   const bool ok = p < nc;
   const int q0 = ok ? mptr[p] : 0, q1 = ok ? mptr[p + 1] : 0;
   T sum = T(0);
-  for (int q = q0; q < q1; ++q) { // the same trip count on the LPR lanes of p
+  for (int q = q0 + grp; q < q1; q += MPR) { // the same trip count on the LPR lanes of a group
     const int i = mem[q];
     const T s = csr_offdiag<LPR, T, false>(i, lane, rp, col, val, x, nullptr, nullptr, T(0), true);
     sum = sum + (b[i] - ac[i] * x[i] - s);
   }
-  if (ok && lane == 0) {
+#pragma unroll
+  for (int off = LPR; off < G; off <<= 1) sum += __shfl_xor(sum, off, G);
+  if (ok && lane == 0 && grp == 0) {
     bc[p] = sum;
     if (mode == 1) xc[p] = omega * sum / acc[p];
     else if (mode == 2) xc[p] = T(0);
@@ -1292,6 +1300,9 @@ static int csr_lpr(const DLevel &D) {
   return a <= 6.0 ? 4 : a <= 16.0 ? 8 : 16;
 }
 
+// member groups per coarse cell in the fused residual-restriction
+constexpr int kRestrictGroups = 4;
+
 // residual of coarse level D restricted into C (plus C's first sweep)
 template <class T>
 static void launch_resid_restrict(fs_mesh *m, DLevel &D, DLevel &C, const T *b, const T *x, T om, int mode,
@@ -1299,18 +1310,20 @@ static void launch_resid_restrict(fs_mesh *m, DLevel &D, DLevel &C, const T *b, 
   hipStream_t s = m->stream;
   const int *rp = D.csr_rp.p, *col = D.csr_col.p;
   const T *val = LV<T>::val(D), *ac = LV<T>::ac(D);
+  constexpr int M = kRestrictGroups;
   switch (csr_lpr(D)) {
   case 4:
-    hipLaunchKernelGGL((k_resid_restrict_csr<4, T>), dim3(gfor(C.n * 4)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p,
-                       rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    hipLaunchKernelGGL((k_resid_restrict_csr<4, M, T>), dim3(gfor(C.n * 4 * M)), dim3(kBlock), 0, s, C.n, C.mptr.p,
+                       C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
     break;
   case 8:
-    hipLaunchKernelGGL((k_resid_restrict_csr<8, T>), dim3(gfor(C.n * 8)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p,
-                       rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    hipLaunchKernelGGL((k_resid_restrict_csr<8, M, T>), dim3(gfor(C.n * 8 * M)), dim3(kBlock), 0, s, C.n, C.mptr.p,
+                       C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
     break;
   default:
-    hipLaunchKernelGGL((k_resid_restrict_csr<16, T>), dim3(gfor(C.n * 16)), dim3(kBlock), 0, s, C.n, C.mptr.p,
-                       C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    hipLaunchKernelGGL((k_resid_restrict_csr<16, M, T>), dim3(gfor(C.n * 16 * M)), dim3(kBlock), 0, s, C.n,
+                       C.mptr.p, C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode,
+                       flags);
   }
 }
 

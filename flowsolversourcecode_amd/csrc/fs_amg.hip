@@ -1300,8 +1300,10 @@ static int csr_lpr(const DLevel &D) {
   return a <= 6.0 ? 4 : a <= 16.0 ? 8 : 16;
 }
 
-// member groups per coarse cell in the fused residual-restriction
-constexpr int kRestrictGroups = 4;
+// member groups per coarse cell in the fused residual-restriction: up to
+// kRestrictGroups, as many as fit one wavefront with LPR lanes each
+constexpr int kRestrictGroups = 8;
+template <int LPR> constexpr int restrict_groups() { return kRestrictGroups < 64 / LPR ? kRestrictGroups : 64 / LPR; }
 
 // residual of coarse level D restricted into C (plus C's first sweep)
 template <class T>
@@ -1310,18 +1312,20 @@ static void launch_resid_restrict(fs_mesh *m, DLevel &D, DLevel &C, const T *b, 
   hipStream_t s = m->stream;
   const int *rp = D.csr_rp.p, *col = D.csr_col.p;
   const T *val = LV<T>::val(D), *ac = LV<T>::ac(D);
-  constexpr int M = kRestrictGroups;
+  constexpr int M4 = restrict_groups<4>(), M8 = restrict_groups<8>(), M16 = restrict_groups<16>();
   switch (csr_lpr(D)) {
   case 4:
-    hipLaunchKernelGGL((k_resid_restrict_csr<4, M, T>), dim3(gfor(C.n * 4 * M)), dim3(kBlock), 0, s, C.n, C.mptr.p,
-                       C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    hipLaunchKernelGGL((k_resid_restrict_csr<4, M4, T>), dim3(gfor(C.n * 4 * M4)), dim3(kBlock), 0, s, C.n,
+                       C.mptr.p, C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode,
+                       flags);
     break;
   case 8:
-    hipLaunchKernelGGL((k_resid_restrict_csr<8, M, T>), dim3(gfor(C.n * 8 * M)), dim3(kBlock), 0, s, C.n, C.mptr.p,
-                       C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    hipLaunchKernelGGL((k_resid_restrict_csr<8, M8, T>), dim3(gfor(C.n * 8 * M8)), dim3(kBlock), 0, s, C.n,
+                       C.mptr.p, C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode,
+                       flags);
     break;
   default:
-    hipLaunchKernelGGL((k_resid_restrict_csr<16, M, T>), dim3(gfor(C.n * 16 * M)), dim3(kBlock), 0, s, C.n,
+    hipLaunchKernelGGL((k_resid_restrict_csr<16, M16, T>), dim3(gfor(C.n * 16 * M16)), dim3(kBlock), 0, s, C.n,
                        C.mptr.p, C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode,
                        flags);
   }

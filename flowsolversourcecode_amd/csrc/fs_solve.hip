@@ -693,14 +693,33 @@ enum FinOp {
 };
 
 // one workgroup of kFinBlock threads combines up to kPartials partials
+// np <= kPartials = kFinR * kFinBlock: a thread's partials i, i + kFinBlock,
+// ... are loaded together, then folded in index order (the slots past np add
+// +0.0, or take the max with the starting -1.0, which changes nothing)
+constexpr int kFinR = kPartials / kFinBlock;
+static_assert(kFinR * kFinBlock == kPartials, "partials per finalise thread");
 __device__ double reduce_partials(const double *part, int np) {
+  double v[kFinR];
+#pragma unroll
+  for (int u = 0; u < kFinR; ++u) {
+    const int i = threadIdx.x + u * kFinBlock;
+    v[u] = i < np ? part[i] : 0.0;
+  }
   double s = 0.0;
-  for (int i = threadIdx.x; i < np; i += kFinBlock) s += part[i];
+#pragma unroll
+  for (int u = 0; u < kFinR; ++u) s += v[u];
   return block_sum<kFinBlock>(s);
 }
 __device__ double reduce_partials_max(const double *part, int np) {
+  double v[kFinR];
+#pragma unroll
+  for (int u = 0; u < kFinR; ++u) {
+    const int i = threadIdx.x + u * kFinBlock;
+    v[u] = i < np ? part[i] : -1.0;
+  }
   double s = -1.0;
-  for (int i = threadIdx.x; i < np; i += kFinBlock) s = fmax(s, part[i]);
+#pragma unroll
+  for (int u = 0; u < kFinR; ++u) s = fmax(s, v[u]);
   return block_max<kFinBlock>(s);
 }
 

@@ -1303,6 +1303,9 @@ static int csr_lpr(const DLevel &D) {
 // member groups per coarse cell in the fused residual-restriction: up to
 // kRestrictGroups, as many as fit one wavefront with LPR lanes each
 constexpr int kRestrictGroups = 8;
+// rows wider than 16 entries are restricted with 8 lanes per member row and
+// 8 member groups rather than 16 lanes and 4 groups
+constexpr bool kRestrictNarrow = true;
 template <int LPR> constexpr int restrict_groups() { return kRestrictGroups < 64 / LPR ? kRestrictGroups : 64 / LPR; }
 
 // residual of coarse level D restricted into C (plus C's first sweep)
@@ -1325,6 +1328,12 @@ static void launch_resid_restrict(fs_mesh *m, DLevel &D, DLevel &C, const T *b, 
                        flags);
     break;
   default:
+    if (kRestrictNarrow) {
+      hipLaunchKernelGGL((k_resid_restrict_csr<8, M8, T>), dim3(gfor(C.n * 8 * M8)), dim3(kBlock), 0, s, C.n,
+                         C.mptr.p, C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode,
+                         flags);
+      break;
+    }
     hipLaunchKernelGGL((k_resid_restrict_csr<16, M16, T>), dim3(gfor(C.n * 16 * M16)), dim3(kBlock), 0, s, C.n,
                        C.mptr.p, C.mem.p, rp, col, val, ac, b, x, LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode,
                        flags);

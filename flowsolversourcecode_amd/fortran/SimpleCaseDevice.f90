@@ -7,9 +7,11 @@
 ! momentum (rrF 0.1, 20 iterations) and p' (rrF 0.01, 200 iterations), no
 ! multigrid.  It writes u, v, w, p (cells) and mdot (interior faces) as raw
 ! doubles to <out>.fields and the restart file of the last iteration in the
-! reference's writeSolution layout to <out>.restart.
+! reference's writeSolution layout to <out>.restart.  With a fifth argument
+! the run starts from that restart file (readSolution, as the reference does
+! with LReadOldSolution, MainProgram.for:171-177).
 !
-!   simple_case_device.x <polyMesh directory> <moving patch> <niter> <out>
+!   simple_case_device.x <polyMesh directory> <moving patch> <niter> <out> [<restart in>]
 !
 ! Exit status 0 on success, 1 with the library's message on any failure.
 program simple_case_device
@@ -18,7 +20,7 @@ program simple_case_device
   use fsamd_case
   implicit none
 
-  character(len=512) :: dir, lid, arg, out
+  character(len=512) :: dir, lid, arg, out, restart_in
   integer :: niter, k, j, nb, u
   integer(c_int) :: counts(5), dims(4), rc
   type(c_ptr) :: pm, cs
@@ -30,8 +32,8 @@ program simple_case_device
   type(fs_simple_stats) :: st
   character(len=4), parameter :: cell_fields(4) = ['u   ', 'v   ', 'w   ', 'p   ']
 
-  if (command_argument_count() /= 4) then
-    print '(a)', 'usage: simple_case_device.x <polyMesh directory> <moving patch> <niter> <out>'
+  if (command_argument_count() /= 4 .and. command_argument_count() /= 5) then
+    print '(a)', 'usage: simple_case_device.x <polyMesh directory> <moving patch> <niter> <out> [<restart in>]'
     stop 1
   end if
   call get_command_argument(1, dir)
@@ -39,6 +41,8 @@ program simple_case_device
   call get_command_argument(3, arg)
   read (arg, *) niter
   call get_command_argument(4, out)
+  restart_in = ' '
+  if (command_argument_count() == 5) call get_command_argument(5, restart_in)
 
   call check(fs_polymesh_read(fs_cstr(dir), counts, pm), 'fs_polymesh_read')
   allocate (x(counts(1)), y(counts(1)), z(counts(1)), face_ptr(counts(2) + 1), face_nodes(counts(3)), &
@@ -69,6 +73,8 @@ program simple_case_device
   call check(fs_case_configure(cs, 0.7d0, 0.3d0, FS_ILU, 0.1d0, 20, FS_MG_OFF, FS_ILU, 0.01d0, 200, &
                                FS_MG_OFF), 'fs_case_configure')
   call check(fs_case_dims(cs, dims), 'fs_case_dims')
+  if (len_trim(restart_in) > 0) &
+    call check(fs_case_read_solution(cs, fs_cstr(restart_in), counts(5), patch_nfaces), 'fs_case_read_solution')
   if (niter > 1) call check(fs_case_iterate(cs, niter - 1, st), 'fs_case_iterate')
   ! the face gradients of the last iteration go into the restart file
   call check(fs_case_set_restart_capture(cs, 1), 'fs_case_set_restart_capture')

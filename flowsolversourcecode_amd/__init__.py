@@ -73,6 +73,7 @@ def lib():
         L.fs_solve_equation_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_int,
                                                C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p, _dp, C.POINTER(SolveStats)]
+        L.fs_solve_equation_device_from_guess.argtypes = L.fs_solve_equation_device.argtypes
         L.fs_amg_export.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]
         L.fs_time_device_copy.argtypes = [C.c_longlong, C.c_int, _dp]
         L.fs_kernel_clock_start.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -214,10 +215,12 @@ class Mesh:
                                             _dptr(dphi), _dptr(res4), C.byref(st)), "fs_solve_equation")
         return rc, fit, dphi, res4, st.as_dict()
 
-    def solve_equation_device(self, variable, fit, rrf, itmax, solver, lmultigrid, ac, anb, bc, dphi):
+    def solve_equation_device(self, variable, fit, rrf, itmax, solver, lmultigrid, ac, anb, bc, dphi,
+                              from_guess=False):
         """SolveEquation on device tensors (torch, float64, on the current
-        device; anb shaped (W, N)).  fit and dphi are updated in place.
-        Returns (rc, res4, stats)."""
+        device; anb shaped (W, N)).  fit and dphi are updated in place.  With
+        from_guess the solve starts from the guess in dphi
+        (fs_solve_equation_device_from_guess).  Returns (rc, res4, stats)."""
         import torch
         nf = NF_CODES[variable] if isinstance(variable, str) else int(variable)
         sv = SOLVERS[solver] if isinstance(solver, str) else int(solver)
@@ -228,10 +231,10 @@ class Mesh:
         torch.cuda.synchronize()
         res4 = np.zeros(4)
         st = SolveStats()
-        rc = _check(lib().fs_solve_equation_device(self.h, nf, fit.data_ptr(), float(rrf), int(itmax), sv,
-                                                   int(lmultigrid), ac.data_ptr(), anb.data_ptr(),
-                                                   bc.data_ptr(), dphi.data_ptr(), _dptr(res4),
-                                                   C.byref(st)), "fs_solve_equation_device")
+        fn = lib().fs_solve_equation_device_from_guess if from_guess else lib().fs_solve_equation_device
+        rc = _check(fn(self.h, nf, fit.data_ptr(), float(rrf), int(itmax), sv, int(lmultigrid), ac.data_ptr(),
+                       anb.data_ptr(), bc.data_ptr(), dphi.data_ptr(), _dptr(res4), C.byref(st)),
+                    "fs_solve_equation_device")
         return rc, res4, st.as_dict()
 
     def amg_level(self, level):

@@ -1603,7 +1603,7 @@ void amg_apply(fs_mesh *m, const double *rhs, double *out, const int *flags, dou
 // 371-416): cycles while the fine L1 residual exceeds rrf*ires and at most
 // MaxMultiGridCycles; each cycle corrects with one V-cycle of the residual.
 int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc, double *dphi,
-              int solver) {
+              int solver, bool warm) {
   (void)solver;
   amg_prepare(m, ac, anb);
   hipStream_t s = m->stream;
@@ -1616,8 +1616,14 @@ int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc,
   // bounded by MaxMultiGridCycles instead
   int itmax_cycles = m->mg_maxcycles;
   FS_HIP_THROW(hipMemcpyAsync(flags + 2, &itmax_cycles, sizeof(int), hipMemcpyHostToDevice, s));
-  FS_HIP_THROW(hipMemcpyAsync(r, bc, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   const int gr = std::min(kPartials, gfor(n));
+  // the first cycle's right-hand side: bc (dphi = 0), or the residual of the
+  // initial guess
+  if (warm)
+    hipLaunchKernelGGL(k_amg_res, dim3(gr), dim3(kBlock), 0, s, n, m->w, m->d_nbr.p, ac, anb, bc, dphi, r,
+                       m->partial.p, (const int *)flags);
+  else
+    FS_HIP_THROW(hipMemcpyAsync(r, bc, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   int fl[4] = {0, 0, 0, 0};
   FS_HIP_THROW(hipMemcpyAsync(fl, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
   FS_HIP_THROW(hipStreamSynchronize(s));

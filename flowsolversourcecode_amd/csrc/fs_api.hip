@@ -238,6 +238,24 @@ int fs_solve_equation_device(fs_mesh *m, int nf, double *d_fit, double rrf, int 
   FS_GUARD_END
 }
 
+int fs_solve_equation_device_from_guess(fs_mesh *m, int nf, double *d_fit, double rrf, int itmax, int solver,
+                                        int lmultigrid, const double *d_ac, const double *d_anb,
+                                        const double *d_bc, double *d_dphi, double *res4,
+                                        fs_solve_stats *stats) {
+  FS_GUARD_BEGIN
+  if (!m || !d_fit || !d_ac || !d_anb || !d_bc || !d_dphi) {
+    set_error("fs_solve_equation_device_from_guess: NULL argument");
+    return -1;
+  }
+  if (lmultigrid < FS_MG_OFF || lmultigrid > FS_MG_PRECOND) {
+    set_error("fs_solve_equation_device_from_guess: LMultigrid must be 0, 1 or 2");
+    return -1;
+  }
+  return fs::solve_device(m, nf, d_fit, rrf, itmax, solver, lmultigrid, d_ac, d_anb, d_bc, d_dphi,
+                          res4, stats, true);
+  FS_GUARD_END
+}
+
 int fs_solve_equation(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int solver,
                       int lmultigrid, const double *ac, const double *anb, const double *bc,
                       double *dphi, double *res4, fs_solve_stats *stats) {

@@ -119,7 +119,7 @@ struct fs_mesh {
   fs::DevBuf<int> d_face_code;  // per (cell, slot): face*4 + role
   // work vectors
   fs::DevBuf<double> v[12];
-  fs::DevBuf<double> partial;  // kPartials * 4
+  fs::DevBuf<double> partial;  // kPartials * 5
   fs::DevBuf<double> scal;     // scalar slots
   fs::DevBuf<int> flags;       // [0] done, [1] iterations
   // options

@@ -41,8 +41,10 @@ void add_inplace(fs_mesh *m, double *a, const double *b);
 // one-block convergence update from np L1 partials (iteration count, done)
 void launch_fin_conv(fs_mesh *m, int np);
 
-// warm: dphi holds an initial guess (BiCGSTAB and CG drop it for a zero start
-// when its residual exceeds |bc|_1); otherwise dphi starts at zero (:47)
+// warm: dphi holds an initial guess, which every solver starts from after it
+// is scaled to its smallest-L2-residual multiple (and dropped for the zero
+// start when its L1 residual still exceeds |bc|_1); otherwise dphi starts
+// at zero (:47)
 int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int solver, int lmg,
                  const double *ac, const double *anb, const double *bc, double *dphi, double *res4,
                  fs_solve_stats *st, bool warm = false);
@@ -54,9 +56,10 @@ void amg_prepare(fs_mesh *m, const double *ac, const double *anb);
 // out = one V-cycle applied to rhs (both FP64 device vectors, not aliased),
 // coarse corrections scaled by alpha.
 void amg_apply(fs_mesh *m, const double *rhs, double *out, const int *flags, double alpha);
-// The reference's multigrid-as-solver loop; returns cycles run.
+// The reference's multigrid-as-solver loop; returns cycles run.  warm: dphi
+// holds an initial guess, so the first cycle smooths bc - A dphi.
 int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc, double *dphi,
-              int solver);
+              int solver, bool warm = false);
 int amg_levels(fs_mesh *m);
 // Average ms of one k_spmv_dot launch (y = A x with (dot_with, y) partials).
 int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, const double *dot_with,

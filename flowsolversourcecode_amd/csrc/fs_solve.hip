@@ -37,7 +37,13 @@ enum Slot {
   S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_TS, S_TT, S_RV, S_RES_ABS, S_RES_MAX,
   S_RES_RMS, S_RES_SCALED, S_ACPHI, S_GAMMA, S_NSLOTS
 };
-enum Flag { F_DONE = 0, F_ITER = 1, F_ITMAX = 2, F_REJECT = 3, F_INNER = 4, F_GATE = 5, F_NFLAGS = 8 };
+enum Flag {
+  F_DONE = 0, F_ITER = 1, F_ITMAX = 2, F_REJECT = 3, F_INNER = 4, F_GATE = 5,
+  F_BAD = 6,     // the input was refused: 2 = a non-finite ac or bc, 1 = a diagonal <= 0
+                 // for a solver that divides by it
+  F_NEEDPOS = 7, // set by the host when the solver divides by the diagonal
+  F_NFLAGS = 8
+};
 // F_INNER: the GMRES cycle in progress has finished its Arnoldi steps;
 // F_GATE = F_DONE | F_INNER, the flag the cycle's kernels (and the V-cycle
 // they call) test
@@ -102,27 +108,39 @@ __device__ __forceinline__ double ell_atx(int i, int n, int wt, const int *__res
 
 // Initial residual norms of SolveEquation (CalculateResidualsRoutine.for:
 // 19-67) with dphi = 0: r_i = bc_i.  Partials: sum|r|, max|r|, sum r^2,
-// max|ac*fit|.
+// max|ac*fit|, and the input check: the largest of 0 (row fine), 1 (ac <= 0)
+// and 2 (ac or bc not finite).
 __global__ void k_init_norms(int n, const double *__restrict__ ac, const double *__restrict__ bc,
                              const double *__restrict__ fit, double *__restrict__ part) {
-  double s1 = 0.0, m1 = -1.0, s2 = 0.0, m2 = -1.0;
+  double s1 = 0.0, m1 = -1.0, s2 = 0.0, m2 = -1.0, bad = 0.0;
   for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const double r = fabs(bc[i]);
+    const double a = ac[i], b = bc[i];
+    const double r = fabs(b);
     s1 += r;
     m1 = fmax(m1, r);
     s2 += r * r;
-    m2 = fmax(m2, fabs(ac[i] * fit[i]));
+    m2 = fmax(m2, fabs(a * fit[i]));
+    bad = fmax(bad, !(isfinite(a) && isfinite(b)) ? 2.0 : (a > 0.0 ? 0.0 : 1.0));
   }
   s1 = block_sum<kBlock>(s1);
   m1 = block_max<kBlock>(m1);
   s2 = block_sum<kBlock>(s2);
   m2 = block_max<kBlock>(m2);
+  bad = block_max<kBlock>(bad);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = s1;
     part[kPartials + blockIdx.x] = m1;
     part[2 * kPartials + blockIdx.x] = s2;
     part[3 * kPartials + blockIdx.x] = m2;
+    part[4 * kPartials + blockIdx.x] = bad;
   }
+}
+
+// dphi = 0 (SolveAlegbraicEquations.for:47), unless the input was refused
+__global__ void k_zero_start(int n, double *__restrict__ x, const int *__restrict__ flags) {
+  if (flags[F_BAD]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = 0.0;
 }
 
 // L1 norm of the true residual bc - A*x (CheckConvergence).
@@ -147,15 +165,11 @@ __global__ void k_warm_scale(int n, double *__restrict__ x, const double *__rest
   if (i < n) x[i] = scal[S_GAMMA] * x[i];
 }
 
-// a rejected initial guess becomes the zero start: x = 0, r = r0 = b
-__global__ void k_warm_fix(int n, double *__restrict__ x, double *__restrict__ r, double *__restrict__ r0,
-                           const double *__restrict__ b, const int *__restrict__ flags) {
+// a rejected initial guess becomes the zero start: x = 0
+__global__ void k_warm_fix(int n, double *__restrict__ x, const int *__restrict__ flags) {
   if (!flags[F_REJECT]) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  x[i] = 0.0;
-  r[i] = b[i];
-  if (r0) r0[i] = b[i];
+  if (i < n) x[i] = 0.0;
 }
 
 // p = z (first iteration) or z + beta*p
@@ -743,11 +757,12 @@ __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ 
                       double *__restrict__ scal, int *__restrict__ flags) {
   if (op != FIN_INIT_NORMS && op != FIN_TRUE_RES && flags[F_DONE]) return;
   const double a = reduce_partials(part, np);
-  double b = 0.0, c = 0.0, d = 0.0;
+  double b = 0.0, c = 0.0, d = 0.0, e = 0.0;
   if (op == FIN_INIT_NORMS) {
     b = reduce_partials_max(part + kPartials, np);
     c = reduce_partials(part + 2 * kPartials, np);
     d = reduce_partials_max(part + 3 * kPartials, np);
+    e = reduce_partials_max(part + 4 * kPartials, np);
   } else if (op == FIN_BICG_OMEGA || op == FIN_WARM_SCALE || op == FIN_CONV_BICG_RHO) {
     b = reduce_partials(part + kPartials, np);
   }
@@ -762,6 +777,9 @@ __global__ void k_fin(int op, int np, int n, int nf, const double *__restrict__ 
     scal[S_FRES] = a;
     flags[F_ITER] = 0;
     flags[F_DONE] = !(a > scal[S_RRF] * a && 0 < flags[F_ITMAX]);
+    // refused input: nothing runs, the caller's arrays stay as they were
+    flags[F_BAD] = e >= 2.0 ? 2 : (e >= 1.0 && flags[F_NEEDPOS]) ? 1 : 0;
+    if (flags[F_BAD]) flags[F_DONE] = 1;
     break;
   }
   case FIN_PBCG_BK: {
@@ -1060,7 +1078,7 @@ int mesh_prepare_solver(fs_mesh *m) {
   sw->color_cells.upload(cells, m->stream);
   sw->dc.alloc(m->n);
   for (auto &v : m->v) v.alloc(m->n);
-  m->partial.alloc(4 * kPartials);
+  m->partial.alloc(5 * kPartials);
   m->scal.alloc(S_NSLOTS);
   m->flags.alloc(F_NFLAGS);
   m->nnz = 0;
@@ -1240,19 +1258,50 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
 
   double h_sc[S_NSLOTS] = {0};
   h_sc[S_RRF] = rrf;
+  if (solver < FS_SOR || solver > FS_CG || solver == FS_DIRECT) {
+    set_error("unsupported solver code " + std::to_string(solver) +
+              (solver == FS_DIRECT ? " ('direct' is the host's dense LU)" : ""));
+    return -3;
+  }
   int h_fl[F_NFLAGS] = {0, 0, itmax, 0};
+  // Gauss-Seidel, D-ILU (also PBCG's preconditioner) and every multigrid
+  // smoother divide by the diagonal
+  h_fl[F_NEEDPOS] = (lmg != FS_MG_OFF || solver == FS_SOR || solver == FS_ILU || solver == FS_PBCG) ? 1 : 0;
   FS_HIP(hipMemcpyAsync(scal, h_sc, sizeof(h_sc), hipMemcpyHostToDevice, s));
   FS_HIP(hipMemcpyAsync(flags, h_fl, sizeof(h_fl), hipMemcpyHostToDevice, s));
-  if (!warm) FS_HIP(hipMemsetAsync(dphi, 0, sizeof(double) * n, s)); // dphi=0. (:47)
   hipLaunchKernelGGL(k_init_norms, dim3(gr), dim3(kBlock), 0, s, n, ac, bc, fit, part);
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_INIT_NORMS, gr, n, nf, part, scal, flags);
+  if (!warm) hipLaunchKernelGGL(k_zero_start, dim3(ge), dim3(kBlock), 0, s, n, dphi, (const int *)flags);
+  auto refuse = [](int bad) {
+    set_error(bad == 2 ? "a coefficient ac or source bc is not finite"
+                       : "a diagonal coefficient ac <= 0, which this solver divides by");
+    return bad == 2 ? -4 : -1;
+  };
+  if (lmg != FS_MG_OFF) {
+    // the hierarchy is built from the operator: refuse bad input before that
+    int fl0[F_NFLAGS];
+    host_flags(m, fl0);
+    if (fl0[F_BAD]) return refuse(fl0[F_BAD]);
+  }
 
   int iters = 0;
   double *V0 = m->v[0].p, *V1 = m->v[1].p, *V2 = m->v[2].p, *V3 = m->v[3].p, *V4 = m->v[4].p,
          *V5 = m->v[5].p, *V6 = m->v[6].p, *V7 = m->v[7].p;
 
+  // An initial guess, whatever the solver: scaled to its multiple with the
+  // smallest L2 residual, then dropped for the zero start when its L1
+  // residual still exceeds that of the zero start (|bc|_1).  Every solver
+  // below starts from dphi as it stands, so the guess can only change the
+  // iteration count, never the stop rule.
+  if (warm) {
+    scale_guess(m, ac, anb, bc, dphi, V3);
+    hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, part, flags);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
+    hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, (const int *)flags);
+  }
+
   if (lmg == FS_MG_SOLVER) {
-    iters = amg_solve(m, ac, anb, bc, dphi, solver);
+    iters = amg_solve(m, ac, anb, bc, dphi, solver, warm);
   } else if (solver == FS_PBCG) {
     // r=V0 rr=V1 p=V2 pp=V3 z=V4 zz=V5 (PBCG1.F90:4)
     ilu_factor(m, ac, anb, sw.dc.p);
@@ -1314,12 +1363,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     // neighbour words (4 per slot, or 4 per pair of slots), anb, x, y, r0
     const double ib = nb.d16 ? 4.0 * ((w + 1) / 2) : 4.0 * w;
     const double spmv_bytes = 8.0 * n + ib * n + 8.0 * m->nnz + 24.0 * n;
-    if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
     hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, V0, V1, part);
-    if (warm) {
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
-      hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, V1, bc, flags);
-    }
     FS_HIP(hipMemsetAsync(V3, 0, sizeof(double) * n, s));
     // rho of the first iteration; every later rho comes with the update
     hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V1, V0, nullptr, nullptr, part, flags);
@@ -1348,13 +1392,8 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
     // y = A p with the (p, y) partials: ac, nbr, anb, p, y
     const double spmv_bytes = 8.0 * n + (nb.d16 ? 4.0 * ((w + 1) / 2) : 4.0 * w) * n + 8.0 * m->nnz + 16.0 * n;
-    if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
     hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, V0, nullptr,
                        part);
-    if (warm) {
-      hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM, gr, n, nf, part, scal, flags);
-      hipLaunchKernelGGL(k_warm_fix, dim3(ge), dim3(kBlock), 0, s, n, dphi, V0, (double *)nullptr, bc, flags);
-    }
     iters = run_batches(m, itmax, nf * 8 + solver, [&]() {
       const double *z = V0;
       if (pre) {
@@ -1393,7 +1432,6 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       hipLaunchKernelGGL(k_gm_restart, dim3(1), dim3(kFinBlock), 0, s, gr, gmm, part, gm, scal, flags);
       hipLaunchKernelGGL(k_gm_v0, dim3(ge), dim3(kBlock), 0, s, n, gmm, V0, V, gm, flags);
     };
-    if (warm) scale_guess(m, ac, anb, bc, dphi, V3);
     restart();
     int fl[F_NFLAGS];
     host_flags(m, fl);
@@ -1464,6 +1502,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
   // true final residual, fit += dphi (:180)
   int fl[F_NFLAGS];
   host_flags(m, fl);
+  if (fl[F_BAD]) return refuse(fl[F_BAD]);
   hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, part, nullptr);
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_TRUE_RES, gr, n, nf, part, scal, flags);
   add_inplace(m, fit, dphi);

@@ -14,8 +14,11 @@
  *
  * Return codes: 0 = converged; 1 = stopped at itmax (the residual is still
  * reported, as the reference does); negative = invalid input or device error
- * (fs_last_error() gives the message).  A negative code leaves the caller's
- * arrays untouched so the host can fall back to its own solver.
+ * (fs_last_error() gives the message): -1 invalid argument, also a diagonal
+ * ac <= 0 for a solver that divides by it (Gauss-Seidel, D-ILU, PBCG and
+ * every multigrid smoother), -2 device error, -3 unsupported solver, -4 a
+ * non-finite ac or bc, or a NaN residual.  A refused input leaves the
+ * caller's arrays untouched so the host can fall back to its own solver.
  *
  * Threading: one call in flight per fs_mesh; each mesh owns one HIP stream
  * and every entry point synchronises before it returns.
@@ -131,6 +134,20 @@ int fs_solve_equation_device(fs_mesh *mesh, int nf, double *d_fit, double rrf,
                              const double *d_ac, const double *d_anb,
                              const double *d_bc, double *d_dphi, double *res4,
                              fs_solve_stats *stats);
+
+/* The same device solve started from the initial guess the caller leaves in
+ * d_dphi instead of from zero.  No counterpart in the reference (its
+ * SolveEquation always starts at dphi = 0, SolveAlegbraicEquations.for:47);
+ * it is the entry of a predicted p' (the learned warm start).  Every solver
+ * first scales the guess to its multiple with the smallest L2 residual and
+ * drops it for the zero start when its L1 residual still exceeds |bc|_1 (a
+ * non-finite guess is always dropped); the residual norms and the stop rule
+ * are those of the zero start, so only the iteration count can change. */
+int fs_solve_equation_device_from_guess(fs_mesh *mesh, int nf, double *d_fit, double rrf,
+                                        int itmax, int solver, int lmultigrid,
+                                        const double *d_ac, const double *d_anb,
+                                        const double *d_bc, double *d_dphi, double *res4,
+                                        fs_solve_stats *stats);
 
 /* Cell-to-face map NGlobalEFaces(N,W) (column-major, 1-based global face
  * numbers: interior faces 1..Fi first, then boundary faces patch by patch)

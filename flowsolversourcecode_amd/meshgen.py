@@ -204,6 +204,44 @@ def box_tet(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, perturb=0.0, seed=0):
                     (nx, ny, nz))
 
 
+def graded_coordinates(n, length, ratio):
+    """n + 1 node coordinates on [0, length], cells symmetric about the middle,
+    each growing by `ratio` from either end toward the middle (n even)."""
+    if n % 2:
+        raise ValueError("graded_coordinates needs an even cell count")
+    h = ratio ** np.arange(n // 2)
+    h = np.concatenate([h, h[::-1]])
+    y = np.concatenate([[0.0], np.cumsum(h)])
+    return y * (length / y[-1])
+
+
+def channel_tet(nx, ny, nz, lx=2.0 * np.pi, ly=2.0, lz=np.pi, grading=1.05, cells="tet"):
+    """A plane channel for the periodic RANS case: nx*ny*nz hexahedra on
+    [0,lx]x[0,ly]x[0,lz], graded in y by `grading` toward both walls (cells
+    growing by that ratio from each wall to the centre plane), each split
+    into 6 tetrahedra (Kuhn, box_tet's cell and face order).  x is the
+    streamwise direction; the Kuhn split triangulates the x = 0 and x = lx
+    planes alike, face for face.  Patches, in this order: outlet (x = lx) and
+    inlet (x = 0), a translational periodic pair; walls (y = 0 and y = ly);
+    sides (z = 0 and z = lz, symmetry planes).  Returns (mesh, patch types,
+    pairs, translations): pairs[q] = the partner patch of patch q (-1 for
+    none) and translations[q] = the vector that carries the partner patch's
+    cells across patch q (the reference's x/y/zTranslation of the partner),
+    the form SetBCPatches and RotationMatrixCoefficients read.  cells="hex"
+    keeps the hexahedra unsplit."""
+    m = (box_tet if cells == "tet" else box_hex)(nx, ny, nz, lx, ly, lz)
+    pts = m.points.copy()
+    j = (np.arange(len(pts)) // (nx + 1)) % (ny + 1)
+    pts[:, 1] = graded_coordinates(ny, ly, grading)[j]
+    m = PolyMesh(pts, m.face_ptr, m.face_nodes, m.owner, m.neighbour, m.patch_names, m.patch_sizes, m.shape)
+    m = regroup(m, [("outlet", ["xmax"]), ("inlet", ["xmin"]), ("walls", ["ymin", "ymax"]),
+                    ("sides", ["zmin", "zmax"])])
+    types = ["periodic", "periodic", "wall", "symmetry"]
+    pairs = [1, 0, -1, -1]
+    translations = [(-lx, 0.0, 0.0), (lx, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)]
+    return m, types, pairs, translations
+
+
 def lid_driven_cavity_walls(mesh, lid="ymax", lid_velocity=(1.0, 0.0, 0.0)):
     """Wall velocity of every boundary face: the lid patch moves, the other
     walls are at rest.  Returns (bu, bv, bw), one entry per boundary face."""

@@ -81,6 +81,7 @@ or_case *or_case_new(or_mesh *m, const double *wall_u, const double *wall_v,
   c->fcf = dz(m->nfaces); c->fff = dz(m->nfaces); c->fvf = dz(m->nfaces); c->ftf = dz(m->nfaces);
   c->fce = dz(N); c->fte = dz(N); c->effdiv = dz(N);
   c->ac = dz(N); c->anb = dz((size_t)N * W); c->bc = dz(N); c->dphi = dz(N);
+  c->bmdot = dz(Fb);
   for (int b = 0; b < Fb; ++b) {
     c->bu[b] = c->bu0[b];
     c->bv[b] = c->bv0[b];
@@ -93,7 +94,8 @@ int or_case_set_patch_types(or_case *c, int npatches, const int *patch_nfaces, c
   int total = 0;
   for (int q = 0; q < npatches; ++q) {
     if (patch_nfaces[q] < 0) return -1;
-    if (patch_type[q] != OR_BC_WALL && patch_type[q] != OR_BC_SYMMETRY) return -1;
+    if (patch_type[q] != OR_BC_WALL && patch_type[q] != OR_BC_SYMMETRY && patch_type[q] != OR_BC_PERIODIC)
+      return -1;
     /* CorrectPressureGradientAtSymmetryBoundary reads BFaceAreanz(i,i): face
      * i of patch i */
     if (patch_type[q] == OR_BC_SYMMETRY && patch_nfaces[q] < q + 1) return -1;
@@ -133,12 +135,231 @@ void or_case_update_symmetry(or_case *c) {
   }
 }
 
+/* ---------------------------------------------------------------------
+ * Translational periodicity with the implicit coupling.
+ */
+
+/* The geometry the periodic branches compute for boundary face b from the
+ * partner cell j1's centroid carried by the partner patch's translation
+ * (e.g. UpdatePeriodic, PeriodicBoundary.for:54-67; the E/T split of
+ * AssembleIsotropicDiffusion.for:663-700). */
+static void periodic_geometry(or_case *c, int b, int q2, const double *trans) {
+  const or_mesh *m = c->m;
+  const int f = c->Fi + b, i1 = m->owner[f], j1 = c->pj1[b];
+  const double xf1 = m->xc[j1] + trans[3 * q2], yf1 = m->yc[j1] + trans[3 * q2 + 1],
+               zf1 = m->zc[j1] + trans[3 * q2 + 2];
+  const double ax = m->fcx[f] - m->xc[i1], ay = m->fcy[f] - m->yc[i1], az = m->fcz[f] - m->zc[i1];
+  const double bx = m->fcx[f] - xf1, by = m->fcy[f] - yf1, bz = m->fcz[f] - zf1;
+  const double d1 = sqrt(ax * ax + ay * ay + az * az), d2 = sqrt(bx * bx + by * by + bz * bz);
+  c->pg[b] = d2 / (d1 + d2);
+  const double dx = xf1 - m->xc[i1], dy = yf1 - m->yc[i1], dz_ = zf1 - m->zc[i1];
+  const double dcf = sqrt(dx * dx + dy * dy + dz_ * dz_);
+  c->pdx[b] = dx;
+  c->pdy[b] = dy;
+  c->pdz[b] = dz_;
+  c->pdcf[b] = dcf;
+  const double ux = dx / dcf, uy = dy / dcf, uz = dz_ / dcf;
+  c->pux[b] = ux;
+  c->puy[b] = uy;
+  c->puz[b] = uz;
+  const double sx = m->Sx[f], sy = m->Sy[f], sz = m->Sz[f], A = m->A[f];
+  double fex, fey, fez, fe;
+  if (c->decomp == 1) {
+    const double dot = ux * sx + uy * sy + uz * sz;
+    fex = dot * ux; fey = dot * uy; fez = dot * uz;
+    fe = fabs(dot);
+  } else if (c->decomp == 2) {
+    fex = A * ux; fey = A * uy; fez = A * uz;
+    fe = A;
+  } else {
+    const double dot = ux * sx + uy * sy + uz * sz;
+    fex = (A * A / dot) * ux; fey = (A * A / dot) * uy; fez = (A * A / dot) * uz;
+    fe = sqrt(fex * fex + fey * fey + fez * fez);
+  }
+  c->pgd[b] = fe / dcf;
+  c->ptx[b] = sx - fex;
+  c->pty[b] = sy - fey;
+  c->ptz[b] = sz - fez;
+}
+
+/* RotationMatrixCoefficients, translational branch
+ * (CalculateRotationMatrixCoefficients.for:171-242): each face of the lower
+ * patch of a pair takes the face of the partner patch whose centroid offset
+ * is closest to the translation (largest per-axis miss, first minimum);
+ * LPeriodicImplicit then puts the partner cell in the face's slot of
+ * ElementNeighbor, the slot searched over NumbOfElementNodes entries of
+ * NGlobalEFaces (:249-289). */
+int or_case_set_periodic(or_case *c, int npatches, const int *pair, const double *trans, double beta0,
+                         int calc_beta, double mdot_target, double relax, int iters, int every) {
+  or_mesh *m = c->m;
+  const int Fi = c->Fi, Fb = c->Fb, N = c->N;
+  if (!c->btype || npatches != m->npatches) return -1;
+  for (int q = 0; q < npatches; ++q) {
+    const int typ = c->btype[m->patch_start[q] - Fi];
+    if (m->patch_start[q + 1] == m->patch_start[q]) continue;
+    if ((typ == OR_BC_PERIODIC) != (pair[q] >= 0)) return -1;
+    if (pair[q] >= 0 && (pair[q] >= npatches || pair[pair[q]] != q || pair[q] == q)) return -1;
+  }
+  free(c->pbf); free(c->pact); free(c->pj1);
+  c->pbf = (int *)malloc(sizeof(int) * (Fb ? Fb : 1));
+  c->pact = (int *)calloc(Fb ? Fb : 1, sizeof(int));
+  c->pj1 = (int *)malloc(sizeof(int) * (Fb ? Fb : 1));
+  for (int b = 0; b < Fb; ++b) c->pbf[b] = c->pj1[b] = -1;
+  double **arrs[] = {&c->pg, &c->pdx, &c->pdy, &c->pdz, &c->pdcf, &c->pux, &c->puy, &c->puz, &c->pgd,
+                     &c->ptx, &c->pty, &c->ptz};
+  for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); ++i) {
+    free(*arrs[i]);
+    *arrs[i] = dz(Fb);
+  }
+  c->nperiodic = 0;
+  for (int q1 = 0; q1 < npatches; ++q1) {
+    const int q2 = pair[q1];
+    if (q2 <= q1) continue;
+    for (int f1 = m->patch_start[q1]; f1 < m->patch_start[q1 + 1]; ++f1) {
+      const double xf = m->fcx[f1], yf = m->fcy[f1], zf = m->fcz[f1];
+      double best = 1.e30; /* big (Constants1.F90) */
+      int fd = -1;
+      for (int f2 = m->patch_start[q2]; f2 < m->patch_start[q2 + 1]; ++f2) {
+        const double d1 = fabs(fabs(m->fcx[f2] - xf) - fabs(trans[3 * q1]));
+        const double d2 = fabs(fabs(m->fcy[f2] - yf) - fabs(trans[3 * q1 + 1]));
+        const double d3 = fabs(fabs(m->fcz[f2] - zf) - fabs(trans[3 * q1 + 2]));
+        double dif = d1;
+        if (d2 > dif) dif = d2;
+        if (d3 > dif) dif = d3;
+        if (dif < best) {
+          best = dif;
+          fd = f2;
+        }
+      }
+      if (fd < 0) return -1;
+      c->pbf[f1 - Fi] = fd - Fi;
+      c->pbf[fd - Fi] = f1 - Fi;
+      c->pact[f1 - Fi] = 1;
+    }
+  }
+  for (int b = 0; b < Fb; ++b) {
+    if (c->btype[b] != OR_BC_PERIODIC) continue;
+    if (c->pbf[b] < 0) return -1;
+    c->pj1[b] = m->owner[Fi + c->pbf[b]];
+    c->nperiodic++;
+  }
+  /* x/y/zTranslationUse: the first periodic patch's translation */
+  for (int q = 0; q < npatches; ++q)
+    if (pair[q] >= 0) {
+      c->tuse[0] = fabs(trans[3 * q]);
+      c->tuse[1] = fabs(trans[3 * q + 1]);
+      c->tuse[2] = fabs(trans[3 * q + 2]);
+      break;
+    }
+  for (int q = 0; q < npatches; ++q) {
+    if (pair[q] < 0) continue;
+    for (int f = m->patch_start[q]; f < m->patch_start[q + 1]; ++f) {
+      const int b = f - Fi, i1 = m->owner[f];
+      periodic_geometry(c, b, pair[q], trans);
+      int found = 0;
+      for (int s = 0; s < m->nen[i1] && s < m->w; ++s)
+        if (m->gef[(size_t)s * N + i1] == f + 1) {
+          m->enbr[(size_t)s * N + i1] = c->pj1[b] + 1;
+          found = 1;
+          break;
+        }
+      if (!found) return -1;
+    }
+  }
+  free(c->ppair);
+  free(c->ptrans);
+  c->ppair = (int *)malloc(sizeof(int) * npatches);
+  c->ptrans = dz(3 * (size_t)npatches);
+  memcpy(c->ppair, pair, sizeof(int) * npatches);
+  memcpy(c->ptrans, trans, sizeof(double) * 3 * npatches);
+  c->beta = beta0;
+  c->beta_calc = calc_beta;
+  c->beta_mdot = mdot_target;
+  c->beta_relax = relax;
+  c->beta_iters = iters;
+  c->beta_every = every > 0 ? every : 1;
+  return 0;
+}
+
+static int is_periodic(const or_case *c, int b) { return c->pbf && c->pbf[b] >= 0; }
+
+/* g phi_C + (1 - g) phi_F of a periodic face */
+static double per_mix(const or_case *c, int b, double pi, double pj) {
+  return c->pg[b] * pi + (1. - c->pg[b]) * pj;
+}
+
+/* the rotation of a translational pair is the identity: the reference's
+ * a1r x + b1r y + c1r z with (1, 0, 0), (0, 1, 0), (0, 0, 1) */
+static double rot(int comp, double x, double y, double z) {
+  const double a = comp == 0 ? 1. : 0., bb = comp == 1 ? 1. : 0., cc = comp == 2 ? 1. : 0.;
+  return a * x + bb * y + cc * z;
+}
+
+/* UpdatePeriodic, translational (PeriodicBoundary.for:54-79): the boundary
+ * velocities of the lower patch interpolated between the two cells, copied
+ * to the partner faces. */
+static void update_periodic(or_case *c) {
+  if (!c->pbf) return;
+  const or_mesh *m = c->m;
+  for (int b = 0; b < c->Fb; ++b) {
+    if (!c->pact[b]) continue;
+    const int i1 = m->owner[c->Fi + b], j1 = c->pj1[b], b2 = c->pbf[b];
+    c->bu[b] = per_mix(c, b, c->u[i1], c->u[j1]);
+    c->bv[b] = per_mix(c, b, c->v[i1], c->v[j1]);
+    c->bw[b] = per_mix(c, b, c->w[i1], c->w[j1]);
+    c->bu[b2] = c->bu[b];
+    c->bv[b2] = c->bv[b];
+    c->bw[b2] = c->bw[b];
+  }
+}
+
+/* CorrectGradientAtPeriodicBoundary, translational (CorrectGradients.for
+ * :141-167): the lower patch's boundary gradients interpolated between the
+ * two cells; the partner faces receive the x and y components only (the
+ * z component keeps the owner's copy). */
+static void periodic_gradient(or_case *c, const double *gr, double *bgr) {
+  if (!c->pbf) return;
+  const or_mesh *m = c->m;
+  const int N = c->N, Fb = c->Fb;
+  for (int b = 0; b < Fb; ++b) {
+    if (!c->pact[b]) continue;
+    const int i1 = m->owner[c->Fi + b], j1 = c->pj1[b], b2 = c->pbf[b];
+    bgr[b] = per_mix(c, b, gr[i1], gr[j1]);
+    bgr[Fb + b] = per_mix(c, b, gr[N + i1], gr[N + j1]);
+    bgr[2 * Fb + b] = per_mix(c, b, gr[2 * N + i1], gr[2 * N + j1]);
+    bgr[b2] = bgr[b];
+    bgr[Fb + b2] = bgr[Fb + b];
+  }
+}
+
+/* a periodic face's cell-to-cell value on the lower patch, copied to the
+ * partner (UpdateBoundaryPressure :293-317, UpdateBoundaryPressureCorrection
+ * :263-290 for p and p') */
+static void periodic_values(or_case *c, const double *phi, double *bphi) {
+  if (!c->pbf) return;
+  const or_mesh *m = c->m;
+  for (int b = 0; b < c->Fb; ++b) {
+    if (!c->pact[b]) continue;
+    const int i1 = m->owner[c->Fi + b], j1 = c->pj1[b];
+    bphi[b] = per_mix(c, b, phi[i1], phi[j1]);
+    bphi[c->pbf[b]] = bphi[b];
+  }
+}
+
 void or_case_delete(or_case *c) {
   free(c->du2);
   free(c->dv2);
   free(c->dw2);
   free(c->btype);
   free(c->bzq);
+  free(c->pbf);
+  free(c->pact);
+  free(c->pj1);
+  free(c->ppair);
+  free(c->ptrans);
+  double *parr[] = {c->pg, c->pdx, c->pdy, c->pdz, c->pdcf, c->pux, c->puy, c->puz, c->pgd, c->ptx, c->pty,
+                    c->ptz, c->bmdot};
+  for (size_t i = 0; i < sizeof(parr) / sizeof(parr[0]); ++i) free(parr[i]);
   double *arrs[] = {c->bu0, c->bv0, c->bw0, c->u, c->v, c->w, c->p, c->pc, c->mdot, c->gu,
                     c->gv, c->gw, c->gp, c->gpc, c->bu, c->bv, c->bw, c->bp, c->bpc, c->bgu,
                     c->bgv, c->bgw, c->bgp, c->bgpc, c->fgu, c->fgv, c->fgw, c->fgp, c->du,
@@ -490,9 +711,16 @@ void or_case_momentum_gradients(or_case *c) {
     or_limit_gradient(c, c->lim_mom, c->w, c->bw, c->gw, c->bgw);
   }
   if (c->lim_cont) or_limit_gradient(c, c->lim_cont, c->p, c->bp, c->gp, c->bgp);
+  /* CorrectGradientAtPeriodicBoundary (CalculateGradients.for:263), before
+   * the pressure's symmetry and wall projections */
+  periodic_gradient(c, c->gu, c->bgu);
+  periodic_gradient(c, c->gv, c->bgv);
+  periodic_gradient(c, c->gw, c->bgw);
+  periodic_gradient(c, c->gp, c->bgp);
   const or_mesh *m = c->m;
   const int Fi = c->Fi, Fb = c->Fb, N = c->N;
   for (int b = 0; b < Fb; ++b) {
+    if (is_periodic(c, b)) continue;
     const int f = Fi + b, i1 = m->owner[f];
     const double nx = m->nx[f], ny = m->ny[f], nz = m->nz[f];
     const double gx = c->gp[i1], gy = c->gp[N + i1], gz = c->gp[2 * N + i1];
@@ -538,6 +766,17 @@ static void scatter(or_case *c) {
     c->ac[k] = c->ac[k] + c->fcf[f];
     c->bc[k] = c->bc[k] - c->ftf[f];
   }
+  /* LPeriodicImplicit: FluxFf of a periodic face into its slot (:99-121) */
+  if (c->pbf)
+    for (int b = 0; b < c->Fb; ++b) {
+      if (!is_periodic(c, b)) continue;
+      const int f = Fi + b, i1 = m->owner[f];
+      for (int s = 0; s < m->nen[i1] && s < m->w; ++s)
+        if (m->gef[(size_t)s * N + i1] == f + 1) {
+          c->anb[(size_t)s * N + i1] = c->anb[(size_t)s * N + i1] + c->fff[f];
+          break;
+        }
+    }
   for (int i = 0; i < N; ++i) {
     c->ac[i] = c->ac[i] + c->fce[i];
     c->bc[i] = c->bc[i] - c->fte[i];
@@ -578,9 +817,24 @@ void or_case_momentum_assemble(or_case *c, int comp) {
     c->fvf[k] = c->fvf[k] + vfl;
     c->ftf[k] = c->ftf[k] + cfl * phi[i] + ffl * phi[j] + vfl;
   }
-  /* no-slip walls */
+  /* no-slip walls, symmetry planes, periodic faces */
   for (int b = 0; b < Fb; ++b) {
     const int f = Fi + b, i1 = m->owner[f];
+    if (is_periodic(c, b)) {
+      /* AssembleIsotropicDiffusion.for:630-700,746-770 (translational,
+       * implicit): BGamaFace interpolated between the two cells
+       * (InterpolateGamaFromCentroidToFace.for:129-178) */
+      const int j1 = c->pj1[b];
+      const double gamf = per_mix(c, b, c->visc[i1], c->visc[j1]);
+      const double *bg = comp == 0 ? c->bgu : comp == 1 ? c->bgv : c->bgw;
+      const double cfl = gamf * c->pgd[b], ffl = -(gamf * c->pgd[b]);
+      const double vfl = -(gamf * (bg[b] * c->ptx[b] + bg[Fb + b] * c->pty[b] + bg[2 * Fb + b] * c->ptz[b]));
+      c->fcf[f] = c->fcf[f] + cfl;
+      c->fff[f] = c->fff[f] + ffl;
+      c->fvf[f] = c->fvf[f] + vfl;
+      c->ftf[f] = c->ftf[f] + cfl * phi[i1] + ffl * phi[j1] + vfl;
+      continue;
+    }
     const double gamf = c->mu; /* BGamaFace = BViscosity */
     const double nx = m->nx[f], ny = m->ny[f], nz = m->nz[f], area = m->A[f];
     const double dn = m->dx[f] * nx + m->dy[f] * ny + m->dz[f] * nz;
@@ -682,7 +936,7 @@ void or_case_momentum_assemble(or_case *c, int comp) {
   }
   for (int b = 0; b < Fb; ++b) {
     const int k = m->owner[Fi + b];
-    c->effdiv[k] = c->effdiv[k] + 0.;
+    c->effdiv[k] = c->effdiv[k] + c->bmdot[b];
   }
   for (int i = 0; i < N; ++i) {
     const double e = c->effdiv[i], ep = e > 0.0 ? e : 0.0;
@@ -691,6 +945,25 @@ void or_case_momentum_assemble(or_case *c, int comp) {
     const double ftel = fcel * phi[i] + fvel;
     c->fce[i] = c->fce[i] + fcel;
     c->fte[i] = c->fte[i] + ftel;
+  }
+  /* periodic faces (AssembleConvection.for:582-627): upwind with the
+   * partner cell's value */
+  if (c->pbf)
+    for (int b = 0; b < Fb; ++b) {
+      if (!is_periodic(c, b)) continue;
+      const int f = Fi + b, i1 = m->owner[f], j1 = c->pj1[b];
+      const double phib = rot(comp, c->u[j1], c->v[j1], c->w[j1]);
+      const double bm = c->bmdot[b];
+      const double cfl = bm > 0. ? bm : 0., ffl = -(-bm > 0. ? -bm : 0.);
+      c->fcf[f] = c->fcf[f] + cfl;
+      c->fff[f] = c->fff[f] + ffl;
+      c->ftf[f] = c->ftf[f] + (cfl * phi[i1] + ffl * phib);
+    }
+  /* AssembleLinearPressureTerm (AssembleLinearPressure.for:18-44) */
+  if (c->pbf) {
+    const double lt = sqrt(c->tuse[0] * c->tuse[0] + c->tuse[1] * c->tuse[1] + c->tuse[2] * c->tuse[2]);
+    const double beta = c->beta * c->tuse[comp] / lt;
+    for (int i = 0; i < N; ++i) c->fte[i] = c->fte[i] - beta * m->vol[i];
   }
   /* pressure gradient */
   const double *gpc = c->gp + comp * N;
@@ -723,6 +996,13 @@ int or_case_set_decompose(or_case *c, int method) {
   if (method < 1 || method > 3) return -1;
   or_mesh_decompose(c->m, method);
   c->decomp = method;
+  if (c->pbf) /* the periodic faces' E/T split follows MethodDecomposeS */
+    for (int b = 0; b < c->Fb; ++b)
+      if (c->pbf[b] >= 0) {
+        int qq = 0;
+        while (c->m->patch_start[qq + 1] <= c->Fi + b) ++qq;
+        periodic_geometry(c, b, c->ppair[qq], c->ptrans);
+      }
   return 0;
 }
 
@@ -747,6 +1027,7 @@ void or_case_momentum_solve(or_case *c, int comp) {
                                       c->mrrf, c->mitmax, c->msolver, comp + 1, c->dphi,
                                       c->res_mom[comp], hist);
   free(hist);
+  update_periodic(c); /* SolveXMomentumEquation.for:299 */
 }
 
 /* AssembleMdot (AssembleMdotUsingRhieChow.for:262-292 face interpolations,
@@ -818,6 +1099,47 @@ void or_case_pc_assemble(or_case *c) {
     fv = fv + (1. - c->urf_u) * (c->mdot[k] - mstar);
     c->mdot[k] = cfl * c->p[i] + ffl * c->p[j] + fv;
   }
+  /* periodic faces, translational (AssembleMdotUsingRhieChow.for:1086-1416,
+   * steady with LRelaxMomentum): the lower patch's Bmdot, its negative on
+   * the partner face */
+  if (c->pbf)
+    for (int b = 0; b < c->Fb; ++b) {
+      if (!c->pact[b]) continue;
+      const int f = Fi + b, i1 = m->owner[f], j1 = c->pj1[b], b2 = c->pbf[b];
+      const double gf = c->pg[b];
+      const double sx = m->Sx[f], sy = m->Sy[f], sz = m->Sz[f];
+      const double ex = c->pux[b], ey = c->puy[b], ez = c->puz[b], cf = c->pdcf[b];
+      const double uf1 = rot(0, c->u[j1], c->v[j1], c->w[j1]), vf1 = rot(1, c->u[j1], c->v[j1], c->w[j1]),
+                   wf1 = rot(2, c->u[j1], c->v[j1], c->w[j1]);
+      const double ub = gf * c->u[i1] + (1. - gf) * uf1, vb = gf * c->v[i1] + (1. - gf) * vf1,
+                   wb = gf * c->w[i1] + (1. - gf) * wf1;
+      const double mbar = ub * sx + vb * sy + wb * sz;
+      double fv = rhof * mbar;
+      double dusf = gf * c->du[i1] + (1. - gf) * c->du[j1];
+      dusf = dusf * sx;
+      double dvsf = gf * c->dv[i1] + (1. - gf) * c->dv[j1];
+      dvsf = dvsf * sy;
+      double dwsf = gf * c->dw[i1] + (1. - gf) * c->dw[j1];
+      dwsf = dwsf * sz;
+      double de[3];
+      const double geo = geo_split(c->decomp, ex, ey, ez, cf, dusf, dvsf, dwsf, de);
+      const double dut = dusf - de[0], dvt = dvsf - de[1], dwt = dwsf - de[2];
+      const double pbx = c->bgp[b], pby = c->bgp[c->Fb + b], pbz = c->bgp[2 * c->Fb + b];
+      double corr = (c->p[j1] - c->p[i1]) / c->pdcf[b];
+      corr = corr - (pbx * c->pux[b] + pby * c->puy[b] + pbz * c->puz[b]);
+      const double pfx = pbx + corr * c->pux[b], pfy = pby + corr * c->puy[b], pfz = pbz + corr * c->puz[b];
+      const double cfl = rhof * geo, ffl = -(rhof * geo);
+      fv = fv - rhof * (pfx * dut + pfy * dvt + pfz * dwt);
+      fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
+      const double uf1s = rot(0, c->us[j1], c->vs[j1], c->ws[j1]), vf1s = rot(1, c->us[j1], c->vs[j1], c->ws[j1]),
+                   wf1s = rot(2, c->us[j1], c->vs[j1], c->ws[j1]);
+      const double usf = gf * c->us[i1] + (1. - gf) * uf1s, vsf = gf * c->vs[i1] + (1. - gf) * vf1s,
+                   wsf = gf * c->ws[i1] + (1. - gf) * wf1s;
+      const double mstar = rhof * (usf * sx + vsf * sy + wsf * sz);
+      fv = fv + (1. - c->urf_u) * (c->bmdot[b] - mstar);
+      c->bmdot[b] = cfl * c->p[i1] + ffl * c->p[j1] + fv;
+      c->bmdot[b2] = -c->bmdot[b];
+    }
   for (int k = 0; k < Fi; ++k) {
     const int i = m->owner[k], j = m->neigh[k];
     const double gf = m->g[k], og = 1. - gf;
@@ -828,8 +1150,109 @@ void or_case_pc_assemble(or_case *c) {
     c->fff[k] = -(rhof * geo);
     c->ftf[k] = c->mdot[k];
   }
-  for (int f = Fi; f < nf; ++f) c->ftf[f] = 0.0; /* Bmdot of walls */
+  for (int f = Fi; f < nf; ++f) c->ftf[f] = c->bmdot[f - Fi]; /* FluxTf = Bmdot (:155-164) */
+  /* periodic faces (AssemblePressureCorrectionFluxes.for:748-830),
+   * translational: the partner's D2 rotated by the identity */
+  if (c->pbf)
+    for (int b = 0; b < c->Fb; ++b) {
+      if (!is_periodic(c, b)) continue;
+      const int f = Fi + b, i1 = m->owner[f], j1 = c->pj1[b];
+      const double *d2u = c->du2 ? c->du2 : c->du, *d2v = c->du2 ? c->dv2 : c->dv, *d2w = c->du2 ? c->dw2 : c->dw;
+      const double gf = c->pg[b];
+      const double sx = m->Sx[f], sy = m->Sy[f], sz = m->Sz[f];
+      double dusf, dvsf, dwsf;
+      if (c->decomp == 3) {
+        dusf = gf * d2u[i1] + (1. - gf) * d2u[j1];
+        dvsf = gf * d2v[i1] + (1. - gf) * d2v[j1];
+        dwsf = gf * d2w[i1] + (1. - gf) * d2w[j1];
+      } else {
+        const double duf1 = rot(0, d2u[j1], d2v[j1], d2w[j1]), dvf1 = rot(1, d2u[j1], d2v[j1], d2w[j1]),
+                     dwf1 = rot(2, d2u[j1], d2v[j1], d2w[j1]);
+        dusf = gf * d2u[i1] + (1. - gf) * duf1;
+        dvsf = gf * d2v[i1] + (1. - gf) * dvf1;
+        dwsf = gf * d2w[i1] + (1. - gf) * dwf1;
+      }
+      dusf = dusf * sx;
+      dvsf = dvsf * sy;
+      dwsf = dwsf * sz;
+      double de[3];
+      const double geo = geo_split(c->decomp, c->pux[b], c->puy[b], c->puz[b], c->pdcf[b], dusf, dvsf, dwsf, de);
+      c->fcf[f] = rhof * geo;
+      c->fff[f] = -(rhof * geo);
+    }
   scatter(c);
+}
+
+/* geoDiff of the D2 S split of a periodic face, as updateperiodicBeta forms
+ * it (PeriodicBeta.for:75-134) */
+static double periodic_beta_geo(const or_case *c, int b) {
+  const or_mesh *m = c->m;
+  const int f = c->Fi + b, i1 = m->owner[f], j1 = c->pj1[b];
+  const double *d2u = c->du2 ? c->du2 : c->du, *d2v = c->du2 ? c->dv2 : c->dv, *d2w = c->du2 ? c->dw2 : c->dw;
+  const double gf = c->pg[b];
+  double dusf = gf * d2u[i1] + (1. - gf) * d2u[j1];
+  dusf = dusf * m->Sx[f];
+  double dvsf = gf * d2v[i1] + (1. - gf) * d2v[j1];
+  dvsf = dvsf * m->Sy[f];
+  double dwsf = gf * d2w[i1] + (1. - gf) * d2w[j1];
+  dwsf = dwsf * m->Sz[f];
+  double de[3];
+  return geo_split(c->decomp, c->pux[b], c->puy[b], c->puz[b], c->pdcf[b], dusf, dvsf, dwsf, de);
+}
+
+/* updateperiodicBeta (PeriodicBeta.for:4-386): the pressure-gradient
+ * correction dp'/dx that brings the mass flow through the lower periodic
+ * patch to periodicMdot, added to periodicBeta, to the lower patch's
+ * boundary velocities and mass fluxes, to every cell velocity along the
+ * translation and to every interior mass flux. */
+static void periodic_beta(or_case *c) {
+  const or_mesh *m = c->m;
+  const int N = c->N, Fi = c->Fi, Fb = c->Fb;
+  const double *d2u = c->du2 ? c->du2 : c->du, *d2v = c->du2 ? c->dv2 : c->dv, *d2w = c->du2 ? c->dw2 : c->dw;
+  double dpdx = 0.;
+  for (int k = 0; k < c->beta_iters; ++k) {
+    double mdin = 0., rgd = 0.;
+    for (int b = 0; b < Fb; ++b) {
+      if (!c->pact[b]) continue;
+      const double rhof = c->rho; /* BDensity */
+      const double geo = periodic_beta_geo(c, b);
+      mdin = mdin + c->bmdot[b];
+      rgd = rgd + rhof * geo;
+    }
+    dpdx = (c->beta_mdot - mdin) / rgd;
+    c->beta = c->beta + c->beta_relax * dpdx;
+  }
+  for (int b = 0; b < Fb; ++b) {
+    if (!c->pact[b]) continue;
+    const int i1 = m->owner[Fi + b], b2 = c->pbf[b];
+    c->bu[b] = c->bu[b] + d2u[i1] * dpdx;
+    c->bv[b] = c->bv[b] + d2v[i1] * dpdx;
+    c->bw[b] = c->bw[b] + d2w[i1] * dpdx;
+    c->bu[b2] = c->bu[b];
+    c->bv[b2] = c->bv[b];
+    c->bw[b2] = c->bw[b];
+    const double rhof = c->rho;
+    const double geo = periodic_beta_geo(c, b);
+    c->bmdot[b] = c->bmdot[b] + rhof * geo * dpdx;
+    c->bmdot[b2] = -c->bmdot[b];
+  }
+  const double lt = sqrt(c->tuse[0] * c->tuse[0] + c->tuse[1] * c->tuse[1] + c->tuse[2] * c->tuse[2]);
+  const double bx = c->tuse[0] / lt, by = c->tuse[1] / lt, bz = c->tuse[2] / lt;
+  for (int i = 0; i < N; ++i) {
+    c->u[i] = c->u[i] + d2u[i] * dpdx * bx;
+    c->v[i] = c->v[i] + d2v[i] * dpdx * by;
+    c->w[i] = c->w[i] + d2w[i] * dpdx * bz;
+  }
+  for (int k = 0; k < Fi; ++k) {
+    const int i = m->owner[k], j = m->neigh[k];
+    const double gf = m->g[k], rhof = c->rho; /* Densityf */
+    const double d1u = gf * d2u[i] + (1. - gf) * d2u[j], d1v = gf * d2v[i] + (1. - gf) * d2v[j],
+                 d1w = gf * d2w[i] + (1. - gf) * d2w[j];
+    double de[3];
+    const double geo = geo_split(c->decomp, m->ex[k], m->ey[k], m->ez[k], m->dcf[k], d1u * m->Sx[k],
+                                 d1v * m->Sy[k], d1w * m->Sz[k], de);
+    c->mdot[k] = c->mdot[k] + rhof * geo * dpdx;
+  }
 }
 
 /* The solve and corrections of SolveContinuity (:76-103):
@@ -847,10 +1270,12 @@ void or_case_pc_solve_correct(or_case *c) {
                                c->pitmax, c->psolver, 4, c->dphi, c->res_pc, hist);
   free(hist);
   for (int b = 0; b < Fb; ++b) c->bpc[b] = c->pc[m->owner[Fi + b]];
+  periodic_values(c, c->pc, c->bpc);
   const double rpc = c->refcell >= 0 ? c->pc[c->refcell] : 0.0;
   for (int i = 0; i < N; ++i) c->p[i] = c->p[i] + c->urf_p * (c->pc[i] - rpc);
   gradient_n(c, c->pc, c->bpc, c->gpc, c->bgpc, c->ngrad_cont);
   if (c->lim_cont) or_limit_gradient(c, c->lim_cont, c->pc, c->bpc, c->gpc, c->bgpc);
+  periodic_gradient(c, c->gpc, c->bgpc);
   for (int i = 0; i < N; ++i) {
     const double d2u = c->du2 ? c->du2[i] : c->du[i], d2v = c->du2 ? c->dv2[i] : c->dv[i],
                  d2w = c->du2 ? c->dw2[i] : c->dw[i];
@@ -863,8 +1288,28 @@ void or_case_pc_solve_correct(or_case *c) {
     c->mdot[k] = c->mdot[k] + c->fcf[k] * c->pc[i] + c->fff[k] * c->pc[j];
   }
   for (int b = 0; b < Fb; ++b) {
+    if (is_periodic(c, b)) continue;
     const int f = Fi + b, i1 = m->owner[f];
     c->bp[b] = c->p[i1] + c->bgp[b] * m->dx[f] + c->bgp[Fb + b] * m->dy[f] + c->bgp[2 * Fb + b] * m->dz[f];
+  }
+  periodic_values(c, c->p, c->bp);
+  if (c->pbf) {
+    /* CorrectBoundarymdot, periodic faces (CorrectMdotandVelocity.for:115-145) */
+    const double *d2u = c->du2 ? c->du2 : c->du, *d2v = c->du2 ? c->dv2 : c->dv, *d2w = c->du2 ? c->dw2 : c->dw;
+    for (int b = 0; b < Fb; ++b) {
+      if (!c->pact[b]) continue;
+      const int f = Fi + b, i1 = m->owner[f], j1 = c->pj1[b], b2 = c->pbf[b];
+      const double rhof = 1.;
+      c->bu[b] = c->bu[b] - d2u[i1] * c->gpc[i1];
+      c->bv[b] = c->bv[b] - d2v[i1] * c->gpc[N + i1];
+      c->bw[b] = c->bw[b] - d2w[i1] * c->gpc[2 * N + i1];
+      c->bmdot[b] = c->bmdot[b] + rhof * c->fcf[f] * c->pc[i1] + rhof * c->fff[f] * c->pc[j1];
+      c->bu[b2] = rot(0, c->bu[b], c->bv[b], c->bw[b]);
+      c->bv[b2] = rot(1, c->bu[b], c->bv[b], c->bw[b]);
+      c->bw[b2] = rot(2, c->bu[b], c->bv[b], c->bw[b]);
+      c->bmdot[b2] = -c->bmdot[b];
+    }
+    if (c->beta_calc && c->iter % c->beta_every == 0) periodic_beta(c);
   }
   or_case_update_symmetry(c); /* SolveContinuityEquation.for:96 */
 }

@@ -84,11 +84,31 @@ typedef struct {
   int decomp; /* MethodDecomposeS of the mesh's E/T split and of Rhie-Chow */
   /* GradientInterpolationSchemeMomentum / Continuity (OR_GI_*) */
   int gi_mom, gi_cont;
+  /* Translational periodicity with the implicit coupling (the case file's
+   * LTranslationalPeriodicity, LPeriodicImplicit; BoundaryConditions2.F90
+   * :130-146).  Per boundary face b: pbf[b] the corresponding face on the
+   * partner patch (Icorrespondingface, a boundary-face index; -1 off the
+   * periodic patches), pact[b] 1 on the lower patch of a pair (the faces
+   * the reference's j2 > i2 branches visit), pj1[b] the partner face's owner;
+   * the geometry every periodic branch computes from the partner's centroid
+   * carried by the partner patch's translation: pg (GFactCF), pdx/pdy/pdz/
+   * pdcf (DistCF), pux/puy/puz (its unit vector), and the E/T split of the
+   * face area by MethodDecomposeS (AssembleIsotropicDiffusion.for:663-700):
+   * pgd (fgDiff), ptx/pty/ptz (FT). */
+  int *pbf, *pact, *pj1;
+  double *pg, *pdx, *pdy, *pdz, *pdcf, *pux, *puy, *puz, *pgd, *ptx, *pty, *ptz;
+  double *bmdot;          /* Bmdot over the boundary faces (0 on walls) */
+  double tuse[3];         /* x/y/zTranslationUse */
+  double beta, beta_mdot, beta_relax; /* periodicBeta, periodicMdot, relaxBeta */
+  int beta_calc, beta_iters, beta_every; /* LcalculateBeta, BetaIterations, nIterCorrectBeta */
+  int nperiodic;
+  int *ppair;             /* partner patch per patch (-1 none) */
+  double *ptrans;         /* x/y/zTranslation per patch */
 } or_case;
 
 enum { OR_GI_AVERAGE = 0, OR_GI_UPWIND = 1, OR_GI_DOWNWIND = 2, OR_GI_AVERAGECORRECTED = 3, OR_GI_HARMONIC = 4 };
 
-enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 1 };
+enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 1, OR_BC_PERIODIC = 2 };
 enum { OR_HR_NONE = 0, OR_HR_TVD = 1, OR_HR_NVF = 2 };
 enum {
   OR_CONV_UPWIND = 0, OR_CONV_DOWNWIND, OR_CONV_FROMM, OR_CONV_SOU, OR_CONV_CD, OR_CONV_QUICK,
@@ -106,6 +126,16 @@ void or_case_delete(or_case *c);
  * symmetry patch p (1-based) with fewer than p faces. */
 int or_case_set_patch_types(or_case *c, int npatches, const int *patch_nfaces, const int *patch_type);
 void or_case_update_symmetry(or_case *c);
+/* Translational periodic pairs after or_case_set_patch_types marked their
+ * patches OR_BC_PERIODIC: pair[q] the partner patch of patch q (0-based, -1
+ * none), trans[3q..3q+2] its x/y/zTranslation.  Pairs the faces and couples
+ * the cells implicitly as RotationMatrixCoefficients does
+ * (CalculateRotationMatrixCoefficients.for:171-291; this modifies the mesh's
+ * ElementNeighbor), then sets the fixed-bulk-flux controls of
+ * updateperiodicBeta (PeriodicBeta.for:4).  Returns -1 when a face finds no
+ * partner or a pair is not mutual. */
+int or_case_set_periodic(or_case *c, int npatches, const int *pair, const double *trans, double beta0,
+                         int calc_beta, double mdot_target, double relax, int iters, int every);
 /* Returns -1 for a scheme the framework does not define (or a steady
  * single-phase case cannot use) and for an unknown framework. */
 int or_case_set_convection(or_case *c, int framework, int scheme, double bleed, int start_iter);

@@ -227,7 +227,7 @@ def ref_simple_available():
     return os.path.exists(REF_SIMPLE)
 
 
-def _simple_input(f, pm, mesh, mode, extra_ints=(), extra_doubles=(), tail=(), decomp=3):
+def _simple_input(f, pm, mesh, mode, extra_ints=(), extra_doubles=(), tail=(), decomp=3, after_doubles=b""):
     """The polyMesh and the Readit topology of ``mesh`` (an OrMesh of ``pm``)
     in ref_simple_driver.f90's input layout."""
     s = mesh.ptr.contents
@@ -248,6 +248,7 @@ def _simple_input(f, pm, mesh, mode, extra_ints=(), extra_doubles=(), tail=(), d
                      np.int32).tobytes())
     if extra_doubles:
         f.write(np.array(extra_doubles, np.float64).tobytes())
+    f.write(after_doubles)
     for d in range(3):
         f.write(_c(pts[:, d], np.float64).tobytes())
     f.write(fnn.tobytes())
@@ -297,25 +298,37 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
                          urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1, mitmax=20, psolver=SOR, prrf=0.01,
                          pitmax=200, framework="none", scheme="upwind", bleed=0.0, start_iter=0, lim_mom=0,
                          lim_cont=0, plmg=0, algorithm="simple", grad_mom=(2, 2), grad_cont=(2, 2),
-                         decomp=3, gi_mom="average", gi_cont="averagecorrected", restart=False, seconds=False):
+                         decomp=3, gi_mom="average", gi_cont="averagecorrected", restart=False, seconds=False,
+                         periodic=None):
     """niter outer iterations of the reference's own SIMPLE (SolveMomentumX/Y/Z,
     SolveContinuity) from rest on the polyMesh ``pm`` with the Readit topology of
     ``mesh``; patch_types per patch "wall"/"symmetry" (default all walls);
     plmg = 1 solves p' with the reference's algebraic multigrid
     (LMultigridContinuity).  Returns a dict u, v, w, p (N) and mdot (Fi), and
-    with seconds=True also the run's wall time ("seconds")."""
+    with seconds=True also the run's wall time ("seconds").  periodic: a
+    dict pairs (partner patch per patch, -1 none), translations (3 per
+    patch), beta (initial periodicBeta), mdot (periodicMdot), calc_beta,
+    relax, iters, every (LcalculateBeta, relaxBeta, BetaIterations,
+    nIterCorrectBeta); the run then also returns "beta" and "bmdot" (the
+    boundary mass fluxes in boundary-face order)."""
     nf, nif, N = mesh.nf, mesh.nif, mesh.N
-    codes = np.array([{"wall": 0, "symmetry": 1}[t] for t in (patch_types or ["wall"] * len(pm.patch_sizes))],
-                     np.int32)
+    codes = np.array([PATCH_CODES[t] for t in (patch_types or ["wall"] * len(pm.patch_sizes))], np.int32)
     ints = (msolver, mitmax, psolver, pitmax, niter, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], lim_mom,
             lim_cont, start_iter, plmg, {"simple": 0, "simplec": 1}[algorithm], *grad_mom, *grad_cont,
-            GRADIENT_INTERPOLATION[gi_mom], GRADIENT_INTERPOLATION[gi_cont])
+            GRADIENT_INTERPOLATION[gi_mom], GRADIENT_INTERPOLATION[gi_cont], 1 if periodic else 0)
     dbl = (rho, mu, urf_u, urf_p, mrrf, prrf, bleed)
+    blob = b""
+    if periodic:
+        blob = (np.asarray(periodic["pairs"], np.int32) + 1).tobytes() + \
+            np.asarray(periodic["translations"], np.float64).ravel().tobytes() + \
+            np.array([periodic.get("beta", 0.0), periodic["mdot"], periodic.get("relax", 1.0)], np.float64).tobytes() + \
+            np.array([int(periodic.get("calc_beta", 1)), periodic.get("iters", 1), periodic.get("every", 1)],
+                     np.int32).tobytes()
     with tempfile.TemporaryDirectory(dir=os.path.join(HERE, "_ref")) as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
             _simple_input(f, pm, mesh, 2, extra_ints=ints, extra_doubles=dbl, tail=(codes.tobytes(),),
-                          decomp=decomp)
+                          decomp=decomp, after_doubles=blob)
             for a in (wall_u, wall_v, wall_w):
                 f.write(_c(a, np.float64).tobytes())
         t0 = time.perf_counter()
@@ -330,6 +343,9 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
         for k, n in (("u", N), ("v", N), ("w", N), ("p", N), ("mdot", nif)):
             out[k] = d[o:o + n].copy()
             o += n
+        if periodic:
+            out["beta"] = float(d[o])
+            out["bmdot"] = d[o + 1:o + 1 + nf - nif].copy()
         if restart:
             out["restart"] = open(fout + ".restart", "rb").read()
     if seconds:
@@ -507,8 +523,16 @@ class _OrCase(C.Structure):
                [(n, C.c_int) for n in ("hr_fw", "hr_scheme", "hr_start", "lim_mom", "lim_cont", "iter")] + \
                [("bleed", C.c_double)] + \
                [(n, C.POINTER(C.c_double)) for n in ("du2", "dv2", "dw2")] + \
-               [("ngrad_cont", C.c_int), ("decomp", C.c_int), ("gi_mom", C.c_int), ("gi_cont", C.c_int)]
+               [("ngrad_cont", C.c_int), ("decomp", C.c_int), ("gi_mom", C.c_int), ("gi_cont", C.c_int)] + \
+               [(n, C.POINTER(C.c_int)) for n in ("pbf", "pact", "pj1")] + \
+               [(n, C.POINTER(C.c_double)) for n in (
+                   "pg", "pdx", "pdy", "pdz", "pdcf", "pux", "puy", "puz", "pgd", "ptx", "pty", "ptz", "bmdot")] + \
+               [("tuse", C.c_double * 3)] + [(n, C.c_double) for n in ("beta", "beta_mdot", "beta_relax")] + \
+               [(n, C.c_int) for n in ("beta_calc", "beta_iters", "beta_every", "nperiodic")] + \
+               [("ppair", C.POINTER(C.c_int)), ("ptrans", C.POINTER(C.c_double))]
 
+
+PATCH_CODES = {"wall": 0, "symmetry": 1, "periodic": 2}
 
 GRADIENT_INTERPOLATION = {"average": 0, "upwind": 1, "downwind": 2, "averagecorrected": 3, "harmonic": 4}
 
@@ -526,7 +550,8 @@ class OrCase:
              "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bpc": "Fb",
              "bgu": "3Fb", "bgv": "3Fb", "bgw": "3Fb", "bgp": "3Fb", "fgu": "3Fi", "fgv": "3Fi", "fgw": "3Fi",
              "fgp": "3Fi", "du": "N", "dv": "N", "dw": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
-             "anb": "NW", "bc": "N", "fcf": "F", "fff": "F", "ftf": "F", "fce": "N", "fte": "N"}
+             "anb": "NW", "bc": "N", "fcf": "F", "fff": "F", "ftf": "F", "fce": "N", "fte": "N", "bmdot": "Fb",
+             "bgpc": "3Fb", "gpc": "3N"}
 
     def __init__(self, pm, bu, bv, bw, rho=1.0, mu=0.01, urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1,
                  mitmax=20, psolver=SOR, prrf=0.01, pitmax=200):
@@ -565,13 +590,30 @@ class OrCase:
         self.view(name)[:] = values
 
     def set_patch_types(self, patch_sizes, patch_types):
-        """Boundary condition per patch, in patch order: "wall" or "symmetry"."""
+        """Boundary condition per patch, in patch order: "wall", "symmetry" or
+        "periodic" (then set_periodic)."""
         L = lib()
         L.or_case_set_patch_types.argtypes = [C.POINTER(_OrCase), C.c_int, _ip, _ip]
-        codes = np.array([{"wall": 0, "symmetry": 1}[t] for t in patch_types], np.int32)
+        codes = np.array([PATCH_CODES[t] for t in patch_types], np.int32)
         sizes = np.ascontiguousarray(patch_sizes, dtype=np.int32)
         if L.or_case_set_patch_types(self.ptr, len(sizes), _c(sizes, np.int32), _c(codes, np.int32)) != 0:
             raise ValueError("or_case_set_patch_types: patches do not fit the boundary")
+
+    def set_periodic(self, pairs, translations, beta=0.0, mdot=0.0, calc_beta=True, relax=1.0, iters=1, every=1):
+        """Translational periodic pairs with the implicit coupling and the
+        fixed-bulk-flux controls (or_case_set_periodic)."""
+        L = lib()
+        L.or_case_set_periodic.argtypes = [C.POINTER(_OrCase), C.c_int, _ip, _dp, C.c_double, C.c_int, C.c_double,
+                                           C.c_double, C.c_int, C.c_int]
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32)
+        tr = np.ascontiguousarray(np.asarray(translations, np.float64).ravel())
+        if L.or_case_set_periodic(self.ptr, len(pairs), _c(pairs, np.int32), _c(tr, np.float64), beta,
+                                  int(calc_beta), mdot, relax, iters, every) != 0:
+            raise ValueError("or_case_set_periodic: the periodic patches do not pair")
+
+    @property
+    def beta(self):
+        return self.ptr.contents.beta
 
     def set_convection(self, framework, scheme, bleed=0.0, start_iter=0):
         """HRFramework ('none', 'tvd', 'nvf'), ConvectionScheme, Bleed and

@@ -34,14 +34,21 @@ program ref_simple_driver
   use Geometry3, only: NIFaces, NFacesTotal, NIFaceOwner, NIFaceNeighbor, NBFaces, NBFaceOwner, &
                        NGlobalEFaces, ElementNeighbor, NumberofElementNeighbors, NElementFaces, &
                        GlobalFaceNumberOfNodes, NIFaceNodes, NBFaceNodes, BCSet, BCRecord, &
-                       iOwnerNeighbor, iNeighborOwner, NBFacesMax
+                       iOwnerNeighbor, iNeighborOwner, NBFacesMax, NBFaceNeighbor
   use Geometry4
+  use BoundaryConditions2, only: PeriodicPair, xTranslation, yTranslation, zTranslation, &
+                                 LTranslationalPeriodicity, LRotationalPeriodicity, LPeriodicImplicit, &
+                                 LcalculateBeta, periodicBeta, periodicMdot, relaxBeta, BetaIterations, &
+                                 nIterCorrectBeta
   implicit none
   character(len=1024) :: fin, fout
   integer :: npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode, idecomp
   integer :: i, j, k, f, q, nbfmax, nb, it
   integer :: msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec
-  integer :: gmm, gmi, gcm, gci, gim, gic
+  integer :: gmm, gmi, gcm, gci, gim, gic, iper, bcalc, biters, bevery
+  integer, allocatable :: perpair(:)
+  double precision, allocatable :: pertrans(:)
+  double precision :: beta0, bmdot0, brelax
   character(len=16), parameter :: gradinterp(0:4) = (/ 'average         ', 'upwind          ', &
       'downwind        ', 'averagecorrected', 'harmonic        ' /)
   double precision :: rho, mu, urfu, urfp, mrrf, prrf, bleed
@@ -59,8 +66,18 @@ program ref_simple_driver
   read(21) npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode, idecomp
   if (mode == 2) then
     read(21) msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec, &
-             gmm, gmi, gcm, gci, gim, gic
+             gmm, gmi, gcm, gci, gim, gic, iper
     read(21) rho, mu, urfu, urfp, mrrf, prrf, bleed
+    if (iper /= 0) then
+      ! translational periodic pairs (the case file's block of
+      ! ProblemDescriptionRoutine.for:1339-1371): partner patch (1-based, 0
+      ! none) and x/y/zTranslation per patch, the bulk-flux controls
+      allocate(perpair(npatches), pertrans(3 * npatches))
+      read(21) perpair
+      read(21) pertrans
+      read(21) beta0, bmdot0, brelax
+      read(21) bcalc, biters, bevery
+    end if
   end if
   allocate(fnn(nfaces), fnodes(nfaces, maxfn), owner(nfaces), neigh(nif), psize(npatches), ptype(npatches))
   NumberOfNodes = npoints
@@ -98,6 +115,8 @@ program ref_simple_driver
   nbfmax = maxval(psize)
   allocate(NIFaceOwner(nif), NIFaceNeighbor(nif), NIFaceNodes(nif, maxfn))
   allocate(NBFaces(npatches), NBFaceOwner(npatches, nbfmax), NBFaceNodes(npatches, nbfmax, maxfn))
+  allocate(NBFaceNeighbor(npatches, nbfmax))
+  NBFaceNeighbor = -1
   allocate(GlobalFaceNumberOfNodes(nfaces), BCSet(nfaces - nif), BCRecord(nfaces - nif))
   GlobalFaceNumberOfNodes = fnn
   do f = 1, nif
@@ -155,6 +174,14 @@ program ref_simple_driver
   if (mode == 2) then
     call run_simple
     write(22) uVelocity, vVelocity, wVelocity, Pressure, mdot
+    if (iper /= 0) then
+      write(22) periodicBeta
+      do q = 1, npatches
+        do j = 1, psize(q)
+          write(22) Bmdot(q, j)
+        end do
+      end do
+    end if
     ! the reference's restart file (WriteSolutionToContinue.for:3) of the
     ! final state, on unit 4 as MainProgram.for:466 opens it
     open(unit=4, file=trim(fout)//'.restart', form='unformatted', status='replace')
@@ -227,6 +254,8 @@ contains
     do q = 1, npatches
       if (ptype(q) == 1) then
         BoundaryType(q) = 'symmetry'
+      else if (ptype(q) == 2) then
+        BoundaryType(q) = 'periodic'
       else
         BoundaryType(q) = 'wall'
       end if
@@ -243,6 +272,25 @@ contains
     end do
     call SetBoundaryConditions
     call SetBCPatches
+    LRotationalPeriodicity = .false.
+    LTranslationalPeriodicity = iper /= 0
+    if (iper /= 0) then
+      LPeriodicImplicit = .true.
+      do q = 1, npatches
+        PeriodicPair(q) = perpair(q)
+        xTranslation(q) = pertrans(3 * q - 2)
+        yTranslation(q) = pertrans(3 * q - 1)
+        zTranslation(q) = pertrans(3 * q)
+      end do
+      LcalculateBeta = bcalc /= 0
+      periodicBeta = beta0
+      periodicMdot = bmdot0
+      relaxBeta = brelax
+      BetaIterations = biters
+      nIterCorrectBeta = bevery
+      ! MainProgram.for:66
+      call RotationMatrixCoefficients
+    end if
     call allocate_flow_storage
     f = 0
     do q = 1, npatches

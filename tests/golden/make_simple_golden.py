@@ -8,7 +8,9 @@ mesh reader, allocateStorage and the case file) on meshes from the package's
 generator, and stores the inputs that name each case and the outputs as .npz.
 Run in the build container only (needs /root/reference):
 
-    make -C oracle ref && python tests/golden/make_simple_golden.py
+    make -C oracle ref && python tests/golden/make_simple_golden.py [case ...]
+
+(with case names, only those SIMPLE fixtures are rewritten).
 """
 import os
 import sys
@@ -58,6 +60,22 @@ SIMPLE = {
                         dict(niter=8, msolver=po.ILU, psolver=po.ILU, framework="nvf", scheme="gamma",
                              decomp=1, grad_mom=(1, 2), grad_cont=(2, 3), gi_mom="averagecorrected",
                              gi_cont="harmonic")),
+    # the periodic channel (graded Kuhn tetrahedra, translational periodic
+    # pair, walls, symmetry sides): the fixed-bulk-flux update every
+    # iteration, and a prescribed pressure gradient.  No gradient limiter:
+    # with the partner cell in ElementNeighbor, LimitGradients.for:132-140
+    # reads FaceCentroidx(kf) at the periodic face's number, beyond the
+    # NIFaces entries ProcessGeometryRoutine.for:35 allocates
+    "channel_ilu": (("channel_tet", (4, 6, 3)),
+                    dict(niter=8, msolver=po.ILU, psolver=po.ILU, prrf=1e-8, pitmax=500,
+                         periodic=dict(beta=0.0, mdot=0.5, calc_beta=1, relax=1.0, iters=1, every=1))),
+    "channel_sor_beta": (("channel_tet", (4, 6, 3)),
+                         dict(niter=8, decomp=1, periodic=dict(beta=0.2, mdot=0.3, calc_beta=1, relax=0.5, iters=2,
+                                                               every=2))),
+    "channel_ilu_fixed_beta": (("channel_tet", (4, 6, 3)),
+                               dict(niter=8, msolver=po.ILU, psolver=po.ILU, prrf=1e-8, pitmax=500,
+                                    framework="tvd", scheme="sou", algorithm="simplec",
+                                    periodic=dict(beta=0.1, mdot=0.0, calc_beta=0))),
     # converged states (the device is compared with these)
     "hex_converged": (("box_hex", (8, 7, 6), 0.15, 3), dict(niter=400)),
     "tet_converged": (("box_tet", (4, 4, 3), 0.15, 4), dict(niter=400)),
@@ -71,7 +89,14 @@ SIMPLE = {
 }
 
 
+CHANNEL_BOX = dict(lx=2.0, ly=2.0, lz=1.0)
+
+
 def build_mesh(spec):
+    if spec[0] == "channel_tet":
+        pm, types, pairs, tr = meshgen.channel_tet(*spec[1], **CHANNEL_BOX)
+        z = np.zeros(pm.n_faces - pm.n_internal)
+        return pm, z, z, z, types
     if spec[0] == "cavity_2d":
         pm, (bu, bv, bw), types = meshgen.cavity_2d(spec[1])
         return pm, bu, bv, bw, types
@@ -81,8 +106,20 @@ def build_mesh(spec):
     return pm, bu, bv, bw, None
 
 
-def main():
+def periodic_settings(spec, kw):
+    """The case's periodic dict completed with the channel's pairs and
+    translations (None off the channel)."""
+    if "periodic" not in kw:
+        return None
+    _, _, pairs, tr = meshgen.channel_tet(*spec[1], **CHANNEL_BOX)
+    return dict(kw["periodic"], pairs=pairs, translations=tr)
+
+
+def main(only=()):
     assert po.ref_simple_available(), "build oracle/_ref/ref_simple.x first (make -C oracle ref)"
+    if only:
+        write_simple(only)
+        return
     out = {}
     for name, (fn, n, perturb, seed) in GEOMETRY.items():
         pm = getattr(meshgen, fn)(*n, perturb=perturb, seed=seed)
@@ -92,12 +129,20 @@ def main():
                 out[f"{name}_{k}" if dec == 3 else f"{name}_dec{dec}_{k}"] = v
     np.savez_compressed(os.path.join(HERE, "geometry_ref.npz"), **out)
     print("wrote geometry_ref.npz")
-    for name, (spec, kw) in SIMPLE.items():
+    write_simple(SIMPLE)
+
+
+def write_simple(names):
+    for name in names:
+        spec, kw = SIMPLE[name]
+        kw = dict(kw)
+        if "periodic" in kw:
+            kw["periodic"] = periodic_settings(spec, kw)
         pm, bu, bv, bw, types = build_mesh(spec)
         mesh = po.OrMesh(pm)
         # the reference's restart file (writeSolution) of the cases the device
         # reproduces bitwise
-        restart = "_ilu" in name
+        restart = "_ilu" in name and not name.startswith("channel")
         r = po.run_reference_simple(pm, mesh, bu, bv, bw, patch_types=types, mu=0.01, restart=restart, **kw)
         if restart:
             r["restart"] = np.frombuffer(r["restart"], np.uint8)
@@ -107,4 +152,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

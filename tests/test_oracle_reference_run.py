@@ -69,6 +69,11 @@ def _oracle_case(name):
     oc.set_gradient_method(kw.get("grad_mom", (2, 2)), kw.get("grad_cont", (2, 2)))
     oc.set_decompose(kw.get("decomp", 3))
     oc.set_gradient_interpolation(kw.get("gi_mom", "average"), kw.get("gi_cont", "averagecorrected"))
+    per = msg.periodic_settings(spec, kw)
+    if per:
+        oc.set_periodic(per["pairs"], per["translations"], beta=per.get("beta", 0.0), mdot=per["mdot"],
+                        calc_beta=per.get("calc_beta", 1), relax=per.get("relax", 1.0), iters=per.get("iters", 1),
+                        every=per.get("every", 1))
     return oc, niter
 
 
@@ -80,4 +85,6 @@ def test_simple_iterations_match_the_reference_build(name):
     oc.iterate(niter)
     for k in ("u", "v", "w", "p", "mdot"):
         assert np.array_equal(oc.get(k), g[k]), (name, k, np.abs(oc.get(k) - g[k]).max())
-    assert np.abs(g["u"]).max() > 0.1 and np.abs(g["p"]).max() > 0.01
+    if "beta" in g:
+        assert oc.beta == float(g["beta"]) and np.array_equal(oc.get("bmdot"), g["bmdot"])
+    assert np.abs(g["u"]).max() > 0.01 and np.abs(g["p"]).max() > 1e-3

@@ -352,7 +352,7 @@ class Geometry:
         return out.reshape(self.W, self.N) if name in ("gef", "enbr") else out
 
 
-BC_TYPES = {"wall": 0, "symmetry": 1}
+BC_TYPES = {"wall": 0, "symmetry": 1, "periodic": 2}
 HR_FRAMEWORKS = {"none": 0, "tvd": 1, "nvf": 2}
 CONV_SCHEMES = {name: i for i, name in enumerate(
     ("upwind", "downwind", "fromm", "sou", "cd", "quick", "superbee", "minmod", "osher", "vanleer", "muscl",
@@ -367,7 +367,7 @@ class Case:
              "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bgp": "3Fb", "du": "N", "dv": "N", "dw": "N", "du2": "N", "dv2": "N", "dw2": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
              "anb": "NW", "bc": "N", "cf": "F", "ff": "F", "tf": "F", "vol": "N", "Sx": "F", "Sy": "F",
              "Sz": "F", "g": "F", "gdiff": "F", "Tx": "F", "Ty": "F", "Tz": "F", "dcf": "F", "A": "F",
-             "kx": "Fi", "ky": "Fi", "kz": "Fi"}
+             "kx": "Fi", "ky": "Fi", "kz": "Fi", "bmdot": "Fb", "beta": "1"}
 
     def __init__(self, pm, wall_u, wall_v, wall_w, rho=1.0, mu=0.01):
         L = lib()
@@ -384,7 +384,7 @@ class Case:
         N, W, F, Fi = (int(v) for v in d)
         self.N, self.W, self.F, self.Fi, self.Fb = N, W, F, Fi, F - Fi
         self.dims = {"N": N, "3N": 3 * N, "Fi": Fi, "12Fi": 12 * Fi, "Fb": F - Fi, "3Fb": 3 * (F - Fi),
-                     "F": F, "NW": N * W}
+                     "F": F, "NW": N * W, "1": 1}
 
     def close(self):
         if getattr(self, "h", None):
@@ -450,6 +450,24 @@ class Case:
         codes = np.array([BC_TYPES[t] if isinstance(t, str) else int(t) for t in patch_types], np.int32)
         _check(lib().fs_case_set_patch_types(self.h, len(sizes), _iptr(sizes), _iptr(codes)),
                "fs_case_set_patch_types")
+
+    def set_periodic(self, patch_sizes, pairs, translations, beta=0.0, calc_beta=False, mdot=0.0, relax=1.0,
+                     iters=1, every=1):
+        """Translational periodic pairs with the implicit coupling on the
+        patches set_patch_types marked "periodic" (fs_case_set_periodic):
+        pairs[q] the partner patch (-1 none), translations[q] its (x, y, z)
+        translation; beta the pressure gradient along the translation
+        (periodicBeta), updated towards the outward mass flow `mdot` of the
+        lower periodic patch when calc_beta."""
+        L = lib()
+        L.fs_case_set_periodic.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, C.c_double, C.c_int, C.c_double,
+                                           C.c_double, C.c_int, C.c_int]
+        sizes = np.ascontiguousarray(patch_sizes, dtype=np.int32)
+        pr = np.ascontiguousarray(pairs, dtype=np.int32)
+        tr = np.ascontiguousarray(np.asarray(translations, np.float64).ravel())
+        _check(L.fs_case_set_periodic(self.h, len(sizes), _iptr(sizes), _iptr(pr), _dptr(tr), float(beta),
+                                      int(bool(calc_beta)), float(mdot), float(relax), int(iters), int(every)),
+               "fs_case_set_periodic")
 
     def set_convection(self, framework, scheme, bleed=0.0, start_iter=0):
         """HRFrameworkMomentum ("none", "tvd", "nvf"), ConvectionSchemeMomentum

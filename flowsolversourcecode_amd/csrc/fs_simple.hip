@@ -190,6 +190,7 @@ __global__ void k_wall_pgrad(int Fb, int Fi, int N, const int *__restrict__ own,
                              double *__restrict__ bgp) {
   const int b = blockIdx.x * kBlock + threadIdx.x;
   if (b >= Fb) return;
+  if (btype && btype[b] == FS_BC_PERIODIC) return; // the periodic correction's faces
   const int f = Fi + b, i1 = own[f];
   const double x = nx[f], y = ny[f], z = nz[f];
   const double gx = gp[i1], gy = gp[N + i1], gz = gp[2 * N + i1];
@@ -516,9 +517,10 @@ __global__ void k_mom_wall(int Fb, int Fi, int N, const int *__restrict__ own,
                            const double *__restrict__ dx, const double *__restrict__ dy,
                            const double *__restrict__ dz, const double *__restrict__ Sx,
                            const double *__restrict__ Sy, const double *__restrict__ Sz, MomArgs a,
-                           double *__restrict__ cf, double *__restrict__ tf) {
+                           double *__restrict__ cf, double *__restrict__ ff, double *__restrict__ tf) {
   const int b = blockIdx.x * kBlock + threadIdx.x;
   if (b >= Fb) return;
+  if (a.btype && a.btype[b] == FS_BC_PERIODIC) return; // k_per_momentum's faces
   const int f = Fi + b, i1 = own[f], c = a.comp;
   const double gamf = a.mu;
   const double x = nx[f], y = ny[f], z = nz[f], area = A[f];
@@ -562,6 +564,7 @@ __global__ void k_mom_wall(int Fb, int Fi, int N, const int *__restrict__ own,
   C = C + 0.;
   T = T + v3;
   cf[f] = C;
+  ff[f] = 0.0;
   tf[f] = T;
 }
 
@@ -572,6 +575,11 @@ __global__ void k_mom_wall(int Fb, int Fi, int N, const int *__restrict__ own,
 // (UnderRelaxation.for) and D = V/a_P, u* = u (StoreTheDValues.for).
 // One slot of the momentum gather (face fluxes cf, ff, tf and mass flux md
 // of the slot's face).
+// PER: periodic faces are boundary faces with the partner cell in the slot
+// (LPeriodicImplicit): the slot takes FluxFf (AssembleGolbalElementAndFace
+// Fluxes.for:99-121) and the effective divergence the face's Bmdot; wall and
+// symmetry faces carry FluxFf = 0 and Bmdot = 0.
+template <bool PER>
 __device__ __forceinline__ double mom_slot(int role, double vc, double vf, double vt, double md, double &a,
                                            double &b, double &e) {
   double an = 0.0;
@@ -587,16 +595,19 @@ __device__ __forceinline__ double mom_slot(int role, double vc, double vf, doubl
     e = e - md;
   } else if (role == ROLE_BOUND) {
     a = a + vc;
+    if constexpr (PER) an = an + vf;
     b = b - vt;
-    e = e + 0.;
+    if constexpr (PER) e = e + md;
+    else e = e + 0.;
   }
   return an;
 }
 
-template <int WC>
-__global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
+template <int WC, bool PER>
+__global__ void k_mom_cell(int N, int W, int Fi, const int *__restrict__ code,
                            const double *__restrict__ cf, const double *__restrict__ ff,
                            const double *__restrict__ tf, const double *__restrict__ mdot,
+                           const double *__restrict__ bmdot, const double *__restrict__ lin,
                            const double *__restrict__ phi, const double *__restrict__ gpc,
                            const double *__restrict__ vol, double urf, const int *__restrict__ perm,
                            double *__restrict__ ac, double *__restrict__ anb, double *__restrict__ bc,
@@ -620,16 +631,19 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
       vc[s] = cf[f];
       vf[s] = ff[f];
       vt[s] = tf[f];
-      vm[s] = mdot[(role == ROLE_OWNER || role == ROLE_NEIGH) ? f : 0];
+      if (PER && role == ROLE_BOUND) vm[s] = bmdot[f - Fi];
+      else vm[s] = mdot[(role == ROLE_OWNER || role == ROLE_NEIGH) ? f : 0];
     }
 #pragma unroll
-    for (int s = 0; s < WC; ++s) anb[(size_t)s * N + q] = mom_slot(cd[s] & 3, vc[s], vf[s], vt[s], vm[s], a, b, e);
+    for (int s = 0; s < WC; ++s)
+      anb[(size_t)s * N + q] = mom_slot<PER>(cd[s] & 3, vc[s], vf[s], vt[s], vm[s], a, b, e);
   } else {
     for (int s = 0; s < W; ++s) {
       const int cd = code[(size_t)s * N + c];
       const int role = cd & 3, f = cd >> 2;
-      const double md = (role == ROLE_OWNER || role == ROLE_NEIGH) ? mdot[f] : 0.0;
-      anb[(size_t)s * N + q] = mom_slot(role, cf[f], ff[f], tf[f], md, a, b, e);
+      const double md = (role == ROLE_OWNER || role == ROLE_NEIGH) ? mdot[f]
+                        : (PER && role == ROLE_BOUND) ? bmdot[f - Fi] : 0.0;
+      anb[(size_t)s * N + q] = mom_slot<PER>(role, cf[f], ff[f], tf[f], md, a, b, e);
     }
   }
   if (eff_out) eff_out[c] = e; // effdiv of the restart file
@@ -637,6 +651,7 @@ __global__ void k_mom_cell(int N, int W, const int *__restrict__ code,
   const double ep = e > 0.0 ? e : 0.0;
   const double fce = 0.0 + (ep - e);
   double fte = 0.0 + ((ep - e) * pc + (-(ep * pc)));
+  if constexpr (PER) fte = fte - lin[0] * vol[c]; // AssembleLinearPressureTerm
   fte = fte + gpc[c] * vol[c];
   a = a + fce;
   b = b - fte;
@@ -769,7 +784,9 @@ __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
   tf[f] = m;
 }
 
-// p' system gather (walls carry no flux: FluxCf = 0, Bmdot = 0).
+// p' system gather (walls carry no flux: FluxCf = 0, Bmdot = 0; PER: the
+// boundary faces' fluxes of k_per_pc_boundary, FluxFf into the slot).
+template <bool PER>
 __device__ __forceinline__ double pc_slot(int role, double vc, double vf, double vt, double &a, double &b) {
   double an = 0.0;
   if (role == ROLE_OWNER) {
@@ -780,11 +797,15 @@ __device__ __forceinline__ double pc_slot(int role, double vc, double vf, double
     a = a - vf;
     an = an - vc;
     b = b + vt;
+  } else if (PER && role == ROLE_BOUND) {
+    a = a + vc;
+    an = an + vf;
+    b = b - vt;
   }
   return an;
 }
 
-template <int WC>
+template <int WC, bool PER>
 __global__ void k_pc_cell(int N, int W, const int *__restrict__ code,
                           const double *__restrict__ cf, const double *__restrict__ ff,
                           const double *__restrict__ tf, double *__restrict__ ac,
@@ -805,12 +826,12 @@ __global__ void k_pc_cell(int N, int W, const int *__restrict__ code,
       vt[s] = tf[f];
     }
 #pragma unroll
-    for (int s = 0; s < WC; ++s) anb[(size_t)s * N + c] = pc_slot(cd[s] & 3, vc[s], vf[s], vt[s], a, b);
+    for (int s = 0; s < WC; ++s) anb[(size_t)s * N + c] = pc_slot<PER>(cd[s] & 3, vc[s], vf[s], vt[s], a, b);
   } else {
     for (int s = 0; s < W; ++s) {
       const int cd = code[(size_t)s * N + c];
       const int f = cd >> 2;
-      anb[(size_t)s * N + c] = pc_slot(cd & 3, cf[f], ff[f], tf[f], a, b);
+      anb[(size_t)s * N + c] = pc_slot<PER>(cd & 3, cf[f], ff[f], tf[f], a, b);
     }
   }
   ac[c] = a;
@@ -850,9 +871,10 @@ __global__ void k_correct_mdot(int Fi, const int *__restrict__ own, const int *_
 __global__ void k_wall_p(int Fb, int Fi, const int *__restrict__ own, const double *__restrict__ dx,
                          const double *__restrict__ dy, const double *__restrict__ dz,
                          const double *__restrict__ p, const double *__restrict__ bgp,
-                         double *__restrict__ bp) {
+                         const int *__restrict__ btype, double *__restrict__ bp) {
   const int b = blockIdx.x * kBlock + threadIdx.x;
   if (b >= Fb) return;
+  if (btype && btype[b] == FS_BC_PERIODIC) return; // interpolated between the pair's cells
   const int f = Fi + b;
   bp[b] = p[own[f]] + bgp[b] * dx[f] + bgp[Fb + b] * dy[f] + bgp[2 * Fb + b] * dz[f];
 }
@@ -876,8 +898,9 @@ __global__ void k_update_symmetry(int Fb, int Fi, const int *__restrict__ own,
 }
 
 // L1 of the cell mass imbalance: per-cell sum of outgoing mdot in slot order
-__global__ void k_mass_l1(int N, int W, const int *__restrict__ code,
-                          const double *__restrict__ mdot, double *__restrict__ part) {
+__global__ void k_mass_l1(int N, int W, int Fi, const int *__restrict__ code,
+                          const double *__restrict__ mdot, const double *__restrict__ bmdot,
+                          double *__restrict__ part) {
   double s = 0.0;
   for (int c = blockIdx.x * kBlock + threadIdx.x; c < N; c += gridDim.x * kBlock) {
     double r = 0.0;
@@ -886,6 +909,7 @@ __global__ void k_mass_l1(int N, int W, const int *__restrict__ code,
       const int role = cd & 3, f = cd >> 2;
       if (role == ROLE_OWNER) r = r + mdot[f];
       else if (role == ROLE_NEIGH) r = r - mdot[f];
+      else if (role == ROLE_BOUND && bmdot) r = r + bmdot[f - Fi];
     }
     s += fabs(r);
   }
@@ -919,6 +943,7 @@ struct SimpleCase {
   int decomp = 3;                  // MethodDecomposeS of the E/T split and of Rhie-Chow
   int gi_mom = FS_GRADINT_AVERAGE, gi_cont = FS_GRADINT_AVERAGECORRECTED; // face-gradient schemes
   std::vector<int> h_own;          // owner of every face (host copy, 0-based)
+  std::vector<int> h_nei;          // neighbour of every interior face
   bool simplec = false;            // Algorithm 'simplec': D2 in du2/dv2/dw2
   // restart-file state the iteration forms but does not keep: the face
   // gradients of u, v, w and p and the effective divergence
@@ -943,6 +968,14 @@ struct SimpleCase {
   fs_simple_stats last{};
   fs_mpnn *warm = nullptr;  // p' initial guess from the network
   fs_mpnn *sink = nullptr;  // p' snapshots into the network's dataset
+  // translational periodicity: the pairs, the boundary mass fluxes (Bmdot,
+  // zero off the periodic faces), the periodic faces' boundary gradients of
+  // u, v, w, the boundary p', and per boundary face its periodic list index
+  Periodic per;
+  DevBuf<double> bmdot, bgu, bgv, bgw, bpc;
+  DevBuf<int> pk;
+  bool periodic() const { return per.np > 0; }
+  PerArgs pargs() const { return periodic_args(per, N, Fi, Fb, own.p); }
 };
 
 static void upload_geometry(SimpleCase &C) {
@@ -1038,6 +1071,13 @@ static void phase_momentum_gradients(SimpleCase &C) {
     const int mc[1] = {C.lim_cont};
     limit_gradients<1>(C, b, mc);
   }
+  if (C.periodic()) {
+    // CorrectGradientAtPeriodicBoundary after the limiters, before the
+    // pressure's symmetry and wall projections (CalculateGradients.for:263)
+    const double *cg[4] = {C.gu.p, C.gv.p, C.gw.p, C.gp.p};
+    double *bg[4] = {C.bgu.p, C.bgv.p, C.bgw.p, C.bgp.p};
+    periodic_bgrad(C.pargs(), C.s, 4, cg, bg);
+  }
   hipLaunchKernelGGL(k_wall_pgrad, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N,
                      C.own.p, C.nx.p, C.ny.p, C.nz.p, C.btype.p, C.bzq.p, C.gp.p, C.bgp.p);
   // the interior momentum face fluxes of all three components, with the
@@ -1081,15 +1121,24 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
   double *tf = C.tf3.p + (size_t)comp * C.F;
   hipLaunchKernelGGL(k_mom_wall, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N, C.own.p,
                      C.A.p, C.nx.p, C.ny.p, C.nz.p, C.dx.p, C.dy.p, C.dz.p, C.Sx.p, C.Sy.p, C.Sz.p, a,
-                     C.cf.p, tf);
+                     C.cf.p, C.ff.p, tf);
+  const bool per = C.periodic();
+  if (per)
+    periodic_momentum(C.pargs(), C.s, comp, C.mu, a.c23, nullptr, a.phi, C.u.p, C.v.p, C.w.p, C.bgu.p, C.bgv.p,
+                      C.bgw.p, C.Sx.p, C.Sy.p, C.Sz.p, C.bmdot.p, C.cf.p, C.ff.p, tf);
   double *D = comp == 0 ? C.du.p : comp == 1 ? C.dv.p : C.dw.p;
   double *st = comp == 0 ? C.us.p : comp == 1 ? C.vs.p : C.ws.p;
+  const double *lin = per ? C.per.scal.p + PSCAL_LIN + comp : nullptr;
   by_width(C.W, [&](auto wc) {
-    hipLaunchKernelGGL(k_mom_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
-                       C.code.p, C.cf.p, C.ff.p, tf, C.mdot.p, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
-                       C.urf_u, perm ? C.mperm.p : nullptr, C.ac.p, C.anb.p, C.bc.p, D, st, C.mx.p,
-                       C.simplec ? (comp == 0 ? C.du2.p : comp == 1 ? C.dv2.p : C.dw2.p) : nullptr,
-                       C.capture ? C.cap_eff.p : nullptr);
+    auto launch = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W, C.Fi, C.code.p, C.cf.p,
+                         C.ff.p, tf, C.mdot.p, C.bmdot.p, lin, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
+                         C.urf_u, perm ? C.mperm.p : nullptr, C.ac.p, C.anb.p, C.bc.p, D, st, C.mx.p,
+                         C.simplec ? (comp == 0 ? C.du2.p : comp == 1 ? C.dv2.p : C.dw2.p) : nullptr,
+                         C.capture ? C.cap_eff.p : nullptr);
+    };
+    if (per) launch(k_mom_cell<decltype(wc)::value, true>);
+    else launch(k_mom_cell<decltype(wc)::value, false>);
   });
 }
 
@@ -1135,6 +1184,11 @@ static int phase_momentum_solve(SimpleCase &C, int comp, fs_simple_stats *st) {
                       C.dphi.p, res4, &ss);
     if (rc < 0) return rc;
   }
+  if (C.periodic()) { // UpdatePeriodic (SolveXMomentumEquation.for:299)
+    const double *ph[3] = {C.u.p, C.v.p, C.w.p};
+    double *bph[3] = {C.bu.p, C.bv.p, C.bw.p};
+    periodic_values(C.pargs(), C.s, 3, ph, bph);
+  }
   if (st) {
     st->momentum_iterations += ss.iterations;
     st->momentum_seconds += ss.seconds;
@@ -1169,9 +1223,22 @@ static void phase_pc_assemble(SimpleCase &C) {
   case 7: rc(k_rhie_chow<true, 3>); break;
   default: rc(k_rhie_chow<false, 3>);
   }
+  const bool per = C.periodic();
+  if (per) {
+    const PerArgs pa = C.pargs();
+    periodic_rhie_chow(pa, C.s, C.decomp, C.rho, C.urf_u, C.u.p, C.v.p, C.w.p, C.us.p, C.vs.p, C.ws.p, C.du.p,
+                       C.dv.p, C.dw.p, C.p.p, C.bgp.p, C.Sx.p, C.Sy.p, C.Sz.p, C.bmdot.p);
+    periodic_pc_boundary(pa, C.s, C.decomp, C.rho, C.pk.p, C.simplec ? C.du2.p : C.du.p,
+                         C.simplec ? C.dv2.p : C.dv.p, C.simplec ? C.dw2.p : C.dw.p, C.Sx.p, C.Sy.p, C.Sz.p,
+                         C.bmdot.p, C.cf.p, C.ff.p, C.tf.p);
+  }
   by_width(C.W, [&](auto wc) {
-    hipLaunchKernelGGL(k_pc_cell<decltype(wc)::value>, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W,
-                       C.code.p, C.cf.p, C.ff.p, C.tf.p, C.ac.p, C.anb.p, C.bc.p);
+    auto launch = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(blocks(C.N)), dim3(kBlock), 0, C.s, C.N, C.W, C.code.p, C.cf.p, C.ff.p,
+                         C.tf.p, C.ac.p, C.anb.p, C.bc.p);
+    };
+    if (per) launch(k_pc_cell<decltype(wc)::value, true>);
+    else launch(k_pc_cell<decltype(wc)::value, false>);
   });
 }
 
@@ -1215,6 +1282,14 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
   a.phi[0] = C.pc.p;
   a.bval[0] = nullptr; // walls: BPressureC = PressureC(owner)
   a.grad[0] = C.gpc.p;
+  if (C.periodic()) { // UpdateBoundaryPressureCorrection (:263-290)
+    const PerArgs pa = C.pargs();
+    owner_values(pa, C.s, C.pc.p, C.bpc.p);
+    const double *ph[1] = {C.pc.p};
+    double *bph[1] = {C.bpc.p};
+    periodic_values(pa, C.s, 1, ph, bph);
+    a.bval[0] = C.bpc.p;
+  }
   gradients<1>(C, a, C.ngrad_cont);
   const int methods[1] = {C.lim_cont};
   limit_gradients<1>(C, a, methods);
@@ -1225,20 +1300,159 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
   hipLaunchKernelGGL(k_correct_mdot, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.own.p, C.nei.p,
                      C.cf.p, C.ff.p, C.pc.p, C.mdot.p);
   hipLaunchKernelGGL(k_wall_p, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.own.p, C.dx.p,
-                     C.dy.p, C.dz.p, C.p.p, C.bgp.p, C.bp.p);
+                     C.dy.p, C.dz.p, C.p.p, C.bgp.p, C.btype.p, C.bp.p);
+  if (C.periodic()) {
+    const PerArgs pa = C.pargs();
+    const double *ph[1] = {C.p.p};
+    double *bph[1] = {C.bp.p};
+    periodic_values(pa, C.s, 1, ph, bph);
+    const double *d2u = C.simplec ? C.du2.p : C.du.p, *d2v = C.simplec ? C.dv2.p : C.dv.p,
+                 *d2w = C.simplec ? C.dw2.p : C.dw.p;
+    periodic_correct_bmdot(pa, C.s, d2u, d2v, d2w, C.gpc.p, C.pc.p, C.cf.p, C.ff.p, C.bu.p, C.bv.p, C.bw.p,
+                           C.bmdot.p);
+    if (C.per.calc_beta && C.iter % C.per.every == 0)
+      periodic_beta(pa, C.s, C.decomp, C.rho, C.per.mdot, C.per.relax, C.per.iters, C.per.tuse, d2u, d2v, d2w,
+                    C.Sx.p, C.Sy.p, C.Sz.p, C.own.p, C.nei.p, C.g.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, C.u.p, C.v.p,
+                    C.w.p, C.bu.p, C.bv.p, C.bw.p, C.bmdot.p, C.mdot.p);
+  }
   update_symmetry(C); // SolveContinuityEquation.for:96
   return 0;
 }
 
 static double mass_l1(SimpleCase &C) {
   const int np = std::min(kPartials, blocks(C.N));
-  hipLaunchKernelGGL(k_mass_l1, dim3(np), dim3(kBlock), 0, C.s, C.N, C.W, C.code.p, C.mdot.p, C.part.p);
+  hipLaunchKernelGGL(k_mass_l1, dim3(np), dim3(kBlock), 0, C.s, C.N, C.W, C.Fi, C.code.p, C.mdot.p,
+                     C.periodic() ? C.bmdot.p : nullptr, C.part.p);
   std::vector<double> h(np);
   FS_HIP_THROW(hipMemcpyAsync(h.data(), C.part.p, sizeof(double) * np, hipMemcpyDeviceToHost, C.s));
   FS_HIP_THROW(hipStreamSynchronize(C.s));
   double t = 0.0;
   for (double x : h) t += x;
   return t;
+}
+
+// The geometry every periodic branch of the reference forms for face k from
+// the partner cell's centroid carried by the partner patch's translation
+// (e.g. PeriodicBoundary.for:54-67; the E/T split of
+// AssembleIsotropicDiffusion.for:663-700 by MethodDecomposeS), in the
+// reference's operation order, uploaded per periodic face.
+void periodic_upload_geometry(SimpleCase &C) {
+  Periodic &P = C.per;
+  const FaceGeom &G = C.geom.face;
+  std::vector<double> g(P.np), ux(P.np), uy(P.np), uz(P.np), dcf(P.np), gd(P.np), tx(P.np), ty(P.np), tz(P.np);
+  for (int k = 0; k < P.np; ++k) {
+    const int f = C.Fi + P.h_b[k], i1 = C.h_own[f], j1 = P.h_j1[k];
+    const int q2 = P.h_pair[P.h_patch[k]];
+    const double xf1 = C.geom.xc[j1] + P.h_trans[3 * q2], yf1 = C.geom.yc[j1] + P.h_trans[3 * q2 + 1],
+                 zf1 = C.geom.zc[j1] + P.h_trans[3 * q2 + 2];
+    const double ax = G.cx[f] - C.geom.xc[i1], ay = G.cy[f] - C.geom.yc[i1], az = G.cz[f] - C.geom.zc[i1];
+    const double bx = G.cx[f] - xf1, by = G.cy[f] - yf1, bz = G.cz[f] - zf1;
+    const double d1 = sqrt(ax * ax + ay * ay + az * az), d2 = sqrt(bx * bx + by * by + bz * bz);
+    g[k] = d2 / (d1 + d2);
+    const double dx = xf1 - C.geom.xc[i1], dy = yf1 - C.geom.yc[i1], dz = zf1 - C.geom.zc[i1];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    dcf[k] = d;
+    const double ex = dx / d, ey = dy / d, ez = dz / d;
+    ux[k] = ex;
+    uy[k] = ey;
+    uz[k] = ez;
+    const double sx = G.Sx[f], sy = G.Sy[f], sz = G.Sz[f], A = G.A[f];
+    double fex, fey, fez, fe;
+    if (C.decomp == 1) {
+      const double dot = ex * sx + ey * sy + ez * sz;
+      fex = dot * ex;
+      fey = dot * ey;
+      fez = dot * ez;
+      fe = fabs(dot);
+    } else if (C.decomp == 2) {
+      fex = A * ex;
+      fey = A * ey;
+      fez = A * ez;
+      fe = A;
+    } else {
+      const double dot = ex * sx + ey * sy + ez * sz;
+      fex = (A * A / dot) * ex;
+      fey = (A * A / dot) * ey;
+      fez = (A * A / dot) * ez;
+      fe = sqrt(fex * fex + fey * fey + fez * fez);
+    }
+    gd[k] = fe / d;
+    tx[k] = sx - fex;
+    ty[k] = sy - fey;
+    tz[k] = sz - fez;
+  }
+  P.g.upload(g, C.s);
+  P.ux.upload(ux, C.s);
+  P.uy.upload(uy, C.s);
+  P.uz.upload(uz, C.s);
+  P.dcf.upload(dcf, C.s);
+  P.gd.upload(gd, C.s);
+  P.tx.upload(tx, C.s);
+  P.ty.upload(ty, C.s);
+  P.tz.upload(tz, C.s);
+}
+
+// The solver contexts of the case from its topology (C.geom.enbr, with the
+// periodic partners once the pairs are set): the mesh context with its face
+// codes, and the momentum context on the colour-major renumbering (the greedy
+// colouring of the mesh, the Gauss-Seidel colours, colour by colour in
+// increasing cell order; greedy colouring of the renumbered mesh gives the
+// same classes, now contiguous), sharing the case's stream.  Replaces any
+// contexts the case held.
+int build_meshes(SimpleCase &C) {
+  const int nif = C.Fi, nfaces = C.F;
+  std::vector<int> own1(nif), nei1(nif);
+  for (int f = 0; f < nif; ++f) {
+    own1[f] = C.h_own[f] + 1;
+    nei1[f] = C.h_nei[f] + 1;
+  }
+  if (C.s) FS_HIP(hipStreamSynchronize(C.s));
+  fs_mesh *mesh = nullptr, *mom = nullptr;
+  int rc = fs_mesh_create(C.N, C.W, C.geom.enbr.data(), C.geom.nef.data(), nif, own1.data(), nei1.data(),
+                          C.geom.iown.data(), C.geom.inbr.data(), &mesh);
+  if (rc == 0) rc = fs_mesh_set_cell_faces(mesh, nfaces, C.geom.gef.data(), C.geom.nef.data());
+  if (rc != 0) {
+    if (mesh) fs_mesh_destroy(mesh);
+    return rc;
+  }
+  std::vector<int> ptr, cells;
+  greedy_colors(C.N, C.W, mesh->h_nbr_dev.data(), ptr, cells);
+  std::vector<int> perm(C.N, 0);
+  for (int q = 0; q < C.N; ++q) perm[cells[q]] = q;
+  std::vector<int> enbr_p((size_t)C.N * C.W, 0), nef_p(C.N, 0);
+  for (int c = 0; c < C.N; ++c) {
+    const int q = perm[c];
+    nef_p[q] = C.geom.nef[c];
+    for (int j = 0; j < C.W; ++j) {
+      const int k = C.geom.enbr[(size_t)j * C.N + c];
+      enbr_p[(size_t)j * C.N + q] = k > 0 ? perm[k - 1] + 1 : 0;
+    }
+  }
+  rc = fs_mesh_create(C.N, C.W, enbr_p.data(), nef_p.data(), 0, nullptr, nullptr, nullptr, nullptr, &mom);
+  if (rc != 0) {
+    fs_mesh_destroy(mesh);
+    return rc;
+  }
+  // the old contexts go (their solver options stay), the momentum one on the
+  // new case stream
+  if (C.mesh) mirror_options(mesh, C.mesh);
+  if (C.mom_mesh) fs_mesh_destroy(C.mom_mesh);
+  if (C.mesh) fs_mesh_destroy(C.mesh);
+  C.mesh = mesh;
+  C.mom_mesh = mom;
+  C.s = mesh->stream;
+  FS_HIP(hipStreamSynchronize(mom->stream));
+  FS_HIP(hipStreamDestroy(mom->stream));
+  mom->stream = C.s;
+  mom->own_stream = false;
+  C.h_mperm = perm;
+  C.mperm.upload(C.h_mperm, C.s);
+  C.mx.alloc(C.N);
+  // the per-slot face code of the mesh context
+  C.code.alloc((size_t)C.N * C.W);
+  FS_HIP(hipMemcpyAsync(C.code.p, mesh->d_face_code.p, sizeof(int) * C.N * C.W, hipMemcpyDeviceToDevice, C.s));
+  FS_HIP(hipStreamSynchronize(C.s));
+  return 0;
 }
 
 int simple_iterate(SimpleCase &C, int niter, fs_simple_stats *st) {
@@ -1248,6 +1462,7 @@ int simple_iterate(SimpleCase &C, int niter, fs_simple_stats *st) {
   for (int it = 0; it < niter; ++it) {
     C.iter = C.iter + 1; // nIter (MainProgram.for:257)
     if (C.capture) C.captured_iter = C.iter;
+    if (C.periodic()) periodic_linear(C.pargs(), C.s, C.per.tuse);
     phase_momentum_gradients(C);
     for (int comp = 0; comp < 3; ++comp) {
       phase_momentum_assemble(C, comp);
@@ -1332,64 +1547,17 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
     C.F = C.geom.F;
     C.Fi = C.geom.Fi;
     C.Fb = C.F - C.Fi;
-    std::vector<int> own1(nif), nei1(nif);
-    for (int f = 0; f < nif; ++f) {
-      own1[f] = owner[f] + 1;
-      nei1[f] = neighbour[f] + 1;
-    }
-    int rc = fs_mesh_create(C.N, C.W, C.geom.enbr.data(), C.geom.nef.data(), nif, own1.data(),
-                            nei1.data(), C.geom.iown.data(), C.geom.inbr.data(), &C.mesh);
-    if (rc == 0) rc = fs_mesh_set_cell_faces(C.mesh, nfaces, C.geom.gef.data(), C.geom.nef.data());
+    C.h_own.assign(owner, owner + nfaces);
+    C.h_nei.assign(neighbour, neighbour + nif);
+    const int rc = fs::build_meshes(C);
     if (rc != 0) {
       std::string msg = fs_last_error();
-      if (C.mesh) fs_mesh_destroy(C.mesh);
       delete cs;
       set_error("fs_case_create: " + msg);
       return rc;
     }
-    C.s = C.mesh->stream;
-    {
-      // colour-major renumbering of the cells for the momentum systems: the
-      // greedy colouring of the mesh (the Gauss-Seidel colours), colour by
-      // colour in increasing cell order.  Greedy colouring of the renumbered
-      // mesh gives the same classes, now contiguous.
-      std::vector<int> ptr, cells;
-      fs::greedy_colors(C.N, C.W, C.mesh->h_nbr_dev.data(), ptr, cells);
-      C.h_mperm.assign(C.N, 0);
-      for (int q = 0; q < C.N; ++q) C.h_mperm[cells[q]] = q;
-      std::vector<int> enbr_p((size_t)C.N * C.W, 0), nef_p(C.N, 0);
-      for (int c = 0; c < C.N; ++c) {
-        const int q = C.h_mperm[c];
-        nef_p[q] = C.geom.nef[c];
-        for (int j = 0; j < C.W; ++j) {
-          const int k = C.geom.enbr[(size_t)j * C.N + c];
-          enbr_p[(size_t)j * C.N + q] = k > 0 ? C.h_mperm[k - 1] + 1 : 0;
-        }
-      }
-      rc = fs_mesh_create(C.N, C.W, enbr_p.data(), nef_p.data(), 0, nullptr, nullptr, nullptr, nullptr,
-                          &C.mom_mesh);
-      if (rc != 0) {
-        std::string msg = fs_last_error();
-        fs_mesh_destroy(C.mesh);
-        delete cs;
-        set_error("fs_case_create: " + msg);
-        return rc;
-      }
-      // the momentum context works on the case's stream
-      FS_HIP_THROW(hipStreamSynchronize(C.mom_mesh->stream));
-      FS_HIP_THROW(hipStreamDestroy(C.mom_mesh->stream));
-      C.mom_mesh->stream = C.s;
-      C.mom_mesh->own_stream = false;
-      C.mperm.upload(C.h_mperm, C.s);
-      C.mx.alloc(C.N);
-    }
     C.own.upload(owner, nfaces, C.s);
-    C.h_own.assign(owner, owner + nfaces);
     C.nei.upload(neighbour, nif, C.s);
-    // the per-slot face code of the mesh context
-    C.code.alloc((size_t)C.N * C.W);
-    FS_HIP_THROW(hipMemcpyAsync(C.code.p, C.mesh->d_face_code.p, sizeof(int) * C.N * C.W,
-                                hipMemcpyDeviceToDevice, C.s));
     fs::upload_geometry(C);
     C.bu.upload(wall_u, C.Fb, C.s);
     C.bv.upload(wall_v, C.Fb, C.s);
@@ -1512,7 +1680,8 @@ static bool case_field(fs::SimpleCase &C, const char *name, double **p, size_t *
       {"ff", {&C.ff, F}}, {"tf", {&C.tf, F}}, {"vol", {&C.vol, N}}, {"Sx", {&C.Sx, F}},
       {"Sy", {&C.Sy, F}}, {"Sz", {&C.Sz, F}}, {"g", {&C.g, F}}, {"gdiff", {&C.gdiff, F}},
       {"Tx", {&C.Tx, F}}, {"Ty", {&C.Ty, F}}, {"Tz", {&C.Tz, F}}, {"dcf", {&C.d, F}},
-      {"A", {&C.A, F}}, {"kx", {&C.kx, Fi}}, {"ky", {&C.ky, Fi}}, {"kz", {&C.kz, Fi}}};
+      {"A", {&C.A, F}}, {"kx", {&C.kx, Fi}}, {"ky", {&C.ky, Fi}}, {"kz", {&C.kz, Fi}},
+      {"bmdot", {&C.bmdot, C.periodic() ? Fb : 0}}, {"beta", {&C.per.scal, C.periodic() ? 1 : 0}}};
   auto it = tab.find(name);
   if (it == tab.end()) return false;
   *p = it->second.first->p;
@@ -1934,8 +2103,9 @@ int fs_case_set_decompose(fs_case *cs, int method) {
   C.kx.upload(F.kx, C.s);
   C.ky.upload(F.ky, C.s);
   C.kz.upload(F.kz, C.s);
-  FS_HIP(hipStreamSynchronize(C.s));
   C.decomp = method;
+  if (C.periodic()) fs::periodic_upload_geometry(C); // the periodic faces' E/T split follows
+  FS_HIP(hipStreamSynchronize(C.s));
   return 0;
   CASE_GUARD_END
 }
@@ -1994,7 +2164,7 @@ int fs_case_set_patch_types(fs_case *cs, int npatches, const int *patch_nfaces, 
   bool any = false;
   for (int q = 0; q < npatches; ++q) {
     const int n = patch_nfaces[q], t = patch_type[q];
-    if (n < 0 || (t != FS_BC_WALL && t != FS_BC_SYMMETRY) || start + n > C.Fb) {
+    if (n < 0 || (t != FS_BC_WALL && t != FS_BC_SYMMETRY && t != FS_BC_PERIODIC) || start + n > C.Fb) {
       set_error("fs_case_set_patch_types: patch " + std::to_string(q) + " has a bad size or type");
       return -1;
     }
@@ -2007,7 +2177,7 @@ int fs_case_set_patch_types(fs_case *cs, int npatches, const int *patch_nfaces, 
       type[start + j] = t;
       zq[start + j] = (int)start + q; // face i of patch i
     }
-    any = any || t == FS_BC_SYMMETRY;
+    any = any || t != FS_BC_WALL;
     start += n;
   }
   if (start != C.Fb) {
@@ -2025,6 +2195,161 @@ int fs_case_set_patch_types(fs_case *cs, int npatches, const int *patch_nfaces, 
   C.bzq.upload(zq, C.s);
   FS_HIP(hipStreamSynchronize(C.s));
   C.sym_dirty = true;
+  return 0;
+  CASE_GUARD_END
+}
+
+int fs_case_set_periodic(fs_case *cs, int npatches, const int *patch_nfaces, const int *pair,
+                         const double *translation, double beta, int calc_beta, double mdot_target, double relax,
+                         int iters, int every) {
+  if (!cs || npatches <= 0 || !patch_nfaces || !pair || !translation || iters < 1) {
+    set_error("fs_case_set_periodic: invalid arguments");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  if (C.periodic()) {
+    set_error("fs_case_set_periodic: the case already has its periodic pairs");
+    return -1;
+  }
+  if (!C.btype.p) {
+    set_error("fs_case_set_periodic: mark the periodic patches with fs_case_set_patch_types first");
+    return -1;
+  }
+  std::vector<int> type(C.Fb);
+  FS_HIP(hipMemcpy(type.data(), C.btype.p, sizeof(int) * C.Fb, hipMemcpyDeviceToHost));
+  std::vector<int> start(npatches + 1, 0);
+  for (int q = 0; q < npatches; ++q) start[q + 1] = start[q] + patch_nfaces[q];
+  if (start[npatches] != C.Fb) {
+    set_error("fs_case_set_periodic: patches do not cover the boundary");
+    return -1;
+  }
+  for (int q = 0; q < npatches; ++q) {
+    const int n = patch_nfaces[q];
+    if (n == 0) continue;
+    const bool per = type[start[q]] == FS_BC_PERIODIC;
+    for (int j = 0; j < n; ++j)
+      if ((type[start[q] + j] == FS_BC_PERIODIC) != per) {
+        set_error("fs_case_set_periodic: patch " + std::to_string(q) + " mixes periodic and other faces");
+        return -1;
+      }
+    if (per != (pair[q] >= 0) ||
+        (pair[q] >= 0 && (pair[q] >= npatches || pair[q] == q || pair[pair[q]] != q ||
+                          patch_nfaces[pair[q]] != n))) {
+      set_error("fs_case_set_periodic: patch " + std::to_string(q) + " has no mutual periodic partner of its size");
+      return -1;
+    }
+  }
+  const fs::FaceGeom &G = C.geom.face;
+  const int Fi = C.Fi;
+  // RotationMatrixCoefficients, translational (CalculateRotationMatrixCoefficients
+  // .for:171-242): the partner face whose centroid offset is closest to the
+  // translation (largest per-axis miss, first minimum)
+  std::vector<int> corr(C.Fb, -1), act(C.Fb, 0);
+  for (int q1 = 0; q1 < npatches; ++q1) {
+    const int q2 = pair[q1];
+    if (q2 <= q1) continue;
+    for (int f1 = Fi + start[q1]; f1 < Fi + start[q1 + 1]; ++f1) {
+      double best = 1.e30;
+      int fd = -1;
+      for (int f2 = Fi + start[q2]; f2 < Fi + start[q2 + 1]; ++f2) {
+        const double d1 = fabs(fabs(G.cx[f2] - G.cx[f1]) - fabs(translation[3 * q1]));
+        const double d2 = fabs(fabs(G.cy[f2] - G.cy[f1]) - fabs(translation[3 * q1 + 1]));
+        const double d3 = fabs(fabs(G.cz[f2] - G.cz[f1]) - fabs(translation[3 * q1 + 2]));
+        double dif = d1;
+        if (d2 > dif) dif = d2;
+        if (d3 > dif) dif = d3;
+        if (dif < best) {
+          best = dif;
+          fd = f2;
+        }
+      }
+      if (fd < 0 || corr[fd - Fi] >= 0) {
+        set_error("fs_case_set_periodic: a periodic face finds no partner of its own");
+        return -1;
+      }
+      corr[f1 - Fi] = fd - Fi;
+      corr[fd - Fi] = f1 - Fi;
+      act[f1 - Fi] = 1;
+    }
+  }
+  fs::Periodic &P = C.per;
+  P.h_pair.assign(pair, pair + npatches);
+  P.h_trans.assign(translation, translation + 3 * npatches);
+  P.h_b.clear();
+  P.h_patch.clear();
+  std::vector<int> kof(C.Fb, -1);
+  for (int q = 0; q < npatches; ++q)
+    if (pair[q] >= 0)
+      for (int b = start[q]; b < start[q + 1]; ++b) {
+        kof[b] = (int)P.h_b.size();
+        P.h_b.push_back(b);
+        P.h_patch.push_back(q);
+      }
+  P.np = (int)P.h_b.size();
+  P.h_part.assign(P.np, 0);
+  P.h_j1.assign(P.np, 0);
+  P.h_act.clear();
+  for (int k = 0; k < P.np; ++k) {
+    const int b = P.h_b[k];
+    P.h_part[k] = kof[corr[b]];
+    P.h_j1[k] = C.h_own[Fi + corr[b]];
+    if (act[b]) P.h_act.push_back(k);
+  }
+  P.na = (int)P.h_act.size();
+  for (int q = 0; q < npatches; ++q)
+    if (pair[q] >= 0) {
+      for (int d = 0; d < 3; ++d) P.tuse[d] = fabs(translation[3 * q + d]);
+      break;
+    }
+  // LPeriodicImplicit: the partner cell in the face's slot of ElementNeighbor,
+  // the slot searched over NumbOfElementNodes entries (:249-289)
+  std::vector<int> enbr = C.geom.enbr;
+  for (int k = 0; k < P.np; ++k) {
+    const int f = Fi + P.h_b[k], i1 = C.h_own[f];
+    const int nen = C.geom.cell_node_ptr[i1 + 1] - C.geom.cell_node_ptr[i1];
+    bool found = false;
+    for (int sl = 0; sl < nen && sl < C.W; ++sl)
+      if (C.geom.gef[(size_t)sl * C.N + i1] == f + 1) {
+        enbr[(size_t)sl * C.N + i1] = P.h_j1[k] + 1;
+        found = true;
+        break;
+      }
+    if (!found) {
+      set_error("fs_case_set_periodic: a periodic face is in no slot of its cell");
+      return -1;
+    }
+  }
+  C.geom.enbr.swap(enbr);
+  int rc = fs::build_meshes(C);
+  if (rc != 0) {
+    C.geom.enbr.swap(enbr);
+    P = fs::Periodic();
+    return rc;
+  }
+  P.b.upload(P.h_b, C.s);
+  P.part.upload(P.h_part, C.s);
+  P.j1.upload(P.h_j1, C.s);
+  P.act.upload(P.h_act.empty() ? std::vector<int>{0} : P.h_act, C.s);
+  fs::periodic_upload_geometry(C);
+  std::vector<double> scal(8, 0.0);
+  scal[fs::PSCAL_BETA] = beta;
+  P.scal.upload(scal, C.s);
+  P.calc_beta = calc_beta != 0;
+  P.mdot = mdot_target;
+  P.relax = relax;
+  P.iters = iters;
+  P.every = every > 0 ? every : 1;
+  std::vector<int> pk(C.Fb, -1);
+  for (int k = 0; k < P.np; ++k) pk[P.h_b[k]] = k;
+  C.pk.upload(pk, C.s);
+  const std::vector<double> zb(3 * (size_t)C.Fb, 0.0);
+  C.bmdot.upload(zb.data(), C.Fb, C.s);
+  C.bpc.upload(zb.data(), C.Fb, C.s);
+  C.bgu.upload(zb, C.s);
+  C.bgv.upload(zb, C.s);
+  C.bgw.upload(zb, C.s);
+  FS_HIP(hipStreamSynchronize(C.s));
   return 0;
   CASE_GUARD_END
 }

@@ -87,7 +87,7 @@ int fs_case_create(int npoints, const double *x, const double *y, const double *
 int fs_case_destroy(fs_case *c);
 
 /* Boundary condition types (BCType of the reference's case definition). */
-enum fs_bc_type { FS_BC_WALL = 0, FS_BC_SYMMETRY = 1 };
+enum fs_bc_type { FS_BC_WALL = 0, FS_BC_SYMMETRY = 1, FS_BC_PERIODIC = 2 };
 /* Boundary condition of each patch, the patches covering the boundary faces
  * in order (npatches sizes summing to NFacesTotal - NIFaces).  A case starts
  * with every boundary face a no-slip wall of the velocity given at creation.
@@ -100,6 +100,33 @@ enum fs_bc_type { FS_BC_WALL = 0, FS_BC_SYMMETRY = 1 };
  * at least p faces).  Fails for sizes that do not cover the boundary. */
 int fs_case_set_patch_types(fs_case *c, int npatches, const int *patch_nfaces,
                             const int *patch_type);
+/* Translational periodicity with the implicit coupling (the case file's
+ * LTranslationalPeriodicity and LPeriodicImplicit, BoundaryConditions2.F90
+ * :130-146, set up by RotationMatrixCoefficients,
+ * CalculateRotationMatrixCoefficients.for:171-291) on patches that
+ * fs_case_set_patch_types marked FS_BC_PERIODIC: pair[q] is the partner
+ * patch of patch q (0-based, -1 for none; mutual, of equal size),
+ * translation[3q..3q+2] its x/y/zTranslation (the vector that carries the
+ * partner patch's cells across patch q).  Each face of the lower patch of a
+ * pair takes the partner face whose centroid offset is closest to the
+ * translation; the partner cell joins the face's slot of ElementNeighbor
+ * (every solver then couples the pair) and the periodic branches of the
+ * gradients, diffusion, convection, Rhie-Chow, p' coefficients and
+ * corrections run (PeriodicBoundary.for, CorrectGradients.for:66-167,
+ * AssembleIsotropicDiffusion.for:630-857, AssembleConvection.for:582-627,
+ * AssembleMdotUsingRhieChow.for:1086-1416,
+ * AssemblePressureCorrectionFluxes.for:748-830, CorrectMdotandVelocity.for
+ * :115-145, CorrectPressure.for:293-317).  The momentum equations take the
+ * pressure-gradient source -beta V along the translation
+ * (AssembleLinearPressure.for:4) with beta = periodicBeta, initially `beta`;
+ * calc_beta (LcalculateBeta) updates it every `every` iterations
+ * (nIterCorrectBeta) towards the mass flow `mdot_target` (periodicMdot,
+ * counted outward through the lower periodic patch) with relaxation `relax`
+ * and `iters` passes (relaxBeta, BetaIterations; updateperiodicBeta,
+ * PeriodicBeta.for:4).  Once per case, after fs_case_set_patch_types. */
+int fs_case_set_periodic(fs_case *c, int npatches, const int *patch_nfaces, const int *pair,
+                         const double *translation, double beta, int calc_beta,
+                         double mdot_target, double relax, int iters, int every);
 /* High-resolution convection of the momentum equations: HRFrameworkMomentum
  * ('none', 'tvd', 'nvf'), ConvectionSchemeMomentum, BleedMomentum and
  * nIterStartApplyingHR (User0.F90:367-410).  Upwind fluxes stay implicit and

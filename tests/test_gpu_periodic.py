@@ -1,0 +1,111 @@
+"""Translational periodicity on the device against the reference build and
+the closed form.
+
+tests/golden/simple_channel_*.npz hold eight outer iterations of the
+reference's own SIMPLE on a graded Kuhn-tetrahedral channel with a
+translational periodic pair, walls and symmetry sides
+(tests/golden/make_simple_golden.py, oracle/_ref/ref_simple.x): the
+fixed-bulk-flux update every iteration, every second iteration with two
+passes and half relaxation (MethodDecomposeS 1), and a prescribed pressure
+gradient with second-order upwind and SIMPLEC.  With the reference's ILU for
+both solves the device iterations are bitwise the reference's (velocities,
+pressure, interior and boundary mass fluxes, periodicBeta); the SOR run's
+settings are compared bitwise with the oracle (pinned to that run) under ILU.  On a uniform hexahedral channel the
+converged device profile is the plane Poiseuille parabola shifted by
+beta h^2 / (8 mu) (the half-cell wall flux), with no cross-flow.
+"""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+fs = pytest.importorskip("flowsolversourcecode_amd")
+pytestmark = pytest.mark.gpu
+
+_spec = importlib.util.spec_from_file_location(
+    "make_simple_golden", os.path.join(os.path.dirname(__file__), "golden", "make_simple_golden.py"))
+msg = importlib.util.module_from_spec(_spec)
+_spec.loader.exec_module(msg)
+
+
+def _device_case(name, msolver="ilu", psolver="ilu", plmg=0):
+    spec, kw = msg.SIMPLE[name]
+    pm, bu, bv, bw, types = msg.build_mesh(spec)
+    per = msg.periodic_settings(spec, kw)
+    dc = fs.Case(pm, bu, bv, bw, rho=1.0, mu=0.01)
+    dc.configure(urf_u=kw.get("urf_u", 0.7), urf_p=kw.get("urf_p", 0.3), msolver=msolver, mrrf=kw.get("mrrf", 0.1),
+                 mitmax=kw.get("mitmax", 20), psolver=psolver, prrf=kw.get("prrf", 0.01),
+                 pitmax=kw.get("pitmax", 200), plmg=plmg)
+    dc.set_patch_types(pm.patch_sizes, types)
+    if kw.get("framework", "none") != "none":
+        dc.set_convection(kw["framework"], kw["scheme"], kw.get("bleed", 0.0), kw.get("start_iter", 0))
+    dc.set_algorithm(kw.get("algorithm", "simple"))
+    dc.set_decompose(kw.get("decomp", 3))
+    dc.set_periodic(pm.patch_sizes, per["pairs"], per["translations"], beta=per.get("beta", 0.0),
+                    calc_beta=per.get("calc_beta", 1), mdot=per["mdot"], relax=per.get("relax", 1.0),
+                    iters=per.get("iters", 1), every=per.get("every", 1))
+    return dc, kw, pm
+
+
+@pytest.mark.parametrize("name", ["channel_ilu", "channel_ilu_fixed_beta"])
+def test_periodic_iterations_bitwise_the_reference_run(name):
+    g = golden(f"simple_{name}.npz")
+    dc, kw, _ = _device_case(name)
+    print(f"periodic bitwise {name}", flush=True)
+    dc.iterate(kw["niter"])
+    for k in ("u", "v", "w", "p", "mdot", "bmdot"):
+        assert np.array_equal(dc.get(k), g[k]), (name, k, np.abs(dc.get(k) - g[k]).max())
+    assert dc.get("beta")[0] == float(g["beta"])
+
+
+def test_periodic_beta_passes_against_the_oracle():
+    """MethodDecomposeS 1 and the beta update every second iteration with two
+    passes and half relaxation (the settings of simple_channel_sor_beta.npz,
+    whose SOR run pins the oracle to the reference build): with the
+    reference's ILU on both sides the device's eight iterations equal the
+    oracle's bitwise."""
+    import pyoracle as po
+    name = "channel_sor_beta"
+    spec, kw = msg.SIMPLE[name]
+    dc, _, pm = _device_case(name, msolver="ilu", psolver="ilu")
+    per = msg.periodic_settings(spec, kw)
+    bu = np.zeros(pm.n_faces - pm.n_internal)
+    oc = po.OrCase(pm, bu, bu, bu, rho=1.0, mu=0.01, urf_u=0.7, urf_p=0.3, msolver=po.ILU, mrrf=0.1, mitmax=20,
+                   psolver=po.ILU, prrf=0.01, pitmax=200)
+    oc.set_patch_types(pm.patch_sizes, msg.build_mesh(spec)[4])
+    oc.set_decompose(1)
+    oc.set_periodic(per["pairs"], per["translations"], beta=per["beta"], mdot=per["mdot"], calc_beta=True,
+                    relax=per["relax"], iters=per["iters"], every=per["every"])
+    print("periodic beta passes vs oracle", flush=True)
+    dc.iterate(8)
+    oc.iterate(8)
+    for k in ("u", "v", "w", "p", "mdot", "bmdot"):
+        assert np.array_equal(dc.get(k), oc.get(k)), (k, np.abs(dc.get(k) - oc.get(k)).max())
+    assert dc.get("beta")[0] == oc.beta
+
+
+@pytest.mark.parametrize("ny", [4, 8])
+def test_device_plane_poiseuille_closed_form(ny):
+    from flowsolversourcecode_amd import meshgen
+    beta, mu = 0.1, 0.05
+    pm, types, pairs, tr = meshgen.channel_tet(4, ny, 3, lx=1.0, ly=2.0, lz=1.0, grading=1.0, cells="hex")
+    z = np.zeros(pm.n_faces - pm.n_internal)
+    dc = fs.Case(pm, z, z, z, rho=1.0, mu=mu)
+    # the reference's ILU for both solves (the oracle reaches round-off in
+    # 800 iterations with these settings)
+    dc.configure(urf_u=0.7, urf_p=0.3, msolver="ilu", mrrf=0.01, mitmax=50, psolver="ilu", prrf=1e-8,
+                 pitmax=2000, plmg=0)
+    dc.set_patch_types(pm.patch_sizes, types)
+    dc.set_periodic(pm.patch_sizes, pairs, tr, beta=beta)
+    print(f"poiseuille ny={ny}", flush=True)
+    dc.iterate(1000)
+    import pyoracle as po
+    yc = po.OrMesh(pm).cell["yc"]
+    h = 2.0 / ny
+    exact = beta / (2 * mu) * yc * (2.0 - yc) + beta * h * h / (8 * mu)
+    u = dc.get("u")
+    assert np.abs(u - exact).max() < 1e-9 * exact.max()
+    assert np.abs(dc.get("v")).max() < 1e-10 and np.abs(dc.get("w")).max() < 1e-10

@@ -1219,7 +1219,9 @@ template <class F> static int run_batches(fs_mesh *m, int itmax, int key, F body
     launched += batch;
     host_flags(m, fl);
     next = pred > 0 ? std::min(grow, m->check_every) : m->check_every;
-    grow *= 2;
+    // stop doubling at check_every: an unbounded doubling overflows int
+    // after 31 batches
+    if (grow < m->check_every) grow *= 2;
   }
   pred = fl[F_ITER];
   // launches of iterations past the converged one returned at once

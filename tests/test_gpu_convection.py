@@ -119,7 +119,11 @@ def test_cavity_2d_second_order_converged_fields_match_the_oracle(fw, scheme, li
     oc = po.OrCase(m, bu, bv, bw, rho=1.0, mu=0.01, urf_u=0.7, urf_p=0.3, msolver=po.SOR, mrrf=0.1, mitmax=10,
                    psolver=po.SOR, prrf=0.01, pitmax=100)
     dc = fs.Case(m, bu, bv, bw, rho=1.0, mu=0.01)
-    dc.configure(urf_u=0.7, urf_p=0.3, msolver="sor", mrrf=0.1, mitmax=10, psolver="bicgs", prrf=1e-8,
+    # the closed cavity's p' system is singular: BiCGSTAB cannot reach 1e-8
+    # on it late in the run (round-off leaves the right-hand side outside the
+    # range), and iterating on at the stall point leaves an erratic iterate;
+    # 1e-4 per solve is reached every time and is ample for the outer loop
+    dc.configure(urf_u=0.7, urf_p=0.3, msolver="sor", mrrf=0.1, mitmax=10, psolver="bicgs", prrf=1e-4,
                  pitmax=300, plmg=fs.MG_PRECOND)
     for c in (oc, dc):
         c.set_patch_types(m.patch_sizes, types)

@@ -182,3 +182,22 @@ def test_wide_numbering_uses_32bit_neighbours(solver, lmg, shift):
         assert rc == 0, st
         sols.append(dphi)
     np.testing.assert_allclose(sols[1][perm], sols[0], rtol=0, atol=1e-7 * np.abs(sols[0]).max())
+
+
+@pytest.mark.parametrize("solver", ["pbcg", "sor"])
+def test_repeat_solve_runs_to_itmax(solver):
+    """A solve repeated on one mesh starts with a batch the length of the
+    previous iteration count, then batches of 2, 4, ... check_every.  The
+    doubling stops at check_every: a repeat solve of more than 262
+    iterations (31 batches) must run to itmax like the first.  PBCG
+    stagnates on this system (the reference's 1e-10 added to its
+    denominators), multicolour Gauss-Seidel is slow on it: neither reaches
+    rrf = 1e-12 in 600 iterations, and the three calls are bitwise alike."""
+    nbr, nnbr, ac, anb, bc = systems.pressure_like(14, 12, 10, 11, shift=1.0 if solver == "pbcg" else 0.05)
+    fit0 = np.linspace(-1.0, 1.0, len(ac))
+    m = fs.Mesh(nbr, nnbr)
+    outs = [m.solve_equation("pressc", fit0, 1e-12, 600, solver, 0, ac, anb, bc) for _ in range(3)]
+    for rc, fit, dphi, res4, st in outs:
+        assert rc == 1 and st["iterations"] == 600, (rc, st)
+        assert np.isfinite(dphi).all()
+        assert np.array_equal(dphi, outs[0][2]) and st["final_l1"] == outs[0][4]["final_l1"]

@@ -366,6 +366,10 @@ void or_case_delete(or_case *c) {
                     c->dv, c->dw, c->us, c->vs, c->ws, c->visc, c->fcf, c->fff, c->fvf, c->ftf,
                     c->fce, c->fte, c->effdiv, c->ac, c->anb, c->bc, c->dphi};
   for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); ++i) free(arrs[i]);
+  double *tarr[] = {c->tk, c->tw, c->mut, c->wd, c->btk, c->btw, c->bmut, c->gk, c->gom, c->bgk, c->bgom,
+                    c->fgk, c->fgom, c->rhok, c->brhok, c->grk, c->bgrk, c->fgrkb, c->fgrkc, c->F1, c->bF1,
+                    c->F2, c->bF2, c->prod, c->sr, c->bsr, c->med, c->bmed};
+  for (size_t i = 0; i < sizeof(tarr) / sizeof(tarr[0]); ++i) free(tarr[i]);
   free(c);
 }
 
@@ -662,8 +666,16 @@ void or_limit_gradient(or_case *c, int method, const double *phi, const double *
 /* HRSchemesCorrections (AssembleHRSchemeCorrection.for:31-133) for a velocity
  * component: the deferred correction mdot (phi_HR - phi_C) of every interior
  * face added to FluxTf, phi_U extrapolated from the upwind cell's gradient. */
+static void hr_correction(or_case *c, int fw, int scheme, double bleed, const double *phi, const double *gr,
+                          double *ftf);
 void or_hr_correction(or_case *c, const double *phi, const double *gr, double *ftf) {
-  if (c->hr_fw == OR_HR_NONE || c->hr_scheme == OR_CONV_UPWIND) return;
+  hr_correction(c, c->hr_fw, c->hr_scheme, c->bleed, phi, gr, ftf);
+}
+
+/* the correction with a variable's own framework, scheme and bleed */
+static void hr_correction(or_case *c, int fw, int scheme, double bleed, const double *phi, const double *gr,
+                          double *ftf) {
+  if (fw == OR_HR_NONE || scheme == OR_CONV_UPWIND) return;
   const or_mesh *m = c->m;
   const int N = c->N;
   const double small = (double)1.e-9f;
@@ -680,18 +692,44 @@ void or_hr_correction(or_case *c, const double *phi, const double *gr, double *f
     }
     const double phiu = phid - 2. * (gr[iu] * dcx + gr[N + iu] * dcy + gr[2 * N + iu] * dcz);
     double phihr;
-    if (c->hr_fw == OR_HR_NVF) {
+    if (fw == OR_HR_NVF) {
       const double diff = phid - phiu;
       const double t = fabs(diff) < small ? 0. : (phic - phiu) / diff;
-      phihr = or_nvf_phif(c->hr_scheme, t) * diff + phiu;
-      if (c->hr_scheme != OR_CONV_GAMMA) phihr = (1. - c->bleed) * phihr + c->bleed * phic;
+      phihr = or_nvf_phif(scheme, t) * diff + phiu;
+      if (scheme != OR_CONV_GAMMA) phihr = (1. - bleed) * phihr + bleed * phic;
     } else {
       const double diff = phid - phic;
       const double r = fabs(diff) < small ? 0. : (phic - phiu) / diff;
-      phihr = phic + 0.5 * or_tvd_psi(c->hr_scheme, r) * diff;
-      phihr = (1. - c->bleed) * phihr + c->bleed * phic;
+      phihr = phic + 0.5 * or_tvd_psi(scheme, r) * diff;
+      phihr = (1. - bleed) * phihr + bleed * phic;
     }
     ftf[k] = ftf[k] + c->mdot[k] * (phihr - phic) * 1.;
+  }
+}
+
+/* The boundary gradients of p (and of 2/3 rho k, CalculateGradients.for
+ * :264-269) at symmetry planes and walls: CorrectPressureGradientAtSymmetry
+ * Boundary (CorrectGradients.for:270-281, its operand order and its z-normal
+ * of face (i,i)) and CorrectPresureGradientAtWalls, the tangential part of
+ * the owner's gradient. */
+static void pressure_boundary_gradient(or_case *c, const double *gr, double *bgr) {
+  const or_mesh *m = c->m;
+  const int Fi = c->Fi, Fb = c->Fb, N = c->N;
+  for (int b = 0; b < Fb; ++b) {
+    if (is_periodic(c, b)) continue;
+    const int f = Fi + b, i1 = m->owner[f];
+    const double nx = m->nx[f], ny = m->ny[f], nz = m->nz[f];
+    const double gx = gr[i1], gy = gr[N + i1], gz = gr[2 * N + i1];
+    if (c->btype && c->btype[b] == OR_BC_SYMMETRY) {
+      const double zq = m->nz[Fi + c->bzq[b]];
+      bgr[b] = gx * (1. - nx * nx) - gy * nx * ny - gz * nx * zq;
+      bgr[Fb + b] = gy * (1. - ny * ny) - gx * ny * nx - gz * ny * zq;
+      bgr[2 * Fb + b] = gz * (1. - zq * zq) - gx * zq * nx - gy * zq * ny;
+      continue;
+    }
+    bgr[b] = (1 - nx * nx) * gx - nx * ny * gy - nx * nz * gz;
+    bgr[Fb + b] = (1 - ny * ny) * gy - ny * nx * gx - ny * nz * gz;
+    bgr[2 * Fb + b] = (1 - nz * nz) * gz - nz * nx * gx - nz * ny * gy;
   }
 }
 
@@ -717,30 +755,33 @@ void or_case_momentum_gradients(or_case *c) {
   periodic_gradient(c, c->gv, c->bgv);
   periodic_gradient(c, c->gw, c->bgw);
   periodic_gradient(c, c->gp, c->bgp);
-  const or_mesh *m = c->m;
-  const int Fi = c->Fi, Fb = c->Fb, N = c->N;
-  for (int b = 0; b < Fb; ++b) {
-    if (is_periodic(c, b)) continue;
-    const int f = Fi + b, i1 = m->owner[f];
-    const double nx = m->nx[f], ny = m->ny[f], nz = m->nz[f];
-    const double gx = c->gp[i1], gy = c->gp[N + i1], gz = c->gp[2 * N + i1];
-    if (c->btype && c->btype[b] == OR_BC_SYMMETRY) {
-      /* CorrectPressureGradientAtSymmetryBoundary (CorrectGradients.for
-       * :270-281), its operand order and its z-normal of face (i,i) */
-      const double zq = m->nz[Fi + c->bzq[b]];
-      c->bgp[b] = gx * (1. - nx * nx) - gy * nx * ny - gz * nx * zq;
-      c->bgp[Fb + b] = gy * (1. - ny * ny) - gx * ny * nx - gz * ny * zq;
-      c->bgp[2 * Fb + b] = gz * (1. - zq * zq) - gx * zq * nx - gy * zq * ny;
-      continue;
-    }
-    c->bgp[b] = (1 - nx * nx) * gx - nx * ny * gy - nx * nz * gz;
-    c->bgp[Fb + b] = (1 - ny * ny) * gy - ny * nx * gx - ny * nz * gz;
-    c->bgp[2 * Fb + b] = (1 - nz * nz) * gz - nz * nx * gx - nz * ny * gy;
-  }
+  pressure_boundary_gradient(c, c->gp, c->bgp);
   face_gradient(c, c->u, c->gu, c->fgu, c->gi_mom);
   face_gradient(c, c->v, c->gv, c->fgv, c->gi_mom);
   face_gradient(c, c->w, c->gw, c->fgw, c->gi_mom);
   face_gradient(c, c->p, c->gp, c->fgp, c->gi_cont);
+  if (c->turb) {
+    const or_mesh *m = c->m;
+    const int N = c->N, Fb = c->Fb;
+    /* CalculateEffectiveDiffusionCoefficient, velocities (Calculate
+     * EffectiveDiffusionCoefficient: Viscosity + TurbulentViscosity) */
+    for (int i = 0; i < N; ++i) c->med[i] = c->visc[i] + c->mut[i];
+    for (int b = 0; b < Fb; ++b) c->bmed[b] = c->mu + c->bmut[b];
+    /* MinusTwoThirdsRhoK('velx') (MinusTwoThirdsRhoK: rhok, its gradient
+     * with the momentum's gradient settings and the pressure's boundary
+     * projections); its face interpolations 'average' and
+     * 'averagecorrected' (AssembleMdotUsingRhieChow.for:294-306) */
+    const double t23 = (double)0.666666666666666666666667f; /* Constants1 TwoThird, a REAL(4) literal */
+    for (int i = 0; i < N; ++i) c->rhok[i] = t23 * c->rho * c->tk[i];
+    for (int b = 0; b < Fb; ++b) c->brhok[b] = t23 * c->rho * c->btk[b];
+    gradient_n(c, c->rhok, c->brhok, c->grk, c->bgrk, c->ngrad);
+    if (c->lim_mom) or_limit_gradient(c, c->lim_mom, c->rhok, c->brhok, c->grk, c->bgrk);
+    periodic_gradient(c, c->grk, c->bgrk);
+    pressure_boundary_gradient(c, c->grk, c->bgrk);
+    face_gradient(c, c->rhok, c->grk, c->fgrkb, OR_GI_AVERAGE);
+    face_gradient(c, c->rhok, c->grk, c->fgrkc, OR_GI_AVERAGECORRECTED);
+    (void)m;
+  }
 }
 
 /* AssembleGlobalMatrixFaceFluxes + ElementFluxes
@@ -805,11 +846,14 @@ void or_case_momentum_assemble(or_case *c, int comp) {
   memset(c->ftf, 0, sizeof(double) * nf);
   memset(c->fce, 0, sizeof(double) * N);
   memset(c->fte, 0, sizeof(double) * N);
+  /* eDiffCoefficient: the viscosity, or with the SST model Viscosity +
+   * TurbulentViscosity in cells and on boundary faces */
+  const double *gam = c->turb ? c->med : c->visc;
   /* isotropic diffusion, interior */
   for (int k = 0; k < Fi; ++k) {
     const int i = m->owner[k], j = m->neigh[k];
     const double gf = m->g[k];
-    const double gamf = gf * c->visc[i] + (1. - gf) * c->visc[j];
+    const double gamf = gf * gam[i] + (1. - gf) * gam[j];
     const double cfl = gamf * m->gdiff[k], ffl = -(gamf * m->gdiff[k]);
     const double vfl = -(gamf * (fg[k] * m->Tx[k] + fg[Fi + k] * m->Ty[k] + fg[2 * Fi + k] * m->Tz[k]));
     c->fcf[k] = c->fcf[k] + cfl;
@@ -825,7 +869,7 @@ void or_case_momentum_assemble(or_case *c, int comp) {
        * implicit): BGamaFace interpolated between the two cells
        * (InterpolateGamaFromCentroidToFace.for:129-178) */
       const int j1 = c->pj1[b];
-      const double gamf = per_mix(c, b, c->visc[i1], c->visc[j1]);
+      const double gamf = per_mix(c, b, gam[i1], gam[j1]);
       const double *bg = comp == 0 ? c->bgu : comp == 1 ? c->bgv : c->bgw;
       const double cfl = gamf * c->pgd[b], ffl = -(gamf * c->pgd[b]);
       const double vfl = -(gamf * (bg[b] * c->ptx[b] + bg[Fb + b] * c->pty[b] + bg[2 * Fb + b] * c->ptz[b]));
@@ -835,7 +879,7 @@ void or_case_momentum_assemble(or_case *c, int comp) {
       c->ftf[f] = c->ftf[f] + cfl * phi[i1] + ffl * phi[j1] + vfl;
       continue;
     }
-    const double gamf = c->mu; /* BGamaFace = BViscosity */
+    const double gamf = c->turb ? c->bmed[b] : c->mu; /* BGamaFace = BeDiffCoefficient */
     const double nx = m->nx[f], ny = m->ny[f], nz = m->nz[f], area = m->A[f];
     const double dn = m->dx[f] * nx + m->dy[f] * ny + m->dz[f] * nz;
     double cfl, vfl;
@@ -876,7 +920,7 @@ void or_case_momentum_assemble(or_case *c, int comp) {
   const double *gc = comp == 0 ? c->fgw : comp == 1 ? c->fgw + Fi : c->fgw + 2 * Fi;
   for (int k = 0; k < Fi; ++k) {
     const int i = m->owner[k], j = m->neigh[k];
-    const double gamf = m->g[k] * c->visc[i] + (1. - m->g[k]) * c->visc[j];
+    const double gamf = m->g[k] * gam[i] + (1. - m->g[k]) * gam[j];
     const double vfl = -(gamf * (ga[k] * m->Sx[k] + gb[k] * m->Sy[k] + gc[k] * m->Sz[k]));
     c->fcf[k] = c->fcf[k] + 0.;
     c->fff[k] = c->fff[k] + 0.;
@@ -886,7 +930,7 @@ void or_case_momentum_assemble(or_case *c, int comp) {
   const double *bga = c->bgu + comp * Fb, *bgb = c->bgv + comp * Fb, *bgc = c->bgw + comp * Fb;
   for (int b = 0; b < Fb; ++b) {
     const int f = Fi + b;
-    const double gamf = c->mu;
+    const double gamf = c->turb ? c->bmed[b] : c->mu;
     const double vfl = -(gamf * (bga[b] * m->Sx[f] + bgb[b] * m->Sy[f] + bgc[b] * m->Sz[f]));
     c->fcf[f] = c->fcf[f] + 0.;
     c->fff[f] = c->fff[f] + 0.;
@@ -897,7 +941,7 @@ void or_case_momentum_assemble(or_case *c, int comp) {
   const double c23 = c_m23();
   for (int k = 0; k < Fi; ++k) {
     const int i = m->owner[k], j = m->neigh[k];
-    const double gamf = m->g[k] * c->visc[i] + (1. - m->g[k]) * c->visc[j];
+    const double gamf = m->g[k] * gam[i] + (1. - m->g[k]) * gam[j];
     const double vfl = -(c23 * gamf * (c->fgu[k] + c->fgv[Fi + k] + c->fgw[2 * Fi + k]) * S[k]);
     c->fcf[k] = c->fcf[k] + 0.;
     c->fff[k] = c->fff[k] + 0.;
@@ -906,7 +950,7 @@ void or_case_momentum_assemble(or_case *c, int comp) {
   }
   for (int b = 0; b < Fb; ++b) {
     const int f = Fi + b;
-    const double gamf = c->mu;
+    const double gamf = c->turb ? c->bmed[b] : c->mu;
     const double vfl = -(c23 * gamf * (c->bgu[b] + c->bgv[Fb + b] + c->bgw[2 * Fb + b]) * S[f]);
     c->fcf[f] = c->fcf[f] + 0.;
     c->fff[f] = c->fff[f] + 0.;
@@ -968,6 +1012,11 @@ void or_case_momentum_assemble(or_case *c, int comp) {
   /* pressure gradient */
   const double *gpc = c->gp + comp * N;
   for (int i = 0; i < N; ++i) c->fte[i] = c->fte[i] + gpc[i] * m->vol[i];
+  /* MinusTwoThirdsRhoK: the gradient of 2/3 rho k */
+  if (c->turb) {
+    const double *grc = c->grk + comp * N;
+    for (int i = 0; i < N; ++i) c->fte[i] = c->fte[i] + grc[i] * m->vol[i];
+  }
   scatter(c);
   for (int i = 0; i < N; ++i) c->ac[i] = c->ac[i] / c->urf_u;
   for (int i = 0; i < N; ++i) {
@@ -1095,6 +1144,11 @@ void or_case_pc_assemble(or_case *c) {
     const double cfl = rhof * geo, ffl = -(rhof * geo);
     fv = fv - rhof * (c->fgp[k] * dut + c->fgp[Fi + k] * dvt + c->fgp[2 * Fi + k] * dwt);
     fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
+    if (c->turb) { /* the -2/3 rho k gradient's difference (:627-635) */
+      fv = fv - rhof * (c->fgrkc[k] * dut + c->fgrkc[Fi + k] * dvt + c->fgrkc[2 * Fi + k] * dwt);
+      fv = fv + rhof * (c->fgrkb[k] * dusf + c->fgrkb[Fi + k] * dvsf + c->fgrkb[2 * Fi + k] * dwsf);
+      fv = fv + cfl * c->rhok[i] + ffl * c->rhok[j];
+    }
     const double mstar = rhof * (usf * sx + vsf * sy + wsf * sz);
     fv = fv + (1. - c->urf_u) * (c->mdot[k] - mstar);
     c->mdot[k] = cfl * c->p[i] + ffl * c->p[j] + fv;
@@ -1131,6 +1185,15 @@ void or_case_pc_assemble(or_case *c) {
       const double cfl = rhof * geo, ffl = -(rhof * geo);
       fv = fv - rhof * (pfx * dut + pfy * dvt + pfz * dwt);
       fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
+      if (c->turb) { /* :1222-1242 */
+        const double kbx = c->bgrk[b], kby = c->bgrk[c->Fb + b], kbz = c->bgrk[2 * c->Fb + b];
+        double kc = (c->rhok[j1] - c->rhok[i1]) / c->pdcf[b];
+        kc = kc - (kbx * c->pux[b] + kby * c->puy[b] + kbz * c->puz[b]);
+        const double kfx = kbx + kc * c->pux[b], kfy = kby + kc * c->puy[b], kfz = kbz + kc * c->puz[b];
+        fv = fv - rhof * (kfx * dut + kfy * dvt + kfz * dwt);
+        fv = fv + rhof * (kbx * dusf + kby * dvsf + kbz * dwsf);
+        fv = fv + cfl * c->rhok[i1] + ffl * c->rhok[j1];
+      }
       const double uf1s = rot(0, c->us[j1], c->vs[j1], c->ws[j1]), vf1s = rot(1, c->us[j1], c->vs[j1], c->ws[j1]),
                    wf1s = rot(2, c->us[j1], c->vs[j1], c->ws[j1]);
       const double usf = gf * c->us[i1] + (1. - gf) * uf1s, vsf = gf * c->vs[i1] + (1. - gf) * vf1s,
@@ -1339,6 +1402,367 @@ void or_case_iterate(or_case *c, int niter) {
     }
     or_case_pc_assemble(c);
     or_case_pc_solve_correct(c);
+    if (c->turb) or_case_sst(c);
   }
   c->mass_l1 = or_case_mass_imbalance(c);
+}
+
+/* ===================================================================== */
+/* k-omega SST, ModelNumber 14, low-Reynolds-number walls ('menter').     */
+
+/* SetTurbulenceModelCoefficients, 'komegasst' (SetModelCoefficients.for
+ * :266-300): REAL(4) literals widened to double on assignment. */
+#define SST_SIGK1 ((double)0.85f)
+#define SST_SIGK2 1.0
+#define SST_SIGW1 ((double)0.5f)
+#define SST_SIGW2 ((double)0.856f)
+#define SST_ALPHA1 ((double)(5.0f / 9.0f))
+#define SST_ALPHA2 ((double)0.44f)
+#define SST_BETA1 ((double)0.075f)
+#define SST_BETA2 ((double)0.0828f)
+#define SST_BETTA ((double)0.075f)
+#define SST_A1 ((double)0.31f)
+#define SST_CMU ((double)0.09f)
+#define SST_C1LIM 10.0 /* c1limiter (SetModelCoefficients.for:18) */
+#define SST_TINY 1.e-15 /* Constants1 tiny */
+
+int or_case_set_sst(or_case *c, const int *io, const double *d, const double *k, const double *omega,
+                    const double *mut, const double *wd, const double *bk, const double *bomega,
+                    const double *bmut) {
+  for (int q = 0; q < 2; ++q) {
+    const int fw = io[4 + 2 * q], sc = io[5 + 2 * q];
+    if (io[2 * q] < 0 || io[2 * q] > 5 || io[1 + 2 * q] < 0 || fw < OR_HR_NONE || fw > OR_HR_NVF ||
+        sc < OR_CONV_UPWIND || sc > OR_CONV_GAMMA || io[8 + 2 * q] < 1 || io[8 + 2 * q] > 2 ||
+        io[9 + 2 * q] < 0 || io[12 + q] < 0 || io[12 + q] > 2)
+      return -1;
+  }
+  if (!(d[0] > 0.) || !(d[1] > 0.) || !(d[2] > 0.) || !(d[5] > 0.)) return -1;
+  const int N = c->N, Fb = c->Fb, Fi = c->Fi;
+  c->turb = 1;
+  c->k_solver = io[0]; c->k_itmax = io[1]; c->w_solver = io[2]; c->w_itmax = io[3];
+  c->k_fw = io[4]; c->k_scheme = io[5]; c->w_fw = io[6]; c->w_scheme = io[7];
+  c->k_ngrad = io[8] == 1 ? 0 : io[9];
+  c->w_ngrad = io[10] == 1 ? 0 : io[11];
+  c->k_lim = io[12]; c->w_lim = io[13];
+  c->urf_k = d[0]; c->urf_w = d[1]; c->urf_mut = d[2]; c->k_rrf = d[3]; c->w_rrf = d[4];
+  c->max_visc_ratio = d[5];
+  double **cells[] = {&c->tk, &c->tw, &c->mut, &c->wd, &c->rhok, &c->F1, &c->F2, &c->prod, &c->sr, &c->med};
+  for (size_t q = 0; q < sizeof(cells) / sizeof(cells[0]); ++q) { free(*cells[q]); *cells[q] = dz(N); }
+  double **bfs[] = {&c->btk, &c->btw, &c->bmut, &c->brhok, &c->bF1, &c->bF2, &c->bsr, &c->bmed};
+  for (size_t q = 0; q < sizeof(bfs) / sizeof(bfs[0]); ++q) { free(*bfs[q]); *bfs[q] = dz(Fb); }
+  double **g3[] = {&c->gk, &c->gom, &c->grk};
+  for (size_t q = 0; q < 3; ++q) { free(*g3[q]); *g3[q] = dz(3 * (size_t)N); }
+  double **b3[] = {&c->bgk, &c->bgom, &c->bgrk};
+  for (size_t q = 0; q < 3; ++q) { free(*b3[q]); *b3[q] = dz(3 * (size_t)Fb); }
+  double **f3[] = {&c->fgk, &c->fgom, &c->fgrkb, &c->fgrkc};
+  for (size_t q = 0; q < 4; ++q) { free(*f3[q]); *f3[q] = dz(3 * (size_t)Fi); }
+  memcpy(c->tk, k, sizeof(double) * N);
+  memcpy(c->tw, omega, sizeof(double) * N);
+  memcpy(c->mut, mut, sizeof(double) * N);
+  memcpy(c->wd, wd, sizeof(double) * N);
+  memcpy(c->btk, bk, sizeof(double) * Fb);
+  memcpy(c->btw, bomega, sizeof(double) * Fb);
+  memcpy(c->bmut, bmut, sizeof(double) * Fb);
+  return 0; /* F1, F2 start at 0 and fr1Coefficient at 1 (AllocateStorageRoutine.for:2618-2631) */
+}
+
+static int is_wall(const or_case *c, int b) {
+  return !is_periodic(c, b) && !(c->btype && c->btype[b] == OR_BC_SYMMETRY);
+}
+
+/* CalculateStrainRateTensor (CalculateStrainRate.for): S_ij of the velocity
+ * gradients and |S| = sqrt(2 S:S), the terms summed in the reference's order */
+static double strain_rate(double ux, double uy, double uz, double vx, double vy, double vz, double wx, double wy,
+                          double wz) {
+  const double s11 = ux, s12 = 0.5 * (uy + vx), s13 = 0.5 * (uz + wx), s22 = vy, s23 = 0.5 * (vz + wy), s33 = wz;
+  return sqrt(2. * (s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 + s23 * s23 + s13 * s13 + s23 * s23 +
+                    s33 * s33));
+}
+
+static void sst_strain(or_case *c) {
+  const int N = c->N, Fb = c->Fb;
+  for (int i = 0; i < N; ++i)
+    c->sr[i] = strain_rate(c->gu[i], c->gu[N + i], c->gu[2 * N + i], c->gv[i], c->gv[N + i], c->gv[2 * N + i],
+                           c->gw[i], c->gw[N + i], c->gw[2 * N + i]);
+  for (int b = 0; b < Fb; ++b)
+    c->bsr[b] = strain_rate(c->bgu[b], c->bgu[Fb + b], c->bgu[2 * Fb + b], c->bgv[b], c->bgv[Fb + b],
+                            c->bgv[2 * Fb + b], c->bgw[b], c->bgw[Fb + b], c->bgw[2 * Fb + b]);
+}
+
+/* CalculateTurbulenceProduction, case 14 (CalculateProductionOfTurbulence.for
+ * :297-316): P_k = max(tau:grad u, 0) with tau = 2 mu_t S - 2/3 rho k I
+ * (CalculateShearStressValues.for, the non-k-epsilon branch), limited to
+ * c1limiter cmu rho max(k omega, 0). */
+static void sst_production(or_case *c) {
+  const int N = c->N;
+  const double t23 = (double)0.666666666666666666666667f;
+  for (int i = 0; i < N; ++i) {
+    const double ux = c->gu[i], uy = c->gu[N + i], uz = c->gu[2 * N + i];
+    const double vx = c->gv[i], vy = c->gv[N + i], vz = c->gv[2 * N + i];
+    const double wx = c->gw[i], wy = c->gw[N + i], wz = c->gw[2 * N + i];
+    const double s11 = ux, s12 = 0.5 * (uy + vx), s13 = 0.5 * (uz + wx), s22 = vy, s23 = 0.5 * (vz + wy), s33 = wz;
+    const double mt = c->mut[i];
+    double t11 = mt * 2. * s11, t12 = mt * 2. * s12, t13 = mt * 2. * s13, t22 = mt * 2. * s22,
+           t23v = mt * 2. * s23, t33 = mt * 2. * s33;
+    const double k1 = mx2(c->tk[i], 0.);
+    t11 = t11 - t23 * c->rho * k1;
+    t22 = t22 - t23 * c->rho * k1;
+    t33 = t33 - t23 * c->rho * k1;
+    const double p = t11 * ux + t12 * uy + t13 * uz + t12 * vx + t22 * vy + t23v * vz + t13 * wx + t23v * wy + t33 * wz;
+    const double pk = mx2(p, 0.);
+    const double lim = SST_C1LIM * SST_CMU * c->rho * mx2(c->tk[i] * c->tw[i], 0.);
+    c->prod[i] = mn2(pk, lim);
+  }
+}
+
+/* CalculateTurbulenceInterpolationFactors (CalculateTurbulenceModelCoefficients
+ * .for:3063-3290) without roughness or rotation corrections: F1 and F2 of
+ * cells and boundary faces (the boundary faces with the owner's wall
+ * distance); F3 = F4 = 1. */
+static double sst_f1(double k, double w, double d, double mu, double rho, double kx, double ky, double kz,
+                     double wx, double wy, double wz) {
+  const double tke1 = mx2(k, 0.), tom1 = mx2(w, SST_TINY);
+  const double f1 = sqrt(tke1) / (SST_CMU * tom1 * d);
+  const double f2 = 500. * mu / (rho * (d * d) * tom1);
+  const double f4 = 2. * rho * SST_SIGW2 / tom1;
+  const double f5 = kx * wx + ky * wy + kz * wz;
+  const double f6 = f4 * f5;
+  const double cd = mx2(f6, 1.e-10);
+  const double f3 = 4. * rho * SST_SIGW2 * tke1 / (cd * (d * d));
+  const double g1 = mn2(mn2(mx2(f1, f2), f3), 10.);
+  return tanh(g1 * g1 * g1 * g1);
+}
+static double sst_f2(double k, double w, double d, double mu, double rho) {
+  const double tke1 = mx2(k, 0.), tom1 = mx2(w, SST_TINY);
+  const double f1 = 2. * sqrt(tke1) / (SST_CMU * tom1 * d);
+  const double f2 = 500. * mu / (rho * (d * d) * tom1);
+  const double g2 = mn2(mx2(f1, f2), 100.);
+  return tanh(g2 * g2);
+}
+static void sst_blending(or_case *c) {
+  const or_mesh *m = c->m;
+  const int N = c->N, Fb = c->Fb;
+  for (int i = 0; i < N; ++i)
+    c->F1[i] = sst_f1(c->tk[i], c->tw[i], c->wd[i], c->visc[i], c->rho, c->gk[i], c->gk[N + i], c->gk[2 * N + i],
+                      c->gom[i], c->gom[N + i], c->gom[2 * N + i]);
+  for (int b = 0; b < Fb; ++b) {
+    const int o = m->owner[c->Fi + b];
+    c->bF1[b] = sst_f1(c->btk[b], c->btw[b], c->wd[o], c->mu, c->rho, c->bgk[b], c->bgk[Fb + b], c->bgk[2 * Fb + b],
+                       c->bgom[b], c->bgom[Fb + b], c->bgom[2 * Fb + b]);
+  }
+  for (int i = 0; i < N; ++i) c->F2[i] = sst_f2(c->tk[i], c->tw[i], c->wd[i], c->visc[i], c->rho);
+  for (int b = 0; b < Fb; ++b)
+    c->bF2[b] = sst_f2(c->btk[b], c->btw[b], c->wd[m->owner[c->Fi + b]], c->mu, c->rho);
+}
+
+/* Gradient of k or omega (CalculateGradients.for: Green-Gauss, the limiter,
+ * CorrectGradientAtPeriodicBoundary) and its face interpolation 'average' */
+static void sst_gradient(or_case *c, const double *phi, const double *bphi, double *gr, double *bgr, double *fg,
+                         int ngrad, int lim) {
+  gradient_n(c, phi, bphi, gr, bgr, ngrad);
+  if (lim) or_limit_gradient(c, lim, phi, bphi, gr, bgr);
+  periodic_gradient(c, gr, bgr);
+  face_gradient(c, phi, gr, fg, OR_GI_AVERAGE);
+}
+
+/* One transport equation of the SST model, var 0 = k ('tke',
+ * SolveTKE.for), 1 = omega ('tomega', SolveTOmega.for): gradient, effective
+ * diffusion coefficient, sources (AssembleTKESources.for:985-1096,
+ * AssembleTOmegaSources.for:558-713, low-Reynolds-number walls), diffusion
+ * (AssembleIsotropicDiffusion.for:2172-2505), convection
+ * (AssembleConvection.for:90-110,156-178,808-966), the empty user source,
+ * scatter, under-relaxation, SolveEquation, bounding and the boundary values
+ * (UpdateVariablesFromBoundaryConditions.for:76-146,431-501). */
+static void sst_equation(or_case *c, int var) {
+  const or_mesh *m = c->m;
+  const int N = c->N, Fi = c->Fi, Fb = c->Fb, nf = m->nfaces;
+  double *phi = var ? c->tw : c->tk, *bphi = var ? c->btw : c->btk;
+  double *gr = var ? c->gom : c->gk, *bgr = var ? c->bgom : c->bgk, *fg = var ? c->fgom : c->fgk;
+  memset(c->fcf, 0, sizeof(double) * nf);
+  memset(c->fff, 0, sizeof(double) * nf);
+  memset(c->fvf, 0, sizeof(double) * nf);
+  memset(c->ftf, 0, sizeof(double) * nf);
+  memset(c->fce, 0, sizeof(double) * N);
+  memset(c->fte, 0, sizeof(double) * N);
+  sst_gradient(c, phi, bphi, gr, bgr, fg, var ? c->w_ngrad : c->k_ngrad, var ? c->w_lim : c->k_lim);
+  /* CalculateEffectiveDiffusionCoefficient: mu + mu_t sigma(F1), F1 of the
+   * last omega assembly */
+  const double s1 = var ? SST_SIGW1 : SST_SIGK1, s2 = var ? SST_SIGW2 : SST_SIGK2;
+  for (int i = 0; i < N; ++i) {
+    const double sl = c->F1[i] * s1 + (1. - c->F1[i]) * s2;
+    c->med[i] = c->visc[i] + c->mut[i] * sl;
+  }
+  for (int b = 0; b < Fb; ++b) {
+    const double sl = c->bF1[b] * s1 + (1. - c->bF1[b]) * s2;
+    c->bmed[b] = c->mu + c->bmut[b] * sl;
+  }
+  if (var == 0) {
+    sst_production(c);
+    for (int i = 0; i < N; ++i) {
+      const double rted = SST_CMU * c->rho * mx2(c->tw[i], 0.);
+      const double pt = 1. * c->prod[i]; /* fr1Coefficient = 1 */
+      c->fce[i] = c->fce[i] + rted * m->vol[i];
+      c->fte[i] = c->fte[i] - pt * m->vol[i] + rted * m->vol[i] * mx2(c->tk[i], 0.);
+    }
+    for (int b = 0; b < Fb; ++b) /* SetTKEWallBC */
+      if (is_wall(c, b)) c->btk[b] = 0.;
+  } else {
+    sst_blending(c);
+    for (int i = 0; i < N; ++i) {
+      const double v = m->vol[i], w = c->tw[i], f1 = c->F1[i];
+      double t1 = c->rho * c->prod[i] * 1. / mx2(c->mut[i], 1.e-10);
+      const double al = f1 * SST_ALPHA1 + (1. - f1) * SST_ALPHA2;
+      c->fte[i] = c->fte[i] - v * mx2(al * t1, 0.);
+      t1 = c->rho * mx2(w, 0.) * v;
+      double be = f1 * SST_BETA1 + (1. - f1) * SST_BETA2;
+      be = 1. * be; /* F4factor = 1 */
+      c->fce[i] = c->fce[i] + 2. * mx2(be * t1, 0.);
+      c->fte[i] = c->fte[i] + mx2(be * t1 * w, 0.);
+      t1 = 2. * c->rho * SST_SIGW2 / mx2(w, SST_TINY);
+      const double t2 = c->gk[i] * c->gom[i] + c->gk[N + i] * c->gom[N + i] + c->gk[2 * N + i] * c->gom[2 * N + i];
+      t1 = (1. - f1) * t1 * t2 * v;
+      if (t1 > 0.) {
+        c->fte[i] = c->fte[i] - t1;
+      } else {
+        const double ted1 = mx2(w, SST_TINY);
+        c->fce[i] = c->fce[i] - t1 / ted1;
+        c->fte[i] = c->fte[i] - t1;
+      }
+    }
+    for (int b = 0; b < Fb; ++b) /* SetTOmegaWallBC, 'menter' */
+      if (is_wall(c, b)) {
+        const double d = c->wd[m->owner[Fi + b]];
+        c->btw[b] = 10. * 6. * c->mu / (c->rho * SST_BETTA * (d * d));
+      }
+  }
+  /* the wall faces' diffusion, assembled with the sources */
+  for (int b = 0; b < Fb; ++b) {
+    if (!is_wall(c, b)) continue;
+    const int f = Fi + b, i1 = m->owner[f];
+    const double gamf = c->bmed[b];
+    const double cfl = gamf * m->gdiff[f], ffl = -(gamf * m->gdiff[f]);
+    const double vfl = -(gamf * (bgr[b] * m->Tx[f] + bgr[Fb + b] * m->Ty[f] + bgr[2 * Fb + b] * m->Tz[f]));
+    c->fcf[f] = c->fcf[f] + cfl;
+    c->fff[f] = c->fff[f] + ffl;
+    c->fvf[f] = c->fvf[f] + vfl;
+    c->ftf[f] = c->ftf[f] + cfl * phi[i1] + ffl * bphi[b] + vfl;
+  }
+  /* diffusion: interior faces */
+  for (int k = 0; k < Fi; ++k) {
+    const int i = m->owner[k], j = m->neigh[k];
+    const double gf = m->g[k];
+    const double gamf = gf * c->med[i] + (1. - gf) * c->med[j];
+    const double cfl = gamf * m->gdiff[k], ffl = -(gamf * m->gdiff[k]);
+    const double vfl = -(gamf * (fg[k] * m->Tx[k] + fg[Fi + k] * m->Ty[k] + fg[2 * Fi + k] * m->Tz[k]));
+    c->fcf[k] = c->fcf[k] + cfl;
+    c->fff[k] = c->fff[k] + ffl;
+    c->fvf[k] = c->fvf[k] + vfl;
+    c->ftf[k] = c->ftf[k] + cfl * phi[i] + ffl * phi[j] + vfl;
+  }
+  /* symmetry planes: no flux, the boundary value the owner's */
+  for (int b = 0; b < Fb; ++b)
+    if (c->btype && c->btype[b] == OR_BC_SYMMETRY) bphi[b] = phi[m->owner[Fi + b]];
+  /* periodic faces: BGamaFace interpolated between the two cells */
+  if (c->pbf)
+    for (int b = 0; b < Fb; ++b) {
+      if (!is_periodic(c, b)) continue;
+      const int f = Fi + b, i1 = m->owner[f], j1 = c->pj1[b];
+      const double gamf = per_mix(c, b, c->med[i1], c->med[j1]);
+      const double cfl = gamf * c->pgd[b], ffl = -(gamf * c->pgd[b]);
+      const double vfl = -(gamf * (bgr[b] * c->ptx[b] + bgr[Fb + b] * c->pty[b] + bgr[2 * Fb + b] * c->ptz[b]));
+      c->fcf[f] = c->fcf[f] + cfl;
+      c->fff[f] = c->fff[f] + ffl;
+      c->fvf[f] = c->fvf[f] + vfl;
+      const double ftl = cfl * phi[i1] + ffl * phi[j1] + vfl;
+      c->ftf[f] = c->ftf[f] + ftl;
+    }
+  /* convection: upwind, the deferred correction, effective divergence */
+  for (int k = 0; k < Fi; ++k) {
+    const int i = m->owner[k], j = m->neigh[k];
+    const double md = c->mdot[k];
+    const double cfl = md > 0. ? md : 0.;
+    const double ffl = -(-md > 0. ? -md : 0.);
+    c->fcf[k] = c->fcf[k] + cfl;
+    c->fff[k] = c->fff[k] + ffl;
+    c->ftf[k] = c->ftf[k] + cfl * phi[i] + ffl * phi[j];
+  }
+  if (c->iter >= c->hr_start)
+    hr_correction(c, var ? c->w_fw : c->k_fw, var ? c->w_scheme : c->k_scheme, 0., phi, gr, c->ftf);
+  memset(c->effdiv, 0, sizeof(double) * N);
+  for (int k = 0; k < Fi; ++k) {
+    const int i = m->owner[k], j = m->neigh[k];
+    c->effdiv[i] = c->effdiv[i] + c->mdot[k];
+    c->effdiv[j] = c->effdiv[j] - c->mdot[k];
+  }
+  for (int b = 0; b < Fb; ++b) {
+    const int k = m->owner[Fi + b];
+    c->effdiv[k] = c->effdiv[k] + c->bmdot[b];
+  }
+  for (int i = 0; i < N; ++i) {
+    const double e = c->effdiv[i], ep = e > 0.0 ? e : 0.0;
+    const double fcel = ep - e;
+    const double fvel = -(ep * phi[i]);
+    const double ftel = fcel * phi[i] + fvel;
+    c->fce[i] = c->fce[i] + fcel;
+    c->fte[i] = c->fte[i] + ftel;
+  }
+  if (c->pbf)
+    for (int b = 0; b < Fb; ++b) {
+      if (!is_periodic(c, b)) continue;
+      const int f = Fi + b, i1 = m->owner[f], j1 = c->pj1[b];
+      const double bm = c->bmdot[b];
+      const double cfl = bm > 0. ? bm : 0., ffl = -(-bm > 0. ? -bm : 0.);
+      c->fcf[f] = c->fcf[f] + cfl;
+      c->fff[f] = c->fff[f] + ffl;
+      c->ftf[f] = c->ftf[f] + cfl * phi[i1] + ffl * phi[j1];
+    }
+  /* AssembleSourceTerm with the zero ScTKE/SbTKE (AssembleSources.for) */
+  for (int i = 0; i < N; ++i) {
+    c->fce[i] = c->fce[i] + 0.;
+    c->fte[i] = c->fte[i] - (0. + 0. * phi[i]) * m->vol[i];
+  }
+  scatter(c);
+  const double urf = var ? c->urf_w : c->urf_k;
+  for (int i = 0; i < N; ++i) c->ac[i] = c->ac[i] / urf;
+  /* ModifyCoefficientsAlongWalls: nothing for the low-Reynolds-number SST */
+  const int itmax = var ? c->w_itmax : c->k_itmax;
+  double *hist = dz(itmax + 1);
+  const int it = or_solve_equation(N, c->W, m->enbr, m->nef, c->ac, c->anb, c->bc, phi, var ? c->w_rrf : c->k_rrf,
+                                   itmax, var ? c->w_solver : c->k_solver, var ? 10 : 8, c->dphi,
+                                   var ? c->res_w : c->res_k, hist);
+  free(hist);
+  if (var) c->it_w = it; else c->it_k = it;
+  /* BoundTurbulentKE / BoundTurbulentOmega (BoundTurbulentVariables.for) */
+  for (int i = 0; i < N; ++i) phi[i] = mx2(phi[i], SST_TINY);
+  for (int b = 0; b < Fb; ++b) bphi[b] = mx2(bphi[b], 0.);
+  for (int b = 0; b < Fb; ++b) {
+    if (c->btype && c->btype[b] == OR_BC_SYMMETRY) bphi[b] = phi[m->owner[Fi + b]];
+    else if (is_periodic(c, b)) bphi[b] = per_mix(c, b, phi[m->owner[Fi + b]], phi[c->pj1[b]]);
+  }
+}
+
+/* CalculateTurbulentViscosity, case 14 (CalculateTurbulentViscosity.for
+ * :483-513,925-938): relaxed rho a1 k / max(a1 omega, |S| F2 F3, tiny),
+ * capped at MaximumViscosityRatio times the viscosity. */
+static void sst_viscosity(or_case *c) {
+  const int N = c->N, Fb = c->Fb;
+  const double u = c->urf_mut;
+  for (int i = 0; i < N; ++i) {
+    const double den = mx2(mx2(SST_A1 * c->tw[i], c->sr[i] * c->F2[i] * 1.), SST_TINY);
+    c->mut[i] = (1. - u) * c->mut[i] + u * c->rho * SST_A1 * mx2(c->tk[i], 0.) / den;
+  }
+  for (int b = 0; b < Fb; ++b) {
+    const double den = mx2(mx2(SST_A1 * c->btw[b], c->bsr[b] * c->bF2[b] * 1.), SST_TINY);
+    c->bmut[b] = (1. - u) * c->bmut[b] + u * c->rho * SST_A1 * mx2(c->btk[b], 0.) / den;
+  }
+  for (int i = 0; i < N; ++i) c->mut[i] = mn2(c->mut[i], c->max_visc_ratio * c->visc[i]);
+  for (int b = 0; b < Fb; ++b) c->bmut[b] = mn2(c->bmut[b], c->max_visc_ratio * c->mu);
+}
+
+void or_case_sst(or_case *c) {
+  if (!c->turb) return;
+  sst_strain(c);
+  sst_equation(c, 0);
+  sst_equation(c, 1);
+  sst_viscosity(c);
 }

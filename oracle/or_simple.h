@@ -104,6 +104,25 @@ typedef struct {
   int nperiodic;
   int *ppair;             /* partner patch per patch (-1 none) */
   double *ptrans;         /* x/y/zTranslation per patch */
+  /* k-omega SST (TurbulenceModel 'komegasst', ModelNumber 14) with
+   * low-Reynolds-number walls (WallBCModel 'menter'); turb = 0 laminar.
+   * Cells: k, omega, mu_t, the wall distance, F1/F2, TurbulenceProduction,
+   * StrainRate, 2/3 rho k and the effective diffusion coefficient of the
+   * equation in assembly (eDiffCoefficient); boundary faces: their B*
+   * counterparts.  Gradients as gu (3N / 3Fb / 3Fi); the 2/3 rho k face
+   * gradients interpolated 'average' (fgrkb) and 'averagecorrected' (fgrkc). */
+  int turb;
+  double *tk, *tw, *mut, *wd, *btk, *btw, *bmut;
+  double *gk, *gom, *bgk, *bgom, *fgk, *fgom;
+  double *rhok, *brhok, *grk, *bgrk, *fgrkb, *fgrkc;
+  double *F1, *bF1, *F2, *bF2, *prod, *sr, *bsr, *med, *bmed;
+  /* ASSolver, ASIter, HRFramework, ConvectionScheme, nIterGradient (0 =
+   * MethodCalcGradient 1) and LimitGradientMethod (0 off) of k and omega */
+  int k_solver, k_itmax, w_solver, w_itmax, k_fw, k_scheme, w_fw, w_scheme, k_ngrad, w_ngrad, k_lim, w_lim;
+  /* urfTKE, urfTOmega, urfTViscosity, rrFTKE, rrFTOmega, MaximumViscosityRatio */
+  double urf_k, urf_w, urf_mut, k_rrf, w_rrf, max_visc_ratio;
+  int it_k, it_w;
+  double res_k[4], res_w[4];
 } or_case;
 
 enum { OR_GI_AVERAGE = 0, OR_GI_UPWIND = 1, OR_GI_DOWNWIND = 2, OR_GI_AVERAGECORRECTED = 3, OR_GI_HARMONIC = 4 };
@@ -159,6 +178,20 @@ void or_case_momentum_solve(or_case *c, int comp);
 void or_case_pc_assemble(or_case *c);
 void or_case_pc_solve_correct(or_case *c);
 void or_case_iterate(or_case *c, int niter);
+/* The k-omega SST case: initial cell fields k, omega, mu_t and the wall
+ * distance, initial boundary-face fields k, omega, mu_t; every outer
+ * iteration then ends with CalculateEffectsOfTurbulence.  iopt: k solver,
+ * k itmax, omega solver, omega itmax, k framework, k scheme, omega
+ * framework, omega scheme, k MethodCalcGradient, k nIterGradient, omega
+ * MethodCalcGradient, omega nIterGradient, k limiter, omega limiter (0 off);
+ * dopt: urfTKE, urfTOmega, urfTViscosity, rrFTKE, rrFTOmega,
+ * MaximumViscosityRatio.  Returns -1 on an invalid option. */
+int or_case_set_sst(or_case *c, const int *iopt, const double *dopt, const double *k, const double *omega,
+                    const double *mut, const double *wd, const double *bk, const double *bomega,
+                    const double *bmut);
+/* CalculateEffectsOfTurbulence for the SST case
+ * (SolveSelectedTurbulenceModelEquations.for:4-13,150-160). */
+void or_case_sst(or_case *c);
 double or_case_mass_imbalance(or_case *c);
 
 #ifdef __cplusplus

@@ -294,12 +294,30 @@ def run_reference_geometry(pm, mesh, decomp=3):
     return out
 
 
+# The k-omega SST case (ref_simple_driver.f90 set_turbulence_case): solvers,
+# iteration limits, convection (framework, scheme), MethodCalcGradient and
+# nIterGradient, gradient limiter (0 off) of k and omega, the wall distance
+# (0 the input's wd, 1 or 2 the reference's MethodCalculateNormalDistance);
+# urfTKE, urfTOmega, urfTViscosity, rrFTKE, rrFTOmega, MaximumViscosityRatio.
+SST_DEFAULTS = dict(ksolver=SOR, kitmax=20, wsolver=SOR, witmax=20, kframework="none", kscheme="upwind",
+                    wframework="none", wscheme="upwind", kgrad=(2, 2), wgrad=(2, 2), klim=0, wlim=0, wdmethod=0,
+                    urf_k=0.5, urf_w=0.5, urf_mut=0.5, krrf=0.1, wrrf=0.1, max_visc_ratio=1.0e6)
+
+
+def sst_settings_blob(t):
+    ints = (t["ksolver"], t["kitmax"], t["wsolver"], t["witmax"], HR_FRAMEWORKS[t["kframework"]],
+            CONV_SCHEMES[t["kscheme"]], HR_FRAMEWORKS[t["wframework"]], CONV_SCHEMES[t["wscheme"]],
+            *t["kgrad"], *t["wgrad"], t["klim"], t["wlim"], t["wdmethod"])
+    dbl = (t["urf_k"], t["urf_w"], t["urf_mut"], t["krrf"], t["wrrf"], t["max_visc_ratio"])
+    return np.array(ints, np.int32).tobytes() + np.array(dbl, np.float64).tobytes()
+
+
 def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, niter=1, rho=1.0, mu=0.01,
                          urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1, mitmax=20, psolver=SOR, prrf=0.01,
                          pitmax=200, framework="none", scheme="upwind", bleed=0.0, start_iter=0, lim_mom=0,
                          lim_cont=0, plmg=0, algorithm="simple", grad_mom=(2, 2), grad_cont=(2, 2),
                          decomp=3, gi_mom="average", gi_cont="averagecorrected", restart=False, seconds=False,
-                         periodic=None):
+                         periodic=None, sst=None):
     """niter outer iterations of the reference's own SIMPLE (SolveMomentumX/Y/Z,
     SolveContinuity) from rest on the polyMesh ``pm`` with the Readit topology of
     ``mesh``; patch_types per patch "wall"/"symmetry" (default all walls);
@@ -310,12 +328,17 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
     patch), beta (initial periodicBeta), mdot (periodicMdot), calc_beta,
     relax, iters, every (LcalculateBeta, relaxBeta, BetaIterations,
     nIterCorrectBeta); the run then also returns "beta" and "bmdot" (the
-    boundary mass fluxes in boundary-face order)."""
+    boundary mass fluxes in boundary-face order).  sst: a dict of the k-omega
+    SST case (see SST_DEFAULTS) with the initial cell fields k, omega, mut,
+    wd and boundary-face fields bk, bomega, bmut; the run then also returns
+    k, omega, mut (N), bk, bomega, bmut (boundary faces), F1 and the wall
+    distance the run used, wd (N)."""
     nf, nif, N = mesh.nf, mesh.nif, mesh.N
     codes = np.array([PATCH_CODES[t] for t in (patch_types or ["wall"] * len(pm.patch_sizes))], np.int32)
     ints = (msolver, mitmax, psolver, pitmax, niter, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], lim_mom,
             lim_cont, start_iter, plmg, {"simple": 0, "simplec": 1}[algorithm], *grad_mom, *grad_cont,
-            GRADIENT_INTERPOLATION[gi_mom], GRADIENT_INTERPOLATION[gi_cont], 1 if periodic else 0)
+            GRADIENT_INTERPOLATION[gi_mom], GRADIENT_INTERPOLATION[gi_cont], 1 if periodic else 0,
+            1 if sst else 0)
     dbl = (rho, mu, urf_u, urf_p, mrrf, prrf, bleed)
     blob = b""
     if periodic:
@@ -324,6 +347,9 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
             np.array([periodic.get("beta", 0.0), periodic["mdot"], periodic.get("relax", 1.0)], np.float64).tobytes() + \
             np.array([int(periodic.get("calc_beta", 1)), periodic.get("iters", 1), periodic.get("every", 1)],
                      np.int32).tobytes()
+    if sst:
+        t = dict(SST_DEFAULTS, **sst)
+        blob += sst_settings_blob(t)
     with tempfile.TemporaryDirectory(dir=os.path.join(HERE, "_ref")) as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
@@ -331,6 +357,9 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
                           decomp=decomp, after_doubles=blob)
             for a in (wall_u, wall_v, wall_w):
                 f.write(_c(a, np.float64).tobytes())
+            if sst:
+                for k in ("k", "omega", "mut", "wd", "bk", "bomega", "bmut"):
+                    f.write(_c(t[k], np.float64).tobytes())
         t0 = time.perf_counter()
         r = subprocess.run([REF_SIMPLE, fin, fout], capture_output=True, text=True, cwd=td,
                            env=dict(os.environ, OMP_NUM_THREADS="1"))
@@ -343,6 +372,11 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
         for k, n in (("u", N), ("v", N), ("w", N), ("p", N), ("mdot", nif)):
             out[k] = d[o:o + n].copy()
             o += n
+        if sst:
+            for k, n in (("k", N), ("omega", N), ("mut", N), ("bk", nf - nif), ("bomega", nf - nif),
+                         ("bmut", nf - nif), ("F1", N), ("wd", N)):
+                out[k] = d[o:o + n].copy()
+                o += n
         if periodic:
             out["beta"] = float(d[o])
             out["bmdot"] = d[o + 1:o + 1 + nf - nif].copy()
@@ -529,7 +563,16 @@ class _OrCase(C.Structure):
                    "pg", "pdx", "pdy", "pdz", "pdcf", "pux", "puy", "puz", "pgd", "ptx", "pty", "ptz", "bmdot")] + \
                [("tuse", C.c_double * 3)] + [(n, C.c_double) for n in ("beta", "beta_mdot", "beta_relax")] + \
                [(n, C.c_int) for n in ("beta_calc", "beta_iters", "beta_every", "nperiodic")] + \
-               [("ppair", C.POINTER(C.c_int)), ("ptrans", C.POINTER(C.c_double))]
+               [("ppair", C.POINTER(C.c_int)), ("ptrans", C.POINTER(C.c_double))] + \
+               [("turb", C.c_int)] + \
+               [(n, C.POINTER(C.c_double)) for n in (
+                   "tk", "tw", "mut", "wd", "btk", "btw", "bmut", "gk", "gom", "bgk", "bgom", "fgk", "fgom",
+                   "rhok", "brhok", "grk", "bgrk", "fgrkb", "fgrkc", "F1", "bF1", "F2", "bF2", "prod", "sr", "bsr",
+                   "med", "bmed")] + \
+               [(n, C.c_int) for n in ("k_solver", "k_itmax", "w_solver", "w_itmax", "k_fw", "k_scheme", "w_fw",
+                                       "w_scheme", "k_ngrad", "w_ngrad", "k_lim", "w_lim")] + \
+               [(n, C.c_double) for n in ("urf_k", "urf_w", "urf_mut", "k_rrf", "w_rrf", "max_visc_ratio")] + \
+               [("it_k", C.c_int), ("it_w", C.c_int), ("res_k", C.c_double * 4), ("res_w", C.c_double * 4)]
 
 
 PATCH_CODES = {"wall": 0, "symmetry": 1, "periodic": 2}
@@ -551,7 +594,9 @@ class OrCase:
              "bgu": "3Fb", "bgv": "3Fb", "bgw": "3Fb", "bgp": "3Fb", "fgu": "3Fi", "fgv": "3Fi", "fgw": "3Fi",
              "fgp": "3Fi", "du": "N", "dv": "N", "dw": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
              "anb": "NW", "bc": "N", "fcf": "F", "fff": "F", "ftf": "F", "fce": "N", "fte": "N", "bmdot": "Fb",
-             "bgpc": "3Fb", "gpc": "3N"}
+             "bgpc": "3Fb", "gpc": "3N", "tk": "N", "tw": "N", "mut": "N", "wd": "N", "btk": "Fb", "btw": "Fb",
+             "bmut": "Fb", "gk": "3N", "gom": "3N", "bgk": "3Fb", "bgom": "3Fb", "F1": "N", "bF1": "Fb", "F2": "N",
+             "bF2": "Fb", "prod": "N", "sr": "N", "bsr": "Fb", "rhok": "N", "grk": "3N", "bgrk": "3Fb"}
 
     def __init__(self, pm, bu, bv, bw, rho=1.0, mu=0.01, urf_u=0.7, urf_p=0.3, msolver=SOR, mrrf=0.1,
                  mitmax=20, psolver=SOR, prrf=0.01, pitmax=200):
@@ -614,6 +659,23 @@ class OrCase:
     @property
     def beta(self):
         return self.ptr.contents.beta
+
+    def set_sst(self, k, omega, mut, wd, bk, bomega, bmut, **opts):
+        """The k-omega SST case (or_case_set_sst): initial cell fields k,
+        omega, mu_t, the wall distance and the initial boundary-face k, omega,
+        mu_t; opts as SST_DEFAULTS (solvers, iteration limits, convection,
+        gradient method, limiters, relaxation, tolerances, viscosity cap)."""
+        t = dict(SST_DEFAULTS, **opts)
+        io = np.array([t["ksolver"], t["kitmax"], t["wsolver"], t["witmax"], HR_FRAMEWORKS[t["kframework"]],
+                       CONV_SCHEMES[t["kscheme"]], HR_FRAMEWORKS[t["wframework"]], CONV_SCHEMES[t["wscheme"]],
+                       *t["kgrad"], *t["wgrad"], t["klim"], t["wlim"]], np.int32)
+        d = np.array([t["urf_k"], t["urf_w"], t["urf_mut"], t["krrf"], t["wrrf"], t["max_visc_ratio"]], np.float64)
+        L = lib()
+        L.or_case_set_sst.argtypes = [C.POINTER(_OrCase), _ip, _dp] + [_dp] * 7
+        if L.or_case_set_sst(self.ptr, _c(io, np.int32), _c(d, np.float64), *(_c(a, np.float64) for a in
+                                                                          (k, omega, mut, wd, bk, bomega, bmut))) != 0:
+            raise ValueError("or_case_set_sst: invalid option")
+        L.or_case_sst.argtypes = [C.POINTER(_OrCase)]
 
     def set_convection(self, framework, scheme, bleed=0.0, start_iter=0):
         """HRFramework ('none', 'tvd', 'nvf'), ConvectionScheme, Bleed and

@@ -13,6 +13,12 @@
 !           symmetry planes (allocateBCStorage, SetBoundaryConditions,
 !           SetBCPatches, CalculateFaceDensity, CalculateInitialmdot,
 !           SolveMomentumX/Y/Z, SolveContinuity) and writes u, v, w, p, mdot.
+!           With iturb = 1 the flow is turbulent with the k-omega SST model
+!           (ModelNumber 14, low-Reynolds-number walls, WallBCModel 'menter')
+!           from the k, omega, mu_t and wall distance the input gives, and
+!           every iteration ends with CalculateEffectsOfTurbulence
+!           (SolveSelectedTurbulenceModelEquations.for:4,150); k, omega and
+!           mu_t are written after the flow fields.
 !           The storage the reference's allocateStorage would allocate for
 !           such a flow is allocated here and zeroed as it does
 !           (AllocateStorageRoutine.for:36-147,208-259,586-700,1617-2193).
@@ -29,6 +35,8 @@ program ref_simple_driver
   use PhysicalProperties1
   use Residuals1
   use MultiGrid1
+  use Turbulence1
+  use WallDistance1, only: WallDistance, BWallDistance, MethodCalculateNormalDistance
   use Geometry1, only: NumberOfNodes, NumberOfElements, NumberOfBCSets, NumbOfElementNodes, &
                        ListOfElementNodes, NumbOfElementFaces, x, y, z
   use Geometry3, only: NIFaces, NFacesTotal, NIFaceOwner, NIFaceNeighbor, NBFaces, NBFaceOwner, &
@@ -45,7 +53,10 @@ program ref_simple_driver
   integer :: npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode, idecomp
   integer :: i, j, k, f, q, nbfmax, nb, it
   integer :: msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec
-  integer :: gmm, gmi, gcm, gci, gim, gic, iper, bcalc, biters, bevery
+  integer :: gmm, gmi, gcm, gci, gim, gic, iper, bcalc, biters, bevery, iturb
+  integer :: tint(15)
+  double precision :: tdbl(6)
+  double precision, allocatable :: tk0(:), tw0(:), tmut0(:), twd(:), tbk0(:), tbw0(:), tbmut0(:)
   integer, allocatable :: perpair(:)
   double precision, allocatable :: pertrans(:)
   double precision :: beta0, bmdot0, brelax
@@ -66,7 +77,7 @@ program ref_simple_driver
   read(21) npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode, idecomp
   if (mode == 2) then
     read(21) msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec, &
-             gmm, gmi, gcm, gci, gim, gic, iper
+             gmm, gmi, gcm, gci, gim, gic, iper, iturb
     read(21) rho, mu, urfu, urfp, mrrf, prrf, bleed
     if (iper /= 0) then
       ! translational periodic pairs (the case file's block of
@@ -77,6 +88,15 @@ program ref_simple_driver
       read(21) pertrans
       read(21) beta0, bmdot0, brelax
       read(21) bcalc, biters, bevery
+    end if
+    if (iturb /= 0) then
+      ! k and omega: solver, itmax (each), HR framework and scheme (each),
+      ! MethodCalcGradient and nIterGradient (each), gradient limiter method
+      ! (each, 0 off), the wall distance (0 the input's, 1 or 2 the
+      ! reference's MethodCalculateNormalDistance); urfTKE, urfTOmega,
+      ! urfTViscosity, rrFTKE, rrFTOmega, MaximumViscosityRatio
+      read(21) tint
+      read(21) tdbl
     end if
   end if
   allocate(fnn(nfaces), fnodes(nfaces, maxfn), owner(nfaces), neigh(nif), psize(npatches), ptype(npatches))
@@ -107,6 +127,12 @@ program ref_simple_driver
   if (mode == 2) then
     allocate(wallu(nfaces - nif), wallv(nfaces - nif), wallw(nfaces - nif))
     read(21) wallu, wallv, wallw
+    if (iturb /= 0) then
+      allocate(tk0(ncells), tw0(ncells), tmut0(ncells), twd(ncells))
+      allocate(tbk0(nfaces - nif), tbw0(nfaces - nif), tbmut0(nfaces - nif))
+      read(21) tk0, tw0, tmut0, twd
+      read(21) tbk0, tbw0, tbmut0
+    end if
   end if
   close(21)
   NumberofElementNeighbors = NumbOfElementFaces
@@ -174,6 +200,13 @@ program ref_simple_driver
   if (mode == 2) then
     call run_simple
     write(22) uVelocity, vVelocity, wVelocity, Pressure, mdot
+    if (iturb /= 0) then
+      write(22) TurbulentKE, TurbulentOmega, TurbulentViscosity
+      call put_bfaces(BTurbulentKE)
+      call put_bfaces(BTurbulentOmega)
+      call put_bfaces(BTurbulentViscosity)
+      write(22) F1factor, WallDistance
+    end if
     if (iper /= 0) then
       write(22) periodicBeta
       do q = 1, npatches
@@ -197,7 +230,7 @@ contains
     LSolveMomentum = .true.
     LSolveContinuity = .true.
     LSolveEnergy = .false.
-    LTurbulentFlow = .false.
+    LTurbulentFlow = iturb /= 0
     LUnsteady = .false.
     Lcompressible = .false.
     LBuoyancy = .false.
@@ -270,6 +303,7 @@ contains
       outletTypeC(q) = 'none'
       outletTypeE(q) = 'none'
     end do
+    if (iturb /= 0) call set_turbulence_case
     call SetBoundaryConditions
     call SetBCPatches
     LRotationalPeriodicity = .false.
@@ -292,6 +326,7 @@ contains
       call RotationMatrixCoefficients
     end if
     call allocate_flow_storage
+    if (iturb /= 0) call allocate_turbulence_storage
     f = 0
     do q = 1, npatches
       do j = 1, psize(q)
@@ -317,8 +352,142 @@ contains
       call SolveMomentumY
       call SolveMomentumZ
       call SolveContinuity
+      if (iturb /= 0) call CalculateEffectsOfTurbulence
     end do
   end subroutine run_simple
+
+  ! The k-omega SST case: the model switches of the case file
+  ! (ProblemDescriptionRoutine.for) and the model constants
+  ! (SetTurbulenceModelCoefficients, SetModelCoefficients.for:266-300).
+  subroutine set_turbulence_case
+    character(len=6), parameter :: tsolvers(0:5) = (/ 'sor   ', 'ilu   ', 'pbcg  ', 'direct', 'gmres ', 'bicgs ' /)
+    TurbulenceModel = 'komegasst'
+    WallTreatment = 'lowreynoldsnumber'
+    WallBCModel = 'menter'
+    LRough = .false.
+    LRotation = .false.
+    LKOmegaSSTRotationCurvatureCorrection = .false.
+    LCompressibilityTemperatureCorrection = .false.
+    LBuoyancyCorrection = .false.
+    LCompressibilityCorrection = .false.
+    LimitTurbulenceProduction = .true.
+    LSolveTurbulenceKineticEnergy = .true.
+    LSolveTurbulenceSpecificDissipationRate = .true.
+    LSolveTurbulenceDissipationRate = .false.
+    ASSolverTKE = tsolvers(tint(1))
+    ASIterTKE = tint(2)
+    ASSolverTOmega = tsolvers(tint(3))
+    ASIterTOmega = tint(4)
+    HRFrameworkTKE = frameworks(tint(5))
+    ConvectionSchemeTKE = schemes(tint(6))
+    HRFrameworkTOmega = frameworks(tint(7))
+    ConvectionSchemeTOmega = schemes(tint(8))
+    MethodCalcGradientTKE = tint(9)
+    nIterGradientTKE = tint(10)
+    MethodCalcGradientTOmega = tint(11)
+    nIterGradientTOmega = tint(12)
+    LimitGradientTKE = tint(13) > 0
+    LimitGradientTKEMethod = max(tint(13), 1)
+    LimitGradientTOmega = tint(14) > 0
+    LimitGradientTOmegaMethod = max(tint(14), 1)
+    BleedTKE = 0.
+    BleedTOmega = 0.
+    LMultigridTKE = .false.
+    LMultigridTOmega = .false.
+    LRelaxTKE = .true.
+    LRelaxTOmega = .true.
+    LFalseTransientTKE = .false.
+    LFalseTransientTOmega = .false.
+    urfTKE = tdbl(1)
+    urfTOmega = tdbl(2)
+    urfTViscosity = tdbl(3)
+    rrFTKE = tdbl(4)
+    rrFTOmega = tdbl(5)
+    MaximumViscosityRatio = tdbl(6)
+    GradientInterpolationSchemeTKE = 'average'
+    GradientInterpolationSchemeTOmega = 'average'
+    InterpolationSchemeGamaTKE = 'average'
+    InterpolationSchemeGamaTOmega = 'average'
+    call SetTurbulenceModelCoefficients
+  end subroutine set_turbulence_case
+
+  ! What allocateStorage holds for the k-omega SST flow
+  ! (AllocateStorageRoutine.for:793-846,1239-1262,1312-1397), zeroed as it
+  ! zeroes it (:2195-2243,2618-2631), with fr1Coefficient 1; then the
+  ! initial k, omega, mu_t and wall distance of the input.
+  subroutine allocate_turbulence_storage
+    integer :: n, nb2, qq, jj, ff
+    n = ncells
+    nb2 = nbfmax
+    allocate(rhok(n), Brhok(npatches, nb2), drhokdx(n), drhokdy(n), drhokdz(n))
+    allocate(Bdrhokdx(npatches, nb2), Bdrhokdy(npatches, nb2), Bdrhokdz(npatches, nb2))
+    allocate(rhoTED(n), BrhoTED(npatches, nb2), ReT(n), BReT(npatches, nb2))
+    allocate(S11(n), S12(n), S13(n), S22(n), S23(n), S33(n))
+    allocate(BS11(npatches, nb2), BS12(npatches, nb2), BS13(npatches, nb2), BS22(npatches, nb2))
+    allocate(BS23(npatches, nb2), BS33(npatches, nb2), StrainRate(n), BStrainRate(npatches, nb2))
+    allocate(W11(n), W12(n), W13(n), W22(n), W23(n), W33(n))
+    allocate(BW11(npatches, nb2), BW12(npatches, nb2), BW13(npatches, nb2), BW22(npatches, nb2))
+    allocate(BW23(npatches, nb2), BW33(npatches, nb2), Vorticity(n), BVorticity(npatches, nb2))
+    allocate(Tau11(n), Tau12(n), Tau13(n), Tau22(n), Tau23(n), Tau33(n))
+    allocate(TurbulentKE(n), BTurbulentKE(npatches, nb2), TKEGradx(n), TKEGrady(n), TKEGradz(n))
+    allocate(BTKEGradx(npatches, nb2), BTKEGrady(npatches, nb2), BTKEGradz(npatches, nb2))
+    allocate(TKEGradfx(nif), TKEGradfy(nif), TKEGradfz(nif), ScTKE(n), SbTKE(n))
+    allocate(TurbulenceProduction(n), BTurbulenceProduction(npatches, nb2))
+    allocate(TurbulentOmega(n), BTurbulentOmega(npatches, nb2), TOmegaGradx(n), TOmegaGrady(n))
+    allocate(TOmegaGradz(n), BTOmegaGradx(npatches, nb2), BTOmegaGrady(npatches, nb2))
+    allocate(BTOmegaGradz(npatches, nb2), TOmegaGradfx(nif), TOmegaGradfy(nif), TOmegaGradfz(nif))
+    allocate(ScTOmega(n), SbTOmega(n))
+    allocate(F1factor(n), BF1factor(npatches, nb2), fr1Coefficient(n), F2factor(n))
+    allocate(BF2factor(npatches, nb2), F3factor(n), BF3factor(npatches, nb2), F4factor(n))
+    allocate(BF4factor(npatches, nb2))
+    allocate(TurbulentViscosity(n), BTurbulentViscosity(npatches, nb2))
+    rhok = 0.; Brhok = 0.; drhokdx = 0.; drhokdy = 0.; drhokdz = 0.
+    Bdrhokdx = 0.; Bdrhokdy = 0.; Bdrhokdz = 0.; rhoTED = 0.; BrhoTED = 0.; ReT = 0.; BReT = 0.
+    S11 = 0.; S12 = 0.; S13 = 0.; S22 = 0.; S23 = 0.; S33 = 0.
+    BS11 = 0.; BS12 = 0.; BS13 = 0.; BS22 = 0.; BS23 = 0.; BS33 = 0.; StrainRate = 0.; BStrainRate = 0.
+    W11 = 0.; W12 = 0.; W13 = 0.; W22 = 0.; W23 = 0.; W33 = 0.
+    BW11 = 0.; BW12 = 0.; BW13 = 0.; BW22 = 0.; BW23 = 0.; BW33 = 0.; Vorticity = 0.; BVorticity = 0.
+    Tau11 = 0.; Tau12 = 0.; Tau13 = 0.; Tau22 = 0.; Tau23 = 0.; Tau33 = 0.
+    TKEGradx = 0.; TKEGrady = 0.; TKEGradz = 0.; BTKEGradx = 0.; BTKEGrady = 0.; BTKEGradz = 0.
+    TKEGradfx = 0.; TKEGradfy = 0.; TKEGradfz = 0.; ScTKE = 0.; SbTKE = 0.
+    TurbulenceProduction = 0.; BTurbulenceProduction = 0.
+    TOmegaGradx = 0.; TOmegaGrady = 0.; TOmegaGradz = 0.; BTOmegaGradx = 0.; BTOmegaGrady = 0.
+    BTOmegaGradz = 0.; TOmegaGradfx = 0.; TOmegaGradfy = 0.; TOmegaGradfz = 0.; ScTOmega = 0.; SbTOmega = 0.
+    F1factor = 0.; BF1factor = 0.; fr1Coefficient = 1.; F2factor = 0.; BF2factor = 0.
+    F3factor = 0.; BF3factor = 0.; F4factor = 0.; BF4factor = 0.
+    BTurbulentKE = 0.; BTurbulentOmega = 0.; BTurbulentViscosity = 0.
+    TurbulentKE = tk0
+    TurbulentOmega = tw0
+    TurbulentViscosity = tmut0
+    ! the wall distance and the nearest wall face of every cell (iTau, jTau)
+    ! from the reference's CalculateNormalDistanceToWall
+    ! (CalculateDistanceToWall.for:4), which writes them to unit 15
+    MethodCalculateNormalDistance = max(tint(15), 1)
+    open(unit=15, status='scratch')
+    call CalculateNormalDistanceToWall
+    close(15)
+    if (tint(15) == 0) WallDistance = twd
+    ff = 0
+    do qq = 1, npatches
+      do jj = 1, psize(qq)
+        ff = ff + 1
+        BTurbulentKE(qq, jj) = tbk0(ff)
+        BTurbulentOmega(qq, jj) = tbw0(ff)
+        BTurbulentViscosity(qq, jj) = tbmut0(ff)
+      end do
+    end do
+  end subroutine allocate_turbulence_storage
+
+  ! a boundary-face array in boundary-face order (patch by patch)
+  subroutine put_bfaces(b)
+    double precision, intent(in) :: b(:,:)
+    integer :: qq, jj
+    do qq = 1, npatches
+      do jj = 1, psize(qq)
+        write(22) b(qq, jj)
+      end do
+    end do
+  end subroutine put_bfaces
 
   ! What allocateStorage holds for this flow, zeroed as it zeroes it; the
   ! densities are ConstantDensity.

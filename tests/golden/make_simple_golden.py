@@ -76,6 +76,25 @@ SIMPLE = {
                                dict(niter=8, msolver=po.ILU, psolver=po.ILU, prrf=1e-8, pitmax=500,
                                     framework="tvd", scheme="sou", algorithm="simplec",
                                     periodic=dict(beta=0.1, mdot=0.0, calc_beta=0))),
+    # the k-omega SST model (low-Reynolds-number walls) on the channel with a
+    # prescribed pressure gradient and on the lid-driven box; the initial k,
+    # omega, mu_t and wall distance are sst_initial's
+    "channel_sst_ilu": (("channel_tet", (4, 6, 3)),
+                        dict(niter=8, msolver=po.ILU, psolver=po.ILU, periodic=dict(beta=0.05, mdot=0.0, calc_beta=0),
+                             sst=dict(ksolver=po.ILU, wsolver=po.ILU))),
+    "channel_sst_hr": (("channel_tet", (4, 6, 3)),
+                       dict(niter=8, msolver=po.ILU, psolver=po.ILU, framework="tvd", scheme="sou",
+                            periodic=dict(beta=0.05, mdot=0.0, calc_beta=0),
+                            sst=dict(ksolver=po.ILU, wsolver=po.ILU, kframework="tvd", kscheme="sou",
+                                     wframework="tvd", wscheme="minmod"))),
+    "channel_sst_sor": (("channel_tet", (4, 6, 3)),
+                        dict(niter=8, periodic=dict(beta=0.05, mdot=0.0, calc_beta=0),
+                             sst=dict(ksolver=po.SOR, wsolver=po.PBCG, kgrad=(1, 0), urf_k=0.7, urf_w=0.6,
+                                      urf_mut=0.8))),
+    "hex_sst_limited": (("box_hex", (5, 4, 3), 0.15, 3),
+                        dict(niter=6, msolver=po.ILU, psolver=po.ILU,
+                             sst=dict(ksolver=po.ILU, wsolver=po.ILU, klim=1, wlim=2, kframework="nvf",
+                                      kscheme="smart", wgrad=(2, 3)))),
     # converged states (the device is compared with these)
     "hex_converged": (("box_hex", (8, 7, 6), 0.15, 3), dict(niter=400)),
     "tet_converged": (("box_tet", (4, 4, 3), 0.15, 4), dict(niter=400)),
@@ -115,6 +134,27 @@ def periodic_settings(spec, kw):
     return dict(kw["periodic"], pairs=pairs, translations=tr)
 
 
+def sst_initial(spec, pm, types, k0=1e-3, w0=1.0):
+    """Initial state of the SST cases: uniform k0, omega0, mu_t = k0/omega0
+    (rho = 1) in the cells; on the boundary faces k = mu_t = 0 at walls and
+    the cell values elsewhere, omega0 everywhere; the wall distance of the
+    cell centroids to the nearest wall plane (the channel's y walls, the
+    box's six faces)."""
+    m = po.OrMesh(pm)
+    nb = pm.n_faces - pm.n_internal
+    xc, yc, zc = m.cell["xc"], m.cell["yc"], m.cell["zc"]
+    if spec[0] == "channel_tet":
+        wd = np.minimum(yc, CHANNEL_BOX["ly"] - yc)
+        wall = np.concatenate([[t == "wall"] * n for t, n in zip(types, pm.patch_sizes)])
+    else:
+        lo, hi = pm.points.min(axis=0), pm.points.max(axis=0)
+        wd = np.minimum.reduce([xc - lo[0], hi[0] - xc, yc - lo[1], hi[1] - yc, zc - lo[2], hi[2] - zc])
+        wall = np.ones(nb, bool)
+    n = m.N
+    return dict(k=np.full(n, k0), omega=np.full(n, w0), mut=np.full(n, k0 / w0), wd=wd,
+                bk=np.where(wall, 0.0, k0), bomega=np.full(nb, w0), bmut=np.where(wall, 0.0, k0 / w0))
+
+
 def main(only=()):
     assert po.ref_simple_available(), "build oracle/_ref/ref_simple.x first (make -C oracle ref)"
     if only:
@@ -140,6 +180,8 @@ def write_simple(names):
             kw["periodic"] = periodic_settings(spec, kw)
         pm, bu, bv, bw, types = build_mesh(spec)
         mesh = po.OrMesh(pm)
+        if "sst" in kw:
+            kw["sst"] = dict(sst_initial(spec, pm, types), **kw["sst"])
         # the reference's restart file (writeSolution) of the cases the device
         # reproduces bitwise
         restart = "_ilu" in name and not name.startswith("channel")

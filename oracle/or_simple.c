@@ -1466,9 +1466,7 @@ int or_case_set_sst(or_case *c, const int *io, const double *d, const double *k,
   return 0; /* F1, F2 start at 0 and fr1Coefficient at 1 (AllocateStorageRoutine.for:2618-2631) */
 }
 
-static int is_wall(const or_case *c, int b) {
-  return !is_periodic(c, b) && !(c->btype && c->btype[b] == OR_BC_SYMMETRY);
-}
+static int is_wall(const or_case *c, int b) { return !c->btype || c->btype[b] == OR_BC_WALL; }
 
 /* CalculateStrainRateTensor (CalculateStrainRate.for): S_ij of the velocity
  * gradients and |S| = sqrt(2 S:S), the terms summed in the reference's order */
@@ -1765,4 +1763,107 @@ void or_case_sst(or_case *c) {
   sst_equation(c, 0);
   sst_equation(c, 1);
   sst_viscosity(c);
+}
+
+/* CalculateNormalDistanceToWall, MethodCalculateNormalDistance 2
+ * (CalculateDistanceToWall.for:154-470): the Poisson problem lap(phi) = -1
+ * with phi = 0 on the no-slip walls and no flux elsewhere (periodic faces
+ * included), Green-Gauss gradients (method 2, two sweeps) and 'average' face
+ * gradients, outer passes of assembly and solve until the solve's final L1
+ * residual is at most tol; then d = -|grad phi| + sqrt(|grad phi|^2 + 2 phi)
+ * with the gradient of the last pass, and the cells next to a wall the
+ * normal distance of their (last) wall face.  The inner solves use
+ * or_solve_equation (solver, rrf, itmax) where the reference uses its
+ * algebraic multigrid: the converged result is the same to within the
+ * tolerance.  Returns the outer passes run, or -1 when maxouter passes do not
+ * converge.  phi_out may be NULL. */
+int or_case_wall_distance(or_case *c, double tol, int maxouter, int solver, double rrf, int itmax, double *wd,
+                          double *phi_out) {
+  const or_mesh *m = c->m;
+  const int N = c->N, Fi = c->Fi, Fb = c->Fb, nf = m->nfaces;
+  double *phi = dz(N), *bphi = dz(Fb), *gr = dz(3 * (size_t)N), *bgr = dz(3 * (size_t)Fb);
+  double *fg = dz(3 * (size_t)Fi), *hist = dz(itmax + 1);
+  for (int i = 0; i < N; ++i) phi[i] = 1.;
+  for (int b = 0; b < Fb; ++b) bphi[b] = is_wall(c, b) ? 0. : 1.;
+  double res = 1.e10;
+  int outer = 1, passes = 0;
+  const double stop = tol < 0. ? -tol * 0.75 : tol;
+  while (outer < maxouter && res > stop) {
+    memset(c->fcf, 0, sizeof(double) * nf);
+    memset(c->fff, 0, sizeof(double) * nf);
+    memset(c->fvf, 0, sizeof(double) * nf);
+    memset(c->ftf, 0, sizeof(double) * nf);
+    memset(c->fce, 0, sizeof(double) * N);
+    memset(c->fte, 0, sizeof(double) * N);
+    gradient_n(c, phi, bphi, gr, bgr, 2);
+    periodic_gradient(c, gr, bgr);
+    face_gradient(c, phi, gr, fg, OR_GI_AVERAGE);
+    for (int k = 0; k < Fi; ++k) {
+      const int i = m->owner[k], j = m->neigh[k];
+      const double cfl = m->gdiff[k], ffl = -m->gdiff[k];
+      const double vfl = -fg[k] * m->Tx[k] - fg[Fi + k] * m->Ty[k] - fg[2 * Fi + k] * m->Tz[k];
+      c->fcf[k] = c->fcf[k] + cfl;
+      c->fff[k] = c->fff[k] + ffl;
+      c->fvf[k] = c->fvf[k] + vfl;
+      c->ftf[k] = c->ftf[k] + cfl * phi[i] + ffl * phi[j] + vfl;
+    }
+    for (int b = 0; b < Fb; ++b) {
+      const int f = Fi + b, i1 = m->owner[f];
+      if (is_wall(c, b)) {
+        bphi[b] = 0.;
+        const double cfl = m->gdiff[f], ffl = -m->gdiff[f];
+        const double vfl = -(bgr[b] * m->Tx[f] + bgr[Fb + b] * m->Ty[f] + bgr[2 * Fb + b] * m->Tz[f]);
+        c->fcf[f] = c->fcf[f] + cfl;
+        c->fff[f] = c->fff[f] + ffl;
+        c->fvf[f] = c->fvf[f] + vfl;
+        c->ftf[f] = c->ftf[f] + cfl * phi[i1] + ffl * bphi[b] + vfl;
+      } else {
+        bphi[b] = phi[i1];
+        c->fcf[f] = c->fcf[f] + 0.;
+        c->fff[f] = c->fff[f] + 0.;
+        c->fvf[f] = c->fvf[f] + 0.;
+        c->ftf[f] = c->ftf[f] + 0.;
+      }
+    }
+    /* AssembleSourceTerm with ScWall = 0, SbWall = 1 */
+    for (int i = 0; i < N; ++i) {
+      c->fce[i] = c->fce[i] + 0.;
+      c->fte[i] = c->fte[i] - (1. + 0. * phi[i]) * m->vol[i];
+    }
+    /* no implicit periodic coupling: the periodic faces carry no flux, so
+     * the scatter's partner slots receive FluxFf = 0 */
+    scatter(c);
+    double r4[4];
+    double ires = 0.;
+    for (int i = 0; i < N; ++i) ires += fabs(c->bc[i]);
+    or_solve_equation(N, c->W, m->enbr, m->nef, c->ac, c->anb, c->bc, phi, rrf, itmax, solver, 5, c->dphi, r4,
+                      hist);
+    /* the solve's final L1 residual (CheckConvergence) of the updated phi */
+    double s = 0.;
+    for (int i = 0; i < N; ++i) {
+      double r = c->bc[i] - c->ac[i] * c->dphi[i];
+      for (int q = 0; q < m->nef[i]; ++q) {
+        const int kq = m->enbr[(size_t)q * N + i];
+        if (kq) r = r - c->anb[(size_t)q * N + i] * c->dphi[kq - 1];
+      }
+      s += fabs(r);
+    }
+    res = tol < 0. ? ires : s; /* tol < 0: stop on the pass's initial residual |bc|_1 <= -tol */
+    if (tol < 0. && res <= -tol) res = -tol * 0.5;
+    outer = outer + 1;
+    ++passes;
+  }
+  for (int i = 0; i < N; ++i) {
+    const double t1 = gr[i] * gr[i] + gr[N + i] * gr[N + i] + gr[2 * N + i] * gr[2 * N + i];
+    const double t2 = t1 + 2. * phi[i];
+    wd[i] = -sqrt(t1) + sqrt(t2);
+  }
+  for (int b = 0; b < Fb; ++b)
+    if (is_wall(c, b)) {
+      const int f = Fi + b;
+      wd[m->owner[f]] = m->dx[f] * m->nx[f] + m->dy[f] * m->ny[f] + m->dz[f] * m->nz[f];
+    }
+  if (phi_out) memcpy(phi_out, phi, sizeof(double) * N);
+  free(phi); free(bphi); free(gr); free(bgr); free(fg); free(hist);
+  return res > stop ? -1 : passes;
 }

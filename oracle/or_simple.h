@@ -192,6 +192,11 @@ int or_case_set_sst(or_case *c, const int *iopt, const double *dopt, const doubl
 /* CalculateEffectsOfTurbulence for the SST case
  * (SolveSelectedTurbulenceModelEquations.for:4-13,150-160). */
 void or_case_sst(or_case *c);
+/* CalculateNormalDistanceToWall, method 2 (the Poisson wall distance) of the
+ * case's no-slip walls into wd (N); phi_out (N, or NULL) the Poisson
+ * solution.  Returns the outer passes, -1 without convergence. */
+int or_case_wall_distance(or_case *c, double tol, int maxouter, int solver, double rrf, int itmax, double *wd,
+                          double *phi_out);
 double or_case_mass_imbalance(or_case *c);
 
 #ifdef __cplusplus

@@ -677,6 +677,19 @@ class OrCase:
             raise ValueError("or_case_set_sst: invalid option")
         L.or_case_sst.argtypes = [C.POINTER(_OrCase)]
 
+    def wall_distance(self, tol=1e-10, maxouter=10000, solver=ILU, rrf=1e-7, itmax=10):
+        """The Poisson wall distance of the case's no-slip walls
+        (or_case_wall_distance): (wd, phi, outer passes)."""
+        L = lib()
+        L.or_case_wall_distance.argtypes = [C.POINTER(_OrCase), C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
+                                            _dp, _dp]
+        n = self.dims["N"]
+        wd, phi = np.zeros(n), np.zeros(n)
+        it = L.or_case_wall_distance(self.ptr, tol, maxouter, solver, rrf, itmax, wd, phi)
+        if it < 0:
+            raise RuntimeError("or_case_wall_distance did not converge")
+        return wd, phi, it
+
     def set_convection(self, framework, scheme, bleed=0.0, start_iter=0):
         """HRFramework ('none', 'tvd', 'nvf'), ConvectionScheme, Bleed and
         nIterStartApplyingHR of the momentum equations."""

@@ -36,7 +36,7 @@ program ref_simple_driver
   use Residuals1
   use MultiGrid1
   use Turbulence1
-  use WallDistance1, only: WallDistance, BWallDistance, MethodCalculateNormalDistance
+  use WallDistance1, only: WallDistance, BWallDistance, MethodCalculateNormalDistance, WallResiduals
   use Geometry1, only: NumberOfNodes, NumberOfElements, NumberOfBCSets, NumbOfElementNodes, &
                        ListOfElementNodes, NumbOfElementFaces, x, y, z
   use Geometry3, only: NIFaces, NFacesTotal, NIFaceOwner, NIFaceNeighbor, NBFaces, NBFaceOwner, &
@@ -462,7 +462,10 @@ contains
     ! the wall distance and the nearest wall face of every cell (iTau, jTau)
     ! from the reference's CalculateNormalDistanceToWall
     ! (CalculateDistanceToWall.for:4), which writes them to unit 15
-    MethodCalculateNormalDistance = max(tint(15), 1)
+    ! (3: method 2 run to an outer residual of 1e-13 instead of WallResiduals
+    ! = 1e-10, WallDistance1.F90:12)
+    MethodCalculateNormalDistance = min(max(tint(15), 1), 2)
+    if (tint(15) == 3) WallResiduals = 1.d-13
     open(unit=15, status='scratch')
     call CalculateNormalDistanceToWall
     close(15)

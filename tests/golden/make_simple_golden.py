@@ -110,6 +110,14 @@ SIMPLE = {
 
 CHANNEL_BOX = dict(lx=2.0, ly=2.0, lz=1.0)
 
+# the reference's Poisson wall distance (CalculateNormalDistanceToWall,
+# method 2) run to an outer residual of 1e-13: the graded tetrahedral
+# channel (periodic faces carry no flux in this problem) and a perturbed box
+WALL_DISTANCE = {
+    "channel_tet": ("channel_tet", (4, 6, 3)),
+    "box_hex": ("box_hex", (5, 4, 3), 0.15, 3),
+}
+
 
 def build_mesh(spec):
     if spec[0] == "channel_tet":
@@ -157,6 +165,9 @@ def sst_initial(spec, pm, types, k0=1e-3, w0=1.0):
 
 def main(only=()):
     assert po.ref_simple_available(), "build oracle/_ref/ref_simple.x first (make -C oracle ref)"
+    if only == ["wall_distance"]:
+        write_wall_distance()
+        return
     if only:
         write_simple(only)
         return
@@ -170,6 +181,23 @@ def main(only=()):
     np.savez_compressed(os.path.join(HERE, "geometry_ref.npz"), **out)
     print("wrote geometry_ref.npz")
     write_simple(SIMPLE)
+    write_wall_distance()
+
+
+def write_wall_distance():
+    out = {}
+    for name, spec in WALL_DISTANCE.items():
+        pm, bu, bv, bw, types = build_mesh(spec)
+        mesh = po.OrMesh(pm)
+        sst = dict(sst_initial(spec, pm, types), wdmethod=3)
+        r = po.run_reference_simple(pm, mesh, bu, bv, bw, patch_types=types, niter=0,
+                                    periodic=periodic_settings(spec, dict(periodic=dict(beta=0.0, mdot=0.0,
+                                                                                          calc_beta=0)))
+                                    if spec[0] == "channel_tet" else None, sst=sst)
+        out[f"{name}_wd"] = r["wd"]
+        out[f"{name}_xc"] = mesh.cell["xc"]
+    np.savez_compressed(os.path.join(HERE, "wall_distance_ref.npz"), **out)
+    print("wrote wall_distance_ref.npz")
 
 
 def write_simple(names):

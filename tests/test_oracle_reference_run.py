@@ -111,3 +111,31 @@ def test_sst_channel_fixture_blends_f1():
     the fixtures cover both branches of the k-omega / k-epsilon blending."""
     g = golden("simple_channel_sst_ilu.npz")
     assert g["F1"].min() < 0.99 and g["F1"].max() == 1.0
+
+
+@pytest.mark.parametrize("name", sorted(msg.WALL_DISTANCE))
+def test_poisson_wall_distance_matches_the_reference_build(name):
+    """CalculateNormalDistanceToWall, method 2 (tests/golden/wall_distance_ref
+    .npz, the reference run to an outer residual of 1e-13): the oracle's
+    converged fixed point (every pass's initial residual below 1e-13) is
+    within 2e-7 of the largest distance; the cells at the walls carry their
+    faces' normal distance exactly; and on the channel the distance follows
+    the planes y = 0, 2 to the discretisation's accuracy."""
+    g = golden("wall_distance_ref.npz")
+    spec = msg.WALL_DISTANCE[name]
+    pm, bu, bv, bw, types = msg.build_mesh(spec)
+    oc = po.OrCase(pm, bu, bv, bw)
+    if types:
+        oc.set_patch_types(pm.patch_sizes, types)
+    assert np.array_equal(oc.mesh.cell["xc"], g[f"{name}_xc"])
+    wd, phi, passes = oc.wall_distance(tol=-1e-13, solver=po.ILU, itmax=200, rrf=1e-12, maxouter=2000)
+    ref = g[f"{name}_wd"]
+    assert np.abs(wd - ref).max() <= 2e-7 * ref.max(), np.abs(wd - ref).max() / ref.max()
+    m = oc.mesh
+    kinds = np.concatenate([[t] * n for t, n in zip(types or ["wall"] * len(pm.patch_sizes), pm.patch_sizes)])
+    at_wall = np.unique(np.asarray(pm.owner)[m.nif:][kinds == "wall"])
+    assert np.array_equal(wd[at_wall], ref[at_wall])
+    assert (wd > 0).all()
+    if spec[0] == "channel_tet":
+        plane = np.minimum(m.cell["yc"], 2.0 - m.cell["yc"])
+        assert np.abs(wd - plane).max() < 0.1 * plane.max()

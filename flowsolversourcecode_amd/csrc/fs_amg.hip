@@ -1608,8 +1608,8 @@ int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc,
   amg_prepare(m, ac, anb);
   hipStream_t s = m->stream;
   const int n = m->n;
-  double *r = m->v[8].p, *c = m->v[9].p;
-  int *flags = m->flags.p;
+  double *r = m->wk.v[8], *c = m->wk.v[9];
+  int *flags = m->wk.flags;
   // additive correction as in the reference unless asked otherwise
   const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.0;
   // flags[2] holds itmax for the Krylov solvers; the multigrid loop is
@@ -1621,7 +1621,7 @@ int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc,
   // initial guess
   if (warm)
     hipLaunchKernelGGL(k_amg_res, dim3(gr), dim3(kBlock), 0, s, n, m->w, m->d_nbr.p, ac, anb, bc, dphi, r,
-                       m->partial.p, (const int *)flags);
+                       m->wk.partial, (const int *)flags);
   else
     FS_HIP_THROW(hipMemcpyAsync(r, bc, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   int fl[4] = {0, 0, 0, 0};
@@ -1634,7 +1634,7 @@ int amg_solve(fs_mesh *m, const double *ac, const double *anb, const double *bc,
       amg_apply(m, r, c, flags, alpha);
       hipLaunchKernelGGL(k_add2, dim3(gfor(n)), dim3(kBlock), 0, s, n, dphi, c, flags);
       hipLaunchKernelGGL(k_amg_res, dim3(gr), dim3(kBlock), 0, s, n, m->w, m->d_nbr.p, ac, anb,
-                         bc, dphi, r, m->partial.p, flags);
+                         bc, dphi, r, m->wk.partial, flags);
       launch_fin_conv(m, gr);
     }
     launched += batch;

@@ -112,6 +112,7 @@ int fs_mesh_create(int n_cells, int width, const int *elem_nbr, const int *n_nbr
   m->w = width;
   int rc = 0;
   try {
+    FS_HIP_THROW(hipGetDevice(&m->device));
     FS_HIP_THROW(hipStreamCreate(&m->stream));
     m->h_nbr.assign(elem_nbr, elem_nbr + (size_t)n_cells * width);
     m->h_nnbr.assign(n_nbr, n_nbr + n_cells);

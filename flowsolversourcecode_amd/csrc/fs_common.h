@@ -97,6 +97,7 @@ struct KernelClock {
 
 struct fs_mesh {
   int n = 0, w = 0;
+  int device = 0;           // the device current at fs_mesh_create
   hipStream_t stream = nullptr;
   bool own_stream = true;   // false: the stream belongs to another context
   std::vector<int> h_nbr;   // reference layout, 1-based, 0 = boundary
@@ -122,6 +123,16 @@ struct fs_mesh {
   fs::DevBuf<double> partial;  // kPartials * 5
   fs::DevBuf<double> scal;     // scalar slots
   fs::DevBuf<int> flags;       // [0] done, [1] iterations
+  // The scratch the solve uses: the mesh's buffers above, or slices of the
+  // library's static device array for the duration of a
+  // fs_solve_equation_device_noalloc call.
+  struct Scratch {
+    double *v[12];
+    double *dc;       // D-ILU diagonal
+    double *partial;  // kPartials * 5
+    double *scal;
+    int *flags;
+  } wk{};
   // options
   int mg_pre = 2, mg_post = 2, mg_maxcycles = 30, mg_maxlevels = 10;
   int mg_minparents = 10, mg_reagg = 300, mg_fp32 = 0, mg_smoother = 0;

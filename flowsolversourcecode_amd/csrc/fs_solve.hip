@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 
 #include "fs_common.h"
@@ -139,6 +140,12 @@ __global__ void k_init_norms(int n, const double *__restrict__ ac, const double 
 // dphi = 0 (SolveAlegbraicEquations.for:47), unless the input was refused
 __global__ void k_zero_start(int n, double *__restrict__ x, const int *__restrict__ flags) {
   if (flags[F_BAD]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+// x = 0 by a kernel (no memset node, so the sequence can be graph-captured)
+__global__ void k_zero(int n, double *__restrict__ x) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) x[i] = 0.0;
 }
@@ -1081,6 +1088,11 @@ int mesh_prepare_solver(fs_mesh *m) {
   m->partial.alloc(5 * kPartials);
   m->scal.alloc(S_NSLOTS);
   m->flags.alloc(F_NFLAGS);
+  for (int k = 0; k < 12; ++k) m->wk.v[k] = m->v[k].p;
+  m->wk.dc = sw->dc.p;
+  m->wk.partial = m->partial.p;
+  m->wk.scal = m->scal.p;
+  m->wk.flags = m->flags.p;
   m->nnz = 0;
   for (int i = 0; i < m->n; ++i)
     for (int j = 0; j < m->w; ++j)
@@ -1193,7 +1205,7 @@ static void ilu_factor(fs_mesh *m, const double *ac, const double *anb, double *
 }
 
 static void host_flags(fs_mesh *m, int *fl) {
-  FS_HIP_THROW(hipMemcpyAsync(fl, m->flags.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, m->stream));
+  FS_HIP_THROW(hipMemcpyAsync(fl, m->wk.flags, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, m->stream));
   FS_HIP_THROW(hipStreamSynchronize(m->stream));
 }
 
@@ -1239,10 +1251,10 @@ static void scale_guess(fs_mesh *m, const double *ac, const double *anb, const d
   const int n = m->n, gr = grid_red(n);
   hipStream_t s = m->stream;
   hipLaunchKernelGGL(k_spmv_dot_ts, dim3(gr), dim3(kBlock), 0, s, n, m->w, EllNbr(m->d_nbr.p, m->d_nbr16.p), ac, anb, x,
-                     v, bc, m->partial.p, m->flags.p);
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM_SCALE, gr, n, 4, m->partial.p, m->scal.p,
-                     m->flags.p);
-  hipLaunchKernelGGL(k_warm_scale, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, m->scal.p, m->flags.p);
+                     v, bc, m->wk.partial, m->wk.flags);
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_WARM_SCALE, gr, n, 4, m->wk.partial, m->wk.scal,
+                     m->wk.flags);
+  hipLaunchKernelGGL(k_warm_scale, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, m->wk.scal, m->wk.flags);
 }
 
 int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int solver, int lmg,
@@ -1252,8 +1264,8 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
   hipStream_t s = m->stream;
   auto t0 = std::chrono::steady_clock::now();
   const int gr = grid_red(n), ge = grid_for(n);
-  double *part = m->partial.p, *scal = m->scal.p;
-  int *flags = m->flags.p;
+  double *part = m->wk.partial, *scal = m->wk.scal;
+  int *flags = m->wk.flags;
   const int *nbr = m->d_nbr.p;
   const EllNbr nb(m->d_nbr.p, m->d_nbr16.p); // the row loops of the hot kernels
   SolveWork &sw = work(m);
@@ -1287,8 +1299,8 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
   }
 
   int iters = 0;
-  double *V0 = m->v[0].p, *V1 = m->v[1].p, *V2 = m->v[2].p, *V3 = m->v[3].p, *V4 = m->v[4].p,
-         *V5 = m->v[5].p, *V6 = m->v[6].p, *V7 = m->v[7].p;
+  double *V0 = m->wk.v[0], *V1 = m->wk.v[1], *V2 = m->wk.v[2], *V3 = m->wk.v[3], *V4 = m->wk.v[4],
+         *V5 = m->wk.v[5], *V6 = m->wk.v[6], *V7 = m->wk.v[7];
 
   // An initial guess, whatever the solver: scaled to its multiple with the
   // smallest L2 residual, then dropped for the zero start when its L1
@@ -1306,7 +1318,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     iters = amg_solve(m, ac, anb, bc, dphi, solver, warm);
   } else if (solver == FS_PBCG) {
     // r=V0 rr=V1 p=V2 pp=V3 z=V4 zz=V5 (PBCG1.F90:4)
-    ilu_factor(m, ac, anb, sw.dc.p);
+    ilu_factor(m, ac, anb, m->wk.dc);
     hipLaunchKernelGGL(k_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, dphi, V0);
     hipLaunchKernelGGL(k_pbcg_start, dim3(ge), dim3(kBlock), 0, s, n, bc, V0, V1);
     hipLaunchKernelGGL(k_spmv, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, V0, V1);
@@ -1316,7 +1328,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     const bool pre = (lmg == FS_MG_PRECOND);
     if (pre) amg_prepare(m, ac, anb);
     const double alpha = m->mg_alpha > 0.0 ? m->mg_alpha : 1.7;
-    const double *dc = pre ? nullptr : sw.dc.p;
+    const double *dc = pre ? nullptr : m->wk.dc;
     iters = run_batches(m, itmax, nf * 8 + solver, [&]() {
       if (pre) {
         amg_apply(m, V0, V6, flags, alpha);
@@ -1366,7 +1378,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     const double ib = nb.d16 ? 4.0 * ((w + 1) / 2) : 4.0 * w;
     const double spmv_bytes = 8.0 * n + ib * n + 8.0 * m->nnz + 24.0 * n;
     hipLaunchKernelGGL(k_resid_vec, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, V0, V1, part);
-    FS_HIP(hipMemsetAsync(V3, 0, sizeof(double) * n, s));
+    hipLaunchKernelGGL(k_zero, dim3(ge), dim3(kBlock), 0, s, n, V3);
     // rho of the first iteration; every later rho comes with the update
     hipLaunchKernelGGL(k_dot2, dim3(gr), dim3(kBlock), 0, s, n, V1, V0, nullptr, nullptr, part, flags);
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_BICG_RHO, gr, n, nf, part, scal, flags);
@@ -1427,7 +1439,7 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
       sw.gm_m = gmm;
     }
     double *V = sw.gm_V.p, *gm = sw.gm_state.p, *gp = sw.gm_part.p;
-    double *Z = m->v[10].p, *U = m->v[11].p;
+    double *Z = m->wk.v[10], *U = m->wk.v[11];
     const size_t ldv = (size_t)n;
     auto restart = [&]() {
       hipLaunchKernelGGL(k_gm_resid, dim3(gr), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, V0, part, flags);
@@ -1473,22 +1485,22 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
     // the reference's ILU iteration: exact D-ILU diagonal, then per iteration
     // the residual and the two level-scheduled substitutions
     if (!sw.ilu.ready) precompute_ilu_sweep(m, sw.ilu);
-    ilu_factor(m, ac, anb, sw.dc.p);
+    ilu_factor(m, ac, anb, m->wk.dc);
     IluSweep &S = sw.ilu;
-    double *rcv = m->v[0].p;
+    double *rcv = m->wk.v[0];
     iters = run_batches(m, itmax, nf * 8 + solver, [&]() {
       hipLaunchKernelGGL(k_ilu_resid, dim3(ge), dim3(kBlock), 0, s, n, w, nbr, ac, anb, bc, dphi, rcv, flags);
       for (size_t l = 0; l + 1 < S.fwd_level_ptr.size(); ++l) {
         const int b0 = S.fwd_level_ptr[l], b1 = S.fwd_level_ptr[l + 1];
         if (b1 > b0)
           hipLaunchKernelGGL(k_ilu_fwd, dim3(grid_for(b1 - b0)), dim3(kBlock), 0, s, b1 - b0, S.fwd_cells.p + b0,
-                             S.fwd_ptr.p, S.fwd_lin.p, S.fwd_src.p, anb, sw.dc.p, rcv, flags);
+                             S.fwd_ptr.p, S.fwd_lin.p, S.fwd_src.p, anb, m->wk.dc, rcv, flags);
       }
       for (size_t l = 0; l + 1 < S.bwd_level_ptr.size(); ++l) {
         const int b0 = S.bwd_level_ptr[l], b1 = S.bwd_level_ptr[l + 1];
         if (b1 > b0)
           hipLaunchKernelGGL(k_ilu_bwd, dim3(grid_for(b1 - b0)), dim3(kBlock), 0, s, b1 - b0, S.bwd_cells.p + b0,
-                             S.bwd_ptr.p, S.bwd_lin.p, S.bwd_src.p, anb, sw.dc.p, rcv, dphi, flags);
+                             S.bwd_ptr.p, S.bwd_lin.p, S.bwd_src.p, anb, m->wk.dc, rcv, dphi, flags);
       }
       hipLaunchKernelGGL(k_resid_l1, dim3(gr), dim3(kBlock), 0, s, n, w, nb, ac, anb, bc, dphi, part, flags);
       hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, s, (int)FIN_CONV, gr, n, nf, part, scal, flags);
@@ -1540,7 +1552,7 @@ __global__ void k_add(int n, double *__restrict__ a, const double *__restrict__ 
 
 void launch_fin_conv(fs_mesh *m, int np) {
   hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinBlock), 0, m->stream, (int)FIN_CONV, np, m->n, 4,
-                     m->partial.p, m->scal.p, m->flags.p);
+                     m->wk.partial, m->wk.scal, m->wk.flags);
 }
 
 void add_inplace(fs_mesh *m, double *a, const double *b) {
@@ -1560,13 +1572,13 @@ int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, 
   FS_HIP(hipEventCreate(&e0));
   FS_HIP(hipEventCreate(&e1));
   int h_fl[F_NFLAGS] = {0, 0, 1, 0};
-  FS_HIP(hipMemcpyAsync(m->flags.p, h_fl, sizeof(h_fl), hipMemcpyHostToDevice, m->stream));
+  FS_HIP(hipMemcpyAsync(m->wk.flags, h_fl, sizeof(h_fl), hipMemcpyHostToDevice, m->stream));
   hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, EllNbr(m->d_nbr.p, m->d_nbr16.p), ac, anb, x,
-                     y, dot_with, m->partial.p, m->flags.p);
+                     y, dot_with, m->wk.partial, m->wk.flags);
   FS_HIP(hipEventRecord(e0, m->stream));
   for (int r = 0; r < reps; ++r)
     hipLaunchKernelGGL(k_spmv_dot, dim3(gr), dim3(kBlock), 0, m->stream, n, w, EllNbr(m->d_nbr.p, m->d_nbr16.p), ac, anb,
-                       x, y, dot_with, m->partial.p, m->flags.p);
+                       x, y, dot_with, m->wk.partial, m->wk.flags);
   FS_HIP(hipEventRecord(e1, m->stream));
   FS_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -1577,3 +1589,101 @@ int time_spmv(fs_mesh *m, const double *ac, const double *anb, const double *x, 
   return 0;
 }
 } // namespace fs
+
+// ---- the solve without allocation: its scratch in one static device array.
+// Layout (doubles, every slice 256-byte aligned): 8 work vectors and the D-ILU
+// diagonal of kNoallocMaxCells each, the reduction partials, the scalar slots,
+// the flags.
+namespace fs {
+constexpr int kNoallocMaxCells = FS_NOALLOC_MAX_CELLS;
+constexpr size_t kNoallocVec = (size_t)kNoallocMaxCells;
+constexpr size_t kNoallocPartial = 5 * (size_t)kPartials;
+constexpr size_t kNoallocScal = 64, kNoallocFlags = 64;
+static_assert(S_NSLOTS <= (int)kNoallocScal && F_NFLAGS * sizeof(int) <= kNoallocFlags * sizeof(double),
+              "scalar and flag slices");
+static_assert(kNoallocVec % 32 == 0 && kNoallocPartial % 32 == 0, "256-byte slices");
+constexpr size_t kNoallocDoubles = 9 * kNoallocVec + kNoallocPartial + kNoallocScal + kNoallocFlags;
+static_assert(kNoallocDoubles * sizeof(double) == FS_NOALLOC_SCRATCH_BYTES, "size stated in fsamd.h");
+} // namespace fs
+
+static __device__ __align__(256) double scratch[fs::kNoallocDoubles];
+
+namespace fs {
+// The device address of `scratch` on each device, taken once per device with
+// that device current (each GPU holds its own copy of the array).
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[64] = {nullptr};
+
+static double *scratch_on(int dev) {
+  if (dev < 0 || dev >= 64) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    double *ptr = nullptr;
+    FS_HIP_THROW(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+int solve_device_noalloc(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int solver, int lmg,
+                         const double *ac, const double *anb, const double *bc, double *dphi, double *res4,
+                         fs_solve_stats *st) {
+  if (m->n > kNoallocMaxCells) {
+    set_error("fs_solve_equation_device_noalloc: " + std::to_string(m->n) + " cells exceed the scratch array's " +
+              std::to_string(kNoallocMaxCells));
+    return -1;
+  }
+  if (lmg != FS_MG_OFF) {
+    set_error("fs_solve_equation_device_noalloc: LMultigrid must be 0 (the AMG hierarchy is allocated)");
+    return -1;
+  }
+  if (solver != FS_SOR && solver != FS_PBCG && solver != FS_BICGS && solver != FS_CG) {
+    set_error("fs_solve_equation_device_noalloc: solver must be SOR, PBCG, BICGS or CG (ILU builds its "
+              "substitution schedule and GMRES its basis on first use)");
+    return -3;
+  }
+  int cur = 0;
+  FS_HIP(hipGetDevice(&cur));
+  if (cur != m->device) {
+    set_error("fs_solve_equation_device_noalloc: the mesh's device " + std::to_string(m->device) +
+              " is not the current device " + std::to_string(cur));
+    return -1;
+  }
+  double *base = scratch_on(cur);
+  fs_mesh::Scratch saved = m->wk, sc{};
+  for (int k = 0; k < 8; ++k) sc.v[k] = base + (size_t)k * kNoallocVec;
+  for (int k = 8; k < 12; ++k) sc.v[k] = nullptr; // the multigrid and GMRES vectors are not used
+  sc.dc = base + 8 * kNoallocVec;
+  sc.partial = base + 9 * kNoallocVec;
+  sc.scal = sc.partial + kNoallocPartial;
+  sc.flags = reinterpret_cast<int *>(sc.scal + kNoallocScal);
+  m->wk = sc;
+  int rc;
+  try {
+    rc = solve_device(m, nf, fit, rrf, itmax, solver, lmg, ac, anb, bc, dphi, res4, st, false);
+  } catch (...) {
+    m->wk = saved;
+    throw;
+  }
+  m->wk = saved;
+  return rc;
+}
+} // namespace fs
+
+extern "C" int fs_solve_equation_device_noalloc(fs_mesh *m, int nf, double *d_fit, double rrf, int itmax, int solver,
+                                                int lmultigrid, const double *d_ac, const double *d_anb,
+                                                const double *d_bc, double *d_dphi, double *res4,
+                                                fs_solve_stats *stats) {
+  if (!m || !d_fit || !d_ac || !d_anb || !d_bc || !d_dphi) {
+    fs::set_error("fs_solve_equation_device_noalloc: NULL argument");
+    return -1;
+  }
+  try {
+    return fs::solve_device_noalloc(m, nf, d_fit, rrf, itmax, solver, lmultigrid, d_ac, d_anb, d_bc, d_dphi, res4,
+                                    stats);
+  } catch (const std::exception &e) {
+    fs::set_error(e.what());
+    return -2;
+  }
+}

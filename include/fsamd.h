@@ -149,6 +149,26 @@ int fs_solve_equation_device_from_guess(fs_mesh *mesh, int nf, double *d_fit, do
                                         const double *d_bc, double *d_dphi, double *res4,
                                         fs_solve_stats *stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the solve of fs_solve_equation_device, same arguments
+ * and results, with lmultigrid 0 and solver SOR, PBCG, BICGS or CG, whose work
+ * vectors, D-ILU diagonal, reduction partials, scalar slots and flags live in
+ * one static device array of the library instead of the mesh's buffers.  The
+ * call makes no hipMalloc/hipFree.  Limits: at most FS_NOALLOC_MAX_CELLS
+ * cells (larger meshes are refused with -1); the array takes
+ * FS_NOALLOC_SCRATCH_BYTES on every GPU the library runs on, until the
+ * process ends; the mesh's device must be current.  All calls on one GPU
+ * share that array (it keeps the last call's contents; everything a solve
+ * reads is initialised by the solve), so calls that overlap on one GPU race. */
+#define FS_NOALLOC_MAX_CELLS 524288
+#define FS_NOALLOC_SCRATCH_BYTES 38077440
+int fs_solve_equation_device_noalloc(fs_mesh *mesh, int nf, double *d_fit, double rrf,
+                                     int itmax, int solver, int lmultigrid,
+                                     const double *d_ac, const double *d_anb,
+                                     const double *d_bc, double *d_dphi, double *res4,
+                                     fs_solve_stats *stats);
+
 /* Cell-to-face map NGlobalEFaces(N,W) (column-major, 1-based global face
  * numbers: interior faces 1..Fi first, then boundary faces patch by patch)
  * with NumbOfElementFaces(N) (ReadFoamMesh.f90:329-342; Geometry1.F90:14).

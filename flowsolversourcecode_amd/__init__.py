@@ -263,7 +263,8 @@ class SimpleStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("momentum_iterations", C.c_int), ("pc_iterations", C.c_int),
                 ("pc_unconverged", C.c_int), ("pc_levels", C.c_int), ("seconds", C.c_double),
                 ("momentum_seconds", C.c_double), ("pc_seconds", C.c_double), ("pc_initial_l1", C.c_double),
-                ("pc_final_l1", C.c_double), ("res_momentum", C.c_double * 3), ("mass_l1", C.c_double)]
+                ("pc_final_l1", C.c_double), ("res_momentum", C.c_double * 3), ("mass_l1", C.c_double),
+                ("turbulence_iterations", C.c_int)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "res_momentum"}
@@ -272,7 +273,8 @@ class SimpleStats(C.Structure):
 
 
 PHASES = {"momentum_gradients": 0, "assemble_u": 1, "assemble_v": 2, "assemble_w": 3, "solve_u": 4,
-          "solve_v": 5, "solve_w": 6, "pc_assemble": 7, "pc_solve_correct": 8, "update_symmetry": 9}
+          "solve_v": 5, "solve_w": 6, "pc_assemble": 7, "pc_solve_correct": 8, "update_symmetry": 9,
+          "turbulence": 10}
 TOPOLOGY = {"nef": 0, "gef": 1, "enbr": 2, "iown": 3, "inbr": 4}
 
 
@@ -367,7 +369,8 @@ class Case:
              "gw": "3N", "gp": "3N", "gpc": "3N", "bu": "Fb", "bv": "Fb", "bw": "Fb", "bp": "Fb", "bgp": "3Fb", "du": "N", "dv": "N", "dw": "N", "du2": "N", "dv2": "N", "dw2": "N", "us": "N", "vs": "N", "ws": "N", "ac": "N",
              "anb": "NW", "bc": "N", "cf": "F", "ff": "F", "tf": "F", "vol": "N", "Sx": "F", "Sy": "F",
              "Sz": "F", "g": "F", "gdiff": "F", "Tx": "F", "Ty": "F", "Tz": "F", "dcf": "F", "A": "F",
-             "kx": "Fi", "ky": "Fi", "kz": "Fi", "bmdot": "Fb", "beta": "1"}
+             "kx": "Fi", "ky": "Fi", "kz": "Fi", "bmdot": "Fb", "beta": "1", "k": "N", "omega": "N", "mut": "N",
+             "wd": "N", "bk": "Fb", "bomega": "Fb", "bmut": "Fb", "F1": "N", "F2": "N", "grk": "3N"}
 
     def __init__(self, pm, wall_u, wall_v, wall_w, rho=1.0, mu=0.01):
         L = lib()
@@ -468,6 +471,44 @@ class Case:
         _check(L.fs_case_set_periodic(self.h, len(sizes), _iptr(sizes), _iptr(pr), _dptr(tr), float(beta),
                                       int(bool(calc_beta)), float(mdot), float(relax), int(iters), int(every)),
                "fs_case_set_periodic")
+
+    def set_sst(self, k, omega, mut, wd, bk, bomega, bmut, ksolver="ilu", kitmax=20, wsolver="ilu", witmax=20,
+                kframework="none", kscheme="upwind", wframework="none", wscheme="upwind", kgrad=(2, 2),
+                wgrad=(2, 2), klim=0, wlim=0, urf_k=0.5, urf_w=0.5, urf_mut=0.5, krrf=0.1, wrrf=0.1,
+                max_visc_ratio=1.0e6):
+        """The k-omega SST model with low-Reynolds-number walls
+        (fs_case_set_sst): initial cell fields k, omega, mu_t and the wall
+        distance, boundary-face k, omega, mu_t; the k and omega solvers,
+        iteration limits, convection, gradient methods and limiters,
+        relaxation factors (urfTKE, urfTOmega, urfTViscosity), tolerances and
+        MaximumViscosityRatio."""
+        L = lib()
+        L.fs_case_set_sst.argtypes = [C.c_void_p, _ip, _dp] + [_dp] * 7
+        sv = lambda x: SOLVERS[x] if isinstance(x, str) else int(x)
+        io = np.array([sv(ksolver), kitmax, sv(wsolver), witmax, HR_FRAMEWORKS[kframework], CONV_SCHEMES[kscheme],
+                       HR_FRAMEWORKS[wframework], CONV_SCHEMES[wscheme], *kgrad, *wgrad, klim, wlim], np.int32)
+        d = np.array([urf_k, urf_w, urf_mut, krrf, wrrf, max_visc_ratio], np.float64)
+        n, nb = self.dims["N"], self.dims["Fb"]
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (k, omega, mut, wd, bk, bomega, bmut)]
+        for a, want in zip(arrs, (n, n, n, n, nb, nb, nb)):
+            assert a is None or a.shape == (want,)
+        _check(L.fs_case_set_sst(self.h, _iptr(io), _dptr(d), *(None if a is None else _dptr(a) for a in arrs)),
+               "fs_case_set_sst")
+
+    def wall_distance(self, tol=1e-10, maxouter=2000):
+        """The Poisson wall distance of the no-slip walls on the device
+        (fs_case_wall_distance); kept for set_sst(wd=None).  Returns (wd,
+        passes); raises when the passes do not reach tol."""
+        L = lib()
+        L.fs_case_wall_distance.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, C.POINTER(C.c_int)]
+        wd = np.zeros(self.dims["N"])
+        passes = C.c_int(0)
+        rc = _check(L.fs_case_wall_distance(self.h, float(tol), int(maxouter), _dptr(wd), C.byref(passes)),
+                    "fs_case_wall_distance")
+        if rc != 0:
+            raise FsError(f"fs_case_wall_distance: {maxouter} passes did not reach {tol}")
+        return wd, passes.value
 
     def set_convection(self, framework, scheme, bleed=0.0, start_iter=0):
         """HRFrameworkMomentum ("none", "tvd", "nvf"), ConvectionSchemeMomentum

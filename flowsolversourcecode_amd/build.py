@@ -15,7 +15,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfsamd.so")
 FORTRAN = os.path.join(PKG, "fortran")
 FORTRAN_HOST = os.path.join(FORTRAN, "_build", "simple_case_device.x")
-SOURCES = ["fs_api.hip", "fs_solve.hip", "fs_amg.hip", "fs_mesh.hip", "fs_assemble.hip", "fs_simple.hip", "fs_probe.hip", "fs_mpnn.hip", "fs_polymesh.hip", "fs_periodic.hip"]
+SOURCES = ["fs_api.hip", "fs_solve.hip", "fs_amg.hip", "fs_mesh.hip", "fs_assemble.hip", "fs_simple.hip", "fs_probe.hip", "fs_mpnn.hip", "fs_polymesh.hip", "fs_periodic.hip", "fs_turb.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -99,7 +99,7 @@ __global__ void k_per_values(PerArgs a, int nf, ValArgs x) {
 // AssembleBulkViscosity.for (BeDiffCoefficient = the viscosity), and
 // AssembleConvection.for:582-627, for one momentum component.
 __global__ void k_per_momentum(PerArgs a, int comp, double mu, double c23, const double *__restrict__ visc,
-                               const double *__restrict__ phi, const double *__restrict__ u,
+                               const double *__restrict__ bvisc, const double *__restrict__ phi, const double *__restrict__ u,
                                const double *__restrict__ v, const double *__restrict__ w,
                                const double *__restrict__ bgu, const double *__restrict__ bgv,
                                const double *__restrict__ bgw, const double *__restrict__ Sx,
@@ -117,7 +117,7 @@ __global__ void k_per_momentum(PerArgs a, int comp, double mu, double c23, const
   double C = 0.0 + cfl, Fv = 0.0 + ffl;
   double T = 0.0 + cfl * phi[i1] + ffl * phi[j1] + vfl;
   // explicit diffusion: -Gamma_b (grad u_comp)_b . S
-  const double gb = visc ? visc[i1] : mu; // BeDiffCoefficient of the face (owner's viscosity)
+  const double gb = bvisc ? bvisc[b] : mu; // the face's BeDiffCoefficient
   const double sx = Sx[f], sy = Sy[f], sz = Sz[f];
   const double v2 = -(gb * (bgu[(size_t)comp * Fb + b] * sx + bgv[(size_t)comp * Fb + b] * sy +
                             bgw[(size_t)comp * Fb + b] * sz));
@@ -147,6 +147,7 @@ __global__ void k_per_momentum(PerArgs a, int comp, double mu, double c23, const
 struct RcPer {
   double rho, urf_u;
   const double *u, *v, *w, *us, *vs, *ws, *du, *dv, *dw, *p, *bgp, *Sx, *Sy, *Sz;
+  const double *rhok, *bgrk; // 2/3 rho k and its boundary gradient, or nullptr
 };
 __global__ void k_per_rhie_chow(PerArgs a, int dec, RcPer r, double *__restrict__ bmdot) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -177,6 +178,15 @@ __global__ void k_per_rhie_chow(PerArgs a, int dec, RcPer r, double *__restrict_
   const double cfl = rhof * geo, ffl = -(rhof * geo);
   fv = fv - rhof * (pfx * dut + pfy * dvt + pfz * dwt);
   fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
+  if (r.rhok) { // the -2/3 rho k gradient's difference (:1222-1242)
+    const double kbx = r.bgrk[b], kby = r.bgrk[Fb + b], kbz = r.bgrk[2 * Fb + b];
+    double kc = (r.rhok[j1] - r.rhok[i1]) / cf;
+    kc = kc - (kbx * ex + kby * ey + kbz * ez);
+    const double kfx = kbx + kc * ex, kfy = kby + kc * ey, kfz = kbz + kc * ez;
+    fv = fv - rhof * (kfx * dut + kfy * dvt + kfz * dwt);
+    fv = fv + rhof * (kbx * dusf + kby * dvsf + kbz * dwsf);
+    fv = fv + cfl * r.rhok[i1] + ffl * r.rhok[j1];
+  }
   const double us1 = rot(0, r.us[j1], r.vs[j1], r.ws[j1]), vs1 = rot(1, r.us[j1], r.vs[j1], r.ws[j1]),
                ws1 = rot(2, r.us[j1], r.vs[j1], r.ws[j1]);
   const double usf = gf * r.us[i1] + (1. - gf) * us1, vsf = gf * r.vs[i1] + (1. - gf) * vs1,
@@ -407,20 +417,20 @@ void periodic_values(const PerArgs &a, hipStream_t s, int nf, const double *cons
 }
 
 void periodic_momentum(const PerArgs &a, hipStream_t s, int comp, double mu, double c23, const double *visc,
-                       const double *phi, const double *u, const double *v, const double *w,
+                       const double *bvisc, const double *phi, const double *u, const double *v, const double *w,
                        const double *bgu, const double *bgv, const double *bgw, const double *Sx,
                        const double *Sy, const double *Sz, const double *bmdot, double *cf, double *ff,
                        double *tf) {
-  hipLaunchKernelGGL(k_per_momentum, dim3(blocks_of(a.np)), dim3(kBlock), 0, s, a, comp, mu, c23, visc, phi, u, v,
-                     w, bgu, bgv, bgw, Sx, Sy, Sz, bmdot, cf, ff, tf);
+  hipLaunchKernelGGL(k_per_momentum, dim3(blocks_of(a.np)), dim3(kBlock), 0, s, a, comp, mu, c23, visc, bvisc, phi,
+                     u, v, w, bgu, bgv, bgw, Sx, Sy, Sz, bmdot, cf, ff, tf);
 }
 
 void periodic_rhie_chow(const PerArgs &a, hipStream_t s, int dec, double rho, double urf_u, const double *u,
                         const double *v, const double *w, const double *us, const double *vs, const double *ws,
                         const double *du, const double *dv, const double *dw, const double *p,
                         const double *bgp, const double *Sx, const double *Sy, const double *Sz,
-                        double *bmdot) {
-  RcPer r{rho, urf_u, u, v, w, us, vs, ws, du, dv, dw, p, bgp, Sx, Sy, Sz};
+                        const double *rhok, const double *bgrk, double *bmdot) {
+  RcPer r{rho, urf_u, u, v, w, us, vs, ws, du, dv, dw, p, bgp, Sx, Sy, Sz, rhok, bgrk};
   hipLaunchKernelGGL(k_per_rhie_chow, dim3(blocks_of(a.na)), dim3(kBlock), 0, s, a, dec, r, bmdot);
 }
 

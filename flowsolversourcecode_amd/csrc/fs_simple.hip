@@ -358,6 +358,7 @@ __global__ void k_limit(int N, int W, int F, const int *__restrict__ code, const
 struct MomArgs {
   int comp;
   double mu, c23;
+  const double *bgam;                      // BeDiffCoefficient per boundary face, nullptr = mu
   const double *phi, *u, *v, *w;          // current velocity components
   const double *bu, *bv, *bw;              // boundary velocities
   const int *btype;                        // per boundary face, nullptr = all walls
@@ -413,7 +414,8 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
                             const double *__restrict__ g, const double *__restrict__ gdiff,
                             const double *__restrict__ Tx, const double *__restrict__ Ty,
                             const double *__restrict__ Tz, const double *__restrict__ Sx,
-                            const double *__restrict__ Sy, const double *__restrict__ Sz, double mu, double c23,
+                            const double *__restrict__ Sy, const double *__restrict__ Sz, double mu,
+                            const double *__restrict__ gam, double c23,
                             const double *__restrict__ u, const double *__restrict__ v,
                             const double *__restrict__ w, const double *__restrict__ gu,
                             const double *__restrict__ gv, const double *__restrict__ gw,
@@ -457,7 +459,8 @@ __global__ void k_mom_face3(int Fi, int F, int N, const int *__restrict__ own, c
     for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int d = 0; d < 3; ++d) fg_out[(size_t)(3 * k + d) * Fi + f] = fgv[k][d];
-  const double gamf = gf * mu + (1. - gf) * mu;
+  // GamaFace: the viscosity, or the eddy-viscosity model's eDiffCoefficient
+  const double gamf = gam ? gf * gam[i] + (1. - gf) * gam[j] : gf * mu + (1. - gf) * mu;
   const double gd = ld_mat<kStreamFaces>(&gdiff[f]);
   const double cfl = gamf * gd, ffl = -(gamf * gd);
   const double md = ld_mat<kStreamFaces>(&mdot[f]);
@@ -522,7 +525,7 @@ __global__ void k_mom_wall(int Fb, int Fi, int N, const int *__restrict__ own,
   if (b >= Fb) return;
   if (a.btype && a.btype[b] == FS_BC_PERIODIC) return; // k_per_momentum's faces
   const int f = Fi + b, i1 = own[f], c = a.comp;
-  const double gamf = a.mu;
+  const double gamf = a.bgam ? a.bgam[b] : a.mu;
   const double x = nx[f], y = ny[f], z = nz[f], area = A[f];
   const double dn = dx[f] * x + dy[f] * y + dz[f] * z;
   const double bphi = c == 0 ? a.bu[b] : c == 1 ? a.bv[b] : a.bw[b];
@@ -612,7 +615,7 @@ __global__ void k_mom_cell(int N, int W, int Fi, const int *__restrict__ code,
                            const double *__restrict__ vol, double urf, const int *__restrict__ perm,
                            double *__restrict__ ac, double *__restrict__ anb, double *__restrict__ bc,
                            double *__restrict__ D, double *__restrict__ star, double *__restrict__ xq,
-                           double *__restrict__ D2, double *__restrict__ eff_out) {
+                           double *__restrict__ D2, double *__restrict__ eff_out, const double *__restrict__ grk) {
   const int c = xcd_block() * kBlock + threadIdx.x;
   if (c >= N) return;
   // the system row of cell c: its colour-major position when perm is given
@@ -653,6 +656,7 @@ __global__ void k_mom_cell(int N, int W, int Fi, const int *__restrict__ code,
   double fte = 0.0 + ((ep - e) * pc + (-(ep * pc)));
   if constexpr (PER) fte = fte - lin[0] * vol[c]; // AssembleLinearPressureTerm
   fte = fte + gpc[c] * vol[c];
+  if (grk) fte = fte + grk[c] * vol[c]; // MinusTwoThirdsRhoK
   a = a + fce;
   b = b - fte;
   a = a / urf;
@@ -681,6 +685,7 @@ struct RcArgs {
   const double *du2, *dv2, *dw2; // SIMPLEC's D2 (the p' coefficients)
   int gint;                      // GradientInterpolationSchemeContinuity
   double *fgp_out;               // the pressure face gradient of the restart file, or nullptr
+  const double *rhok, *grk;      // 2/3 rho k and its cell gradient (the eddy-viscosity model), or nullptr
 };
 
 // geoDiff = |D_E| / d_CF of the face's D S split by MethodDecomposeS DEC
@@ -761,6 +766,19 @@ __global__ void k_rhie_chow(int Fi, int N, const int *__restrict__ own,
   }
   fv = fv - rhof * (fgx * dut + fgy * dvt + fgz * dwt);
   fv = fv + rhof * (pbx * dusf + pby * dvsf + pbz * dwsf);
+  if (a.rhok) {
+    // the -2/3 rho k gradient's difference (AssembleMdotUsingRhieChow.for
+    // :627-635), its face gradient 'average' and 'averagecorrected'
+    // (:294-306)
+    const double kbx = gf * a.grk[i] + og * a.grk[j], kby = gf * a.grk[N + i] + og * a.grk[N + j],
+                 kbz = gf * a.grk[2 * N + i] + og * a.grk[2 * N + j];
+    double kc = (a.rhok[j] - a.rhok[i]) / dcf[f];
+    kc = kc - (kbx * ex[f] + kby * ey[f] + kbz * ez[f]);
+    const double kfx = kbx + kc * ex[f], kfy = kby + kc * ey[f], kfz = kbz + kc * ez[f];
+    fv = fv - rhof * (kfx * dut + kfy * dvt + kfz * dwt);
+    fv = fv + rhof * (kbx * dusf + kby * dvsf + kbz * dwsf);
+    fv = fv + cfl * a.rhok[i] + ffl * a.rhok[j];
+  }
   if (a.fgp_out) {
     a.fgp_out[f] = fgx;
     a.fgp_out[(size_t)Fi + f] = fgy;
@@ -976,6 +994,30 @@ struct SimpleCase {
   DevBuf<int> pk;
   bool periodic() const { return per.np > 0; }
   PerArgs pargs() const { return periodic_args(per, N, Fi, Fb, own.p); }
+  // the k-omega SST model (fs_case_set_sst): its settings, fields and the
+  // iteration counts of the last k and omega solves
+  struct Turb {
+    bool on = false;
+    int ksolver = FS_ILU, kitmax = 20, wsolver = FS_ILU, witmax = 20, kfw = FS_HR_NONE, kscheme = FS_CONV_UPWIND,
+        wfw = FS_HR_NONE, wscheme = FS_CONV_UPWIND, kngrad = 2, wngrad = 2, klim = 0, wlim = 0;
+    double urf_k = 0.5, urf_w = 0.5, urf_mut = 0.5, krrf = 0.1, wrrf = 0.1, max_ratio = 1.0e6;
+    DevBuf<double> k, om, mut, wd, bk, bom, bmut, gk, gom, bgk, bgom, rhok, brhok, grk, bgrk, F1, F2, bF1, bF2,
+        prod, sr, bsr, med, bmed;
+    int it_k = 0, it_w = 0;
+  } tb;
+  SstArgs sst() const {
+    SstArgs a{};
+    a.N = N; a.Fi = Fi; a.Fb = Fb;
+    a.own = own.p; a.nei = nei.p; a.btype = btype.p; a.pk = pk.p;
+    a.rho = rho; a.mu = mu; a.urf_mut = tb.urf_mut; a.max_ratio = tb.max_ratio;
+    a.gu = gu.p; a.gv = gv.p; a.gw = gw.p; a.bgu = bgu.p; a.bgv = bgv.p; a.bgw = bgw.p;
+    a.mdot = mdot.p; a.bmdot = bmdot.p;
+    a.k = tb.k.p; a.om = tb.om.p; a.mut = tb.mut.p; a.wd = tb.wd.p; a.bk = tb.bk.p; a.bom = tb.bom.p;
+    a.bmut = tb.bmut.p; a.gk = tb.gk.p; a.gom = tb.gom.p; a.bgk = tb.bgk.p; a.bgom = tb.bgom.p;
+    a.rhok = tb.rhok.p; a.brhok = tb.brhok.p; a.F1 = tb.F1.p; a.F2 = tb.F2.p; a.bF1 = tb.bF1.p; a.bF2 = tb.bF2.p;
+    a.prod = tb.prod.p; a.sr = tb.sr.p; a.bsr = tb.bsr.p; a.med = tb.med.p; a.bmed = tb.bmed.p;
+    return a;
+  }
 };
 
 static void upload_geometry(SimpleCase &C) {
@@ -1023,6 +1065,19 @@ template <int NF> static void gradients(SimpleCase &C, GradArgs a, int ncorr) {
   }
 }
 
+// The face and cell centroids the limiters read, uploaded once.
+static void upload_centroids(SimpleCase &C) {
+  if (C.xc.p) return;
+  const auto &F = C.geom.face;
+  C.fcx.upload(F.cx, C.s);
+  C.fcy.upload(F.cy, C.s);
+  C.fcz.upload(F.cz, C.s);
+  C.xc.upload(C.geom.xc, C.s);
+  C.yc.upload(C.geom.yc, C.s);
+  C.zc.upload(C.geom.zc, C.s);
+  FS_HIP_THROW(hipStreamSynchronize(C.s));
+}
+
 // The limiters of the fields of the gradient launch just made (its boundary
 // face values are still in fp), methods[k] per field.
 template <int NF> static void limit_gradients(SimpleCase &C, const GradArgs &g, const int *methods) {
@@ -1036,6 +1091,7 @@ template <int NF> static void limit_gradients(SimpleCase &C, const GradArgs &g, 
     range = range || methods[k] == 2;
   }
   if (!any) return;
+  if (!C.xc.p || !C.fcx.p) throw std::runtime_error("gradient limiter without the centroids uploaded");
   if (range) {
     const int nb = std::min(256, blocks(C.N + C.Fb));
     C.lim_part.alloc((size_t)2 * NF * nb);
@@ -1080,6 +1136,25 @@ static void phase_momentum_gradients(SimpleCase &C) {
   }
   hipLaunchKernelGGL(k_wall_pgrad, dim3(blocks(C.Fb)), dim3(kBlock), 0, C.s, C.Fb, C.Fi, C.N,
                      C.own.p, C.nx.p, C.ny.p, C.nz.p, C.btype.p, C.bzq.p, C.gp.p, C.bgp.p);
+  if (C.tb.on) {
+    // CalculateEffectiveDiffusionCoefficient of the velocities (Viscosity +
+    // TurbulentViscosity) and MinusTwoThirdsRhoK('velx'): 2/3 rho k and its
+    // gradient with the momentum's gradient settings (its wall and
+    // symmetry projections feed nothing here)
+    const SstArgs sa = C.sst();
+    sst_ediff(sa, C.s, 2);
+    sst_rhok(sa, C.s);
+    GradArgs r{};
+    r.phi[0] = C.tb.rhok.p; r.bval[0] = C.tb.brhok.p; r.grad[0] = C.tb.grk.p;
+    gradients<1>(C, r, C.ngrad_mom);
+    const int mr[1] = {C.lim_mom};
+    limit_gradients<1>(C, r, mr);
+    if (C.periodic()) {
+      const double *cg[1] = {C.tb.grk.p};
+      double *bg[1] = {C.tb.bgrk.p};
+      periodic_bgrad(C.pargs(), C.s, 1, cg, bg);
+    }
+  }
   // the interior momentum face fluxes of all three components, with the
   // high-resolution correction from iteration nIterStartApplyingHR on
   // (AssembleConvection.for:156-165)
@@ -1088,7 +1163,7 @@ static void phase_momentum_gradients(SimpleCase &C) {
   auto face3 = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.F, C.N, C.own.p, C.nei.p,
                        C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p, C.Sx.p, C.Sy.p, C.Sz.p, C.mu,
-                       (double)(-2.0f / 3.0f), C.u.p, C.v.p, C.w.p, C.gu.p, C.gv.p, C.gw.p, C.mdot.p, C.dx.p,
+                       C.tb.on ? (const double *)C.tb.med.p : nullptr, (double)(-2.0f / 3.0f), C.u.p, C.v.p, C.w.p, C.gu.p, C.gv.p, C.gw.p, C.mdot.p, C.dx.p,
                        C.dy.p, C.dz.p, C.hr_scheme, C.bleed, C.gi_mom, C.rho, C.ex.p, C.ey.p, C.ez.p, C.d.p,
                        C.cf.p, C.ff.p, C.tf3.p, C.capture ? C.cap_fgm.p : nullptr);
   };
@@ -1110,6 +1185,7 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
   a.comp = comp;
   a.mu = C.mu;
   a.c23 = (double)(-2.0f / 3.0f); // REAL(4) literal -2./3. (AssembleBulkViscosity.for:49)
+  a.bgam = C.tb.on ? C.tb.bmed.p : nullptr;
   a.phi = comp == 0 ? C.u.p : comp == 1 ? C.v.p : C.w.p;
   a.u = C.u.p; a.v = C.v.p; a.w = C.w.p;
   a.bu = C.bu.p; a.bv = C.bv.p; a.bw = C.bw.p;
@@ -1124,8 +1200,9 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
                      C.cf.p, C.ff.p, tf);
   const bool per = C.periodic();
   if (per)
-    periodic_momentum(C.pargs(), C.s, comp, C.mu, a.c23, nullptr, a.phi, C.u.p, C.v.p, C.w.p, C.bgu.p, C.bgv.p,
-                      C.bgw.p, C.Sx.p, C.Sy.p, C.Sz.p, C.bmdot.p, C.cf.p, C.ff.p, tf);
+    periodic_momentum(C.pargs(), C.s, comp, C.mu, a.c23, C.tb.on ? C.tb.med.p : nullptr,
+                      C.tb.on ? C.tb.bmed.p : nullptr, a.phi, C.u.p, C.v.p, C.w.p, C.bgu.p, C.bgv.p, C.bgw.p,
+                      C.Sx.p, C.Sy.p, C.Sz.p, C.bmdot.p, C.cf.p, C.ff.p, tf);
   double *D = comp == 0 ? C.du.p : comp == 1 ? C.dv.p : C.dw.p;
   double *st = comp == 0 ? C.us.p : comp == 1 ? C.vs.p : C.ws.p;
   const double *lin = per ? C.per.scal.p + PSCAL_LIN + comp : nullptr;
@@ -1135,7 +1212,8 @@ static void phase_momentum_assemble(SimpleCase &C, int comp) {
                          C.ff.p, tf, C.mdot.p, C.bmdot.p, lin, a.phi, C.gp.p + (size_t)comp * C.N, C.vol.p,
                          C.urf_u, perm ? C.mperm.p : nullptr, C.ac.p, C.anb.p, C.bc.p, D, st, C.mx.p,
                          C.simplec ? (comp == 0 ? C.du2.p : comp == 1 ? C.dv2.p : C.dw2.p) : nullptr,
-                         C.capture ? C.cap_eff.p : nullptr);
+                         C.capture ? C.cap_eff.p : nullptr,
+                         C.tb.on ? (const double *)C.tb.grk.p + (size_t)comp * C.N : nullptr);
     };
     if (per) launch(k_mom_cell<decltype(wc)::value, true>);
     else launch(k_mom_cell<decltype(wc)::value, false>);
@@ -1210,6 +1288,8 @@ static void phase_pc_assemble(SimpleCase &C) {
   a.du2 = C.du2.p; a.dv2 = C.dv2.p; a.dw2 = C.dw2.p;
   a.gint = C.gi_cont;
   a.fgp_out = C.capture ? C.cap_fgp.p : nullptr;
+  a.rhok = C.tb.on ? C.tb.rhok.p : nullptr;
+  a.grk = C.tb.on ? C.tb.grk.p : nullptr;
   auto rc = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(blocks(C.Fi)), dim3(kBlock), 0, C.s, C.Fi, C.N, C.own.p, C.nei.p, C.g.p,
                        C.Sx.p, C.Sy.p, C.Sz.p, C.ex.p, C.ey.p, C.ez.p, C.d.p, a, C.mdot.p, C.cf.p, C.ff.p,
@@ -1227,7 +1307,8 @@ static void phase_pc_assemble(SimpleCase &C) {
   if (per) {
     const PerArgs pa = C.pargs();
     periodic_rhie_chow(pa, C.s, C.decomp, C.rho, C.urf_u, C.u.p, C.v.p, C.w.p, C.us.p, C.vs.p, C.ws.p, C.du.p,
-                       C.dv.p, C.dw.p, C.p.p, C.bgp.p, C.Sx.p, C.Sy.p, C.Sz.p, C.bmdot.p);
+                       C.dv.p, C.dw.p, C.p.p, C.bgp.p, C.Sx.p, C.Sy.p, C.Sz.p,
+                       C.tb.on ? C.tb.rhok.p : nullptr, C.tb.on ? C.tb.bgrk.p : nullptr, C.bmdot.p);
     periodic_pc_boundary(pa, C.s, C.decomp, C.rho, C.pk.p, C.simplec ? C.du2.p : C.du.p,
                          C.simplec ? C.dv2.p : C.dv.p, C.simplec ? C.dw2.p : C.dw.p, C.Sx.p, C.Sy.p, C.Sz.p,
                          C.bmdot.p, C.cf.p, C.ff.p, C.tf.p);
@@ -1316,6 +1397,114 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
                     C.w.p, C.bu.p, C.bv.p, C.bw.p, C.bmdot.p, C.mdot.p);
   }
   update_symmetry(C); // SolveContinuityEquation.for:96
+  return 0;
+}
+
+// One k or omega equation of the SST model (SolveTKE.for, SolveTOmega.for):
+// gradient (Green-Gauss, limiter, periodic correction), effective diffusion
+// coefficient, sources (production, or the blending functions), face fluxes,
+// the cell gather, SolveEquation, bounding and boundary values.
+static int phase_sst_equation(SimpleCase &C, int var, fs_simple_stats *st) {
+  auto &T = C.tb;
+  const SstArgs a = C.sst();
+  const bool per = C.periodic();
+  const PerArgs pa = C.pargs();
+  GradArgs g{};
+  g.phi[0] = var ? T.om.p : T.k.p;
+  g.bval[0] = var ? T.bom.p : T.bk.p;
+  g.grad[0] = var ? T.gom.p : T.gk.p;
+  gradients<1>(C, g, var ? T.wngrad : T.kngrad);
+  const int m[1] = {var ? T.wlim : T.klim};
+  limit_gradients<1>(C, g, m);
+  if (per) {
+    const double *cg[1] = {g.grad[0]};
+    double *bg[1] = {var ? T.bgom.p : T.bgk.p};
+    periodic_bgrad(pa, C.s, 1, cg, bg);
+  }
+  sst_ediff(a, C.s, var);
+  if (var == 0) sst_production(a, C.s);
+  else sst_blending(a, C.s);
+  const int fw = C.iter >= C.hr_start ? (var ? T.wfw : T.kfw) : FS_HR_NONE;
+  sst_faces(a, per ? &pa : nullptr, C.s, var, fw, var ? T.wscheme : T.kscheme, C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p,
+            C.Tz.p, C.dx.p, C.dy.p, C.dz.p, C.cf.p, C.ff.p, C.tf.p);
+  C.sys_perm = false;
+  sst_cells(a, per, C.s, var, C.W, C.code.p, C.cf.p, C.ff.p, C.tf.p, C.vol.p, var ? T.urf_w : T.urf_k, C.ac.p,
+            C.anb.p, C.bc.p);
+  fs_solve_stats ss{};
+  double res4[4];
+  const int rc = solve_device(C.mesh, var ? 10 : 8, var ? T.om.p : T.k.p, var ? T.wrrf : T.krrf,
+                              var ? T.witmax : T.kitmax, var ? T.wsolver : T.ksolver, 0, C.ac.p, C.anb.p, C.bc.p,
+                              C.dphi.p, res4, &ss);
+  if (rc < 0) return rc;
+  (var ? T.it_w : T.it_k) = ss.iterations;
+  if (st) st->turbulence_iterations += ss.iterations;
+  sst_bound(a, pa, C.s, var);
+  return 0;
+}
+
+// CalculateNormalDistanceToWall, method 2 (CalculateDistanceToWall.for
+// :154-470): passes of assembly (Green-Gauss gradient, method 2 with two
+// sweeps, 'average' face gradients) and solve until a pass starts from a
+// residual |bc|_1 at most tol times the first pass's, then the distance from
+// the last pass's gradient, the cells at the walls their faces' normal
+// distance.  The reference stops on the inner solve's final residual (1e-10)
+// instead; the fixed point is the same.  The solves are CG preconditioned by
+// the AMG V-cycle; the hierarchy is dropped afterwards so that the p' solves
+// aggregate their own operator.
+static int wall_distance(SimpleCase &C, double tol, int maxouter, int *passes) {
+  const int N = C.N, Fb = C.Fb;
+  auto &T = C.tb;
+  std::vector<int> bt(Fb, FS_BC_WALL);
+  if (C.btype.p) FS_HIP(hipMemcpy(bt.data(), C.btype.p, sizeof(int) * Fb, hipMemcpyDeviceToHost));
+  std::vector<double> hb(Fb);
+  for (int b = 0; b < Fb; ++b) hb[b] = bt[b] == FS_BC_WALL ? 0. : 1.;
+  DevBuf<double> phi, bphi, gr;
+  phi.upload(std::vector<double>(N, 1.0), C.s);
+  bphi.upload(hb, C.s);
+  gr.alloc(3 * (size_t)N);
+  T.wd.alloc(N);
+  double first = -1.0;
+  int pass = 0;
+  bool done = false;
+  while (pass < maxouter && !done) {
+    GradArgs g{};
+    g.phi[0] = phi.p; g.bval[0] = bphi.p; g.grad[0] = gr.p;
+    gradients<1>(C, g, 2);
+    wall_distance_faces(C.s, C.Fi, Fb, N, C.own.p, C.nei.p, C.btype.p, C.g.p, C.gdiff.p, C.Tx.p, C.Ty.p, C.Tz.p,
+                        phi.p, gr.p, bphi.p, C.cf.p, C.ff.p, C.tf.p);
+    wall_distance_cells(C.s, N, C.W, C.code.p, C.cf.p, C.ff.p, C.tf.p, C.vol.p, phi.p, C.ac.p, C.anb.p, C.bc.p);
+    C.sys_perm = false;
+    fs_solve_stats ss{};
+    double res4[4];
+    const int rc = solve_device(C.mesh, 5, phi.p, 1e-8, 500, FS_CG, FS_MG_PRECOND, C.ac.p, C.anb.p, C.bc.p,
+                                C.dphi.p, res4, &ss);
+    if (rc < 0) return rc;
+    if (first < 0.0) first = ss.initial_l1;
+    done = !(ss.initial_l1 > tol * first);
+    ++pass;
+  }
+  wall_distance_final(C.s, N, C.W, C.Fi, C.code.p, C.btype.p, phi.p, gr.p, C.dx.p, C.dy.p, C.dz.p, C.nx.p, C.ny.p,
+                      C.nz.p, T.wd.p);
+  FS_HIP(hipStreamSynchronize(C.s));
+  C.mesh->amg.reset();
+  if (passes) *passes = pass;
+  if (!done) {
+    set_error("fs_case_wall_distance: the passes did not reach the tolerance");
+    return 1;
+  }
+  return 0;
+}
+
+// CalculateEffectsOfTurbulence for the SST model
+// (SolveSelectedTurbulenceModelEquations.for:4-13,150-160): strain rate, the
+// k and omega equations, the eddy viscosity.
+static int phase_turbulence(SimpleCase &C, fs_simple_stats *st) {
+  sst_strain(C.sst(), C.s);
+  int rc = phase_sst_equation(C, 0, st);
+  if (rc < 0) return rc;
+  rc = phase_sst_equation(C, 1, st);
+  if (rc < 0) return rc;
+  sst_viscosity(C.sst(), C.s);
   return 0;
 }
 
@@ -1470,8 +1659,12 @@ int simple_iterate(SimpleCase &C, int niter, fs_simple_stats *st) {
       if (rc < 0) return rc;
     }
     phase_pc_assemble(C);
-    const int rc = phase_pc_solve_correct(C, &loc);
+    int rc = phase_pc_solve_correct(C, &loc);
     if (rc < 0) return rc;
+    if (C.tb.on) {
+      rc = phase_turbulence(C, &loc);
+      if (rc < 0) return rc;
+    }
   }
   FS_HIP(hipStreamSynchronize(C.s));
   loc.iterations = niter;
@@ -1656,6 +1849,15 @@ int fs_case_phase(fs_case *cs, int phase) {
   case FS_PHASE_PC_ASSEMBLE: fs::phase_pc_assemble(C); break;
   case FS_PHASE_PC_SOLVE_CORRECT: { int rc = fs::phase_pc_solve_correct(C, nullptr); if (rc < 0) return rc; break; }
   case FS_PHASE_UPDATE_SYMMETRY: fs::update_symmetry(C); break;
+  case FS_PHASE_TURBULENCE: {
+    if (!C.tb.on) {
+      set_error("fs_case_phase: the case has no turbulence model");
+      return -1;
+    }
+    int rc = fs::phase_turbulence(C, nullptr);
+    if (rc < 0) return rc;
+    break;
+  }
   default:
     set_error("fs_case_phase: unknown phase");
     return -1;
@@ -1681,7 +1883,12 @@ static bool case_field(fs::SimpleCase &C, const char *name, double **p, size_t *
       {"Sy", {&C.Sy, F}}, {"Sz", {&C.Sz, F}}, {"g", {&C.g, F}}, {"gdiff", {&C.gdiff, F}},
       {"Tx", {&C.Tx, F}}, {"Ty", {&C.Ty, F}}, {"Tz", {&C.Tz, F}}, {"dcf", {&C.d, F}},
       {"A", {&C.A, F}}, {"kx", {&C.kx, Fi}}, {"ky", {&C.ky, Fi}}, {"kz", {&C.kz, Fi}},
-      {"bmdot", {&C.bmdot, C.periodic() ? Fb : 0}}, {"beta", {&C.per.scal, C.periodic() ? 1 : 0}}};
+      {"bmdot", {&C.bmdot, C.periodic() ? Fb : 0}}, {"beta", {&C.per.scal, C.periodic() ? 1 : 0}},
+      {"k", {&C.tb.k, C.tb.on ? N : 0}}, {"omega", {&C.tb.om, C.tb.on ? N : 0}},
+      {"mut", {&C.tb.mut, C.tb.on ? N : 0}}, {"wd", {&C.tb.wd, C.tb.on ? N : 0}},
+      {"bk", {&C.tb.bk, C.tb.on ? Fb : 0}}, {"bomega", {&C.tb.bom, C.tb.on ? Fb : 0}},
+      {"bmut", {&C.tb.bmut, C.tb.on ? Fb : 0}}, {"F1", {&C.tb.F1, C.tb.on ? N : 0}},
+      {"F2", {&C.tb.F2, C.tb.on ? N : 0}}, {"grk", {&C.tb.grk, C.tb.on ? 3 * N : 0}}};
   auto it = tab.find(name);
   if (it == tab.end()) return false;
   *p = it->second.first->p;
@@ -2136,16 +2343,7 @@ int fs_case_set_gradient_limiters(fs_case *cs, int momentum_method, int continui
   }
   CASE_GUARD_BEGIN
   auto &C = cs->c;
-  if ((momentum_method || continuity_method) && !C.xc.p) {
-    const auto &F = C.geom.face;
-    C.fcx.upload(F.cx, C.s);
-    C.fcy.upload(F.cy, C.s);
-    C.fcz.upload(F.cz, C.s);
-    C.xc.upload(C.geom.xc, C.s);
-    C.yc.upload(C.geom.yc, C.s);
-    C.zc.upload(C.geom.zc, C.s);
-    FS_HIP(hipStreamSynchronize(C.s));
-  }
+  if (momentum_method || continuity_method) fs::upload_centroids(C);
   C.lim_mom = momentum_method;
   C.lim_cont = continuity_method;
   return 0;
@@ -2349,6 +2547,94 @@ int fs_case_set_periodic(fs_case *cs, int npatches, const int *patch_nfaces, con
   C.bgu.upload(zb, C.s);
   C.bgv.upload(zb, C.s);
   C.bgw.upload(zb, C.s);
+  FS_HIP(hipStreamSynchronize(C.s));
+  return 0;
+  CASE_GUARD_END
+}
+
+int fs_case_wall_distance(fs_case *cs, double tol, int maxouter, double *wd, int *passes) {
+  if (!cs || !(tol > 0.0) || maxouter < 1) {
+    set_error("fs_case_wall_distance: NULL case, non-positive tolerance or no passes");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  int n = 0;
+  const int rc = fs::wall_distance(C, tol, maxouter, &n);
+  if (passes) *passes = n;
+  if (rc < 0) return rc;
+  if (wd) {
+    FS_HIP(hipMemcpyAsync(wd, C.tb.wd.p, sizeof(double) * C.N, hipMemcpyDeviceToHost, C.s));
+    FS_HIP(hipStreamSynchronize(C.s));
+  }
+  return rc;
+  CASE_GUARD_END
+}
+
+int fs_case_set_sst(fs_case *cs, const int *io, const double *d, const double *k, const double *omega,
+                    const double *mut, const double *wd, const double *bk, const double *bomega,
+                    const double *bmut) {
+  if (!cs || !io || !d || !k || !omega || !mut || !bk || !bomega || !bmut) {
+    set_error("fs_case_set_sst: NULL argument");
+    return -1;
+  }
+  if (!wd && cs->c.tb.wd.n != (size_t)cs->c.N) {
+    set_error("fs_case_set_sst: no wall distance given and none computed (fs_case_wall_distance)");
+    return -1;
+  }
+  for (int q = 0; q < 2; ++q) {
+    const int sv = io[2 * q], fw = io[4 + 2 * q], sc = io[5 + 2 * q];
+    if (sv < FS_SOR || sv > FS_CG || sv == FS_DIRECT) {
+      set_error("fs_case_set_sst: k and omega take SOR, ILU, PBCG, GMRES, BICGS or CG");
+      return -3;
+    }
+    if (io[1 + 2 * q] < 0 || fw < FS_HR_NONE || fw > FS_HR_NVF || sc < FS_CONV_UPWIND || sc > FS_CONV_GAMMA ||
+        io[8 + 2 * q] < 1 || io[8 + 2 * q] > 2 || io[9 + 2 * q] < 0 || io[12 + q] < 0 || io[12 + q] > 2) {
+      set_error("fs_case_set_sst: invalid option");
+      return -1;
+    }
+  }
+  if (!(d[0] > 0.0 && d[0] <= 1.0) || !(d[1] > 0.0 && d[1] <= 1.0) || !(d[2] > 0.0 && d[2] <= 1.0) ||
+      !(d[5] > 0.0)) {
+    set_error("fs_case_set_sst: relaxation factors must lie in (0, 1] and the viscosity cap be positive");
+    return -1;
+  }
+  CASE_GUARD_BEGIN
+  auto &C = cs->c;
+  auto &T = C.tb;
+  const size_t N = C.N, Fb = C.Fb, Fi = C.Fi;
+  for (size_t i = 0; i < N; ++i)
+    if ((wd && !(wd[i] > 0.0)) || !std::isfinite(k[i]) || !std::isfinite(omega[i]) || !std::isfinite(mut[i])) {
+      set_error("fs_case_set_sst: the wall distance must be positive and k, omega, mu_t finite");
+      return -1;
+    }
+  T.ksolver = io[0]; T.kitmax = io[1]; T.wsolver = io[2]; T.witmax = io[3];
+  T.kfw = io[4]; T.kscheme = io[5]; T.wfw = io[6]; T.wscheme = io[7];
+  T.kngrad = io[8] == 1 ? 0 : io[9];
+  T.wngrad = io[10] == 1 ? 0 : io[11];
+  T.klim = io[12]; T.wlim = io[13];
+  if (T.klim || T.wlim) fs::upload_centroids(C);
+  T.urf_k = d[0]; T.urf_w = d[1]; T.urf_mut = d[2]; T.krrf = d[3]; T.wrrf = d[4]; T.max_ratio = d[5];
+  T.k.upload(k, N, C.s);
+  T.om.upload(omega, N, C.s);
+  T.mut.upload(mut, N, C.s);
+  if (wd) T.wd.upload(wd, N, C.s);
+  T.bk.upload(bk, Fb, C.s);
+  T.bom.upload(bomega, Fb, C.s);
+  T.bmut.upload(bmut, Fb, C.s);
+  // F1 and F2 start at 0 (AllocateStorageRoutine.for:2618-2631)
+  const std::vector<double> zn(3 * N, 0.0), zb(3 * Fb, 0.0);
+  T.F1.upload(zn.data(), N, C.s);
+  T.F2.upload(zn.data(), N, C.s);
+  T.bF1.upload(zb.data(), Fb, C.s);
+  T.bF2.upload(zb.data(), Fb, C.s);
+  for (auto *b : {&T.gk, &T.gom, &T.grk}) b->upload(zn, C.s);
+  for (auto *b : {&T.bgk, &T.bgom, &T.bgrk}) b->upload(zb, C.s);
+  for (auto *b : {&T.rhok, &T.prod, &T.sr, &T.med}) b->upload(zn.data(), N, C.s);
+  for (auto *b : {&T.brhok, &T.bsr, &T.bmed}) b->upload(zb.data(), Fb, C.s);
+  if (C.bmdot.n != Fb) C.bmdot.upload(zb.data(), Fb, C.s); // zero off the periodic faces
+  (void)Fi;
+  T.on = true;
   FS_HIP(hipStreamSynchronize(C.s));
   return 0;
   CASE_GUARD_END

@@ -63,6 +63,7 @@ typedef struct fs_simple_stats {
   double pc_final_l1;        /* last p' solve: sum|bc - A p'|                */
   double res_momentum[3];    /* last ResorAbs of u, v, w                     */
   double mass_l1;            /* sum over cells of |net mass outflow| at end  */
+  int turbulence_iterations; /* summed k and omega solver iterations         */
 } fs_simple_stats;
 
 enum fs_phase {
@@ -75,7 +76,8 @@ enum fs_phase {
   FS_PHASE_SOLVE_W = 6,
   FS_PHASE_PC_ASSEMBLE = 7,
   FS_PHASE_PC_SOLVE_CORRECT = 8,
-  FS_PHASE_UPDATE_SYMMETRY = 9
+  FS_PHASE_UPDATE_SYMMETRY = 9,
+  FS_PHASE_TURBULENCE = 10    /* CalculateEffectsOfTurbulence (SST)        */
 };
 
 /* wall_u/v/w: velocity of every boundary face (nfaces - nif entries). */
@@ -127,6 +129,37 @@ int fs_case_set_patch_types(fs_case *c, int npatches, const int *patch_nfaces,
 int fs_case_set_periodic(fs_case *c, int npatches, const int *patch_nfaces, const int *pair,
                          const double *translation, double beta, int calc_beta,
                          double mdot_target, double relax, int iters, int every);
+/* The k-omega SST model (TurbulenceModel 'komegasst', ModelNumber 14;
+ * SolveSelectedTurbulenceModelEquations.for:150-160) with low-Reynolds-number
+ * walls (WallTreatment 'lowreynoldsnumber', WallBCModel 'menter'): every
+ * outer iteration then ends with CalculateEffectsOfTurbulence (strain rate,
+ * the k and omega equations, mu_t), and the momentum equations take
+ * Viscosity + TurbulentViscosity and the 2/3 rho k gradient
+ * (MinusTwoThirdsRhoK, with its difference in the Rhie-Chow flux).
+ * iopt[14]: k solver, k itmax, omega solver, omega itmax (FS_* solver
+ * codes), k HR framework, k scheme, omega framework, omega scheme (as
+ * fs_case_set_convection), k MethodCalcGradient (1 or 2), k nIterGradient,
+ * omega MethodCalcGradient, omega nIterGradient, k limiter method, omega
+ * limiter method (0 off, 1, 2).  dopt[6]: urfTKE, urfTOmega, urfTViscosity,
+ * rrFTKE, rrFTOmega, MaximumViscosityRatio.  Initial fields (host arrays):
+ * k, omega, mu_t and the wall distance of every cell; k, omega, mu_t of every
+ * boundary face.  Fields by name afterwards: k omega mut wd bk bomega bmut F1
+ * F2.  wall_distance may be NULL after fs_case_wall_distance: the computed
+ * distance is used.  Returns -1 on an invalid option, -3 for a solver code
+ * the k and omega solves cannot take. */
+/* The distance of every cell from the no-slip walls of the case's patch
+ * types by the reference's Poisson method (CalculateNormalDistanceToWall,
+ * MethodCalculateNormalDistance 2, CalculateDistanceToWall.for:154-470):
+ * lap(phi) = -1, phi = 0 at the walls, no flux elsewhere, passes of
+ * assembly and CG + AMG solve until a pass starts from at most tol times the
+ * first pass's residual |bc|_1; d = -|grad phi| + sqrt(|grad phi|^2 + 2 phi),
+ * the cells at a wall their face's normal distance.  Kept on the device for
+ * fs_case_set_sst; copied to wd (N, host) unless NULL; passes: the passes
+ * run.  Returns 0, 1 when maxouter passes did not reach tol, < 0 on error. */
+int fs_case_wall_distance(fs_case *c, double tol, int maxouter, double *wd, int *passes);
+int fs_case_set_sst(fs_case *c, const int *iopt, const double *dopt, const double *k,
+                    const double *omega, const double *mut, const double *wall_distance,
+                    const double *bk, const double *bomega, const double *bmut);
 /* High-resolution convection of the momentum equations: HRFrameworkMomentum
  * ('none', 'tvd', 'nvf'), ConvectionSchemeMomentum, BleedMomentum and
  * nIterStartApplyingHR (User0.F90:367-410).  Upwind fluxes stay implicit and

@@ -15,7 +15,7 @@ program ref_solve_driver
   use User0
   use MultiGrid1
   use MultiGrid2
-  use Geometry1, only: NumberOfElements
+  use Geometry1, only: NumberOfElements, MaximumNumberofElementFaces
   use Geometry3, only: ElementNeighbor, NumberofElementNeighbors, NElementFaces
   use Variables2, only: ac, anb, bc, dphi, dc, rc
   use Residuals1
@@ -49,6 +49,8 @@ program ref_solve_driver
   read(21) n, w, mode, isolver, itmax, lmg, icycle, npre, npost, maxcyc, maxlev, minpar, ncalls
   read(21) rrf
   NumberOfElements = n
+  ! SolveAlgebraicSystemInParallel sizes its row lists by it
+  MaximumNumberofElementFaces = w
   NElementFaces = w
   allocate(ElementNeighbor(n, w), NumberofElementNeighbors(n))
   allocate(ac(n), anb(n, w), bc(n), dphi(n), dc(n), rc(n), fit(n), bc0(n))
@@ -75,6 +77,7 @@ program ref_solve_driver
   case (0); solver = 'sor'
   case (1); solver = 'ilu'
   case (2); solver = 'pbcg'
+  case (3); solver = 'direct'
   case (4); solver = 'gmres'
   case (5); solver = 'bicgs'
   case default; solver = 'sor'
@@ -94,6 +97,7 @@ program ref_solve_driver
   LTestMultiGrid = .false.
   LPrintMultiGridResiduals = .false.
   nIterStartApplyingMG = 1
+  iLevelMax = 0
   reAgglomorate = 1000000
   lmultigrid = (lmg .ne. 0)
   variable = 'pressc'
@@ -139,7 +143,9 @@ program ref_solve_driver
     write(22) fit
     write(22) ResorAbs(4), ResorMax(4), ResorRMS(4), ResorScaled(4)
     write(22) hist(1:max(nhist,1))
-    if (mode == 0 .and. lmultigrid) then
+    ! the reference's hierarchy, when its AlgebraicMultigrid built one (a
+    ! drop-in that solved on the device leaves iLevelMax at 0)
+    if (mode == 0 .and. lmultigrid .and. iLevelMax > 0) then
       write(22) iLevelMax
       write(22) NumberOfElementsMG(1:iLevelMax), ijbeginE(1:iLevelMax), ijbeginN(1:iLevelMax), &
                 NElementFacesMG(1:iLevelMax)

@@ -60,6 +60,9 @@ int max_children(int ilevel) { // AgglomorateALevelAlgebraicly.for:79-105
   return 2;
 }
 
+void finish_level(HostLevel &f, HostLevel &c, const std::vector<int> &parents, const std::vector<int> &seeds,
+                  const std::vector<std::vector<int>> &kids, int nc);
+
 void aggregate(HostLevel &f, HostLevel &c, int ilevel) {
   const int nf = f.n, maxch = max_children(ilevel);
   const double ratiomin = 0.5, tiny = 1.0e-15;
@@ -143,7 +146,15 @@ void aggregate(HostLevel &f, HostLevel &c, int ilevel) {
       ++ip;
     }
   }
-  const int nc = ip - 1;
+  finish_level(f, c, parents, seeds, kids, ip - 1);
+}
+
+// The coarse level of an aggregation: parents (1-based) of the fine cells,
+// seed and children (in insertion order) of aggregates 1..nc; the coarse
+// neighbour lists in the order the fine cells and their slots reach them.
+void finish_level(HostLevel &f, HostLevel &c, const std::vector<int> &parents, const std::vector<int> &seeds,
+                  const std::vector<std::vector<int>> &kids, int nc) {
+  const int nf = f.n;
   std::vector<std::vector<int>> cn(nc + 1);
   int wmax = 1;
   for (int e = 1; e <= nf; ++e)
@@ -175,6 +186,121 @@ void aggregate(HostLevel &f, HostLevel &c, int ilevel) {
   }
   c.ac.assign(nc, 0.0);
   c.anb.assign((size_t)nc * wmax, 0.0);
+}
+
+// Off-diagonal couplings of a level as compressed rows (the diagonal is not
+// needed to judge strength).
+struct Couplings {
+  int n = 0;
+  std::vector<int> rp, col;
+  std::vector<double> val;
+};
+
+Couplings couplings_of(const HostLevel &f) {
+  Couplings A;
+  A.n = f.n;
+  A.rp.assign(f.n + 1, 0);
+  for (int i = 0; i < f.n; ++i) {
+    for (int l = 0; l < f.nnbr[i]; ++l) {
+      const int j = f.nbr[(size_t)l * f.n + i];
+      if (j <= 0) continue;
+      A.col.push_back(j - 1);
+      A.val.push_back(f.anb[(size_t)l * f.n + i]);
+    }
+    A.rp[i + 1] = (int)A.col.size();
+  }
+  return A;
+}
+
+// One pass of pairwise matching: each unmatched cell, in order, is paired
+// with the unmatched neighbour of its strongest negative coupling among those
+// at least `beta` times the row's strongest (Notay's pairwise aggregation);
+// a cell without one stays alone.  agg[i] = 0-based pair number.
+int pair_pass(const Couplings &A, double beta, std::vector<int> &agg) {
+  const int n = A.n;
+  agg.assign(n, -1);
+  int na = 0;
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    double mx = 0.0;
+    for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) mx = std::fmax(mx, -A.val[k]);
+    int best = -1;
+    double bv = 0.0;
+    for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+      const int j = A.col[k];
+      const double sv = -A.val[k];
+      if (j == i || agg[j] >= 0 || !(sv > 0.0) || sv < beta * mx) continue;
+      if (sv > bv) {
+        bv = sv;
+        best = j;
+      }
+    }
+    agg[i] = na;
+    if (best >= 0) agg[best] = na;
+    ++na;
+  }
+  return na;
+}
+
+// The couplings between pairs: the fine couplings summed over pair members,
+// those inside a pair dropped.
+Couplings pair_couplings(const Couplings &A, const std::vector<int> &agg, int na) {
+  std::vector<std::vector<int>> mem(na);
+  for (int i = 0; i < A.n; ++i) mem[agg[i]].push_back(i);
+  Couplings B;
+  B.n = na;
+  B.rp.assign(na + 1, 0);
+  std::vector<int> where(na, -1);
+  std::vector<int> touched;
+  for (int p = 0; p < na; ++p) {
+    touched.clear();
+    for (int i : mem[p])
+      for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+        const int q = agg[A.col[k]];
+        if (q == p) continue;
+        if (where[q] < 0) {
+          where[q] = (int)B.col.size();
+          B.col.push_back(q);
+          B.val.push_back(0.0);
+          touched.push_back(q);
+        }
+        B.val[where[q]] += A.val[k];
+      }
+    for (int q : touched) where[q] = -1;
+    B.rp[p + 1] = (int)B.col.size();
+  }
+  return B;
+}
+
+// Aggregation by `passes` pairwise passes (aggregates of up to 2^passes
+// cells).  Aggregates are numbered in the order of their first fine cell,
+// which is also their seed; the other members follow in increasing order.
+void aggregate_pairwise(HostLevel &f, HostLevel &c, int passes) {
+  const double beta = 0.25;
+  const Couplings A = couplings_of(f);
+  std::vector<int> agg;
+  int nc = pair_pass(A, beta, agg);
+  if (passes > 1) {
+    const Couplings B = pair_couplings(A, agg, nc);
+    std::vector<int> agg2;
+    nc = pair_pass(B, beta, agg2);
+    for (int i = 0; i < f.n; ++i) agg[i] = agg2[agg[i]];
+  }
+  std::vector<int> ren(nc, 0);
+  int t = 0;
+  std::vector<int> parents(f.n);
+  for (int i = 0; i < f.n; ++i) {
+    if (ren[agg[i]] == 0) ren[agg[i]] = ++t;
+    parents[i] = ren[agg[i]];
+  }
+  std::vector<int> seeds(nc + 1, 0);
+  std::vector<std::vector<int>> kids(nc + 1);
+  for (int i = 0; i < f.n; ++i) {
+    const int p = parents[i];
+    if (seeds[p] == 0) seeds[p] = i + 1;
+    else kids[p].push_back(i + 1);
+  }
+  finish_level(f, c, parents, seeds, kids, nc);
 }
 
 // Galerkin contribution lists of level c in the reference's order and the
@@ -1100,7 +1226,8 @@ static void build_hierarchy(fs_mesh *m, const double *ac, const double *anb) {
   int nl = 1;
   while (nl < m->mg_maxlevels && H->host[nl - 1].n > 2 * m->mg_minparents) {
     HostLevel c;
-    aggregate(H->host[nl - 1], c, nl + 1);
+    if (m->mg_agg == 0) aggregate(H->host[nl - 1], c, nl + 1);
+    else aggregate_pairwise(H->host[nl - 1], c, m->mg_agg);
     std::vector<int> gp;
     std::vector<long long> gs;
     galerkin_lists(H->host[nl - 1], c, gp, gs);

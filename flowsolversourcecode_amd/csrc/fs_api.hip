@@ -186,7 +186,7 @@ int fs_set_option(fs_mesh *m, int key, double value) {
   // keys that change the hierarchy's structure force a rebuild, but only when
   // their value actually changes
   const int old_levels = m->mg_maxlevels, old_parents = m->mg_minparents, old_tail = m->mg_tail_cells,
-            old_sell = m->mg_sell_rows;
+            old_sell = m->mg_sell_rows, old_agg = m->mg_agg;
   switch (key) {
   case FS_OPT_MG_PRESWEEP: m->mg_pre = iv < 0 ? 0 : iv; break;
   case FS_OPT_MG_POSTSWEEP: m->mg_post = iv < 0 ? 0 : iv; break;
@@ -211,12 +211,19 @@ int fs_set_option(fs_mesh *m, int key, double value) {
   case FS_OPT_MG_COARSE_PRESWEEP: m->mg_cpre = iv < -1 ? -1 : iv; break;
   case FS_OPT_MG_COARSE_POSTSWEEP: m->mg_cpost = iv < -1 ? -1 : iv; break;
   case FS_OPT_MG_SELL_ROWS: m->mg_sell_rows = iv < 0 ? 0 : iv; break;
+  case FS_OPT_MG_AGGREGATION:
+    if (iv < 0 || iv > 2) {
+      set_error("fs_set_option: MG_AGGREGATION must be 0, 1 or 2");
+      return -1;
+    }
+    m->mg_agg = iv;
+    break;
   default:
     set_error("fs_set_option: unknown key " + std::to_string(key));
     return -1;
   }
   if (m->mg_maxlevels != old_levels || m->mg_minparents != old_parents || m->mg_tail_cells != old_tail ||
-      m->mg_sell_rows != old_sell)
+      m->mg_sell_rows != old_sell || m->mg_agg != old_agg)
     m->amg.reset();
   return 0;
 }

@@ -88,10 +88,16 @@ enum fs_option {
                                  preconditioner [0]                           */
   FS_OPT_MG_COARSE_PRESWEEP = 15, /* pre-sweeps on levels >= 1 (-1: as level 0) [-1] */
   FS_OPT_MG_COARSE_POSTSWEEP = 16, /* post-sweeps on levels >= 1 (-1: as level 0) [-1] */
-  FS_OPT_MG_SELL_ROWS = 17    /* coarse levels with at least this many rows keep
+  FS_OPT_MG_SELL_ROWS = 17,   /* coarse levels with at least this many rows keep
                                  their operator in sliced ELL (64-row slices,
                                  rows sorted by length within 256-row windows,
                                  one row per lane); 0 = CSR everywhere [100000] */
+  FS_OPT_MG_AGGREGATION = 18  /* 0 = the reference's greedy agglomeration
+                                 (AgglomorateALevelAlgebraicly, bitwise its
+                                 maps); 1 = pairwise matching of each cell with
+                                 its strongest negative coupling, 2 = two
+                                 pairwise passes per level (aggregates of up to
+                                 4 cells) -- no reference counterpart [0]      */
 };
 
 /* Statistics of the last solve. */

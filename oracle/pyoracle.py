@@ -317,7 +317,7 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
                          pitmax=200, framework="none", scheme="upwind", bleed=0.0, start_iter=0, lim_mom=0,
                          lim_cont=0, plmg=0, algorithm="simple", grad_mom=(2, 2), grad_cont=(2, 2),
                          decomp=3, gi_mom="average", gi_cont="averagecorrected", restart=False, seconds=False,
-                         periodic=None, sst=None):
+                         periodic=None, sst=None, u0=None):
     """niter outer iterations of the reference's own SIMPLE (SolveMomentumX/Y/Z,
     SolveContinuity) from rest on the polyMesh ``pm`` with the Readit topology of
     ``mesh``; patch_types per patch "wall"/"symmetry" (default all walls);
@@ -332,7 +332,8 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
     SST case (see SST_DEFAULTS) with the initial cell fields k, omega, mut,
     wd and boundary-face fields bk, bomega, bmut; the run then also returns
     k, omega, mut (N), bk, bomega, bmut (boundary faces), F1 and the wall
-    distance the run used, wd (N)."""
+    distance the run used, wd (N).  u0: an initial velocity, (u, v, w) cell
+    arrays (default rest)."""
     nf, nif, N = mesh.nf, mesh.nif, mesh.N
     codes = np.array([PATCH_CODES[t] for t in (patch_types or ["wall"] * len(pm.patch_sizes))], np.int32)
     ints = (msolver, mitmax, psolver, pitmax, niter, HR_FRAMEWORKS[framework], CONV_SCHEMES[scheme], lim_mom,
@@ -360,6 +361,10 @@ def run_reference_simple(pm, mesh, wall_u, wall_v, wall_w, patch_types=None, nit
             if sst:
                 for k in ("k", "omega", "mut", "wd", "bk", "bomega", "bmut"):
                     f.write(_c(t[k], np.float64).tobytes())
+            f.write(np.array([0 if u0 is None else 1], np.int32).tobytes())
+            if u0 is not None:
+                for a in u0:
+                    f.write(_c(a, np.float64).tobytes())
         t0 = time.perf_counter()
         r = subprocess.run([REF_SIMPLE, fin, fout], capture_output=True, text=True, cwd=td,
                            env=dict(os.environ, OMP_NUM_THREADS="1"))

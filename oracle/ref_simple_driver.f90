@@ -53,7 +53,8 @@ program ref_simple_driver
   integer :: npoints, nfaces, nif, ncells, npatches, w, lenw, maxfn, mode, idecomp
   integer :: i, j, k, f, q, nbfmax, nb, it
   integer :: msolver, mitmax, psolver, pitmax, nouter, hrfw, hrscheme, limmom, limcont, hrstart, plmg, isimplec
-  integer :: gmm, gmi, gcm, gci, gim, gic, iper, bcalc, biters, bevery, iturb
+  integer :: gmm, gmi, gcm, gci, gim, gic, iper, bcalc, biters, bevery, iturb, hasu0
+  double precision, allocatable :: u0in(:), v0in(:), w0in(:)
   integer :: tint(15)
   double precision :: tdbl(6)
   double precision, allocatable :: tk0(:), tw0(:), tmut0(:), twd(:), tbk0(:), tbw0(:), tbmut0(:)
@@ -132,6 +133,12 @@ program ref_simple_driver
       allocate(tbk0(nfaces - nif), tbw0(nfaces - nif), tbmut0(nfaces - nif))
       read(21) tk0, tw0, tmut0, twd
       read(21) tbk0, tbw0, tbmut0
+    end if
+    ! an initial velocity field (hasu0 = 1) instead of rest
+    read(21) hasu0
+    if (hasu0 /= 0) then
+      allocate(u0in(ncells), v0in(ncells), w0in(ncells))
+      read(21) u0in, v0in, w0in
     end if
   end if
   close(21)
@@ -326,6 +333,9 @@ contains
       call RotationMatrixCoefficients
     end if
     call allocate_flow_storage
+    if (hasu0 /= 0) then
+      uVelocity = u0in; vVelocity = v0in; wVelocity = w0in
+    end if
     if (iturb /= 0) call allocate_turbulence_storage
     f = 0
     do q = 1, npatches

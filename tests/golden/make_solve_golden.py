@@ -5,7 +5,7 @@ PBCG and algebraic-multigrid sources, compiled by oracle/build_ref.sh) on small
 seeded systems from tests/systems.py and stores inputs and outputs as .npz.
 Run in the build container only (needs /root/reference):
 
-    make -C oracle ref && python tests/golden/make_solve_golden.py
+    make -C oracle ref && python tests/golden/make_solve_golden.py [amg_scale]
 """
 import os
 import sys
@@ -47,9 +47,35 @@ def flatten_mg(levels):
     return out
 
 
+# the hierarchy at scale: 32^3 cells, 9 levels with minNumberofParents 1 (the
+# child caps of levels 2, 3, 4 and beyond all in use); the system is rebuilt
+# from its seed, the file holds the maps and the coarse operators only
+SCALE = dict(dims=(32, 32, 32), seed=5, minparents=1, maxlevels=12)
+
+
+def write_amg_scale():
+    nbr, nnbr, ac, anb, bc = systems.quantized_pressure(*SCALE["dims"], SCALE["seed"])
+    r = po.run_reference(nbr, nnbr, ac, anb, bc, np.zeros(len(ac)), 1e-8, 30, 0, mode=0, lmg=1,
+                         minparents=SCALE["minparents"], maxlevels=SCALE["maxlevels"])[0]
+    full = flatten_mg(r["mg"])
+    n0 = len(ac)
+    k0 = int(full["mg_nbr_ptr"][n0])  # the fine level's lists are the input itself
+    rec = {k: full[k] for k in ("mg_n", "mg_parent", "mg_seed", "mg_child", "mg_child_ptr")}
+    rec["mg_nbr"] = full["mg_nbr"][k0:]
+    rec["mg_nbr_ptr"] = full["mg_nbr_ptr"][n0:] - k0
+    rec["mg_anb"] = full["mg_anb"][k0:]
+    rec["mg_ac"] = full["mg_ac"][n0:]
+    path = os.path.join(HERE, "solve_amg_scale.npz")
+    np.savez_compressed(path, **rec)
+    print("wrote", path, os.path.getsize(path), "levels", list(full["mg_n"]))
+
+
 def main():
     if not po.ref_available():
         sys.exit("reference build missing: run make -C oracle ref")
+    if sys.argv[1:] == ["amg_scale"]:
+        write_amg_scale()
+        return
     for name, fn in CASES.items():
         nbr, nnbr, ac, anb, bc = fn()
         n = len(ac)
@@ -67,6 +93,7 @@ def main():
         path = os.path.join(HERE, f"solve_{name}.npz")
         np.savez_compressed(path, **rec)
         print("wrote", path, os.path.getsize(path))
+    write_amg_scale()
 
 
 if __name__ == "__main__":

@@ -59,6 +59,20 @@ def pressure_like(nx, ny, nz, seed, shift=0.0):
     return nbr, nnbr, ac, anb, bc
 
 
+def quantized_pressure(nx, ny, nz, seed):
+    """pressure_like with every coupling rounded to a multiple of 1/256 (at
+    least 1/256 in magnitude), ac = -sum(anb) and a zero-mean source of
+    multiples of 1/8: every Galerkin sum of its coarse levels is exact in
+    FP64, so a fixture of its hierarchy can hold only the maps and the coarse
+    operators and rebuild the system from the seed."""
+    nbr, nnbr, ac, anb, bc = pressure_like(nx, ny, nz, seed)
+    anb = np.where(nbr > 0, np.minimum(-np.round(-anb * 256) / 256, -1 / 256), 0.0)
+    ac = -anb.sum(axis=0)
+    bc = np.random.default_rng(seed + 1000).integers(-8, 9, len(ac)) / 8.0
+    bc = bc - bc.mean()
+    return nbr, nnbr, ac, anb, bc
+
+
 def momentum_like(nx, ny, nz, seed, urf=0.7):
     """Non-symmetric upwind convection-diffusion with implicit under-relaxation
     (ac/urf), the shape of a momentum system."""

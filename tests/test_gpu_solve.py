@@ -111,6 +111,51 @@ def test_amg_maps_and_galerkin_bitwise(case):
     assert np.array_equal(acs, g["mg_ac"]) and np.array_equal(anbs, g["mg_anb"])
 
 
+def test_amg_maps_at_scale_bitwise():
+    """The hierarchy of a 32^3 system (9 levels, minNumberofParents 1: the
+    child caps of levels 2, 3, 4 and every later level in use,
+    AgglomorateALevelAlgebraicly.for:79-105) against the reference build's
+    (tests/golden/solve_amg_scale.npz, the system rebuilt from its seed):
+    parents, seeds, children, coarse neighbour lists and coarse operators of
+    every coarse level bitwise."""
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_solve_golden import SCALE
+    g = golden("solve_amg_scale.npz")
+    nbr, nnbr, ac, anb, bc = systems.quantized_pressure(*SCALE["dims"], SCALE["seed"])
+    m = fs.Mesh(nbr, nnbr)
+    m.set_option("mg_minparents", SCALE["minparents"])
+    m.set_option("mg_maxlevels", SCALE["maxlevels"])
+    print("amg maps at 32^3", flush=True)
+    rc, fit, dphi, res4, st = m.solve_equation("pressc", np.zeros(len(ac)), 1e-8, 200, "bicgs", fs.MG_PRECOND,
+                                               ac, anb, bc)
+    L = m.amg_level(0)["levels"]
+    assert L == len(g["mg_n"]) == 9
+    parents, seeds, kids, kptr, nbs, nbptr, anbs, acs = [], [], [], [0], [], [0], [], []
+    for l in range(L):
+        a = m.amg_level(l)
+        assert a["n"] == g["mg_n"][l]
+        if l < L - 1:
+            parents.append(a["parent"])
+        if l == 0:
+            continue
+        seeds += list(a["seed"])
+        for p in range(a["n"]):
+            kids += list(a["child"][a["child_ptr"][p]:a["child_ptr"][p + 1]])
+            kptr.append(len(kids))
+        acs += list(a["ac"])
+        for p in range(a["n"]):
+            nbs += list(a["nbr"][:a["nnbr"][p], p])
+            anbs += list(a["anb"][:a["nnbr"][p], p])
+            nbptr.append(len(nbs))
+    assert np.array_equal(np.concatenate(parents), g["mg_parent"])
+    assert np.array_equal(seeds, g["mg_seed"])
+    assert np.array_equal(kids, g["mg_child"]) and np.array_equal(kptr, g["mg_child_ptr"])
+    assert np.array_equal(nbs, g["mg_nbr"]) and np.array_equal(nbptr, g["mg_nbr_ptr"])
+    assert np.array_equal(acs, g["mg_ac"]) and np.array_equal(anbs, g["mg_anb"])
+    assert rc == 0
+
+
 @pytest.mark.parametrize("case", ["pshift", "momentum"])
 @pytest.mark.parametrize("solver,lmg,fp32", [("sor", 0, 0), ("bicgs", 0, 0), ("bicgs", 2, 0), ("bicgs", 2, 1),
                                              ("sor", 1, 0), ("sor", 1, 1), ("gmres", 0, 0), ("gmres", 2, 1)])

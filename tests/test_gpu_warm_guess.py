@@ -1,0 +1,78 @@
+"""Initial guesses of the p' solve (fs_solve_equation_device_from_guess,
+the path of the learned warm start): a guess is first scaled to its multiple
+with the smallest L2 residual, then dropped for the zero start when its L1
+residual still exceeds that of the zero start.
+
+Expected behaviour, written in each test: a garbage guess -- NaN or infinite
+entries, as a diverged network would give, or huge noise -- is dropped, and
+the solve is then bitwise the zero-start solve (same dphi, FiT, norms and
+iteration count); a good guess (the solution itself, sign flipped and scaled)
+is kept, rescaled, and converges in fewer iterations; every solve still
+meets the stop rule of the zero start (1e-8 of the zero start's residual)."""
+import numpy as np
+import pytest
+
+import systems
+
+fs = pytest.importorskip("flowsolversourcecode_amd")
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _t(a):
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=DEV)
+
+
+@pytest.fixture(scope="module")
+def system():
+    nbr, nnbr, ac, anb, bc = systems.pressure_like(24, 20, 16, 7, shift=0.02)
+    return nbr, nnbr, ac, anb, bc, systems.sparse(nbr, nnbr, ac, anb)
+
+
+def _solve(system, guess, solver):
+    nbr, nnbr, ac, anb, bc, A = system
+    m = fs.Mesh(nbr, nnbr)
+    fit = _t(np.zeros(len(ac)))
+    dphi = _t(np.zeros(len(ac)) if guess is None else guess)
+    rc, res4, st = m.solve_equation_device("pressc", fit, 1e-8, 400, solver, fs.MG_PRECOND, _t(ac), _t(anb), _t(bc),
+                                           dphi, from_guess=guess is not None)
+    return rc, fit.cpu().numpy(), dphi.cpu().numpy(), res4, st
+
+
+@pytest.mark.parametrize("solver", ["bicgs", "cg", "pbcg"])
+@pytest.mark.parametrize("kind", ["nan", "inf", "noise"])
+def test_garbage_guess_is_dropped(system, solver, kind):
+    nbr, nnbr, ac, anb, bc, A = system
+    n = len(ac)
+    rng = np.random.default_rng(3)
+    guess = {"nan": np.full(n, np.nan), "inf": np.where(np.arange(n) % 2, np.inf, -np.inf),
+             "noise": 1e12 * rng.standard_normal(n)}[kind]
+    if kind == "nan":
+        guess[::3] = 1.0  # some finite entries among the NaN
+    rc0, fit0, dphi0, res0, st0 = _solve(system, None, solver)
+    rc, fit, dphi, res, st = _solve(system, guess, solver)
+    print(f"{kind} guess, {solver}: {st['iterations']} iterations (zero start {st0['iterations']})", flush=True)
+    assert rc == rc0 == 0, (rc, rc0)
+    assert np.abs(bc - A @ dphi).sum() <= 1.0001e-8 * np.abs(bc).sum()
+    if kind != "noise":
+        assert st["iterations"] == st0["iterations"]
+        assert np.array_equal(dphi, dphi0) and np.array_equal(fit, fit0) and np.array_equal(res, res0)
+    else:
+        # noise scaled to its best multiple is almost zero: never worse than a
+        # few extra iterations, and dropped outright when its residual exceeds
+        # the zero start's
+        assert st["iterations"] <= st0["iterations"] + 2
+
+
+@pytest.mark.parametrize("solver", ["bicgs", "cg"])
+def test_good_guess_is_kept_and_saves_iterations(system, solver):
+    nbr, nnbr, ac, anb, bc, A = system
+    rc0, fit0, dphi0, res0, st0 = _solve(system, None, solver)
+    near = -3.0 * dphi0 * (1.0 + 1e-4 * np.sin(np.arange(len(ac))))  # the solution, flipped, scaled, perturbed
+    rc, fit, dphi, res, st = _solve(system, near, solver)
+    print(f"good guess, {solver}: {st['iterations']} iterations (zero start {st0['iterations']})", flush=True)
+    assert rc == 0
+    assert np.abs(bc - A @ dphi).sum() <= 1.0001e-8 * np.abs(bc).sum()
+    assert st["iterations"] < st0["iterations"]

@@ -56,6 +56,41 @@ def test_amg_maps_and_operators_match_reference(case):
     assert np.array_equal(acs, g["mg_ac"]) and np.array_equal(anbs, g["mg_anb"])
 
 
+def test_amg_maps_at_scale_match_reference():
+    """The oracle's agglomeration of the 32^3 system of
+    tests/golden/solve_amg_scale.npz (9 levels, every child cap in use):
+    maps and coarse operators bitwise the reference build's."""
+    import os, sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_solve_golden import SCALE
+    g = golden("solve_amg_scale.npz")
+    nbr, nnbr, ac, anb, bc = systems.quantized_pressure(*SCALE["dims"], SCALE["seed"])
+    h = po.Amg(nbr, nnbr, ac, anb, maxlevels=SCALE["maxlevels"], minparents=SCALE["minparents"])
+    assert h.nlevels == len(g["mg_n"]) == 9
+    parents, seeds, kids, kptr, nbs, nbptr, anbs, acs = [], [], [], [0], [], [0], [], []
+    for l in range(h.nlevels):
+        a = h.level(l)
+        assert a["n"] == g["mg_n"][l]
+        if l < h.nlevels - 1:
+            parents.append(a["parent"])
+        if l == 0:
+            continue
+        seeds += list(a["seed"])
+        for p in range(a["n"]):
+            kids += list(a["child"][a["child_ptr"][p]:a["child_ptr"][p + 1]])
+            kptr.append(len(kids))
+        acs += list(a["ac"])
+        for p in range(a["n"]):
+            nbs += list(a["nbr"][:a["nnbr"][p], p])
+            anbs += list(a["anb"][:a["nnbr"][p], p])
+            nbptr.append(len(nbs))
+    assert np.array_equal(np.concatenate(parents), g["mg_parent"])
+    assert np.array_equal(seeds, g["mg_seed"])
+    assert np.array_equal(kids, g["mg_child"]) and np.array_equal(kptr, g["mg_child_ptr"])
+    assert np.array_equal(nbs, g["mg_nbr"]) and np.array_equal(nbptr, g["mg_nbr_ptr"])
+    assert np.array_equal(acs, g["mg_ac"]) and np.array_equal(anbs, g["mg_anb"])
+
+
 @pytest.mark.parametrize("case", CASES + ["pbig"])
 def test_amg_vcycle_solution_matches_reference(case):
     g = golden(f"solve_{case}.npz")

@@ -1613,15 +1613,23 @@ static T *post_smooth(fs_mesh *m, DLevel &D, int level, int nc, const T *b, T *x
     }
     return x;
   }
+  // the prolongation fused into the first sweep (each row adds the coarse
+  // value of itself and of its neighbours), or as a pass of its own first
+  // (FS_OPT_MG_FUSE_PROLONG 0: cheaper where the neighbours' parent lookups
+  // are scattered)
+  const bool fuse = m->mg_fuse_prolong != 0;
+  if (!fuse)
+    hipLaunchKernelGGL((k_prolong<T, T>), dim3(gfor(D.n)), dim3(kBlock), 0, s, D.n, D.parent.p, xc, x, al, x,
+                       flags);
   T *cur = x, *oth = other;
   for (int k = 0; k < sweeps; ++k) {
-    const bool last = k + 1 == sweeps;
+    const bool last = k + 1 == sweeps, pro = fuse && k == 0;
     if (last && dst) {
-      if (k == 0) launch_sweep<T, TO, true>(m, D, level, b, cur, D.parent.p, xc, nc, al, om, dst, flags);
+      if (pro) launch_sweep<T, TO, true>(m, D, level, b, cur, D.parent.p, xc, nc, al, om, dst, flags);
       else launch_sweep<T, TO, false>(m, D, level, b, cur, nullptr, nullptr, 0, T(0), om, dst, flags);
       return cur;
     }
-    if (k == 0) launch_sweep<T, T, true>(m, D, level, b, cur, D.parent.p, xc, nc, al, om, oth, flags);
+    if (pro) launch_sweep<T, T, true>(m, D, level, b, cur, D.parent.p, xc, nc, al, om, oth, flags);
     else launch_sweep<T, T, false>(m, D, level, b, cur, nullptr, nullptr, 0, T(0), om, oth, flags);
     std::swap(cur, oth);
   }

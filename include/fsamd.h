@@ -92,12 +92,15 @@ enum fs_option {
                                  their operator in sliced ELL (64-row slices,
                                  rows sorted by length within 256-row windows,
                                  one row per lane); 0 = CSR everywhere [100000] */
-  FS_OPT_MG_AGGREGATION = 18  /* 0 = the reference's greedy agglomeration
+  FS_OPT_MG_AGGREGATION = 18, /* 0 = the reference's greedy agglomeration
                                  (AgglomorateALevelAlgebraicly, bitwise its
                                  maps); 1 = pairwise matching of each cell with
                                  its strongest negative coupling, 2 = two
                                  pairwise passes per level (aggregates of up to
                                  4 cells) -- no reference counterpart [0]      */
+  FS_OPT_MG_FUSE_PROLONG = 19 /* 1 = the prolongation of the coarse correction
+                                 fused into the first post-sweep of grid-wide
+                                 levels, 0 = a pass of its own before it [1]   */
 };
 
 /* Statistics of the last solve. */

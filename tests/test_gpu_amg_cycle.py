@@ -52,6 +52,8 @@ VARIANTS = [
     {"mg_sell_rows": 1, "mg_tail_cells": 0, "mg_fp32": 1},
     {"mg_sell_rows": 1, "mg_coarse_presweep": 0, "mg_coarse_postsweep": 2},
     {"mg_sell_rows": 0},                        # CSR on every coarse level
+    {"mg_fuse_prolong": 0},                     # prolongation as a pass of its own
+    {"mg_fuse_prolong": 0, "mg_fp32": 1, "mg_sell_rows": 1},
 ]
 
 
@@ -92,6 +94,22 @@ def test_shifted_variants_match_direct(shifted, solver, opts):
     assert rc == 0, st
     assert np.abs(bc - A @ dphi).sum() <= 1.0001e-8 * np.abs(bc).sum()
     np.testing.assert_allclose(dphi, x, rtol=0, atol=1e-6 * np.abs(x).max())
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_unfused_prolongation_is_bitwise_the_fused(neumann, fp32):
+    """x + alpha*xc[parent] formed in the first post-sweep's loads or written
+    by a pass of its own first: the same values in the same precision, so
+    the same iterates."""
+    nbr, nnbr, ac, anb, bc, A = neumann
+    out = []
+    for fuse in (1, 0):
+        rc, fit, dphi, res4, st = _solve(nbr, nnbr, ac, anb, bc, "cg",
+                                         {"mg_fp32": fp32, "mg_fuse_prolong": fuse, "mg_sell_rows": 1})
+        assert rc == 0, st
+        out.append((dphi, st["iterations"]))
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
 
 
 def test_overcorrection_saves_iterations(neumann):

@@ -950,6 +950,7 @@ struct SimpleCase {
   DevBuf<double> fp, fv, cf, ff, tf, ac, anb, bc, dphi, part;
   DevBuf<double> tf3; // momentum FluxTf of u, v, w (F each)
   DevBuf<double> pcprev; // the last converged p' (input of the warm-start network)
+  bool have_pcprev = false;
   DevBuf<int> btype, bzq; // boundary condition per boundary face (empty: all walls)
   // high-resolution convection and gradient limiting; iter = nIter, the
   // number of the outer iteration in progress (0 before the first)
@@ -1343,11 +1344,16 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
     const int rp = mpnn_predict_device(C.warm, C.s, C.bc.p, C.ac.p, C.anb.p, C.vol.p, C.pcprev.p, C.dphi.p);
     if (rp < 0) return rp;
   }
+  // FS_OPT_WARM_PREVIOUS: without a network, the previous converged p' as the
+  // initial guess (scaled and accepted or dropped like any guess)
+  const bool prev = !warm && C.mesh->warm_prev && C.have_pcprev;
+  if (prev) FS_HIP(hipMemcpyAsync(C.dphi.p, C.pcprev.p, sizeof(double) * C.N, hipMemcpyDeviceToDevice, C.s));
   const int rc = solve_device(C.mesh, 4, C.pc.p, C.prrf, C.pitmax, C.psolver, C.plmg, C.ac.p,
-                              C.anb.p, C.bc.p, C.dphi.p, res4, &ss, warm);
+                              C.anb.p, C.bc.p, C.dphi.p, res4, &ss, warm || prev);
   if (rc < 0) return rc;
   if (C.sink) mpnn_capture(C.sink, C.s, C.bc.p, C.dphi.p, C.ac.p, C.anb.p, C.vol.p, C.pcprev.p);
   FS_HIP(hipMemcpyAsync(C.pcprev.p, C.dphi.p, sizeof(double) * C.N, hipMemcpyDeviceToDevice, C.s));
+  C.have_pcprev = true;
   if (st) {
     st->pc_iterations += ss.iterations;
     st->pc_seconds += ss.seconds;

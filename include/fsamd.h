@@ -98,9 +98,14 @@ enum fs_option {
                                  its strongest negative coupling, 2 = two
                                  pairwise passes per level (aggregates of up to
                                  4 cells) -- no reference counterpart [0]      */
-  FS_OPT_MG_FUSE_PROLONG = 19 /* 1 = the prolongation of the coarse correction
+  FS_OPT_MG_FUSE_PROLONG = 19, /* 1 = the prolongation of the coarse correction
                                  fused into the first post-sweep of grid-wide
                                  levels, 0 = a pass of its own before it [1]   */
+  FS_OPT_WARM_PREVIOUS = 20   /* device SIMPLE case (fsamd_simple.h) without a
+                                 warm-start network: 1 = start each p' solve
+                                 from the previous iteration's p', scaled to
+                                 its best multiple and dropped when worse than
+                                 the zero start [0]                            */
 };
 
 /* Statistics of the last solve. */

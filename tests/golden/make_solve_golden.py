@@ -5,7 +5,7 @@ PBCG and algebraic-multigrid sources, compiled by oracle/build_ref.sh) on small
 seeded systems from tests/systems.py and stores inputs and outputs as .npz.
 Run in the build container only (needs /root/reference):
 
-    make -C oracle ref && python tests/golden/make_solve_golden.py [amg_scale]
+    make -C oracle ref && python tests/golden/make_solve_golden.py [amg_scale | omp_krylov]
 """
 import os
 import sys
@@ -70,11 +70,32 @@ def write_amg_scale():
     print("wrote", path, os.path.getsize(path), "levels", list(full["mg_n"]))
 
 
+def write_omp_krylov():
+    """dphi and the residual norms of the reference's OpenMP 'gmres' and
+    'bicgs' (SolveAlgebraicSystemInParallel, one thread) on the pshift and
+    momentum systems: their fixed tolerances, not the caller's rrF, end
+    these solves."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    out = {}
+    for case in ("pshift", "momentum"):
+        g = np.load(os.path.join(HERE, f"solve_{case}.npz"))
+        for solver, tag in ((4, "gmres"), (5, "bicgs")):
+            r = po.run_reference(g["nbr"], g["nnbr"], g["ac"], g["anb"], g["bc"], g["fit"], 1e-8, 200, solver,
+                                 mode=0)[0]
+            out[f"{case}_{tag}_dphi"], out[f"{case}_{tag}_res"] = r["dphi"], r["res"]
+    path = os.path.join(HERE, "solve_omp_krylov.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
 def main():
     if not po.ref_available():
         sys.exit("reference build missing: run make -C oracle ref")
     if sys.argv[1:] == ["amg_scale"]:
         write_amg_scale()
+        return
+    if sys.argv[1:] == ["omp_krylov"]:
+        write_omp_krylov()
         return
     for name, fn in CASES.items():
         nbr, nnbr, ac, anb, bc = fn()
@@ -94,6 +115,7 @@ def main():
         np.savez_compressed(path, **rec)
         print("wrote", path, os.path.getsize(path))
     write_amg_scale()
+    write_omp_krylov()
 
 
 if __name__ == "__main__":

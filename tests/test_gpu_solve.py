@@ -111,6 +111,33 @@ def test_amg_maps_and_galerkin_bitwise(case):
     assert np.array_equal(acs, g["mg_ac"]) and np.array_equal(anbs, g["mg_anb"])
 
 
+@pytest.mark.parametrize("case", ["pshift", "momentum"])
+@pytest.mark.parametrize("tag", ["gmres", "bicgs"])
+def test_device_krylov_against_the_reference_omp_solvers(case, tag):
+    """The reference's 'gmres' and 'bicgs' (SolveAlgebraicSystemInParallel,
+    tests/golden/solve_omp_krylov.npz) stop at their own fixed tolerances
+    (GMRES_tol 1e-9; BICGSTAB_OMP 1e-5 and 100 iterations) whatever rrF says;
+    the device's versions run to the caller's rrF (1e-8 here).  Expected:
+    the same initial norms (1e-13), the device's true L1 residual within the
+    stop rule (the reference's BiCGSTAB leaves 5e-4 of it on pshift), and the
+    two solutions apart by at most the reference's distance from the exact
+    solution plus 1e-6 of its largest entry."""
+    g = golden(f"solve_{case}.npz")
+    k = golden("solve_omp_krylov.npz")
+    A = systems.dense(g["nbr"], g["nnbr"], g["ac"], g["anb"])
+    x = np.linalg.solve(A, g["bc"])
+    m = _mesh(g)
+    rc, fit, dphi, res4, st = m.solve_equation("pressc", g["fit"], 1e-8, 400, tag, 0, g["ac"], g["anb"], g["bc"])
+    assert rc == 0, st
+    ref = k[f"{case}_{tag}_dphi"]
+    l1, l1_ref = np.abs(g["bc"] - A @ dphi).sum(), np.abs(g["bc"] - A @ ref).sum()
+    print(f"{case} {tag}: device L1 {l1 / np.abs(g['bc']).sum():.1e}, reference {l1_ref / np.abs(g['bc']).sum():.1e}",
+          flush=True)
+    np.testing.assert_allclose(res4, k[f"{case}_{tag}_res"], rtol=1e-13)
+    assert l1 <= 1.0001e-8 * np.abs(g["bc"]).sum()
+    assert np.abs(dphi - ref).max() <= np.abs(ref - x).max() + 1e-6 * np.abs(x).max()
+
+
 def test_amg_maps_at_scale_bitwise():
     """The hierarchy of a 32^3 system (9 levels, minNumberofParents 1: the
     child caps of levels 2, 3, 4 and every later level in use,

@@ -341,29 +341,58 @@ static void prolong_add(or_amg_level *f, const or_amg_level *c) {
   for (int i = 0; i < f->n; ++i) f->dphi[i] = f->dphi[i] + c->dphi[f->parent[i] - 1];
 }
 
-/* vMultiGridCycle (MultiGridCycle.for:32-47) */
-static void vcycle(or_amg *h, double ires, double rrf, int npre, int npost) {
-  int L = h->nlevels;
-  for (int l = 0; l < L - 1; ++l) { /* Restrict, RestrictSolution.for:17-20 */
-    level_smooth(&h->levels[l], ires, rrf, npre);
-    level_residual(&h->levels[l]);
-    restrict_rhs(&h->levels[l], &h->levels[l + 1]);
+/* Restrict(iLevel) (RestrictSolution.for:17-20), levels numbered from 1 as
+ * in the reference: smooth level iLevel, its residual, the next level's
+ * right-hand side */
+static void restrict_at(or_amg *h, int ilevel, double ires, double rrf, int npre) {
+  or_amg_level *f = &h->levels[ilevel - 1];
+  level_smooth(f, ires, rrf, npre);
+  level_residual(f);
+  restrict_rhs(f, &h->levels[ilevel]);
+}
+
+/* Prolongate(iLevel) (ProlongateSolution.for:15-16): level iLevel's
+ * correction into level iLevel-1, then smooth level iLevel-1 */
+static void prolongate_at(or_amg *h, int ilevel, double ires, double rrf, int npost) {
+  prolong_add(&h->levels[ilevel - 2], &h->levels[ilevel - 1]);
+  level_smooth(&h->levels[ilevel - 2], ires, rrf, npost);
+}
+
+/* SolveEquationMG + CalculateResidualsMG on the coarsest level */
+static void coarsest(or_amg *h, double ires, double rrf, int npost) {
+  level_smooth(&h->levels[h->nlevels - 1], ires, rrf, npost);
+  level_residual(&h->levels[h->nlevels - 1]);
+}
+
+/* vMultiGridCycle, fMultiGridCycle, wMultiGridCycle (MultiGridCycle.for:32-47,
+ * 84-127, 162-220): cycle 0 = V, 1 = F, 2 = W */
+static void cycle(or_amg *h, int kind, double ires, double rrf, int npre, int npost) {
+  const int lmax = h->nlevels;
+  for (int il = 1; il <= lmax - 1; ++il) restrict_at(h, il, ires, rrf, npre);
+  coarsest(h, ires, rrf, npost);
+  if (kind >= 1) {
+    for (int l1 = lmax - 1; l1 >= 2; --l1) {
+      for (int il = lmax; il >= l1 + 1; --il) prolongate_at(h, il, ires, rrf, npost);
+      for (int il = l1; il <= lmax - 1; ++il) restrict_at(h, il, ires, rrf, npre);
+      coarsest(h, ires, rrf, npost);
+    }
   }
-  level_smooth(&h->levels[L - 1], ires, rrf, npost);
-  level_residual(&h->levels[L - 1]);
-  for (int l = L - 1; l >= 1; --l) { /* Prolongate, ProlongateSolution.for:15-16 */
-    prolong_add(&h->levels[l - 1], &h->levels[l]);
-    level_smooth(&h->levels[l - 1], ires, rrf, npost);
+  if (kind == 2) {
+    for (int l1 = 3; l1 <= lmax - 1; ++l1) {
+      for (int il = lmax; il >= l1 + 1; --il) prolongate_at(h, il, ires, rrf, npost);
+      for (int il = l1; il <= lmax - 1; ++il) restrict_at(h, il, ires, rrf, npre);
+      coarsest(h, ires, rrf, npost);
+    }
   }
+  for (int il = lmax; il >= 2; --il) prolongate_at(h, il, ires, rrf, npost);
 }
 
 /* Cycle loop of AlgebraicMultigrid (AlgebraicMultiGridMainRoutine.for:371-416)
- * with V-cycles: every level's dphi and residual start at zero once, the
+ * with V, F or W cycles (kind 0, 1, 2): every level's dphi and residual start at zero once, the
  * fine right-hand side is bc, and cycles repeat while the fine L1 residual
  * exceeds rrf*ires.  ires is ResorAbs(NF) of the caller.  Returns cycles run. */
-int or_amg_solve(or_amg *h, const double *bc, double ires, double rrf,
-                 int maxcycles, int npre, int npost, double *dphi,
-                 double *hist) {
+int or_amg_solve_cycle(or_amg *h, int kind, const double *bc, double ires, double rrf,
+                       int maxcycles, int npre, int npost, double *dphi, double *hist) {
   for (int l = 0; l < h->nlevels; ++l) {
     or_amg_level *L = &h->levels[l];
     memset(L->dphi, 0, sizeof(double) * L->n);
@@ -373,13 +402,19 @@ int or_amg_solve(or_amg *h, const double *bc, double ires, double rrf,
   double fres = ires;
   int ncycle = 1;
   while (ncycle <= maxcycles && fres > rrf * ires) {
-    vcycle(h, ires, rrf, npre, npost);
+    cycle(h, kind, ires, rrf, npre, npost);
     fres = level_l1(&h->levels[0]);
     if (hist) hist[ncycle - 1] = fres;
     ncycle++;
   }
   memcpy(dphi, h->levels[0].dphi, sizeof(double) * h->levels[0].n);
   return ncycle - 1;
+}
+
+int or_amg_solve(or_amg *h, const double *bc, double ires, double rrf,
+                 int maxcycles, int npre, int npost, double *dphi,
+                 double *hist) {
+  return or_amg_solve_cycle(h, 0, bc, ires, rrf, maxcycles, npre, npost, dphi, hist);
 }
 
 /* ---- flat accessors for the ctypes wrapper ---- */

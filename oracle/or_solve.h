@@ -72,6 +72,8 @@ void or_amg_update(or_amg *h, const double *ac, const double *anb);
 int or_amg_solve(or_amg *h, const double *bc, double ires, double rrf,
                  int maxcycles, int npre, int npost, double *dphi,
                  double *hist);
+int or_amg_solve_cycle(or_amg *h, int kind, const double *bc, double ires, double rrf,
+                       int maxcycles, int npre, int npost, double *dphi, double *hist);
 void or_amg_free(or_amg *h);
 or_amg *or_amg_new(int n, int w, const int *nbr, const int *nnbr,
                    const double *ac, const double *anb, int maxlevels,

@@ -452,6 +452,9 @@ class Amg:
         L.or_amg_level_copy.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, _ip, _ip, _ip, _ip, _ip]
         L.or_amg_solve.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.or_amg_solve.restype = C.c_int
+        L.or_amg_solve_cycle.argtypes = [C.c_void_p, C.c_int, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                         C.c_int, _dp, _dp]
+        L.or_amg_solve_cycle.restype = C.c_int
         L.or_amg_update.argtypes = [C.c_void_p, _dp, _dp]
         w, n = nbr.shape
         self.n = n
@@ -485,10 +488,12 @@ class Amg:
     def update(self, ac, anb):
         lib().or_amg_update(self.h, _c(ac, np.float64), _c(anb, np.float64))
 
-    def solve(self, bc, ires, rrf, maxcycles=30, npre=3, npost=5):
+    def solve(self, bc, ires, rrf, maxcycles=30, npre=3, npost=5, cycle=0):
+        """cycle 0 = V, 1 = F, 2 = W (MultiGridCycleType)."""
         dphi = np.zeros(self.n)
         hist = np.zeros(max(maxcycles, 1))
-        k = lib().or_amg_solve(self.h, _c(bc, np.float64), ires, rrf, maxcycles, npre, npost, dphi, hist)
+        k = lib().or_amg_solve_cycle(self.h, int(cycle), _c(bc, np.float64), ires, rrf, maxcycles, npre, npost,
+                                     dphi, hist)
         return dphi, hist[:k].copy()
 
 

@@ -5,7 +5,7 @@ PBCG and algebraic-multigrid sources, compiled by oracle/build_ref.sh) on small
 seeded systems from tests/systems.py and stores inputs and outputs as .npz.
 Run in the build container only (needs /root/reference):
 
-    make -C oracle ref && python tests/golden/make_solve_golden.py [amg_scale | omp_krylov]
+    make -C oracle ref && python tests/golden/make_solve_golden.py [amg_scale | omp_krylov | amg_cycles]
 """
 import os
 import sys
@@ -88,6 +88,21 @@ def write_omp_krylov():
     print("wrote", path, os.path.getsize(path))
 
 
+def write_amg_cycles():
+    """dphi of the reference's multigrid as the solver with F and W cycles
+    (2 pre- and 3 post-sweeps, minNumberofParents 2) on pshift and pbig."""
+    out = {}
+    for case in ("pshift", "pbig"):
+        g = np.load(os.path.join(HERE, f"solve_{case}.npz"))
+        for cycle, tag in ((1, "f"), (2, "w")):
+            r = po.run_reference(g["nbr"], g["nnbr"], g["ac"], g["anb"], g["bc"], g["fit"], float(g["rrf"]), 30, 0,
+                                 mode=0, lmg=1, cycle=cycle, npre=2, npost=3, minparents=2)[0]
+            out[f"{case}_{tag}_dphi"] = r["dphi"]
+    path = os.path.join(HERE, "solve_amg_cycles.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
 def main():
     if not po.ref_available():
         sys.exit("reference build missing: run make -C oracle ref")
@@ -96,6 +111,9 @@ def main():
         return
     if sys.argv[1:] == ["omp_krylov"]:
         write_omp_krylov()
+        return
+    if sys.argv[1:] == ["amg_cycles"]:
+        write_amg_cycles()
         return
     for name, fn in CASES.items():
         nbr, nnbr, ac, anb, bc = fn()
@@ -116,6 +134,7 @@ def main():
         print("wrote", path, os.path.getsize(path))
     write_amg_scale()
     write_omp_krylov()
+    write_amg_cycles()
 
 
 if __name__ == "__main__":

@@ -101,6 +101,27 @@ def test_amg_vcycle_solution_matches_reference(case):
     assert np.array_equal(dphi, g["amg_dphi"])
 
 
+@pytest.mark.parametrize("cycle", [1, 2])
+@pytest.mark.parametrize("case", ["pshift", "pbig"])
+def test_amg_f_and_w_cycles_match_reference(case, cycle):
+    """F (1) and W (2) cycles (MultiGridCycle.for:55,133): the oracle's dphi
+    after the cycle loop equals the reference build's SolveEquation with
+    MultiGridCycleType 'fcycle' / 'wcycle' bitwise
+    (tests/golden/solve_amg_cycles.npz, and ref_solve.x itself where it is
+    built)."""
+    g = golden(f"solve_{case}.npz")
+    n = len(g["ac"])
+    h = po.Amg(g["nbr"], g["nnbr"], g["ac"], g["anb"], maxlevels=10, minparents=2)
+    res0 = po.residuals(g["nbr"], g["nnbr"], g["ac"], g["anb"], g["bc"], np.zeros(n), g["fit"])
+    dphi, hist = h.solve(g["bc"], res0[0], float(g["rrf"]), 30, 2, 3, cycle=cycle)
+    assert hist[-1] < 0.5 * res0[0]
+    assert np.array_equal(dphi, golden("solve_amg_cycles.npz")[f"{case}_{'fw'[cycle - 1]}_dphi"])
+    if po.ref_available():
+        r = po.run_reference(g["nbr"], g["nnbr"], g["ac"], g["anb"], g["bc"], g["fit"], float(g["rrf"]), 30, 0,
+                             mode=0, lmg=1, cycle=cycle, npre=2, npost=3, minparents=2)[0]
+        assert np.array_equal(dphi, r["dphi"])
+
+
 # ---------------------------------------------------------------- pins by maths
 
 def test_galerkin_equals_PtAP():

@@ -296,7 +296,7 @@ def cell_heights(pm):
 
 
 def channel_case(args, fs, pm, types, pairs, tr, fp32, alpha, pre=2, post=2, omega=0.8, wd=None, agg=0,
-                 psolver="bicgs", fuse=1):
+                 psolver="bicgs", fuse=1, maxlevels=10):
     """The headline's case: SIMPLE with second-order upwind (TVD sou) and the
     min/max limiter on the velocity gradients, momentum and k, omega by
     multicolour Gauss-Seidel (rrF 0.1, <= 10 sweeps), p' BiCGSTAB + AMG to
@@ -315,7 +315,8 @@ def channel_case(args, fs, pm, types, pairs, tr, fp32, alpha, pre=2, post=2, ome
     c.set_convection("tvd", "sou")
     c.set_gradient_limiters(momentum=1)
     for key, v in (("mg_fp32", fp32), ("mg_alpha", alpha), ("mg_presweep", pre), ("mg_postsweep", post),
-                   ("mg_omega", omega), ("mg_aggregation", agg), ("mg_fuse_prolong", fuse)):
+                   ("mg_omega", omega), ("mg_aggregation", agg), ("mg_fuse_prolong", fuse),
+                   ("mg_maxlevels", maxlevels)):
         c.set_option(key, v)
     passes = 0
     if wd is None and args.poisson_wall_distance:
@@ -394,7 +395,8 @@ def channel_main(args):
     pm, types, pairs, tr = channel_mesh(dims, meshgen)
     print(f"# mesh {dims} x 6: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     tuned = dict(alpha=args.ch_alpha, pre=args.ch_pre, post=args.ch_post, omega=args.ch_omega,
-                 agg=args.mg_aggregation, psolver=args.ch_psolver, fuse=args.ch_fuse_prolong)
+                 agg=args.mg_aggregation, psolver=args.ch_psolver, fuse=args.ch_fuse_prolong,
+                 maxlevels=args.ch_maxlevels)
     t0 = time.perf_counter()
     case, wd, passes = channel_case(args, fs, pm, types, pairs, tr, 1 if args.precision == "fp32amg" else 0,
                                     **tuned)
@@ -528,6 +530,7 @@ def channel_main(args):
                    "p_initial_guess": "previous p', scaled" if args.ch_warm_previous else "zero",
                    "vcycle": {"presweeps": args.ch_pre, "postsweeps": args.ch_post, "alpha": args.ch_alpha,
                               "omega": args.ch_omega, "fused_prolongation": bool(args.ch_fuse_prolong),
+                              "max_levels": args.ch_maxlevels,
                               "aggregation": ["reference greedy", "pairwise, 1 pass",
                                               "pairwise, 2 passes"][args.mg_aggregation]}},
         "pc_solve_ms": 1e3 * st["pc_seconds"] / args.steps,
@@ -582,6 +585,10 @@ def main():
     # (scaled to its best multiple, dropped when worse than zero): 10.05
     # against 9.13 it/s from zero on the 7.86M-tet channel
     ap.add_argument("--ch-warm-previous", type=int, default=1, choices=[0, 1])
+    # 12 levels bring the coarsest level of the 7.86M-tet hierarchy under the
+    # 64 cells of the direct FP64 solve (10 leave about 130 cells to 16 sweeps):
+    # 57 instead of 61 CG iterations per solve
+    ap.add_argument("--ch-maxlevels", type=int, default=12)
     ap.add_argument("--tet-n", type=int, default=64, help="hexahedra per side of the tetrahedral cavity variant")
     ap.add_argument("--clock-stride", type=int, default=5,
                     help="time every n-th launch of the roofline kernels with HIP events")

@@ -196,3 +196,79 @@ def iterations_with_and_without(net, run, n=16, iterations=60):
     diff = max(np.abs(fields["zero"][k] - fields["learned"][k]).max() / max(np.abs(fields["zero"][k]).max(), 1e-30)
                for k in fields["zero"])
     return out, diff
+
+
+def c1_case(re, perturb, seed, n=64, warm_previous=0):
+    """A member of the two-dimensional cavity family of the C1 settings:
+    n x n x 1 hexahedra, interior nodes perturbed by U(-perturb, perturb) of
+    the spacing, lid speed 1 (Reynolds number re, rho 1), urf 0.9 / 0.1,
+    momentum by Gauss-Seidel (rrF 0.1, <= 10 sweeps), p' by PBCG with one AMG
+    V-cycle as its preconditioner to 1e-8."""
+    m, (bu, bv, bw), types = meshgen.cavity_2d(n, perturb=perturb, seed=seed)
+    case = Case(m, bu, bv, bw, rho=1.0, mu=1.0 / re)
+    case.configure(urf_u=0.9, urf_p=0.1, msolver="sor", mrrf=0.1, mitmax=10, psolver="pbcg", prrf=1e-8,
+                   pitmax=500, plmg=MG_PRECOND)
+    case.set_patch_types(m.patch_sizes, types)
+    case.set_option("warm_previous", warm_previous)
+    return case
+
+
+def c4_study(snapshots=20000, per_run=200, epochs=6, batch=16, heldout=4, iterations=100, seed=0, log=None):
+    """The learned form at its configured scale: `snapshots` (bc, p') pairs
+    written by the SIMPLE loops of C1-family cavities (Re U(100, 2000),
+    perturbation U(0, 0.2), fresh seeds; `per_run` outer iterations each,
+    the previous p' among the inputs), the network trained on the device,
+    then held-out members of the family run `iterations` outer iterations
+    three times: p' from zero, from the previous iteration's p' and from the
+    network's guess.  Returns a dict of the p' iteration totals."""
+    rng = np.random.default_rng(seed)
+    probe = c1_case(1000.0, 0.1, 1)
+    net = Mpnn.for_case(probe, layers=3, seed=7, capacity=snapshots, use_prev=True)
+    del probe
+    runs = 0
+    while net.count < snapshots:
+        re, pert = float(rng.uniform(100.0, 2000.0)), float(rng.uniform(0.0, 0.2))
+        case = c1_case(re, pert, 1000 + runs)
+        attach(case, sink=net)
+        case.iterate(min(per_run, snapshots - net.count))
+        attach(case)
+        runs += 1
+        if log:
+            log("collect", runs, net.count)
+    hist = net.train(epochs=epochs, batch=batch, lr=2e-3, seed=seed, log=(lambda e, l: log("epoch", e, l))
+                     if log else None)
+    out = {"snapshots": net.count, "runs": runs, "epochs": epochs, "loss_first_epoch": hist[0],
+           "loss_last_epoch": hist[-1], "heldout": []}
+    for h in range(heldout):
+        re, pert = float(rng.uniform(100.0, 2000.0)), float(rng.uniform(0.0, 0.2))
+        rec = {"re": re, "perturb": pert}
+        for mode in ("zero", "previous", "learned"):
+            case = c1_case(re, pert, 900000 + h, warm_previous=int(mode == "previous"))
+            if mode == "learned":
+                attach(case, warm=net)
+            st = case.iterate(iterations)
+            attach(case)
+            rec[mode] = int(st["pc_iterations"])
+            rec[mode + "_unconverged"] = int(st["pc_unconverged"])
+        out["heldout"].append(rec)
+        if log:
+            log("heldout", rec)
+    for mode in ("zero", "previous", "learned"):
+        out[mode] = sum(r[mode] for r in out["heldout"])
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="the learned warm start on the C1 cavity family (C4)")
+    ap.add_argument("--snapshots", type=int, default=20000)
+    ap.add_argument("--per-run", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--heldout", type=int, default=4)
+    ap.add_argument("--iterations", type=int, default=100)
+    a = ap.parse_args()
+    res = c4_study(a.snapshots, a.per_run, a.epochs, heldout=a.heldout, iterations=a.iterations,
+                   log=lambda *x: print("#", *x, file=sys.stderr, flush=True))
+    print(json.dumps(res))

@@ -527,7 +527,8 @@ def channel_main(args):
                    "wall_distance": (f"Poisson method on the device, {passes} passes to 1e-6"
                                      if args.poisson_wall_distance else "min(y, 2 - y) (plane channel)"),
                    "p_solver": args.ch_psolver,
-                   "p_initial_guess": "previous p', scaled" if args.ch_warm_previous else "zero",
+                   "p_initial_guess": {0: "zero", 1: "previous p', scaled"}.get(
+                       args.ch_warm_previous, f"projection on the last {args.ch_warm_previous} p' fields, scaled"),
                    "vcycle": {"presweeps": args.ch_pre, "postsweeps": args.ch_post, "alpha": args.ch_alpha,
                               "omega": args.ch_omega, "fused_prolongation": bool(args.ch_fuse_prolong),
                               "max_levels": args.ch_maxlevels,
@@ -584,7 +585,7 @@ def main():
     # each p' solve of the headline starts from the previous iteration's p'
     # (scaled to its best multiple, dropped when worse than zero): 10.05
     # against 9.13 it/s from zero on the 7.86M-tet channel
-    ap.add_argument("--ch-warm-previous", type=int, default=1, choices=[0, 1])
+    ap.add_argument("--ch-warm-previous", type=int, default=2, choices=[0, 1, 2, 3, 4])
     # 12 levels bring the coarsest level of the 7.86M-tet hierarchy under the
     # 64 cells of the direct FP64 solve (10 leave about 130 cells to 16 sweeps):
     # 57 instead of 61 CG iterations per solve

@@ -951,6 +951,10 @@ struct SimpleCase {
   DevBuf<double> tf3; // momentum FluxTf of u, v, w (F each)
   DevBuf<double> pcprev; // the last converged p' (input of the warm-start network)
   bool have_pcprev = false;
+  // the last ring_m converged p' fields (FS_OPT_WARM_PREVIOUS >= 2), the
+  // latest at ring[(ring_head + ring_m - 1) % ring_m]
+  DevBuf<double> ring[kProjMax];
+  int ring_m = 0, ring_n = 0, ring_head = 0;
   DevBuf<int> btype, bzq; // boundary condition per boundary face (empty: all walls)
   // high-resolution convection and gradient limiting; iter = nIter, the
   // number of the outer iteration in progress (0 before the first)
@@ -1345,15 +1349,41 @@ static int phase_pc_solve_correct(SimpleCase &C, fs_simple_stats *st) {
     if (rp < 0) return rp;
   }
   // FS_OPT_WARM_PREVIOUS: without a network, the previous converged p' as the
-  // initial guess (scaled and accepted or dropped like any guess)
-  const bool prev = !warm && C.mesh->warm_prev && C.have_pcprev;
-  if (prev) FS_HIP(hipMemcpyAsync(C.dphi.p, C.pcprev.p, sizeof(double) * C.N, hipMemcpyDeviceToDevice, C.s));
+  // initial guess, or (2..4) the combination of the last m p' fields with the
+  // smallest residual for this system; either is then scaled and accepted or
+  // dropped like any guess
+  const int pm = C.mesh->warm_prev;
+  if (pm >= 2 && pm != C.ring_m) {
+    C.ring_m = pm;
+    C.ring_n = C.ring_head = 0;
+  }
+  const bool proj = !warm && pm >= 2 && C.ring_n > 0;
+  const bool prev = !warm && (proj || (pm == 1 && C.have_pcprev));
+  if (proj) {
+    const double *X[kProjMax];
+    for (int a = 0; a < C.ring_n; ++a) X[a] = C.ring[(C.ring_head + 2 * C.ring_m - 1 - a) % C.ring_m].p;
+    try {
+      project_guess(C.mesh, C.ac.p, C.anb.p, C.bc.p, X, C.ring_n, C.dphi.p);
+    } catch (const std::exception &e) {
+      set_error(e.what());
+      return -2;
+    }
+  } else if (prev) {
+    FS_HIP(hipMemcpyAsync(C.dphi.p, C.pcprev.p, sizeof(double) * C.N, hipMemcpyDeviceToDevice, C.s));
+  }
   const int rc = solve_device(C.mesh, 4, C.pc.p, C.prrf, C.pitmax, C.psolver, C.plmg, C.ac.p,
                               C.anb.p, C.bc.p, C.dphi.p, res4, &ss, warm || prev);
   if (rc < 0) return rc;
   if (C.sink) mpnn_capture(C.sink, C.s, C.bc.p, C.dphi.p, C.ac.p, C.anb.p, C.vol.p, C.pcprev.p);
   FS_HIP(hipMemcpyAsync(C.pcprev.p, C.dphi.p, sizeof(double) * C.N, hipMemcpyDeviceToDevice, C.s));
   C.have_pcprev = true;
+  if (pm >= 2) {
+    DevBuf<double> &r = C.ring[C.ring_head];
+    if (r.n != (size_t)C.N) r.alloc(C.N);
+    FS_HIP(hipMemcpyAsync(r.p, C.dphi.p, sizeof(double) * C.N, hipMemcpyDeviceToDevice, C.s));
+    C.ring_head = (C.ring_head + 1) % C.ring_m;
+    if (C.ring_n < C.ring_m) ++C.ring_n;
+  }
   if (st) {
     st->pc_iterations += ss.iterations;
     st->pc_seconds += ss.seconds;

@@ -49,6 +49,14 @@ int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int sol
                  const double *ac, const double *anb, const double *bc, double *dphi, double *res4,
                  fs_solve_stats *st, bool warm = false);
 
+// x0 = the combination of the k (at most kProjMax) device vectors X[0..k-1]
+// with the smallest L2 residual |bc - A x0|; vectors that add nothing to the
+// span of those before them get weight 0.  Uses wk.v[0..k-1] and the
+// partials; synchronises the stream once.  Returns the vectors used.
+constexpr int kProjMax = 4;
+int project_guess(fs_mesh *m, const double *ac, const double *anb, const double *bc, const double *const *X, int k,
+                  double *x0);
+
 // AMG (fs_amg.hip).  amg_prepare builds or refreshes the hierarchy for the
 // fine operator (ac, anb): aggregation on the first call and every
 // mg_reagg calls, device Galerkin operators otherwise.

@@ -1257,6 +1257,135 @@ static void scale_guess(fs_mesh *m, const double *ac, const double *anb, const d
   hipLaunchKernelGGL(k_warm_scale, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, m->wk.scal, m->wk.flags);
 }
 
+// ---------------------------------------------------------------- projection
+// Projection of the right-hand side onto earlier solutions (Fischer, "Projection
+// techniques for iterative solution of Ax = b with successive right-hand
+// sides", CMAME 163, 1998): x0 = sum_a c_a x_a with the c that minimise
+// |b - A x0|_2, i.e. G c = h with G_ab = (A x_a, A x_b), h_a = (A x_a, b).
+
+constexpr int kProjGrid = 2048; // blocks of the Gram pass: 14 partial rows fit the partial buffer
+
+struct ProjVecs {
+  const double *p[kProjMax];
+};
+struct ProjCoef {
+  double c[kProjMax];
+};
+
+// Partials of the upper triangle of G and of h, row u of the order
+// (G_00, h_0, G_01, ..., G_11, h_1, ...) at part[u * gridDim.x + block].
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_proj_gram(int n, ProjVecs y, const double *__restrict__ b,
+                                                      double *__restrict__ part) {
+  constexpr int NS = K * (K + 1) / 2 + K;
+  double s[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) s[u] = 0.0;
+  for (int i = xcd_block() * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double v[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) v[a] = y.p[a][i];
+    const double bi = b[i];
+    int u = 0;
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+#pragma unroll
+      for (int c = a; c < K; ++c) s[u++] += v[a] * v[c];
+      s[u++] += v[a] * bi;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const double t = block_sum<kBlock>(s[u]);
+    if (threadIdx.x == 0) part[(size_t)u * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// out[u] = sum of row u of the partials, in a fixed order
+__global__ void __launch_bounds__(kBlock) k_proj_fin(int ns, int gp, const double *__restrict__ part,
+                                                     double *__restrict__ out) {
+  for (int u = 0; u < ns; ++u) {
+    double t = 0.0;
+    for (int j = threadIdx.x; j < gp; j += kBlock) t += part[(size_t)u * gp + j];
+    t = block_sum<kBlock>(t);
+    if (threadIdx.x == 0) out[u] = t;
+  }
+}
+
+__global__ void k_proj_comb(int n, int k, ProjVecs x, ProjCoef c, double *__restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int a = 0; a < k; ++a) s += c.c[a] * x.p[a][i];
+  out[i] = s;
+}
+
+int project_guess(fs_mesh *m, const double *ac, const double *anb, const double *bc, const double *const *X, int k,
+                  double *x0) {
+  if (k > kProjMax) k = kProjMax;
+  if (k <= 0) return 0;
+  const int n = m->n;
+  hipStream_t s = m->stream;
+  ProjVecs yv{}, xv{};
+  for (int a = 0; a < k; ++a) {
+    yv.p[a] = m->wk.v[a];
+    xv.p[a] = X[a];
+    hipLaunchKernelGGL(k_spmv, dim3(grid_for(n)), dim3(kBlock), 0, s, n, m->w, m->d_nbr.p, ac, anb, X[a],
+                       m->wk.v[a]);
+  }
+  const int ns = k * (k + 1) / 2 + k;
+  double *part = m->wk.partial, *out = m->wk.partial + (size_t)ns * kProjGrid;
+  switch (k) {
+  case 1: hipLaunchKernelGGL(k_proj_gram<1>, dim3(kProjGrid), dim3(kBlock), 0, s, n, yv, bc, part); break;
+  case 2: hipLaunchKernelGGL(k_proj_gram<2>, dim3(kProjGrid), dim3(kBlock), 0, s, n, yv, bc, part); break;
+  case 3: hipLaunchKernelGGL(k_proj_gram<3>, dim3(kProjGrid), dim3(kBlock), 0, s, n, yv, bc, part); break;
+  default: hipLaunchKernelGGL(k_proj_gram<4>, dim3(kProjGrid), dim3(kBlock), 0, s, n, yv, bc, part); break;
+  }
+  hipLaunchKernelGGL(k_proj_fin, dim3(1), dim3(kBlock), 0, s, ns, kProjGrid, part, out);
+  double g[kProjMax * (kProjMax + 1) / 2 + kProjMax];
+  FS_HIP_THROW(hipMemcpyAsync(g, out, sizeof(double) * ns, hipMemcpyDeviceToHost, s));
+  FS_HIP_THROW(hipStreamSynchronize(s));
+  double G[kProjMax][kProjMax] = {}, h[kProjMax] = {};
+  for (int a = 0, u = 0; a < k; ++a) {
+    for (int c = a; c < k; ++c) G[a][c] = G[c][a] = g[u++];
+    h[a] = g[u++];
+  }
+  // Cholesky of G over the vectors in order; a vector whose part outside
+  // the span of those before it is below 1e-10 of its norm (or not finite)
+  // is left out, so the earliest vectors (the latest solutions) are kept
+  double L[kProjMax][kProjMax] = {}, z[kProjMax] = {};
+  bool use[kProjMax] = {};
+  int used = 0;
+  for (int j = 0; j < k; ++j) {
+    double d = G[j][j];
+    for (int p = 0; p < j; ++p) d -= L[j][p] * L[j][p];
+    if (!(G[j][j] > 0.0) || !std::isfinite(G[j][j]) || !(d > 1e-10 * G[j][j])) continue;
+    use[j] = true;
+    ++used;
+    L[j][j] = std::sqrt(d);
+    for (int i = j + 1; i < k; ++i) {
+      double t = G[i][j];
+      for (int p = 0; p < j; ++p) t -= L[i][p] * L[j][p];
+      L[i][j] = t / L[j][j];
+    }
+  }
+  for (int j = 0; j < k; ++j) {
+    if (!use[j]) continue;
+    double t = h[j];
+    for (int p = 0; p < j; ++p) t -= L[j][p] * z[p];
+    z[j] = t / L[j][j];
+  }
+  ProjCoef cf{};
+  for (int j = k - 1; j >= 0; --j) {
+    if (!use[j]) continue;
+    double t = z[j];
+    for (int i = j + 1; i < k; ++i) t -= L[i][j] * cf.c[i];
+    cf.c[j] = t / L[j][j];
+  }
+  hipLaunchKernelGGL(k_proj_comb, dim3(grid_for(n)), dim3(kBlock), 0, s, n, k, xv, cf, x0);
+  return used;
+}
+
 int solve_device(fs_mesh *m, int nf, double *fit, double rrf, int itmax, int solver, int lmg,
                  const double *ac, const double *anb, const double *bc, double *dphi, double *res4,
                  fs_solve_stats *st, bool warm) {

@@ -105,7 +105,10 @@ enum fs_option {
                                  warm-start network: 1 = start each p' solve
                                  from the previous iteration's p', scaled to
                                  its best multiple and dropped when worse than
-                                 the zero start [0]                            */
+                                 the zero start; 2..4 = from the combination of
+                                 the last 2..4 p' fields with the smallest L2
+                                 residual (projection), scaled and dropped the
+                                 same way [0]                                  */
 };
 
 /* Statistics of the last solve. */

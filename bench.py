@@ -401,6 +401,7 @@ def channel_main(args):
     case, wd, passes = channel_case(args, fs, pm, types, pairs, tr, 1 if args.precision == "fp32amg" else 0,
                                     **tuned)
     case.set_option("warm_previous", args.ch_warm_previous)
+    case.set_option("mg_cycle", args.ch_cycle)
     setup_s = time.perf_counter() - t0
     n_cells, n_faces = case.N, case.F
     print(f"# setup {setup_s:.1f} s: N={n_cells} faces={n_faces}", file=sys.stderr, flush=True)
@@ -529,7 +530,7 @@ def channel_main(args):
                    "p_solver": args.ch_psolver,
                    "p_initial_guess": {0: "zero", 1: "previous p', scaled"}.get(
                        args.ch_warm_previous, f"projection on the last {args.ch_warm_previous} p' fields, scaled"),
-                   "vcycle": {"presweeps": args.ch_pre, "postsweeps": args.ch_post, "alpha": args.ch_alpha,
+                   "vcycle": {"cycle": "VFW"[args.ch_cycle], "presweeps": args.ch_pre, "postsweeps": args.ch_post, "alpha": args.ch_alpha,
                               "omega": args.ch_omega, "fused_prolongation": bool(args.ch_fuse_prolong),
                               "max_levels": args.ch_maxlevels,
                               "aggregation": ["reference greedy", "pairwise, 1 pass",
@@ -585,6 +586,8 @@ def main():
     # each p' solve of the headline starts from the previous iteration's p'
     # (scaled to its best multiple, dropped when worse than zero): 10.05
     # against 9.13 it/s from zero on the 7.86M-tet channel
+    ap.add_argument("--ch-cycle", type=int, default=0, choices=[0, 1, 2],
+                    help="channel p' AMG cycle: 0 V, 1 F, 2 W")
     ap.add_argument("--ch-warm-previous", type=int, default=2, choices=[0, 1, 2, 3, 4])
     # 12 levels bring the coarsest level of the 7.86M-tet hierarchy under the
     # 64 cells of the direct FP64 solve (10 leave about 130 cells to 16 sweeps):

@@ -36,7 +36,7 @@ OPTIONS = {"mg_presweep": 1, "mg_postsweep": 2, "mg_maxcycles": 3, "mg_maxlevels
            "mg_smoother": 9, "gmres_restart": 10, "check_every": 11,
            "mg_tail_cells": 12, "mg_coarse_sweeps": 13, "mg_alpha": 14, "mg_coarse_presweep": 15,
            "mg_coarse_postsweep": 16, "mg_sell_rows": 17, "mg_aggregation": 18,
-           "mg_fuse_prolong": 19, "warm_previous": 20}
+           "mg_fuse_prolong": 19, "warm_previous": 20, "mg_cycle": 21}
 
 
 class FsError(RuntimeError):

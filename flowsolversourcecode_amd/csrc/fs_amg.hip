@@ -1669,6 +1669,82 @@ static void vcycle(fs_mesh *m, const double *rhs, double *out, double alpha, con
   std::vector<T *> res(L, nullptr), oth(L, nullptr);
   // levels above the bottom: grid-wide kernels
   const int top = ts < L ? ts : L - 1;
+  // the restriction from level l into l + 1 also starts level l + 1: mode 0
+  // hands it to the tail kernel, 1 makes the first Jacobi sweep from zero,
+  // 2 zeroes it (Gauss-Seidel)
+  auto mode_from = [&](int l) {
+    if (l + 1 == ts) return 0;                                // the tail kernel takes over
+    if (l + 1 == L - 1) return 1;                             // grid-wide coarsest: `coarse` sweeps
+    return (!use_gs(m, H.lev[l + 1]) && cpre > 0) ? 1 : 2;
+  };
+  auto restrict_from = [&](int l, const T *b, const T *x) {
+    DLevel &D = H.lev[l], &C = H.lev[l + 1];
+    const int mode = mode_from(l);
+    if (l > 0 && !use_gs(m, D) && !D.sell) {
+      launch_resid_restrict<T>(m, D, C, b, x, om, mode, flags);
+    } else {
+      launch_resid<T>(m, D, b, x, LV<T>::r(D), flags);
+      hipLaunchKernelGGL(k_restrict<T>, dim3(gfor(C.n)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p, LV<T>::r(D),
+                         LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    }
+  };
+  // the levels below the grid-wide ones
+  auto bottom = [&]() {
+    if (ts < L) {
+      DLevel &Dt = H.lev[ts];
+      hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), ti.desc.lds_bytes, s, ti.ddesc.p, ti.img.p,
+                         LV<T>::b(Dt), LV<T>::x(Dt), cpre, cpost, coarse, om, al, flags);
+      FS_HIP_THROW(hipGetLastError()); // a refused launch (LDS request) must not pass silently
+      res[ts] = LV<T>::x(Dt);
+    } else {
+      DLevel &Dc = H.lev[L - 1];
+      // the restriction took the first of the coarse sweeps
+      T *o = nullptr;
+      res[L - 1] = pre_smooth<T>(m, Dc, L - 1, LV<T>::b(Dc), coarse, om, o, flags);
+    }
+  };
+  // prolongation and post-smoothing from the bottom up to level stop (level
+  // 0 writes out)
+  auto ascend = [&](int stop) {
+    for (int l = top - 1; l >= stop; --l) {
+      DLevel &D = H.lev[l];
+      const T *b = l == 0 ? b0 : LV<T>::b(D);
+      const int npost = l == 0 ? post0 : cpost;
+      const int nc = H.lev[l + 1].n;
+      if (l == 0) {
+        post_smooth<T, double>(m, D, 0, nc, b, res[0], oth[0], res[1], npost, om, al, out, flags);
+      } else {
+        res[l] = post_smooth<T, T>(m, D, l, nc, b, res[l], oth[l], res[l + 1], npost, om, al, (T *)nullptr, flags);
+        oth[l] = res[l] == LV<T>::x(D) ? LV<T>::t(D) : LV<T>::x(D);
+      }
+    }
+  };
+  // a second coarse-grid correction of level l1 >= 1 (F and W cycles):
+  // pre-sweeps from its current iterate, then a fresh descent (every coarser
+  // level from zero) to the bottom
+  auto redescend = [&](int l1) {
+    DLevel &D = H.lev[l1];
+    const T *b = LV<T>::b(D);
+    T *cur = res[l1], *o = oth[l1];
+    if (use_gs(m, D)) {
+      gs_sweeps<T>(m, D, b, cur, cpre, flags);
+    } else {
+      for (int k = 0; k < cpre; ++k) {
+        launch_sweep<T, T, false>(m, D, l1, b, cur, nullptr, nullptr, 0, T(0), om, o, flags);
+        std::swap(cur, o);
+      }
+    }
+    res[l1] = cur;
+    oth[l1] = o;
+    restrict_from(l1, b, cur);
+    for (int l = l1 + 1; l < top; ++l) {
+      DLevel &Dl = H.lev[l];
+      res[l] = pre_smooth<T>(m, Dl, l, LV<T>::b(Dl), cpre, om, oth[l], flags);
+      restrict_from(l, LV<T>::b(Dl), res[l]);
+    }
+    bottom();
+  };
+
   // one Jacobi pre-sweep on the finest level: the entry and the residual of
   // that sweep in one kernel
   const bool fuse0 = !use_gs(m, D0) && pre0 == 1 && top > 0;
@@ -1681,52 +1757,37 @@ static void vcycle(fs_mesh *m, const double *rhs, double *out, double alpha, con
     DLevel &D = H.lev[l];
     const T *b = l == 0 ? b0 : LV<T>::b(D);
     const int npre = l == 0 ? pre0 : cpre;
-    DLevel &C = H.lev[l + 1];
-    int mode;
-    if (l + 1 == ts) mode = 0;                                  // the tail kernel takes over
-    else if (l + 1 == L - 1) mode = 1;                          // grid-wide coarsest: `coarse` sweeps
-    else mode = (!use_gs(m, C) && cpre > 0) ? 1 : 2;
     if (l == 0 && fuse0) {
+      DLevel &C = H.lev[1];
       hipLaunchKernelGGL(k_enter_resid<T>, dim3(gfor(D0.n)), dim3(kBlock), 0, s, D0.n, D0.w,
                          EllNbr(D0.nbr, D0.nbr16), LV<T>::ac(D0), LV<T>::anb(D0), rhs, fp64 ? nullptr : LV<T>::b(D0),
                          LV<T>::x(D0), LV<T>::r(D0), om, flags);
       res[0] = LV<T>::x(D0);
       oth[0] = LV<T>::t(D0);
       hipLaunchKernelGGL(k_restrict<T>, dim3(gfor(C.n)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p, LV<T>::r(D0),
-                         LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+                         LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode_from(0), flags);
       continue;
     }
     res[l] = pre_smooth<T>(m, D, l, b, npre, om, oth[l], flags);
-    if (l > 0 && !use_gs(m, D) && !D.sell) {
-      launch_resid_restrict<T>(m, D, C, b, res[l], om, mode, flags);
-    } else {
-      launch_resid<T>(m, D, b, res[l], LV<T>::r(D), flags);
-      hipLaunchKernelGGL(k_restrict<T>, dim3(gfor(C.n)), dim3(kBlock), 0, s, C.n, C.mptr.p, C.mem.p, LV<T>::r(D),
-                         LV<T>::b(C), LV<T>::ac(C), LV<T>::x(C), om, mode, flags);
+    restrict_from(l, b, res[l]);
+  }
+  bottom();
+  // F and W cycles (FS_OPT_MG_CYCLE 1, 2) in the pattern of the reference's
+  // fMultiGridCycle and wMultiGridCycle (MultiGridCycle.for:84-127, 162-220):
+  // back up to level l1 and down again, for l1 from the deepest grid-wide
+  // level up to level 1, and for W once more from level 2 down to the
+  // deepest; the levels from the tail's first level down count as one
+  if (m->mg_cycle >= 1)
+    for (int l1 = top - 1; l1 >= 1; --l1) {
+      ascend(l1);
+      redescend(l1);
     }
-  }
-  if (ts < L) {
-    DLevel &Dt = H.lev[ts];
-    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), ti.desc.lds_bytes, s, ti.ddesc.p, ti.img.p,
-                       LV<T>::b(Dt), LV<T>::x(Dt), cpre, cpost, coarse, om, al, flags);
-    FS_HIP_THROW(hipGetLastError()); // a refused launch (LDS request) must not pass silently
-    res[ts] = LV<T>::x(Dt);
-  } else {
-    DLevel &Dc = H.lev[L - 1];
-    // the restriction took the first of the coarse sweeps
-    T *o = nullptr;
-    res[L - 1] = pre_smooth<T>(m, Dc, L - 1, LV<T>::b(Dc), coarse, om, o, flags);
-  }
-  for (int l = top - 1; l >= 0; --l) {
-    DLevel &D = H.lev[l];
-    const T *b = l == 0 ? b0 : LV<T>::b(D);
-    const int npost = l == 0 ? post0 : cpost;
-    const int nc = H.lev[l + 1].n;
-    if (l == 0)
-      post_smooth<T, double>(m, D, 0, nc, b, res[0], oth[0], res[1], npost, om, al, out, flags);
-    else
-      res[l] = post_smooth<T, T>(m, D, l, nc, b, res[l], oth[l], res[l + 1], npost, om, al, (T *)nullptr, flags);
-  }
+  if (m->mg_cycle == 2)
+    for (int l1 = 2; l1 <= top - 1; ++l1) {
+      ascend(l1);
+      redescend(l1);
+    }
+  ascend(0);
 }
 
 void amg_apply(fs_mesh *m, const double *rhs, double *out, const int *flags, double alpha) {

@@ -212,6 +212,13 @@ int fs_set_option(fs_mesh *m, int key, double value) {
   case FS_OPT_MG_COARSE_POSTSWEEP: m->mg_cpost = iv < -1 ? -1 : iv; break;
   case FS_OPT_MG_SELL_ROWS: m->mg_sell_rows = iv < 0 ? 0 : iv; break;
   case FS_OPT_MG_FUSE_PROLONG: m->mg_fuse_prolong = iv != 0; break;
+  case FS_OPT_MG_CYCLE:
+    if (iv < 0 || iv > 2) {
+      set_error("fs_set_option: MG_CYCLE must be 0 (V), 1 (F) or 2 (W)");
+      return -1;
+    }
+    m->mg_cycle = iv;
+    break;
   case FS_OPT_WARM_PREVIOUS:
     if (iv < 0 || iv > fs::kProjMax) {
       set_error("fs_set_option: WARM_PREVIOUS must lie in [0, 4]");

@@ -138,6 +138,7 @@ struct fs_mesh {
   int mg_minparents = 10, mg_reagg = 300, mg_fp32 = 0, mg_smoother = 0;
   int mg_agg = 0;           // FS_OPT_MG_AGGREGATION
   int mg_fuse_prolong = 1;  // FS_OPT_MG_FUSE_PROLONG
+  int mg_cycle = 0;         // FS_OPT_MG_CYCLE: 0 V, 1 F, 2 W
   int warm_prev = 0;        // FS_OPT_WARM_PREVIOUS (read by the SIMPLE case)
   double mg_omega = 0.8;
   int mg_tail_cells = 1024, mg_coarse_sweeps = 16;

@@ -108,7 +108,14 @@ enum fs_option {
                                  the zero start; 2..4 = from the combination of
                                  the last 2..4 p' fields with the smallest L2
                                  residual (projection), scaled and dropped the
-                                 same way [0]                                  */
+                                 same way [0]                                  */,
+  FS_OPT_MG_CYCLE = 21        /* device AMG cycle: 0 = V, 1 = F, 2 = W in the
+                                 pattern of MultiGridCycle.for:84-127, 162-220
+                                 (back up to level l and down again, for l from
+                                 the deepest grid-wide level up to 1; W once
+                                 more from 2 down), every coarser level of a
+                                 descent starting from zero; the one-workgroup
+                                 tail counts as one level [0]                  */
 };
 
 /* Statistics of the last solve. */

@@ -54,6 +54,11 @@ VARIANTS = [
     {"mg_sell_rows": 0},                        # CSR on every coarse level
     {"mg_fuse_prolong": 0},                     # prolongation as a pass of its own
     {"mg_fuse_prolong": 0, "mg_fp32": 1, "mg_sell_rows": 1},
+    {"mg_cycle": 1},                            # F cycle
+    {"mg_cycle": 2, "mg_fp32": 1},              # W cycle
+    {"mg_cycle": 2, "mg_tail_cells": 0, "mg_fp32": 1},
+    {"mg_cycle": 1, "mg_sell_rows": 1, "mg_fuse_prolong": 0},
+    {"mg_cycle": 2, "mg_smoother": 1},
 ]
 
 
@@ -73,8 +78,9 @@ def _solve(nbr, nnbr, ac, anb, bc, solver, opts, lmg=2):
 def test_neumann_variants_reach_1e8(neumann, solver, opts):
     asym = opts.get("mg_presweep", 2) != opts.get("mg_postsweep", 2) or \
         opts.get("mg_coarse_presweep", -1) != opts.get("mg_coarse_postsweep", -1)
-    if solver == "cg" and asym:
-        pytest.skip("unequal pre/post sweeps give a non-symmetric cycle: not a CG preconditioner")
+    if solver == "cg" and (asym or opts.get("mg_cycle", 0)):
+        pytest.skip("unequal pre/post sweeps, or the extra descents of F and W cycles, give a non-symmetric "
+                    "cycle: not a CG preconditioner")
     nbr, nnbr, ac, anb, bc, A = neumann
     rc, fit, dphi, res4, st = _solve(nbr, nnbr, ac, anb, bc, solver, opts)
     assert rc == 0, st
@@ -146,13 +152,28 @@ def test_gmres_on_nonsymmetric_momentum_system():
 def test_multigrid_as_solver_with_tail(shifted):
     """The reference's multigrid-as-solver loop (alpha 1) on the new cycle."""
     nbr, nnbr, ac, anb, bc, A, x = shifted
-    for fp32 in (0, 1):
-        m = fs.Mesh(nbr, nnbr)
-        m.set_option("mg_fp32", fp32)
-        m.set_option("mg_maxcycles", 300)
-        rc, fit, dphi, res4, st = m.solve_equation("pressc", np.zeros(len(ac)), 1e-8, 300, "sor", 1, ac, anb, bc)
-        assert rc == 0, st
-        np.testing.assert_allclose(dphi, x, rtol=0, atol=1e-6 * np.abs(x).max())
+    its = {}
+    for cyc in (0, 1, 2):
+        for fp32 in (0, 1):
+            m = fs.Mesh(nbr, nnbr)
+            m.set_option("mg_fp32", fp32)
+            m.set_option("mg_maxcycles", 300)
+            m.set_option("mg_cycle", cyc)
+            rc, fit, dphi, res4, st = m.solve_equation("pressc", np.zeros(len(ac)), 1e-8, 300, "sor", 1, ac, anb,
+                                                       bc)
+            assert rc == 0, st
+            np.testing.assert_allclose(dphi, x, rtol=0, atol=1e-6 * np.abs(x).max())
+            its[cyc, fp32] = st["iterations"]
+    print("cycles to 1e-8 (V, F, W):", [its[c, 0] for c in (0, 1, 2)], flush=True)
+    # the extra coarse-grid corrections of F and W cycles never cost cycles
+    assert its[1, 0] <= its[0, 0] and its[2, 0] <= its[0, 0]
+
+
+def test_bad_cycle_rejected():
+    nbr, nnbr, ac, anb, bc = systems.pressure_like(4, 4, 4, 1)
+    m = fs.Mesh(nbr, nnbr)
+    with pytest.raises(fs.FsError):
+        m.set_option("mg_cycle", 3)
 
 
 def test_bad_alpha_rejected():

@@ -402,6 +402,9 @@ def channel_main(args):
                                     **tuned)
     case.set_option("warm_previous", args.ch_warm_previous)
     case.set_option("mg_cycle", args.ch_cycle)
+    if args.ch_coarse_sweeps is not None:
+        case.set_option("mg_coarse_presweep", args.ch_coarse_sweeps[0])
+        case.set_option("mg_coarse_postsweep", args.ch_coarse_sweeps[1])
     setup_s = time.perf_counter() - t0
     n_cells, n_faces = case.N, case.F
     print(f"# setup {setup_s:.1f} s: N={n_cells} faces={n_faces}", file=sys.stderr, flush=True)
@@ -530,7 +533,8 @@ def channel_main(args):
                    "p_solver": args.ch_psolver,
                    "p_initial_guess": {0: "zero", 1: "previous p', scaled"}.get(
                        args.ch_warm_previous, f"projection on the last {args.ch_warm_previous} p' fields, scaled"),
-                   "vcycle": {"cycle": "VFW"[args.ch_cycle], "presweeps": args.ch_pre, "postsweeps": args.ch_post, "alpha": args.ch_alpha,
+                   "vcycle": {"cycle": "VFW"[args.ch_cycle], "coarse_sweeps": args.ch_coarse_sweeps,
+                              "presweeps": args.ch_pre, "postsweeps": args.ch_post, "alpha": args.ch_alpha,
                               "omega": args.ch_omega, "fused_prolongation": bool(args.ch_fuse_prolong),
                               "max_levels": args.ch_maxlevels,
                               "aggregation": ["reference greedy", "pairwise, 1 pass",
@@ -586,6 +590,8 @@ def main():
     # each p' solve of the headline starts from the previous iteration's p'
     # (scaled to its best multiple, dropped when worse than zero): 10.05
     # against 9.13 it/s from zero on the 7.86M-tet channel
+    ap.add_argument("--ch-coarse-sweeps", type=int, nargs=2, default=None, metavar=("PRE", "POST"),
+                    help="channel p' AMG sweeps on the levels below the finest (default: as on the finest)")
     ap.add_argument("--ch-cycle", type=int, default=0, choices=[0, 1, 2],
                     help="channel p' AMG cycle: 0 V, 1 F, 2 W")
     ap.add_argument("--ch-warm-previous", type=int, default=2, choices=[0, 1, 2, 3, 4])

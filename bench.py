@@ -5,7 +5,7 @@ hexahedra graded 1.05 toward both walls, each split into 6 tetrahedra (Kuhn;
 the prescribed mean pressure gradient, periodic streamwise, symmetry
 spanwise, second-order upwind momentum with the min/max gradient limiter,
 FP64 assembly, p' by CG to 1e-8 (L1, relative) preconditioned by an
-aggregation-AMG V-cycle whose hierarchy runs in FP32; variants on the same
+aggregation-AMG F-cycle whose hierarchy runs in FP32; variants on the same
 line: the plain FP64 form (BiCGSTAB + FP64 V-cycle), the headline's cycle in
 FP64, and the learned warm start.  ``--workload cavity`` runs the 128^3
 perturbed lid-driven cavity instead (Re = 1000, BiCGSTAB + FP32 AMG).
@@ -296,7 +296,7 @@ def cell_heights(pm):
 
 
 def channel_case(args, fs, pm, types, pairs, tr, fp32, alpha, pre=2, post=2, omega=0.8, wd=None, agg=0,
-                 psolver="bicgs", fuse=1, maxlevels=10):
+                 psolver="bicgs", fuse=1, maxlevels=10, cycle=0, csweeps=None):
     """The headline's case: SIMPLE with second-order upwind (TVD sou) and the
     min/max limiter on the velocity gradients, momentum and k, omega by
     multicolour Gauss-Seidel (rrF 0.1, <= 10 sweeps), p' BiCGSTAB + AMG to
@@ -316,8 +316,11 @@ def channel_case(args, fs, pm, types, pairs, tr, fp32, alpha, pre=2, post=2, ome
     c.set_gradient_limiters(momentum=1)
     for key, v in (("mg_fp32", fp32), ("mg_alpha", alpha), ("mg_presweep", pre), ("mg_postsweep", post),
                    ("mg_omega", omega), ("mg_aggregation", agg), ("mg_fuse_prolong", fuse),
-                   ("mg_maxlevels", maxlevels)):
+                   ("mg_maxlevels", maxlevels), ("mg_cycle", cycle)):
         c.set_option(key, v)
+    if csweeps is not None:
+        c.set_option("mg_coarse_presweep", csweeps[0])
+        c.set_option("mg_coarse_postsweep", csweeps[1])
     passes = 0
     if wd is None and args.poisson_wall_distance:
         t0 = time.perf_counter()
@@ -396,15 +399,11 @@ def channel_main(args):
     print(f"# mesh {dims} x 6: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     tuned = dict(alpha=args.ch_alpha, pre=args.ch_pre, post=args.ch_post, omega=args.ch_omega,
                  agg=args.mg_aggregation, psolver=args.ch_psolver, fuse=args.ch_fuse_prolong,
-                 maxlevels=args.ch_maxlevels)
+                 maxlevels=args.ch_maxlevels, cycle=args.ch_cycle, csweeps=args.ch_coarse_sweeps)
     t0 = time.perf_counter()
     case, wd, passes = channel_case(args, fs, pm, types, pairs, tr, 1 if args.precision == "fp32amg" else 0,
                                     **tuned)
     case.set_option("warm_previous", args.ch_warm_previous)
-    case.set_option("mg_cycle", args.ch_cycle)
-    if args.ch_coarse_sweeps is not None:
-        case.set_option("mg_coarse_presweep", args.ch_coarse_sweeps[0])
-        case.set_option("mg_coarse_postsweep", args.ch_coarse_sweeps[1])
     setup_s = time.perf_counter() - t0
     n_cells, n_faces = case.N, case.F
     print(f"# setup {setup_s:.1f} s: N={n_cells} faces={n_faces}", file=sys.stderr, flush=True)
@@ -575,24 +574,28 @@ def main():
     ap.add_argument("--mg-aggregation", type=int, default=0, choices=[0, 1, 2],
                     help="aggregation of the channel's headline hierarchy: 0 the reference's greedy agglomeration, "
                          "1 or 2 pairwise passes per level")
-    # the channel's headline p' solve: CG (the p' matrix is symmetric) with the
-    # symmetric V-cycle, 2/2 damped-Jacobi sweeps (0.9), additive correction,
-    # the prolongation as a pass of its own; on the 7.86M-tet channel CG beats
-    # BiCGSTAB with the same cycle (81 against 91 ms per p' solve) and every
-    # tuned or pairwise variant measured, damping 0.9 and the unfused
-    # prolongation each take a further 5-6 %
+    # the channel's headline p' solve: CG (the p' matrix is symmetric) with an
+    # F-cycle, 1/1 damped-Jacobi sweeps (0.9) on every level, coarse
+    # corrections over-relaxed by 1.3, the prolongation as a pass of its own.
+    # On the 7.86M-tet channel (10 iterations after 3, ms per p' solve): the
+    # V-cycle with 2/2 sweeps and alpha 1 76, F with 1/1 sweeps 64, F with
+    # alpha 1.2 48 and 1.4 43; from alpha 1.6 the cycle stops being a usable
+    # CG preconditioner (solves hit itmax), so 1.3 keeps a margin.  Among V
+    # cycles CG beats BiCGSTAB (81 against 91 ms) and every tuned or pairwise
+    # variant measured; damping 0.9 and the unfused prolongation each take a
+    # further 5-6 %
     ap.add_argument("--ch-psolver", default="cg", choices=["cg", "bicgs", "gmres", "pbcg"])
-    ap.add_argument("--ch-alpha", type=float, default=1.0)
-    ap.add_argument("--ch-pre", type=int, default=2)
-    ap.add_argument("--ch-post", type=int, default=2)
+    ap.add_argument("--ch-alpha", type=float, default=1.3)
+    ap.add_argument("--ch-pre", type=int, default=1)
+    ap.add_argument("--ch-post", type=int, default=1)
     ap.add_argument("--ch-omega", type=float, default=0.9)
     ap.add_argument("--ch-fuse-prolong", type=int, default=0, choices=[0, 1])
     # each p' solve of the headline starts from the previous iteration's p'
     # (scaled to its best multiple, dropped when worse than zero): 10.05
     # against 9.13 it/s from zero on the 7.86M-tet channel
-    ap.add_argument("--ch-coarse-sweeps", type=int, nargs=2, default=None, metavar=("PRE", "POST"),
-                    help="channel p' AMG sweeps on the levels below the finest (default: as on the finest)")
-    ap.add_argument("--ch-cycle", type=int, default=0, choices=[0, 1, 2],
+    ap.add_argument("--ch-coarse-sweeps", type=int, nargs=2, default=[1, 1], metavar=("PRE", "POST"),
+                    help="channel p' AMG sweeps on the levels below the finest")
+    ap.add_argument("--ch-cycle", type=int, default=1, choices=[0, 1, 2],
                     help="channel p' AMG cycle: 0 V, 1 F, 2 W")
     ap.add_argument("--ch-warm-previous", type=int, default=2, choices=[0, 1, 2, 3, 4])
     # 12 levels bring the coarsest level of the 7.86M-tet hierarchy under the

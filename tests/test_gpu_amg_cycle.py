@@ -169,6 +169,22 @@ def test_multigrid_as_solver_with_tail(shifted):
     assert its[1, 0] <= its[0, 0] and its[2, 0] <= its[0, 0]
 
 
+@pytest.mark.parametrize("alpha", [1.0, 1.3])
+def test_cg_with_the_channel_f_cycle(neumann, alpha):
+    """The channel's p' composition: CG preconditioned by the FP32 F-cycle with
+    1/1 sweeps on every level and the coarse corrections over-relaxed.  The
+    cycle is not symmetric, but CG's residual recurrence does not depend on
+    the preconditioner, so a converged solve meets the stop rule on the true
+    residual."""
+    nbr, nnbr, ac, anb, bc, A = neumann
+    opts = {"mg_cycle": 1, "mg_alpha": alpha, "mg_presweep": 1, "mg_postsweep": 1, "mg_coarse_presweep": 1,
+            "mg_coarse_postsweep": 1, "mg_fp32": 1, "mg_fuse_prolong": 0}
+    rc, fit, dphi, res4, st = _solve(nbr, nnbr, ac, anb, bc, "cg", opts)
+    assert rc == 0, st
+    assert np.abs(bc - A @ dphi).sum() <= 1.0001e-8 * np.abs(bc).sum()
+    assert st["iterations"] < 60, st
+
+
 def test_bad_cycle_rejected():
     nbr, nnbr, ac, anb, bc = systems.pressure_like(4, 4, 4, 1)
     m = fs.Mesh(nbr, nnbr)
